@@ -1,0 +1,34 @@
+"""geo_simulator_amd -- MI355X (gfx950) assemble-and-solve path of
+geo_simulator: Tet10 element kernels, deterministic CSR assembly with
+Dirichlet elimination, and Jacobi-preconditioned CG, in hand-written HIP
+behind a C ABI (include/geofem.h, libgeofem.so).
+
+The Python names mirror the reference's Rust API:
+    ImprovedMeshGenerator.generate_cube, Mesh, DofManager,
+    Assembler.assemble_thermal_stiffness_parallel / assemble_thermal_load /
+    assemble_elasticity_stiffness_parallel / assemble_gravity_load /
+    apply_dirichlet_bcs, IsotropicElasticity, ConjugateGradient.solve,
+    SolverStats, JacobiPreconditioner.
+Importing the package needs no GPU; compute calls need one and the built
+library (there is no CPU fallback).
+"""
+from ._lib import GeoFemError, LIB_PATH
+from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
+from .dof import DofManager
+from .linalg import ConjugateGradient, CsrMatrix, JacobiPreconditioner, SolverStats, norm
+from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
+                       thermal_element_backward)
+from .runtime import Context, default_context
+
+__all__ = [
+    "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
+    "DofManager", "ConjugateGradient", "CsrMatrix", "JacobiPreconditioner",
+    "SolverStats", "norm", "Assembler", "IsotropicElasticity",
+    "thermal_element_backward", "elastic_element_backward", "Context",
+    "default_context",
+]
+
+
+def build(verbose: bool = False) -> str:
+    from .csrc.build import build as _build
+    return _build(verbose)

@@ -1,0 +1,205 @@
+"""Global assembly on the GPU, mirroring `Assembler` (src/fem/assembly.rs) and
+`IsotropicElasticity` (src/mechanics/constitutive.rs:16-135).
+
+Stiffness matrices come back as device-resident `CsrMatrix` handles (export
+with `.to_scipy()`); load vectors come back as numpy arrays like the
+reference's Vec<f64>, or as device tensors with device=True.  The reference's
+panics surface as `GeoFemError` (or ValueError for invalid materials, as the
+constructor asserts).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import lib
+from .linalg import CsrMatrix, _is_torch
+from .runtime import default_context, dptr
+
+
+class IsotropicElasticity:
+    """src/mechanics/constitutive.rs:16-135."""
+
+    def __init__(self, youngs_modulus: float, poisson_ratio: float):
+        if not youngs_modulus > 0.0:
+            raise ValueError("Young's modulus must be positive")
+        if not (-1.0 < poisson_ratio < 0.5):
+            raise ValueError("Poisson's ratio must be in (-1, 0.5) for physical stability")
+        self.youngs_modulus = float(youngs_modulus)
+        self.poisson_ratio = float(poisson_ratio)
+
+    @classmethod
+    def new(cls, E, nu):
+        return cls(E, nu)
+
+    def constitutive_matrix(self) -> np.ndarray:
+        E, nu = self.youngs_modulus, self.poisson_ratio
+        factor = E / ((1.0 + nu) * (1.0 - 2.0 * nu))
+        D = np.zeros((6, 6))
+        D[:3, :3] = nu
+        np.fill_diagonal(D[:3, :3], 1.0 - nu)
+        D[3, 3] = D[4, 4] = D[5, 5] = (1.0 - 2.0 * nu) / 2.0
+        return D * factor
+
+    def lame_parameters(self):
+        E, nu = self.youngs_modulus, self.poisson_ratio
+        return (E * nu) / ((1.0 + nu) * (1.0 - 2.0 * nu)), E / (2.0 * (1.0 + nu))
+
+    def bulk_modulus(self) -> float:
+        E, nu = self.youngs_modulus, self.poisson_ratio
+        return E / (3.0 * (1.0 - 2.0 * nu))
+
+
+def _elem_array(ctx, a, n_elems):
+    if a is None:
+        return None
+    t = ctx.to_device(a)
+    if t.numel() != n_elems:
+        raise ValueError("per-element array length differs from the element count")
+    return t
+
+
+def _out(ctx, t, device):
+    return t if device else t.cpu().numpy()
+
+
+class Assembler:
+    # ---- thermal -----------------------------------------------------------
+    @staticmethod
+    def assemble_thermal_stiffness_parallel(mesh, dof_mgr, conductivity,
+                                            k_per_elem=None, ctx=None):
+        """src/fem/assembly.rs:69-117 (per-element k: additive overload)."""
+        ctx = ctx or default_context()
+        ke = _elem_array(ctx, k_per_elem, mesh.num_elements())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_thermal_stiffness(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), float(conductivity),
+            dptr(ke), C.byref(h)), "assemble_thermal_stiffness_parallel")
+        return CsrMatrix(ctx, h)
+
+    assemble_thermal_stiffness_serial = assemble_thermal_stiffness_parallel
+
+    @staticmethod
+    def assemble_thermal_load(mesh, dof_mgr, source, device=False, ctx=None):
+        """src/fem/assembly.rs:130-154."""
+        ctx = ctx or default_context()
+        f = ctx.empty(dof_mgr.total_dofs())
+        ctx.check(lib().gs_assemble_thermal_load(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), float(source),
+            dptr(f)), "assemble_thermal_load")
+        return _out(ctx, f, device)
+
+    # ---- elasticity -----------------------------------------------------------
+    @staticmethod
+    def assemble_elasticity_stiffness_parallel(mesh, dof_mgr, material,
+                                               E_per_elem=None, ctx=None):
+        """src/fem/assembly.rs:389-437 (per-element E: additive overload)."""
+        ctx = ctx or default_context()
+        ee = _elem_array(ctx, E_per_elem, mesh.num_elements())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_elasticity_stiffness(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx),
+            material.youngs_modulus, material.poisson_ratio, dptr(ee),
+            C.byref(h)), "assemble_elasticity_stiffness_parallel")
+        return CsrMatrix(ctx, h)
+
+    assemble_elasticity_stiffness_serial = assemble_elasticity_stiffness_parallel
+
+    @staticmethod
+    def assemble_gravity_load(mesh, dof_mgr, density, gravity, device=False,
+                              ctx=None):
+        """src/fem/assembly.rs:906-937."""
+        ctx = ctx or default_context()
+        g = np.ascontiguousarray(gravity, dtype=np.float64).reshape(3)
+        f = ctx.empty(dof_mgr.total_dofs())
+        ctx.check(lib().gs_assemble_gravity_load(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), float(density),
+            g.ctypes.data_as(C.POINTER(C.c_double)), dptr(f)),
+            "assemble_gravity_load")
+        return _out(ctx, f, device)
+
+    # ---- Dirichlet ---------------------------------------------------------
+    @staticmethod
+    def apply_dirichlet_bcs(K, f, dof_mgr, device=None):
+        """src/fem/assembly.rs:168-230 -> (K_bc, f_bc)."""
+        if not isinstance(K, CsrMatrix):
+            K = CsrMatrix.from_scipy(K)
+        ctx = K.ctx
+        host = not _is_torch(f) if device is None else not device
+        fd = ctx.to_device(f)
+        if fd.numel() != K.shape[0]:
+            raise ValueError("load vector length differs from the matrix rows")
+        fbc = ctx.empty(K.shape[0])
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_apply_dirichlet_bcs(ctx.handle, K.handle, dptr(fd),
+                                               dof_mgr.handle(ctx), C.byref(h),
+                                               dptr(fbc)), "apply_dirichlet_bcs")
+        return CsrMatrix(ctx, h), (fbc.cpu().numpy() if host else fbc)
+
+    # ---- fused device-resident pipelines --------------------------------------
+    @staticmethod
+    def assemble_thermal_system(mesh, dof_mgr, conductivity, source,
+                                k_per_elem=None, ctx=None):
+        """assemble_thermal_stiffness_parallel + assemble_thermal_load +
+        apply_dirichlet_bcs on the device -> (K_bc, f_bc tensor)."""
+        ctx = ctx or default_context()
+        ke = _elem_array(ctx, k_per_elem, mesh.num_elements())
+        fbc = ctx.empty(dof_mgr.total_dofs())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_thermal_system(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), float(conductivity),
+            dptr(ke), float(source), C.byref(h), dptr(fbc)),
+            "assemble_thermal_system")
+        return CsrMatrix(ctx, h), fbc
+
+    @staticmethod
+    def assemble_elasticity_system(mesh, dof_mgr, material, density, gravity,
+                                   E_per_elem=None, ctx=None):
+        """assemble_elasticity_stiffness_parallel + assemble_gravity_load +
+        apply_dirichlet_bcs on the device -> (K_bc, f_bc tensor)."""
+        ctx = ctx or default_context()
+        ee = _elem_array(ctx, E_per_elem, mesh.num_elements())
+        g = np.ascontiguousarray(gravity, dtype=np.float64).reshape(3)
+        fbc = ctx.empty(dof_mgr.total_dofs())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_elasticity_system(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx),
+            material.youngs_modulus, material.poisson_ratio, dptr(ee),
+            float(density), g.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h),
+            dptr(fbc)), "assemble_elasticity_system")
+        return CsrMatrix(ctx, h), fbc
+
+
+def thermal_element_backward(mesh, k_per_elem, u, lam, ctx=None):
+    """grad_e = -lambda_e^T K_e(k_e) u_e = dL/dlog k_e (device tensors)."""
+    ctx = ctx or default_context()
+    k = ctx.to_device(k_per_elem)
+    ud, ld = ctx.to_device(u), ctx.to_device(lam)
+    g = ctx.empty(mesh.num_elements())
+    ctx.sync_torch()
+    ctx.check(lib().gs_thermal_element_backward(ctx.handle, mesh.handle(ctx),
+                                                dptr(k), dptr(ud), dptr(ld),
+                                                dptr(g)),
+              "thermal_element_backward")
+    return g
+
+
+def elastic_element_backward(mesh, E_per_elem, poisson, u, lam, ctx=None):
+    """grad_e = -lambda_e^T K_e(E_e) u_e = dL/dlog E_e (device tensors)."""
+    ctx = ctx or default_context()
+    E = ctx.to_device(E_per_elem)
+    ud, ld = ctx.to_device(u), ctx.to_device(lam)
+    g = ctx.empty(mesh.num_elements())
+    ctx.sync_torch()
+    ctx.check(lib().gs_elastic_element_backward(ctx.handle, mesh.handle(ctx),
+                                                dptr(E), float(poisson),
+                                                dptr(ud), dptr(ld), dptr(g)),
+              "elastic_element_backward")
+    return g

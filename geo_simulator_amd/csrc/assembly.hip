@@ -1,0 +1,490 @@
+// Deterministic CSR assembly and Dirichlet elimination on gfx950.
+//
+// Symbolic phase (once per mesh, cached): every element contributes the 100
+// node pairs (conn[e][a], conn[e][b]); a stable LSD radix sort of the packed
+// (row, col) keys carrying the triplet id t = e*100 + a*10 + b yields, in one
+// pass, the sorted/unique CSR pattern (sprs TriMat::to_csr, src/fem/
+// assembly.rs:57,116) and, for every CSR slot, its contributors in ascending
+// element order.
+//
+// Numeric phase: a segmented sum per slot over that contributor list (no
+// float atomics), in ascending element order -- the order in which the
+// reference's element-major triplet stream (src/fem/assembly.rs:77-113,
+// 413-434) presents duplicates.  Loads are gathered the same way through each
+// node's diagonal slot (src/fem/assembly.rs:130-154,906-937).
+//
+// Compiled without FP contraction: the Dirichlet RHS lift
+// f_j -= K_ji g_i (src/fem/assembly.rs:198-208) stays a separate multiply and
+// subtract, as in the reference.
+#pragma clang fp contract(off)
+
+#include <cstring>
+
+#include <rocprim/rocprim.hpp>
+
+#include "internal.hpp"
+
+namespace gs {
+namespace {
+
+constexpr int BS = 256;
+
+inline unsigned nblk(int64_t n, int bs = BS) {
+  return (unsigned)((n + bs - 1) / bs);
+}
+
+// ---------------------------------------------------------------------------
+// symbolic phase
+// ---------------------------------------------------------------------------
+__global__ void k_make_keys(int64_t E, const int *__restrict__ conn, int bits,
+                            uint64_t *__restrict__ keys,
+                            int *__restrict__ vals) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= E * 100) return;
+  const int64_t e = t / 100;
+  const int ab = (int)(t - e * 100);
+  const int a = ab / 10, b = ab - 10 * (ab / 10);
+  const uint64_t r = (uint32_t)conn[e * 10 + a];
+  const uint64_t c = (uint32_t)conn[e * 10 + b];
+  keys[t] = (r << bits) | c;
+  vals[t] = (int)t;
+}
+
+__global__ void k_mark_heads(int64_t N, const uint64_t *__restrict__ keys,
+                             int *__restrict__ head) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  head[k] = (k == 0 || keys[k] != keys[k - 1]) ? 1 : 0;
+}
+
+// For every run head k (slot s = incl[k]-1): record column, row, contributor
+// start, diagonal slot, and count the row's slots.
+__global__ void k_slots(int64_t N, const uint64_t *__restrict__ keys,
+                        const int *__restrict__ incl, int bits,
+                        int *__restrict__ col, int *__restrict__ slot_row,
+                        int *__restrict__ contrib_ptr,
+                        int *__restrict__ diag_slot, int *__restrict__ row_cnt) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  if (k != 0 && keys[k] == keys[k - 1]) return;
+  const int s = incl[k] - 1;
+  const uint64_t key = keys[k];
+  const int r = (int)(key >> bits);
+  const int c = (int)(key & ((1ull << bits) - 1));
+  col[s] = c;
+  slot_row[s] = r;
+  contrib_ptr[s] = (int)k;
+  if (r == c) diag_slot[r] = s;
+  atomicAdd(&row_cnt[r], 1);  // integer count: order-independent
+}
+
+__global__ void k_fill_int(int64_t n, int *__restrict__ p, int v) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_set_int(int *__restrict__ p, int v) { *p = v; }
+
+__global__ void k_set_u64(unsigned long long *__restrict__ p,
+                          unsigned long long v) {
+  *p = v;
+}
+
+// ---------------------------------------------------------------------------
+// elastic DOF pattern: node pattern (x) 3x3, DOF = 3*node + c
+// (src/fem/dof.rs:87-106)
+// ---------------------------------------------------------------------------
+__global__ void k_dof3_rowptr(int64_t n_nodes, const int *__restrict__ rp,
+                              int *__restrict__ drp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i > 3 * n_nodes) return;
+  if (i == 3 * n_nodes) { drp[i] = 9 * rp[n_nodes]; return; }
+  const int64_t r = i / 3;
+  const int c = (int)(i - 3 * r);
+  const int deg = rp[r + 1] - rp[r];
+  drp[i] = 9 * rp[r] + 3 * c * deg;
+}
+
+__global__ void k_dof3_cols(int64_t nnz, const int *__restrict__ rp,
+                            const int *__restrict__ col,
+                            const int *__restrict__ slot_row,
+                            int *__restrict__ dcol) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int r = slot_row[s];
+  const int base = rp[r], deg = rp[r + 1] - base;
+  const int j = (int)(s - base);
+  const int cn = col[s];
+#pragma unroll
+  for (int ci = 0; ci < 3; ++ci) {
+    const int64_t d = 9ll * base + 3ll * ci * deg + 3ll * j;
+#pragma unroll
+    for (int cj = 0; cj < 3; ++cj) dcol[d + cj] = 3 * cn + cj;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// numeric gathers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int tri_index(int a, int b) {
+  // packed upper triangle of a 10x10 symmetric matrix, a <= b
+  return 10 * a - (a * (a - 1)) / 2 + (b - a);
+}
+
+__global__ void k_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
+                                 const int *__restrict__ contrib,
+                                 const double *__restrict__ Ke,
+                                 double *__restrict__ val) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int b0 = cptr[s], b1 = cptr[s + 1];
+  double v = 0.0;
+  for (int k = b0; k < b1; ++k) {
+    const int t = contrib[k];
+    const int e = t / 100;
+    const int ab = t - 100 * e;
+    const int a = ab / 10, b = ab - 10 * (ab / 10);
+    const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
+    const double x = Ke[(int64_t)e * 55 + p];
+    v = (k == b0) ? x : v + x;
+  }
+  val[s] = v;
+}
+
+__global__ void k_gather_elastic(int64_t nnz, const int *__restrict__ rp,
+                                 const int *__restrict__ slot_row,
+                                 const int *__restrict__ cptr,
+                                 const int *__restrict__ contrib,
+                                 const double *__restrict__ Ke,
+                                 double *__restrict__ val) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int r = slot_row[s];
+  const int base = rp[r], deg = rp[r + 1] - base;
+  const int j = (int)(s - base);
+  const int b0 = cptr[s], b1 = cptr[s + 1];
+  double v[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) v[m] = 0.0;
+  for (int k = b0; k < b1; ++k) {
+    const int t = contrib[k];
+    const int e = t / 100;
+    const int ab = t - 100 * e;
+    const int a = ab / 10, b = ab - 10 * (ab / 10);
+    const double *Kr = Ke + (int64_t)e * 900 + (3 * a) * 30 + 3 * b;
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+      for (int cj = 0; cj < 3; ++cj) {
+        const double x = Kr[ci * 30 + cj];
+        v[3 * ci + cj] = (k == b0) ? x : v[3 * ci + cj] + x;
+      }
+  }
+#pragma unroll
+  for (int ci = 0; ci < 3; ++ci) {
+    const int64_t d = 9ll * base + 3ll * ci * deg + 3ll * j;
+#pragma unroll
+    for (int cj = 0; cj < 3; ++cj) val[d + cj] = v[3 * ci + cj];
+  }
+}
+
+// f[r] = sum over (e,a) with conn[e][a] == r, ascending e, of Fe[e][a]:
+// exactly the contributors of the diagonal slot (r,r).
+template <int DPN>
+__global__ void k_gather_load(int64_t n_nodes, const int *__restrict__ diag,
+                              const int *__restrict__ cptr,
+                              const int *__restrict__ contrib,
+                              const double *__restrict__ Fe,
+                              double *__restrict__ f) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n_nodes) return;
+  double v[DPN];
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) v[c] = 0.0;
+  const int s = diag[r];
+  if (s >= 0) {
+    for (int k = cptr[s]; k < cptr[s + 1]; ++k) {
+      const int t = contrib[k];
+      const int e = t / 100;
+      const int a = (t - 100 * e) / 10;
+#pragma unroll
+      for (int c = 0; c < DPN; ++c) v[c] += Fe[((int64_t)e * 10 + a) * DPN + c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < DPN; ++c) f[r * DPN + c] = v[c];
+}
+
+// ---------------------------------------------------------------------------
+// Dirichlet elimination (src/fem/assembly.rs:168-230), one thread per row.
+// ---------------------------------------------------------------------------
+// Pass 1: kept-entry count per row and max |K_ii| (as the ordered bit pattern
+// of a non-negative double; max is order independent, so this is exact).
+__global__ void k_bc_count(int64_t n, const int *__restrict__ rp,
+                           const int *__restrict__ col,
+                           const double *__restrict__ val,
+                           const uint8_t *__restrict__ dir,
+                           int *__restrict__ cnt,
+                           unsigned long long *__restrict__ maxbits) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long mine = 0;
+  if (r < n) {
+    int c = 0;
+    for (int k = rp[r]; k < rp[r + 1]; ++k) {
+      const int j = col[k];
+      if (!dir[j]) ++c;
+      if (j == r) {
+        const double a = fabs(val[k]);
+        if (a == a) mine = (unsigned long long)__double_as_longlong(a);
+      }
+    }
+    cnt[r] = dir[r] ? 1 : c;
+  }
+  // wave-level max, then one atomic per wavefront
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off, 64);
+    mine = o > mine ? o : mine;
+  }
+  if ((threadIdx.x & 63) == 0 && mine != 0) atomicMax(maxbits, mine);
+}
+
+__global__ void k_bc_fill(int64_t n, const int *__restrict__ rp,
+                          const int *__restrict__ col,
+                          const double *__restrict__ val,
+                          const uint8_t *__restrict__ dir,
+                          const double *__restrict__ g,
+                          const double *__restrict__ f,
+                          const unsigned long long *__restrict__ maxbits,
+                          const int *__restrict__ nrp, int *__restrict__ ncol,
+                          double *__restrict__ nval, double *__restrict__ fbc) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  int p = nrp[r];
+  if (dir[r]) {
+    const double c = __longlong_as_double((long long)*maxbits);
+    ncol[p] = (int)r;
+    nval[p] = c;
+    fbc[r] = g[r] * c;
+    return;
+  }
+  double fr = f[r];
+  for (int k = rp[r]; k < rp[r + 1]; ++k) {
+    const int j = col[k];
+    if (dir[j]) {
+      fr -= val[k] * g[j];
+    } else {
+      ncol[p] = j;
+      nval[p] = val[k];
+      ++p;
+    }
+  }
+  fbc[r] = fr;
+}
+
+template <class F> void scan_inclusive(gs_ctx *ctx, const int *in, int *out,
+                                       int64_t n, F op) {
+  size_t bytes = 0;
+  GS_HIP(rocprim::inclusive_scan(nullptr, bytes, in, out, (size_t)n, op,
+                                 ctx->stream));
+  auto tmp = dalloc<char>((int64_t)bytes);
+  GS_HIP(rocprim::inclusive_scan(tmp.get(), bytes, in, out, (size_t)n, op,
+                                 ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+}
+
+int read_int(gs_ctx *ctx, const int *p) {
+  int v = 0;
+  GS_HIP(hipMemcpyAsync(ctx->pinned, p, sizeof(int), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  memcpy(&v, ctx->pinned, sizeof(int));
+  return v;
+}
+
+}  // namespace
+
+int read_flag_and_clear(gs_ctx *ctx, int *flag_dev) {
+  const int v = read_int(ctx, flag_dev);
+  hipLaunchKernelGGL(k_set_int, 1, 1, 0, ctx->stream, flag_dev, 0);
+  GS_LAUNCH_CHECK("k_set_int");
+  return v;
+}
+
+void build_node_pattern(gs_ctx *ctx, gs_mesh *m) {
+  if (m->has_pattern) return;
+  const int64_t E = m->n_elems, n = m->n_nodes;
+  const int64_t N = E * 100;
+  if (N >= (int64_t)INT32_MAX)
+    throw Error(GS_ERR_TOO_LARGE, "mesh too large: 100*n_elems must be < 2^31");
+  int bits = 1;
+  while ((1ll << bits) < n) ++bits;
+  hipStream_t s = ctx->stream;
+  NodePattern p;
+  p.row_ptr = dalloc<int>(n + 1);
+  p.diag_slot = dalloc<int>(n);
+  hipLaunchKernelGGL(k_fill_int, nblk(n), BS, 0, s, n, p.diag_slot.get(), -1);
+  GS_LAUNCH_CHECK("k_fill_int");
+  if (N == 0) {
+    GS_HIP(hipMemsetAsync(p.row_ptr.get(), 0, (n + 1) * sizeof(int), s));
+    p.col = dalloc<int>(0);
+    p.slot_row = dalloc<int>(0);
+    p.contrib_ptr = dalloc<int>(1);
+    GS_HIP(hipMemsetAsync(p.contrib_ptr.get(), 0, sizeof(int), s));
+    p.contrib = dalloc<int>(0);
+    GS_HIP(hipStreamSynchronize(s));
+    m->pat = p;
+    m->has_pattern = true;
+    return;
+  }
+  DArr<int> head, incl;
+  {
+    auto keys = dalloc<uint64_t>(N);
+    auto keys2 = dalloc<uint64_t>(N);
+    auto vals = dalloc<int>(N);
+    p.contrib = dalloc<int>(N);
+    hipLaunchKernelGGL(k_make_keys, nblk(N), BS, 0, s, E, m->conn.get(), bits,
+                       keys.get(), vals.get());
+    GS_LAUNCH_CHECK("k_make_keys");
+    size_t bytes = 0;
+    GS_HIP(rocprim::radix_sort_pairs(nullptr, bytes, keys.get(), keys2.get(),
+                                     vals.get(), p.contrib.get(), (size_t)N, 0u,
+                                     (unsigned)(2 * bits), s));
+    {
+      auto tmp = dalloc<char>((int64_t)bytes);
+      GS_HIP(rocprim::radix_sort_pairs(tmp.get(), bytes, keys.get(),
+                                       keys2.get(), vals.get(), p.contrib.get(),
+                                       (size_t)N, 0u, (unsigned)(2 * bits), s));
+      GS_HIP(hipStreamSynchronize(s));
+    }
+    keys = DArr<uint64_t>();
+    vals = DArr<int>();
+    head = dalloc<int>(N);
+    incl = dalloc<int>(N);
+    hipLaunchKernelGGL(k_mark_heads, nblk(N), BS, 0, s, N, keys2.get(),
+                       head.get());
+    GS_LAUNCH_CHECK("k_mark_heads");
+    scan_inclusive(ctx, head.get(), incl.get(), N, rocprim::plus<int>());
+    const int nnz = read_int(ctx, incl.get() + (N - 1));
+    p.nnz = nnz;
+    p.col = dalloc<int>(nnz);
+    p.slot_row = dalloc<int>(nnz);
+    p.contrib_ptr = dalloc<int>(nnz + 1);
+    auto row_cnt = dalloc<int>(n + 1);
+    GS_HIP(hipMemsetAsync(row_cnt.get(), 0, (n + 1) * sizeof(int), s));
+    hipLaunchKernelGGL(k_slots, nblk(N), BS, 0, s, N, keys2.get(), incl.get(),
+                       bits, p.col.get(), p.slot_row.get(),
+                       p.contrib_ptr.get(), p.diag_slot.get(), row_cnt.get());
+    GS_LAUNCH_CHECK("k_slots");
+    hipLaunchKernelGGL(k_set_int, 1, 1, 0, s, p.contrib_ptr.get() + nnz, (int)N);
+    // row_ptr[0] = 0, row_ptr[1..n] = inclusive scan of counts
+    GS_HIP(hipMemsetAsync(p.row_ptr.get(), 0, sizeof(int), s));
+    scan_inclusive(ctx, row_cnt.get(), p.row_ptr.get() + 1, n,
+                   rocprim::plus<int>());
+  }
+  GS_HIP(hipStreamSynchronize(s));
+  m->pat = p;
+  m->has_pattern = true;
+}
+
+void build_dof3_pattern(gs_ctx *ctx, gs_mesh *m) {
+  if (m->has_dof3) return;
+  build_node_pattern(ctx, m);
+  const NodePattern &p = m->pat;
+  const int64_t n = m->n_nodes;
+  if (9 * p.nnz >= (int64_t)INT32_MAX || 3 * n >= (int64_t)INT32_MAX)
+    throw Error(GS_ERR_TOO_LARGE, "elastic operator too large for int32 indices");
+  DofPattern3 d;
+  d.nnz = 9 * p.nnz;
+  d.row_ptr = dalloc<int>(3 * n + 1);
+  d.col = dalloc<int>(d.nnz);
+  hipLaunchKernelGGL(k_dof3_rowptr, nblk(3 * n + 1), BS, 0, ctx->stream, n,
+                     p.row_ptr.get(), d.row_ptr.get());
+  GS_LAUNCH_CHECK("k_dof3_rowptr");
+  if (p.nnz > 0) {
+    hipLaunchKernelGGL(k_dof3_cols, nblk(p.nnz), BS, 0, ctx->stream, p.nnz,
+                       p.row_ptr.get(), p.col.get(), p.slot_row.get(),
+                       d.col.get());
+    GS_LAUNCH_CHECK("k_dof3_cols");
+  }
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  m->dof3 = d;
+  m->has_dof3 = true;
+}
+
+void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
+                    const double *Ke, double *val) {
+  (void)E;
+  if (p.nnz == 0) return;
+  hipLaunchKernelGGL(k_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
+                     p.contrib_ptr.get(), p.contrib.get(), Ke, val);
+  GS_LAUNCH_CHECK("k_gather_thermal");
+}
+
+void gather_thermal_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
+                         const double *Fe, double *f) {
+  if (n_nodes == 0) return;
+  hipLaunchKernelGGL((k_gather_load<1>), nblk(n_nodes), BS, 0, s, n_nodes,
+                     p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(),
+                     Fe, f);
+  GS_LAUNCH_CHECK("k_gather_load<1>");
+}
+
+void gather_elastic(hipStream_t s, const NodePattern &p, int64_t n_nodes,
+                    const double *Ke, double *val) {
+  (void)n_nodes;
+  if (p.nnz == 0) return;
+  hipLaunchKernelGGL(k_gather_elastic, nblk(p.nnz), BS, 0, s, p.nnz,
+                     p.row_ptr.get(), p.slot_row.get(), p.contrib_ptr.get(),
+                     p.contrib.get(), Ke, val);
+  GS_LAUNCH_CHECK("k_gather_elastic");
+}
+
+void gather_gravity_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
+                         const double *Fe, double *f) {
+  if (n_nodes == 0) return;
+  hipLaunchKernelGGL((k_gather_load<3>), nblk(n_nodes), BS, 0, s, n_nodes,
+                     p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(),
+                     Fe, f);
+  GS_LAUNCH_CHECK("k_gather_load<3>");
+}
+
+gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
+                            const uint8_t *is_dir, const double *g,
+                            double *fbc) {
+  const int64_t n = K->rows;
+  hipStream_t s = ctx->stream;
+  auto cnt = dalloc<int>(n > 0 ? n : 1);
+  auto maxbits = dalloc<unsigned long long>(1);
+  const double one = 1.0;
+  unsigned long long one_bits;
+  memcpy(&one_bits, &one, sizeof(one));
+  hipLaunchKernelGGL(k_set_u64, 1, 1, 0, s, maxbits.get(), one_bits);
+  GS_LAUNCH_CHECK("k_set_u64");
+  std::unique_ptr<gs_csr> out(new gs_csr());
+  out->ctx = ctx;
+  out->rows = n;
+  out->cols = K->cols;
+  out->row_ptr = dalloc<int>(n + 1);
+  GS_HIP(hipMemsetAsync(out->row_ptr.get(), 0, sizeof(int), s));
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bc_count, nblk(n), BS, 0, s, n, K->row_ptr.get(),
+                       K->col.get(), K->val.get(), is_dir, cnt.get(),
+                       maxbits.get());
+    GS_LAUNCH_CHECK("k_bc_count");
+    scan_inclusive(ctx, cnt.get(), out->row_ptr.get() + 1, n,
+                   rocprim::plus<int>());
+    out->nnz = read_int(ctx, out->row_ptr.get() + n);
+  }
+  out->col = dalloc<int>(out->nnz);
+  out->val = dalloc<double>(out->nnz);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_bc_fill, nblk(n), BS, 0, s, n, K->row_ptr.get(),
+                       K->col.get(), K->val.get(), is_dir, g, f, maxbits.get(),
+                       out->row_ptr.get(), out->col.get(), out->val.get(), fbc);
+    GS_LAUNCH_CHECK("k_bc_fill");
+  }
+  GS_HIP(hipStreamSynchronize(s));
+  return out.release();
+}
+
+}  // namespace gs

@@ -1,0 +1,702 @@
+// C ABI of libgeofem (include/geofem.h): handle management, argument
+// validation, and the assemble/solve pipelines built from the device kernels.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "internal.hpp"
+
+using namespace gs;
+
+namespace {
+
+template <class F> gs_status guard(gs_ctx *ctx, F &&f) {
+  try {
+    f();
+    if (ctx) ctx->err.clear();
+    return GS_OK;
+  } catch (const Error &e) {
+    if (ctx) ctx->err = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    if (ctx) ctx->err = "host out of memory";
+    return GS_ERR_OOM;
+  } catch (const std::exception &e) {
+    if (ctx) ctx->err = e.what();
+    return GS_ERR_INVALID_ARG;
+  }
+}
+
+void require(bool ok, gs_status code, const char *msg) {
+  if (!ok) throw Error(code, msg);
+}
+
+void set_device(gs_ctx *ctx) { GS_HIP(hipSetDevice(ctx->device)); }
+
+// Element-kernel failure flags -> reference panic conditions as statuses.
+void check_flags(int flags) {
+  if (flags & 4)
+    throw Error(GS_ERR_INVALID_MATERIAL, "Young's modulus must be positive");
+  if (flags & 1) throw Error(GS_ERR_SINGULAR_JACOBIAN, "Singular Jacobian");
+  if (flags & 2)
+    throw Error(GS_ERR_NONFINITE, "non-finite Jacobian determinant");
+}
+
+void check_material(double E, double nu) {
+  // src/mechanics/constitutive.rs:30-41
+  require(E > 0.0, GS_ERR_INVALID_MATERIAL, "Young's modulus must be positive");
+  require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
+          "Poisson's ratio must be in (-1, 0.5) for physical stability");
+}
+
+gs_csr *new_csr(gs_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
+                DArr<int> rp, DArr<int> col, DArr<double> val) {
+  auto *c = new gs_csr();
+  c->ctx = ctx;
+  c->rows = rows;
+  c->cols = cols;
+  c->nnz = nnz;
+  c->row_ptr = std::move(rp);
+  c->col = std::move(col);
+  c->val = std::move(val);
+  return c;
+}
+
+// thermal K (full structural pattern) [+ element loads Fe]
+gs_csr *assemble_thermal_K(gs_ctx *ctx, gs_mesh *m, double k,
+                           const double *k_elem, bool with_load, double Q,
+                           double *f_dev) {
+  build_node_pattern(ctx, m);
+  const NodePattern &p = m->pat;
+  hipStream_t s = ctx->stream;
+  auto Ke = dalloc<double>(m->n_elems * 55);
+  DArr<double> Fe;
+  if (with_load) Fe = dalloc<double>(m->n_elems * 10);
+  auto flag = dalloc<int>(1);
+  GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+  launch_thermal_elements(s, m->n_elems, m->xyz.get(), m->conn.get(), k, k_elem,
+                          Q, with_load, Ke.get(), with_load ? Fe.get() : nullptr,
+                          flag.get());
+  check_flags(read_flag_and_clear(ctx, flag.get()));
+  auto val = dalloc<double>(p.nnz);
+  gather_thermal(s, p, m->n_elems, Ke.get(), val.get());
+  if (with_load) gather_thermal_load(s, p, m->n_nodes, Fe.get(), f_dev);
+  GS_HIP(hipStreamSynchronize(s));
+  return new_csr(ctx, m->n_nodes, m->n_nodes, p.nnz, p.row_ptr, p.col, val);
+}
+
+gs_csr *assemble_elastic_K(gs_ctx *ctx, gs_mesh *m, double E, double nu,
+                           const double *E_elem, bool with_load, double rho,
+                           const double g[3], double *f_dev) {
+  build_dof3_pattern(ctx, m);
+  const NodePattern &p = m->pat;
+  hipStream_t s = ctx->stream;
+  auto Ke = dalloc<double>(m->n_elems * 900);
+  DArr<double> Fe;
+  if (with_load) Fe = dalloc<double>(m->n_elems * 30);
+  auto flag = dalloc<int>(1);
+  GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+  const double g0[3] = {0.0, 0.0, 0.0};
+  launch_elastic_elements(s, m->n_elems, m->xyz.get(), m->conn.get(), E, nu,
+                          E_elem, rho, g ? g : g0, with_load, Ke.get(),
+                          with_load ? Fe.get() : nullptr, flag.get());
+  check_flags(read_flag_and_clear(ctx, flag.get()));
+  auto val = dalloc<double>(m->dof3.nnz);
+  gather_elastic(s, p, m->n_nodes, Ke.get(), val.get());
+  if (with_load) gather_gravity_load(s, p, m->n_nodes, Fe.get(), f_dev);
+  GS_HIP(hipStreamSynchronize(s));
+  return new_csr(ctx, 3 * m->n_nodes, 3 * m->n_nodes, m->dof3.nnz,
+                 m->dof3.row_ptr, m->dof3.col, val);
+}
+
+void check_mesh_dof(const gs_mesh *m, const gs_dofmap *d) {
+  require(m && d, GS_ERR_INVALID_ARG, "null mesh or dofmap");
+  require(d->n_nodes == m->n_nodes, GS_ERR_INVALID_ARG,
+          "dofmap node count differs from the mesh");
+}
+
+// ---- host mesh generator (src/mesh_generator_improved.rs:19-169) ----------
+struct EdgeTable {
+  std::vector<uint64_t> keys;
+  std::vector<int64_t> vals;
+  uint64_t mask = 0;
+  explicit EdgeTable(int64_t expect) {
+    uint64_t cap = 16;
+    while (cap < (uint64_t)(2 * expect)) cap <<= 1;
+    keys.assign(cap, 0);
+    vals.assign(cap, -1);
+    mask = cap - 1;
+  }
+  static uint64_t hash(uint64_t x) {
+    x ^= x >> 31;
+    x *= 0x7fb5d329728ea185ULL;
+    x ^= x >> 27;
+    x *= 0x81dadef4bc2dd44dULL;
+    x ^= x >> 33;
+    return x;
+  }
+  // index of the midpoint node of edge (a,b), created at `next` when new
+  int64_t get_or_add(int64_t a, int64_t b, int64_t next, bool &added) {
+    const int64_t lo = a < b ? a : b, hi = a < b ? b : a;
+    const uint64_t key = (((uint64_t)lo << 32) | (uint64_t)hi) + 1;
+    uint64_t h = hash(key) & mask;
+    while (keys[h] != 0) {
+      if (keys[h] == key) { added = false; return vals[h]; }
+      h = (h + 1) & mask;
+    }
+    keys[h] = key;
+    vals[h] = next;
+    added = true;
+    return next;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+gs_status gs_ctx_create(int device, gs_ctx **out) {
+  if (!out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  auto *ctx = new gs_ctx();
+  ctx->device = device;
+  const gs_status st = guard(ctx, [&] {
+    int count = 0;
+    GS_HIP(hipGetDeviceCount(&count));
+    require(device >= 0 && device < count, GS_ERR_INVALID_ARG,
+            "device index out of range");
+    GS_HIP(hipSetDevice(device));
+    GS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    GS_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+  });
+  if (st != GS_OK) {
+    gs_ctx_destroy(ctx);
+    return st;
+  }
+  *out = ctx;
+  return GS_OK;
+}
+
+void gs_ctx_destroy(gs_ctx *ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->ws = DArr<char>();
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+}
+
+const char *gs_last_error(const gs_ctx *ctx) {
+  return ctx ? ctx->err.c_str() : "null context";
+}
+
+gs_status gs_ctx_synchronize(gs_ctx *ctx) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] { GS_HIP(hipStreamSynchronize(ctx->stream)); });
+}
+
+gs_status gs_device_alloc(gs_ctx *ctx, size_t bytes, void **out_dev) {
+  if (!ctx || !out_dev) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    set_device(ctx);
+    GS_HIP(hipMalloc(out_dev, bytes ? bytes : 1));
+  });
+}
+
+gs_status gs_device_free(gs_ctx *ctx, void *p) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] { GS_HIP(hipFree(p)); });
+}
+
+gs_status gs_memcpy_h2d(gs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+gs_status gs_memcpy_d2h(gs_ctx *ctx, void *dst, const void *src, size_t bytes) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    GS_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+gs_status gs_generate_cube_sizes(int64_t nx, int64_t ny, int64_t nz,
+                                 int64_t *n_nodes, int64_t *n_elems) {
+  if (nx <= 0 || ny <= 0 || nz <= 0 || !n_nodes || !n_elems)
+    return GS_ERR_INVALID_ARG;
+  *n_nodes = (2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1);
+  *n_elems = 6 * nx * ny * nz;
+  return GS_OK;
+}
+
+gs_status gs_generate_cube(int64_t nx, int64_t ny, int64_t nz, double lx,
+                           double ly, double lz, double *xyz, int64_t *conn) {
+  int64_t nn = 0, ne = 0;
+  if (gs_generate_cube_sizes(nx, ny, nz, &nn, &ne) != GS_OK || !xyz || !conn)
+    return GS_ERR_INVALID_ARG;
+  try {
+    const double dx = lx / (double)nx, dy = ly / (double)ny, dz = lz / (double)nz;
+    // vertex grid, ix fastest (src/mesh_generator_improved.rs:29-39)
+    int64_t n = 0;
+    for (int64_t iz = 0; iz <= nz; ++iz)
+      for (int64_t iy = 0; iy <= ny; ++iy)
+        for (int64_t ix = 0; ix <= nx; ++ix, ++n) {
+          xyz[3 * n] = (double)ix * dx;
+          xyz[3 * n + 1] = (double)iy * dy;
+          xyz[3 * n + 2] = (double)iz * dz;
+        }
+    EdgeTable edges(nn);
+    const int64_t sy = nx + 1, sz = (nx + 1) * (ny + 1);
+    int64_t e = 0;
+    auto mid = [&](int64_t a, int64_t b) {
+      bool added = false;
+      const int64_t id = edges.get_or_add(a, b, n, added);
+      if (added) {
+        for (int c = 0; c < 3; ++c)
+          xyz[3 * n + c] = (xyz[3 * a + c] + xyz[3 * b + c]) / 2.0;
+        ++n;
+      }
+      return id;
+    };
+    for (int64_t iz = 0; iz < nz; ++iz)
+      for (int64_t iy = 0; iy < ny; ++iy)
+        for (int64_t ix = 0; ix < nx; ++ix) {
+          const int64_t v000 = ix + iy * sy + iz * sz;
+          const int64_t v100 = v000 + 1, v010 = v000 + sy, v110 = v000 + 1 + sy;
+          const int64_t v001 = v000 + sz, v101 = v001 + 1, v011 = v001 + sy,
+                        v111 = v001 + 1 + sy;
+          // six tets around the v000-v111 diagonal (:105-118)
+          const int64_t T[6][4] = {{v000, v100, v110, v111}, {v000, v100, v111, v101},
+                                   {v000, v010, v110, v111}, {v000, v010, v111, v011},
+                                   {v000, v001, v111, v101}, {v000, v001, v111, v011}};
+          for (const auto &t : T) {
+            int64_t v[4] = {t[0], t[1], t[2], t[3]};
+            double ed[3][3];
+            for (int k = 0; k < 3; ++k)
+              for (int c = 0; c < 3; ++c)
+                ed[k][c] = xyz[3 * v[k + 1] + c] - xyz[3 * v[0] + c];
+            // orientation: e1 . (e2 x e3) < 0 -> swap v2, v3 (:139-155)
+            const double det =
+                ed[0][0] * (ed[1][1] * ed[2][2] - ed[1][2] * ed[2][1]) +
+                ed[0][1] * (ed[1][2] * ed[2][0] - ed[1][0] * ed[2][2]) +
+                ed[0][2] * (ed[1][0] * ed[2][1] - ed[1][1] * ed[2][0]);
+            if (det < 0.0) std::swap(v[2], v[3]);
+            int64_t *c = conn + 10 * e;
+            c[0] = v[0]; c[1] = v[1]; c[2] = v[2]; c[3] = v[3];
+            // midpoints in the order e01, e12, e20, e03, e13, e23 (:158-163)
+            c[4] = mid(v[0], v[1]);
+            c[5] = mid(v[1], v[2]);
+            c[6] = mid(v[2], v[0]);
+            c[7] = mid(v[0], v[3]);
+            c[8] = mid(v[1], v[3]);
+            c[9] = mid(v[2], v[3]);
+            ++e;
+          }
+        }
+    return (n == nn && e == ne) ? GS_OK : GS_ERR_INVALID_ARG;
+  } catch (...) {
+    return GS_ERR_OOM;
+  }
+}
+
+gs_status gs_mesh_create(gs_ctx *ctx, const double *xyz, int64_t n_nodes,
+                         const int64_t *conn, int64_t n_elems, gs_mesh **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    require(n_nodes >= 0 && n_elems >= 0 && (xyz || n_nodes == 0) &&
+                (conn || n_elems == 0),
+            GS_ERR_INVALID_ARG, "invalid mesh arrays");
+    require(n_nodes < INT32_MAX / 3 && n_elems * 100 < INT32_MAX,
+            GS_ERR_TOO_LARGE, "mesh exceeds int32 indexing");
+    set_device(ctx);
+    std::vector<int> c32((size_t)n_elems * 10);
+    for (int64_t i = 0; i < n_elems * 10; ++i) {
+      require(conn[i] >= 0 && conn[i] < n_nodes, GS_ERR_INVALID_ARG,
+              "connectivity references a node out of range");
+      c32[i] = (int)conn[i];
+    }
+    std::unique_ptr<gs_mesh> m(new gs_mesh());
+    m->ctx = ctx;
+    m->n_nodes = n_nodes;
+    m->n_elems = n_elems;
+    m->xyz = dalloc<double>(n_nodes * 3);
+    m->conn = dalloc<int>(n_elems * 10);
+    if (n_nodes)
+      GS_HIP(hipMemcpyAsync(m->xyz.get(), xyz, n_nodes * 3 * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+    if (n_elems)
+      GS_HIP(hipMemcpyAsync(m->conn.get(), c32.data(), n_elems * 10 * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+  });
+}
+
+void gs_mesh_destroy(gs_mesh *m) { delete m; }
+
+gs_status gs_dofmap_create(gs_ctx *ctx, int64_t n_nodes, int dpn,
+                           const uint8_t *is_dir, const double *values,
+                           gs_dofmap **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    require(n_nodes >= 0 && dpn >= 1, GS_ERR_INVALID_ARG, "invalid dofmap size");
+    const int64_t nd = n_nodes * dpn;
+    require(nd < INT32_MAX, GS_ERR_TOO_LARGE, "too many DOFs");
+    set_device(ctx);
+    std::vector<uint8_t> mask((size_t)nd, 0);
+    std::vector<double> vals((size_t)nd, 0.0);
+    int64_t fixed = 0;
+    for (int64_t i = 0; i < nd; ++i) {
+      mask[i] = (is_dir && is_dir[i]) ? 1 : 0;
+      vals[i] = (mask[i] && values) ? values[i] : 0.0;
+      fixed += mask[i];
+    }
+    std::unique_ptr<gs_dofmap> d(new gs_dofmap());
+    d->ctx = ctx;
+    d->n_nodes = n_nodes;
+    d->dpn = dpn;
+    d->n_dofs = nd;
+    d->n_fixed = fixed;
+    d->is_dir = dalloc<uint8_t>(nd);
+    d->values = dalloc<double>(nd);
+    if (nd) {
+      GS_HIP(hipMemcpyAsync(d->is_dir.get(), mask.data(), nd, hipMemcpyHostToDevice,
+                            ctx->stream));
+      GS_HIP(hipMemcpyAsync(d->values.get(), vals.data(), nd * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+    }
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    *out = d.release();
+  });
+}
+
+void gs_dofmap_destroy(gs_dofmap *d) { delete d; }
+
+gs_status gs_assemble_thermal_stiffness(gs_ctx *ctx, gs_mesh *mesh,
+                                        const gs_dofmap *dof, double k,
+                                        const double *k_elem, gs_csr **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    // DOF = node id (src/fem/assembly.rs:95-97); n_dofs must equal n_nodes.
+    require(dof->dpn == 1, GS_ERR_BAD_DOFS, "thermal assembly needs 1 DOF per node");
+    set_device(ctx);
+    *out = assemble_thermal_K(ctx, mesh, k, k_elem, false, 0.0, nullptr);
+  });
+}
+
+gs_status gs_assemble_thermal_load(gs_ctx *ctx, gs_mesh *mesh,
+                                   const gs_dofmap *dof, double Q, double *f) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 1, GS_ERR_BAD_DOFS, "thermal load needs 1 DOF per node");
+    require(f || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    build_node_pattern(ctx, mesh);
+    hipStream_t s = ctx->stream;
+    auto Ke = dalloc<double>(mesh->n_elems * 55);
+    auto Fe = dalloc<double>(mesh->n_elems * 10);
+    auto flag = dalloc<int>(1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    launch_thermal_elements(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(),
+                            1.0, nullptr, Q, true, Ke.get(), Fe.get(), flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()) & ~1);
+    gather_thermal_load(s, mesh->pat, mesh->n_nodes, Fe.get(), f);
+    GS_HIP(hipStreamSynchronize(s));
+  });
+}
+
+gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
+                                           const gs_dofmap *dof, double E,
+                                           double nu, const double *E_elem,
+                                           gs_csr **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Elasticity requires 3 DOF per node");
+    if (E_elem) {
+      require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
+              "Poisson's ratio must be in (-1, 0.5) for physical stability");
+    } else {
+      check_material(E, nu);
+    }
+    set_device(ctx);
+    *out = assemble_elastic_K(ctx, mesh, E, nu, E_elem, false, 0.0, nullptr, nullptr);
+  });
+}
+
+gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
+                                   const gs_dofmap *dof, double rho,
+                                   const double g[3], double *f) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Gravity load requires 3 DOF per node");
+    require(g && (f || mesh->n_nodes == 0), GS_ERR_INVALID_ARG, "null argument");
+    set_device(ctx);
+    build_node_pattern(ctx, mesh);
+    hipStream_t s = ctx->stream;
+    // The 30-entry element load needs no K_e; the elastic kernel computes both,
+    // so run it with a valid material and keep only Fe.
+    auto Ke = dalloc<double>(mesh->n_elems * 900);
+    auto Fe = dalloc<double>(mesh->n_elems * 30);
+    auto flag = dalloc<int>(1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    launch_elastic_elements(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(),
+                            1.0, 0.25, nullptr, rho, g, true, Ke.get(), Fe.get(),
+                            flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()) & ~1);
+    gather_gravity_load(s, mesh->pat, mesh->n_nodes, Fe.get(), f);
+    GS_HIP(hipStreamSynchronize(s));
+  });
+}
+
+gs_status gs_apply_dirichlet_bcs(gs_ctx *ctx, const gs_csr *K, const double *f,
+                                 const gs_dofmap *dof, gs_csr **K_bc,
+                                 double *f_bc) {
+  if (!ctx || !K_bc) return GS_ERR_INVALID_ARG;
+  *K_bc = nullptr;
+  return guard(ctx, [&] {
+    require(K && dof && (f || K->rows == 0) && (f_bc || K->rows == 0),
+            GS_ERR_INVALID_ARG, "null argument");
+    require(K->rows == dof->n_dofs && K->cols == dof->n_dofs, GS_ERR_INVALID_ARG,
+            "matrix size differs from the DOF count");
+    set_device(ctx);
+    *K_bc = dirichlet_eliminate(ctx, K, f, dof->is_dir.get(), dof->values.get(),
+                                f_bc);
+  });
+}
+
+gs_status gs_assemble_thermal_system(gs_ctx *ctx, gs_mesh *mesh,
+                                     const gs_dofmap *dof, double k,
+                                     const double *k_elem, double Q,
+                                     gs_csr **K_bc, double *f_bc) {
+  if (!ctx || !K_bc) return GS_ERR_INVALID_ARG;
+  *K_bc = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 1, GS_ERR_BAD_DOFS, "thermal assembly needs 1 DOF per node");
+    require(f_bc || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    auto f = dalloc<double>(mesh->n_nodes);
+    std::unique_ptr<gs_csr> K(
+        assemble_thermal_K(ctx, mesh, k, k_elem, true, Q, f.get()));
+    *K_bc = dirichlet_eliminate(ctx, K.get(), f.get(), dof->is_dir.get(),
+                                dof->values.get(), f_bc);
+  });
+}
+
+gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
+                                        const gs_dofmap *dof, double E,
+                                        double nu, const double *E_elem,
+                                        double rho, const double g[3],
+                                        gs_csr **K_bc, double *f_bc) {
+  if (!ctx || !K_bc) return GS_ERR_INVALID_ARG;
+  *K_bc = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Elasticity requires 3 DOF per node");
+    require(g && (f_bc || mesh->n_nodes == 0), GS_ERR_INVALID_ARG, "null argument");
+    if (E_elem) {
+      require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
+              "Poisson's ratio must be in (-1, 0.5) for physical stability");
+    } else {
+      check_material(E, nu);
+    }
+    set_device(ctx);
+    auto f = dalloc<double>(3 * mesh->n_nodes);
+    std::unique_ptr<gs_csr> K(
+        assemble_elastic_K(ctx, mesh, E, nu, E_elem, true, rho, g, f.get()));
+    *K_bc = dirichlet_eliminate(ctx, K.get(), f.get(), dof->is_dir.get(),
+                                dof->values.get(), f_bc);
+  });
+}
+
+gs_status gs_csr_info(const gs_csr *A, int64_t *rows, int64_t *cols,
+                      int64_t *nnz) {
+  if (!A) return GS_ERR_INVALID_ARG;
+  if (rows) *rows = A->rows;
+  if (cols) *cols = A->cols;
+  if (nnz) *nnz = A->nnz;
+  return GS_OK;
+}
+
+gs_status gs_csr_export(gs_ctx *ctx, const gs_csr *A, int64_t *indptr,
+                        int64_t *indices, double *data) {
+  if (!ctx || !A) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    set_device(ctx);
+    std::vector<int> rp((size_t)A->rows + 1), ci((size_t)A->nnz);
+    GS_HIP(hipMemcpyAsync(rp.data(), A->row_ptr.get(), (A->rows + 1) * sizeof(int),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (A->nnz) {
+      GS_HIP(hipMemcpyAsync(ci.data(), A->col.get(), A->nnz * sizeof(int),
+                            hipMemcpyDeviceToHost, ctx->stream));
+      if (data)
+        GS_HIP(hipMemcpyAsync(data, A->val.get(), A->nnz * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    if (indptr)
+      for (int64_t i = 0; i <= A->rows; ++i) indptr[i] = rp[i];
+    if (indices)
+      for (int64_t i = 0; i < A->nnz; ++i) indices[i] = ci[i];
+  });
+}
+
+gs_status gs_csr_from_host(gs_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
+                           const int64_t *indptr, const int64_t *indices,
+                           const double *data, gs_csr **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    require(rows >= 0 && cols >= 0 && nnz >= 0 && indptr, GS_ERR_INVALID_ARG,
+            "invalid CSR arguments");
+    require(rows < INT32_MAX && cols < INT32_MAX && nnz < INT32_MAX,
+            GS_ERR_TOO_LARGE, "CSR exceeds int32 indexing");
+    require(indptr[0] == 0 && indptr[rows] == nnz, GS_ERR_INVALID_ARG,
+            "indptr does not span nnz");
+    std::vector<int> rp((size_t)rows + 1), ci((size_t)nnz);
+    for (int64_t r = 0; r < rows; ++r) {
+      require(indptr[r + 1] >= indptr[r], GS_ERR_INVALID_ARG, "indptr not monotone");
+      for (int64_t k = indptr[r]; k < indptr[r + 1]; ++k) {
+        require(indices[k] >= 0 && indices[k] < cols, GS_ERR_INVALID_ARG,
+                "column index out of range");
+        require(k == indptr[r] || indices[k] > indices[k - 1], GS_ERR_INVALID_ARG,
+                "columns must be sorted and unique within a row");
+        ci[k] = (int)indices[k];
+      }
+    }
+    for (int64_t r = 0; r <= rows; ++r) rp[r] = (int)indptr[r];
+    set_device(ctx);
+    auto drp = dalloc<int>(rows + 1);
+    auto dci = dalloc<int>(nnz);
+    auto dv = dalloc<double>(nnz);
+    GS_HIP(hipMemcpyAsync(drp.get(), rp.data(), (rows + 1) * sizeof(int),
+                          hipMemcpyHostToDevice, ctx->stream));
+    if (nnz) {
+      GS_HIP(hipMemcpyAsync(dci.get(), ci.data(), nnz * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+      GS_HIP(hipMemcpyAsync(dv.get(), data, nnz * sizeof(double),
+                            hipMemcpyHostToDevice, ctx->stream));
+    }
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    *out = new_csr(ctx, rows, cols, nnz, drp, dci, dv);
+  });
+}
+
+void gs_csr_destroy(gs_csr *A) {
+  if (!A) return;
+  if (A->ctx) (void)hipStreamSynchronize(A->ctx->stream);
+  delete A;
+}
+
+gs_status gs_csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y) {
+  if (!ctx || !A) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((x && y) || A->rows == 0, GS_ERR_INVALID_ARG, "null vector");
+    set_device(ctx);
+    csr_spmv(ctx, A, x, y);
+  });
+}
+
+gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv) {
+  if (!ctx || !A) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(dinv || A->rows == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    jacobi_diag_inv(ctx->stream, A, dinv);
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+gs_status gs_cg_opts_default(gs_cg_opts *o) {
+  if (!o) return GS_ERR_INVALID_ARG;
+  // src/linalg/iterative.rs:16-24
+  o->max_iterations = 1000;
+  o->tolerance = 1e-8;
+  o->abs_tolerance = 1e-12;
+  o->use_preconditioner = 1;
+  o->precision = GS_PREC_FP64;
+  o->inner_tolerance = 1e-4;
+  o->max_inner_iterations = 0;
+  o->max_outer_iterations = 50;
+  return GS_OK;
+}
+
+gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
+                      const gs_cg_opts *opts, double *x, gs_solver_stats *st) {
+  if (!ctx || !A || !st) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((b && x) || A->rows == 0, GS_ERR_INVALID_ARG, "null vector");
+    gs_cg_opts o;
+    gs_cg_opts_default(&o);
+    if (opts) o = *opts;
+    require(o.precision == GS_PREC_FP64 || o.precision == GS_PREC_MIXED,
+            GS_ERR_INVALID_ARG, "unknown precision");
+    require(o.precision == GS_PREC_FP64 || o.inner_tolerance > 0.0,
+            GS_ERR_INVALID_ARG, "mixed precision needs inner_tolerance > 0");
+    set_device(ctx);
+    cg_solve(ctx, A, b, o, x, st);
+  });
+}
+
+gs_status gs_thermal_element_backward(gs_ctx *ctx, gs_mesh *mesh,
+                                      const double *k_elem, const double *u,
+                                      const double *lam, double *grad) {
+  if (!ctx || !mesh) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((k_elem && u && lam && grad) || mesh->n_elems == 0,
+            GS_ERR_INVALID_ARG, "null argument");
+    set_device(ctx);
+    auto flag = dalloc<int>(1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), ctx->stream));
+    launch_thermal_backward(ctx->stream, mesh->n_elems, mesh->xyz.get(),
+                            mesh->conn.get(), k_elem, u, lam, grad, flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
+                                      const double *E_elem, double nu,
+                                      const double *u, const double *lam,
+                                      double *grad) {
+  if (!ctx || !mesh) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((E_elem && u && lam && grad) || mesh->n_elems == 0,
+            GS_ERR_INVALID_ARG, "null argument");
+    require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
+            "Poisson's ratio must be in (-1, 0.5) for physical stability");
+    set_device(ctx);
+    auto flag = dalloc<int>(1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), ctx->stream));
+    launch_elastic_backward(ctx->stream, mesh->n_elems, mesh->xyz.get(),
+                            mesh->conn.get(), E_elem, nu, u, lam, grad, flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()));
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+  });
+}
+
+gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
+                       int32_t reps, double *avg_ms) {
+  if (!ctx || !A || !avg_ms) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(precision == GS_PREC_FP64 || precision == GS_PREC_MIXED,
+            GS_ERR_INVALID_ARG, "unknown precision");
+    set_device(ctx);
+    *avg_ms = time_spmv(ctx, A, precision, reps);
+  });
+}
+
+}  // extern "C"

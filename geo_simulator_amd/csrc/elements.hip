@@ -1,0 +1,447 @@
+// Tet10 element kernels for gfx950.
+//
+// Arithmetic follows the reference's operation order term for term, and this
+// file is compiled without floating-point contraction, so every element
+// matrix and load vector is bit-identical to the FP64 restatement of
+//   src/fem/element.rs:23-61,117-140       (thermal stiffness / load)
+//   src/mechanics/element.rs:33-65         (elastic B^T D B, 4-point rule)
+//   src/mechanics/body_force.rs:49-84      (gravity load)
+//   src/fem/basis.rs:85-110,196-276        (dN/dL, Jacobian, J^-T)
+// The shape-function derivative tables at the quadrature points are
+// compile-time constants; the Jacobian is recomputed per quadrature point
+// from all ten nodes, so curved (non-affine) Tet10s are handled.
+#pragma clang fp contract(off)
+
+#include "internal.hpp"
+
+namespace gs {
+namespace {
+
+// src/fem/quadrature.rs:24-38 (literal constants)
+constexpr double QA = 0.5854101966249685;
+constexpr double QB = 0.1381966011250105;
+constexpr double QW = 1.0 / 24.0;
+
+struct Tables {
+  double dl[4][10][3];  // (dN_i/dL1 - dN_i/dL0, .../dL2 - ..., .../dL3 - ...)
+  double N[4][10];      // N_i at the quadrature points
+};
+
+constexpr double qpl(int q, int m) { return q == m ? QA : QB; }
+
+// dN_i/dL_m (src/fem/basis.rs:85-110)
+constexpr double dndl(int i, int m, const double L[4]) {
+  switch (i) {
+  case 0: return m == 0 ? 4.0 * L[0] - 1.0 : 0.0;
+  case 1: return m == 1 ? 4.0 * L[1] - 1.0 : 0.0;
+  case 2: return m == 2 ? 4.0 * L[2] - 1.0 : 0.0;
+  case 3: return m == 3 ? 4.0 * L[3] - 1.0 : 0.0;
+  case 4: return m == 0 ? 4.0 * L[1] : (m == 1 ? 4.0 * L[0] : 0.0);
+  case 5: return m == 1 ? 4.0 * L[2] : (m == 2 ? 4.0 * L[1] : 0.0);
+  case 6: return m == 0 ? 4.0 * L[2] : (m == 2 ? 4.0 * L[0] : 0.0);
+  case 7: return m == 0 ? 4.0 * L[3] : (m == 3 ? 4.0 * L[0] : 0.0);
+  case 8: return m == 1 ? 4.0 * L[3] : (m == 3 ? 4.0 * L[1] : 0.0);
+  default: return m == 2 ? 4.0 * L[3] : (m == 3 ? 4.0 * L[2] : 0.0);
+  }
+}
+
+// N_i (src/fem/basis.rs:56-73)
+constexpr double shape(int i, const double L[4]) {
+  switch (i) {
+  case 0: return L[0] * (2.0 * L[0] - 1.0);
+  case 1: return L[1] * (2.0 * L[1] - 1.0);
+  case 2: return L[2] * (2.0 * L[2] - 1.0);
+  case 3: return L[3] * (2.0 * L[3] - 1.0);
+  case 4: return 4.0 * L[0] * L[1];
+  case 5: return 4.0 * L[1] * L[2];
+  case 6: return 4.0 * L[2] * L[0];
+  case 7: return 4.0 * L[0] * L[3];
+  case 8: return 4.0 * L[1] * L[3];
+  default: return 4.0 * L[2] * L[3];
+  }
+}
+
+constexpr Tables make_tables() {
+  Tables t{};
+  for (int q = 0; q < 4; ++q) {
+    const double L[4] = {qpl(q, 0), qpl(q, 1), qpl(q, 2), qpl(q, 3)};
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = dndl(i, 0, L);
+      t.dl[q][i][0] = dndl(i, 1, L) - d0;
+      t.dl[q][i][1] = dndl(i, 2, L) - d0;
+      t.dl[q][i][2] = dndl(i, 3, L) - d0;
+      t.N[q][i] = shape(i, L);
+    }
+  }
+  return t;
+}
+
+__device__ constexpr Tables kT = make_tables();
+
+enum : int { FLAG_SINGULAR = 1, FLAG_NONFINITE = 2, FLAG_MATERIAL = 4 };
+
+// J (src/fem/basis.rs:196-223), accumulated over the ten nodes in order.
+__device__ __forceinline__ void jacobian(int q, const double X[10][3],
+                                         double J[3][3]) {
+#pragma unroll
+  for (int r = 0; r < 3; ++r) J[r][0] = J[r][1] = J[r][2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const double d1 = kT.dl[q][i][0], d2 = kT.dl[q][i][1], d3 = kT.dl[q][i][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      J[r][0] += X[i][r] * d1;
+      J[r][1] += X[i][r] * d2;
+      J[r][2] += X[i][r] * d3;
+    }
+  }
+}
+
+// det (cofactor expansion on the first row) and adjugate/det inverse; returns
+// det.  Same formulas as the FP64 restatement's 3x3 helpers.
+__device__ __forceinline__ double inverse3(const double m[3][3], double o[3][3]) {
+  const double m12_23 = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  const double m11_23 = m[1][0] * m[2][2] - m[2][0] * m[1][2];
+  const double m11_22 = m[1][0] * m[2][1] - m[2][0] * m[1][1];
+  const double det = m[0][0] * m12_23 - m[0][1] * m11_23 + m[0][2] * m11_22;
+  o[0][0] = m12_23 / det;
+  o[0][1] = (m[0][2] * m[2][1] - m[2][2] * m[0][1]) / det;
+  o[0][2] = (m[0][1] * m[1][2] - m[1][1] * m[0][2]) / det;
+  o[1][0] = -m11_23 / det;
+  o[1][1] = (m[0][0] * m[2][2] - m[2][0] * m[0][2]) / det;
+  o[1][2] = (m[0][2] * m[1][0] - m[1][2] * m[0][0]) / det;
+  o[2][0] = m11_22 / det;
+  o[2][1] = (m[0][1] * m[2][0] - m[2][1] * m[0][0]) / det;
+  o[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) / det;
+  return det;
+}
+
+__device__ __forceinline__ int det_flags(double det) {
+  if (!isfinite(det)) return FLAG_NONFINITE;
+  return det == 0.0 ? FLAG_SINGULAR : 0;
+}
+
+// grad N_i = J^-T (dN/dL1 - dN/dL0, ...) (src/fem/basis.rs:265-272)
+__device__ __forceinline__ void gradients(int q, const double Ji[3][3],
+                                          double G[10][3]) {
+#pragma unroll
+  for (int i = 0; i < 10; ++i) {
+    const double v0 = kT.dl[q][i][0], v1 = kT.dl[q][i][1], v2 = kT.dl[q][i][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) G[i][r] = Ji[0][r] * v0 + Ji[1][r] * v1 + Ji[2][r] * v2;
+  }
+}
+
+__device__ __forceinline__ void load_nodes(const double *__restrict__ xyz,
+                                           const int *__restrict__ conn,
+                                           int64_t e, double X[10][3]) {
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) X[a][r] = xyz[nd * 3 + r];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Thermal: one thread per element (a wavefront handles a 64-element batch).
+// Writes the 55 upper-triangle entries of K_e (bitwise symmetric, because
+// grad_i . grad_j is evaluated with the same products in either order) in
+// element-major layout Ke[e*55 + p(a,b)], and optionally f_e (Fe[e*10+a]).
+// ---------------------------------------------------------------------------
+template <bool PER_ELEM, bool WITH_LOAD>
+__global__ __launch_bounds__(128) void k_thermal_elements(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    double k, const double *__restrict__ k_elem, double Q,
+    double *__restrict__ Ke, double *__restrict__ Fe, int *__restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double X[10][3];
+  load_nodes(xyz, conn, e, X);
+  const double kk = PER_ELEM ? k_elem[e] : k;
+  double K[55];
+#pragma unroll
+  for (int p = 0; p < 55; ++p) K[p] = 0.0;
+  double F[10];
+#pragma unroll
+  for (int i = 0; i < 10; ++i) F[i] = 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+    jacobian(q, X, J);
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+    gradients(q, Ji, G);
+    const double w = QW * fabs(det);
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int j = i; j < 10; ++j) {
+        const double gd = G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2];
+        K[p] += kk * gd * w;
+        ++p;
+      }
+    if (WITH_LOAD) {
+#pragma unroll
+      for (int i = 0; i < 10; ++i) F[i] += kT.N[q][i] * Q * w;
+    }
+  }
+  if (bad) atomicOr(flag, bad);
+  double *out = Ke + e * 55;
+#pragma unroll
+  for (int p = 0; p < 55; ++p) out[p] = K[p];
+  if (WITH_LOAD) {
+#pragma unroll
+    for (int i = 0; i < 10; ++i) Fe[e * 10 + i] = F[i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elasticity: 32 lanes per element (two elements per wavefront); lane a < 30
+// produces row a of the 30x30 K_e and, optionally, f_e[a].
+//
+// Row a = 3i+ca of B^T (D B) only meets the three Voigt rows k where B[k][a]
+// is structurally nonzero (ca=0: k=0,3,5; ca=1: 1,3,4; ca=2: 2,4,5) and
+// (D B)[k][b] has a single structural term, so the sparse evaluation below
+// performs exactly the nonzero products of the dense reference product in
+// the same order; the dropped terms are exact zeros.
+// ---------------------------------------------------------------------------
+template <bool PER_ELEM, bool WITH_LOAD>
+__global__ __launch_bounds__(128) void k_elastic_elements(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    double Emod, double nu, const double *__restrict__ E_elem, double bfx,
+    double bfy, double bfz, double *__restrict__ Ke, double *__restrict__ Fe,
+    int *__restrict__ flag) {
+  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t e = gt >> 5;
+  const int a = (int)(gt & 31);
+  if (e >= E || a >= 30) return;
+  const int ia = a / 3, ca = a - 3 * (a / 3);
+  double X[10][3];
+  load_nodes(xyz, conn, e, X);
+  const double Ee = PER_ELEM ? E_elem[e] : Emod;
+  int bad = (Ee > 0.0) ? 0 : FLAG_MATERIAL;
+  // D (src/mechanics/constitutive.rs:63-102): unscaled entries times factor.
+  const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double Dd = (1.0 - nu) * factor;
+  const double Do = nu * factor;
+  const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+  double K[30];
+#pragma unroll
+  for (int b = 0; b < 30; ++b) K[b] = 0.0;
+  double f = 0.0;
+  const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+    jacobian(q, X, J);
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+    gradients(q, Ji, G);
+    const double w = QW * fabs(det);
+    // B[k][a] for the three structural Voigt rows of column a
+    const double gi0 = G[ia][0], gi1 = G[ia][1], gi2 = G[ia][2];
+    const double Bk1 = ca == 0 ? gi0 : (ca == 1 ? gi1 : gi2);  // k1 = ca
+    const double Bk2 = ca == 0 ? gi1 : (ca == 1 ? gi0 : gi1);  // k2 = 3,3,4
+    const double Bk3 = ca == 0 ? gi2 : (ca == 1 ? gi2 : gi0);  // k3 = 5,4,5
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+      const double g0 = G[j][0], g1 = G[j][1], g2 = G[j][2];
+#pragma unroll
+      for (int cb = 0; cb < 3; ++cb) {
+        const double gjc = cb == 0 ? g0 : (cb == 1 ? g1 : g2);
+        // (D B)[k1][b] = D[ca][cb] * B[cb][b]
+        const double db1 = (ca == cb ? Dd : Do) * gjc;
+        // (D B)[k][b] = Ds * B[k][b] for shear rows k = 3, 4, 5
+        const double B3 = cb == 0 ? g1 : (cb == 1 ? g0 : 0.0);
+        const double B4 = cb == 0 ? 0.0 : (cb == 1 ? g2 : g1);
+        const double B5 = cb == 0 ? g2 : (cb == 1 ? 0.0 : g0);
+        const double db2 = Ds * (ca == 2 ? B4 : B3);
+        const double db3 = Ds * (ca == 1 ? B4 : B5);
+        const double s = Bk1 * db1 + Bk2 * db2 + Bk3 * db3;
+        K[3 * j + cb] += w * s;
+      }
+    }
+    if (WITH_LOAD) f += w * kT.N[q][ia] * bfa;
+  }
+  if (bad && a == 0) atomicOr(flag, bad);
+  double *out = Ke + e * 900 + a * 30;
+#pragma unroll
+  for (int b = 0; b < 30; ++b) out[b] = K[b];
+  if (WITH_LOAD) Fe[e * 30 + a] = f;
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint element backward: grad_e = -lambda_e^T K_e u_e, evaluated per
+// quadrature point as -sum_q w_q k (G^T lambda).(G^T u) (thermal) and
+// -sum_q w_q (B lambda)^T D (B u) (elastic), one thread per element.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(128) void k_thermal_backward(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    const double *__restrict__ k_elem, const double *__restrict__ u,
+    const double *__restrict__ lam, double *__restrict__ grad,
+    int *__restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double X[10][3], ue[10], le[10];
+  load_nodes(xyz, conn, e, X);
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    const int nd = conn[e * 10 + a];
+    ue[a] = u[nd];
+    le[a] = lam[nd];
+  }
+  const double kk = k_elem[e];
+  double s = 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+    jacobian(q, X, J);
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+    gradients(q, Ji, G);
+    const double w = QW * fabs(det);
+    double gu[3] = {0.0, 0.0, 0.0}, gl[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        gu[r] += G[i][r] * ue[i];
+        gl[r] += G[i][r] * le[i];
+      }
+    s += kk * (gl[0] * gu[0] + gl[1] * gu[1] + gl[2] * gu[2]) * w;
+  }
+  if (bad) atomicOr(flag, bad);
+  grad[e] = -s;
+}
+
+__global__ __launch_bounds__(128) void k_elastic_backward(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    const double *__restrict__ E_elem, double nu, const double *__restrict__ u,
+    const double *__restrict__ lam, double *__restrict__ grad,
+    int *__restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double X[10][3], ue[10][3], le[10][3];
+  load_nodes(xyz, conn, e, X);
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      ue[a][c] = u[nd * 3 + c];
+      le[a][c] = lam[nd * 3 + c];
+    }
+  }
+  const double Ee = E_elem[e];
+  int bad = (Ee > 0.0) ? 0 : FLAG_MATERIAL;
+  const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double Dd = (1.0 - nu) * factor, Do = nu * factor;
+  const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+  double s = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+    jacobian(q, X, J);
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+    gradients(q, Ji, G);
+    const double w = QW * fabs(det);
+    // displacement gradients H[c][r] = d u_c / d x_r
+    double Hu[3][3] = {}, Hl[3][3] = {};
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) {
+          Hu[c][r] += ue[i][c] * G[i][r];
+          Hl[c][r] += le[i][c] * G[i][r];
+        }
+    // Voigt strains [xx, yy, zz, xy, yz, zx] (engineering shear)
+    const double eu[6] = {Hu[0][0], Hu[1][1], Hu[2][2], Hu[0][1] + Hu[1][0],
+                          Hu[1][2] + Hu[2][1], Hu[2][0] + Hu[0][2]};
+    const double el[6] = {Hl[0][0], Hl[1][1], Hl[2][2], Hl[0][1] + Hl[1][0],
+                          Hl[1][2] + Hl[2][1], Hl[2][0] + Hl[0][2]};
+    const double sig0 = Dd * eu[0] + Do * eu[1] + Do * eu[2];
+    const double sig1 = Do * eu[0] + Dd * eu[1] + Do * eu[2];
+    const double sig2 = Do * eu[0] + Do * eu[1] + Dd * eu[2];
+    const double en = el[0] * sig0 + el[1] * sig1 + el[2] * sig2 +
+                      Ds * (el[3] * eu[3] + el[4] * eu[4] + el[5] * eu[5]);
+    s += w * en;
+  }
+  if (bad) atomicOr(flag, bad);
+  grad[e] = -s;
+}
+
+inline unsigned blocks_for(int64_t work, int bs) {
+  return (unsigned)((work + bs - 1) / bs);
+}
+
+}  // namespace
+
+void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
+                             const int *conn, double k, const double *k_elem,
+                             double Q, bool with_load, double *Ke, double *Fe,
+                             int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E, 128);
+  if (k_elem) {
+    if (with_load)
+      hipLaunchKernelGGL((k_thermal_elements<true, true>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_thermal_elements<true, false>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
+  } else {
+    if (with_load)
+      hipLaunchKernelGGL((k_thermal_elements<false, true>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_thermal_elements<false, false>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
+  }
+  GS_LAUNCH_CHECK("k_thermal_elements");
+}
+
+void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
+                             const int *conn, double Emod, double nu,
+                             const double *E_elem, double rho, const double g3[3],
+                             bool with_load, double *Ke, double *Fe, int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E * 32, 128);
+  const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
+  if (E_elem) {
+    if (with_load)
+      hipLaunchKernelGGL((k_elastic_elements<true, true>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_elastic_elements<true, false>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+  } else {
+    if (with_load)
+      hipLaunchKernelGGL((k_elastic_elements<false, true>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_elastic_elements<false, false>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+  }
+  GS_LAUNCH_CHECK("k_elastic_elements");
+}
+
+void launch_thermal_backward(hipStream_t s, int64_t E, const double *xyz,
+                             const int *conn, const double *k_elem,
+                             const double *u, const double *lam, double *grad,
+                             int *flag) {
+  if (E == 0) return;
+  hipLaunchKernelGGL(k_thermal_backward, blocks_for(E, 128), 128, 0, s, E, xyz,
+                     conn, k_elem, u, lam, grad, flag);
+  GS_LAUNCH_CHECK("k_thermal_backward");
+}
+
+void launch_elastic_backward(hipStream_t s, int64_t E, const double *xyz,
+                             const int *conn, const double *E_elem, double nu,
+                             const double *u, const double *lam, double *grad,
+                             int *flag) {
+  if (E == 0) return;
+  hipLaunchKernelGGL(k_elastic_backward, blocks_for(E, 128), 128, 0, s, E, xyz,
+                     conn, E_elem, nu, u, lam, grad, flag);
+  GS_LAUNCH_CHECK("k_elastic_backward");
+}
+
+}  // namespace gs

@@ -1,0 +1,242 @@
+"""Device-resident CSR matrix and Jacobi-preconditioned CG.
+
+Mirrors `CsMat<f64>` + `impl LinearOperator for CsMat<f64>`
+(src/linalg/solver.rs:61-92), `SolverStats` (src/linalg/solver.rs:5-20),
+`JacobiPreconditioner` (src/linalg/preconditioner.rs:29-62) and
+`ConjugateGradient` (src/linalg/iterative.rs:7-147).  Vectors may be numpy
+arrays (host, like Vec<f64>: results come back as numpy) or torch tensors on
+the device (results stay on the device).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+from . import _lib
+from ._lib import lib
+from .runtime import default_context, dptr
+
+
+def _is_torch(x):
+    try:
+        import torch
+        return isinstance(x, torch.Tensor)
+    except ImportError:  # pragma: no cover
+        return False
+
+
+class CsrMatrix:
+    """A CSR matrix owned by libgeofem on one device (gs_csr)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+        r, c, z = C.c_int64(), C.c_int64(), C.c_int64()
+        ctx.check(lib().gs_csr_info(handle, C.byref(r), C.byref(c), C.byref(z)),
+                  "gs_csr_info")
+        self.shape = (r.value, c.value)
+        self._nnz = z.value
+
+    @classmethod
+    def from_scipy(cls, A, ctx=None):
+        ctx = ctx or default_context()
+        A = sp.csr_matrix(A)
+        A.sum_duplicates()
+        A.sort_indices()
+        indptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+        indices = np.ascontiguousarray(A.indices, dtype=np.int64)
+        data = np.ascontiguousarray(A.data, dtype=np.float64)
+        h = C.c_void_p()
+        ctx.check(lib().gs_csr_from_host(
+            ctx.handle, A.shape[0], A.shape[1], A.nnz,
+            indptr.ctypes.data_as(C.POINTER(C.c_int64)),
+            indices.ctypes.data_as(C.POINTER(C.c_int64)),
+            data.ctypes.data_as(C.POINTER(C.c_double)), C.byref(h)),
+            "gs_csr_from_host")
+        return cls(ctx, h)
+
+    # sprs-like accessors
+    def rows(self) -> int:
+        return self.shape[0]
+
+    def cols(self) -> int:
+        return self.shape[1]
+
+    def nnz(self) -> int:
+        return self._nnz
+
+    def to_scipy(self) -> sp.csr_matrix:
+        """Export in the sprs shape: int64 indptr/indices, sorted columns."""
+        indptr = np.empty(self.shape[0] + 1, dtype=np.int64)
+        indices = np.empty(self._nnz, dtype=np.int64)
+        data = np.empty(self._nnz, dtype=np.float64)
+        self.ctx.check(lib().gs_csr_export(
+            self.ctx.handle, self.handle,
+            indptr.ctypes.data_as(C.POINTER(C.c_int64)),
+            indices.ctypes.data_as(C.POINTER(C.c_int64)),
+            data.ctypes.data_as(C.POINTER(C.c_double))), "gs_csr_export")
+        return sp.csr_matrix((data, indices, indptr), shape=self.shape)
+
+    def apply(self, v):
+        """LinearOperator::apply: y = A v (src/linalg/solver.rs:62-73)."""
+        ctx = self.ctx
+        host = not _is_torch(v)
+        x = ctx.to_device(v)
+        if x.numel() != self.shape[1]:
+            raise ValueError("vector length differs from the matrix columns")
+        y = ctx.empty(self.shape[0])
+        ctx.sync_torch()
+        ctx.check(lib().gs_csr_spmv(ctx.handle, self.handle, dptr(x), dptr(y)),
+                  "gs_csr_spmv")
+        return y.cpu().numpy() if host else y
+
+    def jacobi_diag_inv(self):
+        d = self.ctx.empty(self.shape[0])
+        self.ctx.sync_torch()
+        self.ctx.check(lib().gs_jacobi_diag_inv(self.ctx.handle, self.handle,
+                                                dptr(d)), "gs_jacobi_diag_inv")
+        return d
+
+    def time_spmv(self, precision: str = "fp64", reps: int = 20) -> float:
+        """Average duration (ms) of the CG SpMV kernel over `reps` launches."""
+        out = C.c_double()
+        prec = _lib.GS_PREC_FP64 if precision == "fp64" else _lib.GS_PREC_MIXED
+        self.ctx.check(lib().gs_time_spmv(self.ctx.handle, self.handle, prec,
+                                          reps, C.byref(out)), "gs_time_spmv")
+        return out.value
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                lib().gs_csr_destroy(self.handle)
+            self.handle = None
+        except Exception:
+            pass
+
+
+@dataclass
+class SolverStats:
+    iterations: int
+    residual_norm: float
+    relative_residual: float
+    converged: bool
+    solve_time: float
+    outer_iterations: int = 0
+    spmv_count: int = 0
+
+
+class JacobiPreconditioner:
+    """M = diag(A): d_i = 1/A_ii if |A_ii| > 1e-14 else 1
+    (src/linalg/preconditioner.rs:37-51)."""
+
+    def __init__(self, A: CsrMatrix):
+        self.diag_inv = A.jacobi_diag_inv()
+
+    def apply(self, r):
+        if _is_torch(r):
+            return r * self.diag_inv
+        return np.asarray(r) * self.diag_inv.cpu().numpy()
+
+
+class ConjugateGradient:
+    """Builder and solver with the reference's defaults
+    (max_iterations 1000, tolerance 1e-8, abs_tolerance 1e-12, Jacobi on).
+
+    Extensions: `with_precision("mixed")` runs FP32 inner PCG inside an FP64
+    defect-correction loop (residual, update and RHS in FP64), stopping on the
+    same FP64 criterion ||b - A x|| < tol ||b|| (or < abs_tol)."""
+
+    name = "ConjugateGradient"
+
+    def __init__(self):
+        self.max_iterations = 1000
+        self.tolerance = 1e-8
+        self.abs_tolerance = 1e-12
+        self.use_preconditioner = True
+        self.precision = "fp64"
+        self.inner_tolerance = 1e-4
+        self.max_inner_iterations = 0
+        self.max_outer_iterations = 50
+
+    @classmethod
+    def new(cls):
+        return cls()
+
+    def with_max_iterations(self, n):
+        self.max_iterations = int(n)
+        return self
+
+    def with_tolerance(self, t):
+        self.tolerance = float(t)
+        return self
+
+    def with_abs_tolerance(self, t):
+        self.abs_tolerance = float(t)
+        return self
+
+    def with_preconditioner(self, use):
+        self.use_preconditioner = bool(use)
+        return self
+
+    def with_precision(self, precision: str):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        return self
+
+    def with_inner_tolerance(self, t):
+        self.inner_tolerance = float(t)
+        return self
+
+    def with_max_outer_iterations(self, n):
+        self.max_outer_iterations = int(n)
+        return self
+
+    def opts(self) -> _lib.CgOpts:
+        o = _lib.CgOpts()
+        lib().gs_cg_opts_default(C.byref(o))
+        o.max_iterations = self.max_iterations
+        o.tolerance = self.tolerance
+        o.abs_tolerance = self.abs_tolerance
+        o.use_preconditioner = 1 if self.use_preconditioner else 0
+        o.precision = _lib.GS_PREC_FP64 if self.precision == "fp64" else _lib.GS_PREC_MIXED
+        o.inner_tolerance = self.inner_tolerance
+        o.max_inner_iterations = self.max_inner_iterations
+        o.max_outer_iterations = self.max_outer_iterations
+        return o
+
+    def solve(self, A, b):
+        """Solver::solve (src/linalg/iterative.rs:123-131) -> (x, SolverStats)."""
+        if not isinstance(A, CsrMatrix):
+            A = CsrMatrix.from_scipy(A)
+        ctx = A.ctx
+        host = not _is_torch(b)
+        bd = ctx.to_device(b)
+        if bd.numel() != A.shape[0]:
+            raise ValueError("right-hand side length differs from the matrix rows")
+        x = ctx.empty(A.shape[0])
+        st = _lib.SolverStatsC()
+        o = self.opts()
+        ctx.sync_torch()
+        ctx.check(lib().gs_cg_solve(ctx.handle, A.handle, dptr(bd), C.byref(o),
+                                    dptr(x), C.byref(st)), "gs_cg_solve")
+        stats = SolverStats(int(st.iterations), float(st.residual_norm),
+                            float(st.relative_residual), bool(st.converged),
+                            float(st.solve_time), int(st.outer_iterations),
+                            int(st.spmv_count))
+        return (x.cpu().numpy() if host else x), stats
+
+    # Solver trait accessors (src/linalg/solver.rs:122-135)
+    def set_tolerance(self, t):
+        self.tolerance = float(t)
+
+    def set_abs_tolerance(self, t):
+        self.abs_tolerance = float(t)
+
+
+def norm(v) -> float:
+    """SolverUtils::norm (src/linalg/solver.rs:153-155)."""
+    return float(np.sqrt(np.sum(np.asarray(v, dtype=np.float64) ** 2)))

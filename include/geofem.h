@@ -1,0 +1,201 @@
+/*
+ * geofem.h -- C ABI of the MI355X (gfx950) Tet10 assemble-and-solve library
+ * (libgeofem.so, built from geo_simulator_amd/csrc/).
+ *
+ * Each entry point replaces one call of geo_simulator's Rust API; the
+ * reference interface is cited beside it (paths relative to the reference
+ * repository root).  Conventions:
+ *  - Every call returns gs_status; GS_OK == 0.  A condition on which the
+ *    reference panics returns a non-zero status instead, and
+ *    gs_last_error(ctx) gives the message.  CG non-convergence is not an
+ *    error (stats->converged == 0), as in the reference.
+ *  - Pointers whose name ends in _dev are device pointers (HBM of the ctx's
+ *    device, e.g. from gs_device_alloc or any allocator on that device);
+ *    every other pointer is host memory.  Device handles (mesh, dofmap, csr)
+ *    are owned by the library and freed with their *_destroy call.
+ *  - One HIP stream per context; all work of a context is ordered on it and
+ *    every entry point returns after its work is complete.  A context is not
+ *    re-entrant; distinct contexts may be used from distinct threads.
+ *  - Indices inside device handles are int32: rows, columns and element
+ *    counts must stay below 2^31 (GS_ERR_TOO_LARGE otherwise).
+ */
+#ifndef GEOFEM_H
+#define GEOFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  GS_OK = 0,
+  GS_ERR_INVALID_ARG = 1,
+  GS_ERR_HIP = 2,
+  GS_ERR_SINGULAR_JACOBIAN = 3, /* src/fem/basis.rs:247 "Singular Jacobian" */
+  GS_ERR_BAD_DOFS = 4,          /* src/fem/assembly.rs:394 dpn != 3 */
+  GS_ERR_INVALID_MATERIAL = 5,  /* src/mechanics/constitutive.rs:31-35 */
+  GS_ERR_NONFINITE = 6,         /* non-finite Jacobian determinant */
+  GS_ERR_TOO_LARGE = 7,         /* exceeds int32 indexing or a fixed limit */
+  GS_ERR_OOM = 8
+} gs_status;
+
+typedef enum { GS_PREC_FP64 = 0, GS_PREC_MIXED = 1 } gs_precision;
+
+typedef struct gs_ctx gs_ctx;
+typedef struct gs_mesh gs_mesh;
+typedef struct gs_dofmap gs_dofmap;
+typedef struct gs_csr gs_csr;
+
+/* ---- context / memory (no reference counterpart: device plumbing) ---- */
+gs_status gs_ctx_create(int device, gs_ctx **out);
+void gs_ctx_destroy(gs_ctx *ctx);
+const char *gs_last_error(const gs_ctx *ctx);
+gs_status gs_ctx_synchronize(gs_ctx *ctx);
+gs_status gs_device_alloc(gs_ctx *ctx, size_t bytes, void **out_dev);
+gs_status gs_device_free(gs_ctx *ctx, void *p_dev);
+gs_status gs_memcpy_h2d(gs_ctx *ctx, void *dst_dev, const void *src, size_t bytes);
+gs_status gs_memcpy_d2h(gs_ctx *ctx, void *dst, const void *src_dev, size_t bytes);
+
+/* ---- mesh (host-side generator + device upload) ----
+ * ImprovedMeshGenerator::generate_cube (src/mesh_generator_improved.rs:19-169):
+ * same node numbering (vertex grid ix-fastest, then edge midpoints in
+ * first-touch order) and element order. */
+gs_status gs_generate_cube_sizes(int64_t nx, int64_t ny, int64_t nz,
+                                 int64_t *n_nodes, int64_t *n_elems);
+gs_status gs_generate_cube(int64_t nx, int64_t ny, int64_t nz, double lx,
+                           double ly, double lz, double *xyz /* n_nodes*3 */,
+                           int64_t *conn /* n_elems*10 */);
+/* Mesh{geometry.nodes, connectivity.tet10_elements}
+ * (src/mesh/geometry.rs:8-11,41-50, src/mesh/topology.rs:13-15,55-56). */
+gs_status gs_mesh_create(gs_ctx *ctx, const double *xyz, int64_t n_nodes,
+                         const int64_t *conn, int64_t n_elems, gs_mesh **out);
+void gs_mesh_destroy(gs_mesh *mesh);
+
+/* DofManager::new + set_dirichlet (src/fem/dof.rs:37-50,113-118):
+ * n_nodes*dofs_per_node DOFs numbered node*dpn+c; is_dirichlet/values are
+ * per DOF (values ignored where is_dirichlet == 0). */
+gs_status gs_dofmap_create(gs_ctx *ctx, int64_t n_nodes, int dofs_per_node,
+                           const uint8_t *is_dirichlet, const double *values,
+                           gs_dofmap **out);
+void gs_dofmap_destroy(gs_dofmap *dof);
+
+/* ---- assembly ---- */
+/* Assembler::assemble_thermal_stiffness_parallel (src/fem/assembly.rs:69-117).
+ * k_per_elem_dev (nullable, n_elems doubles) overrides the uniform k
+ * (additive overload for heterogeneous conductivity). */
+gs_status gs_assemble_thermal_stiffness(gs_ctx *ctx, gs_mesh *mesh,
+                                        const gs_dofmap *dof, double conductivity,
+                                        const double *k_per_elem_dev,
+                                        gs_csr **out);
+/* Assembler::assemble_thermal_load (src/fem/assembly.rs:130-154). */
+gs_status gs_assemble_thermal_load(gs_ctx *ctx, gs_mesh *mesh,
+                                   const gs_dofmap *dof, double source,
+                                   double *f_dev);
+/* Assembler::assemble_elasticity_stiffness_parallel
+ * (src/fem/assembly.rs:389-437); E_per_elem_dev nullable (heterogeneous E). */
+gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
+                                           const gs_dofmap *dof, double youngs,
+                                           double poisson,
+                                           const double *E_per_elem_dev,
+                                           gs_csr **out);
+/* Assembler::assemble_gravity_load (src/fem/assembly.rs:906-937). */
+gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
+                                   const gs_dofmap *dof, double density,
+                                   const double gravity[3], double *f_dev);
+/* Assembler::apply_dirichlet_bcs (src/fem/assembly.rs:168-230). */
+gs_status gs_apply_dirichlet_bcs(gs_ctx *ctx, const gs_csr *K,
+                                 const double *f_dev, const gs_dofmap *dof,
+                                 gs_csr **K_bc, double *f_bc_dev);
+
+/* Fused assemble_thermal_stiffness_parallel + assemble_thermal_load +
+ * apply_dirichlet_bcs, device-resident end to end (the benchmark callers'
+ * sequence, benchmarks/thermal_1d_improved_mesh.rs:57-59). */
+gs_status gs_assemble_thermal_system(gs_ctx *ctx, gs_mesh *mesh,
+                                     const gs_dofmap *dof, double conductivity,
+                                     const double *k_per_elem_dev, double source,
+                                     gs_csr **K_bc, double *f_bc_dev);
+/* Same for elasticity + gravity (benchmarks/elasticity_halfspace_gravity.rs:112-122). */
+gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
+                                        const gs_dofmap *dof, double youngs,
+                                        double poisson,
+                                        const double *E_per_elem_dev,
+                                        double density, const double gravity[3],
+                                        gs_csr **K_bc, double *f_bc_dev);
+
+/* ---- sparse matrix handle (sprs CsMat<f64> counterpart) ---- */
+gs_status gs_csr_info(const gs_csr *A, int64_t *rows, int64_t *cols,
+                      int64_t *nnz);
+/* Copy to the sprs shape: int64 indptr (rows+1), sorted int64 indices, data. */
+gs_status gs_csr_export(gs_ctx *ctx, const gs_csr *A, int64_t *indptr,
+                        int64_t *indices, double *data);
+/* Upload a host CSR (sorted, duplicate-free rows) for Solver::solve on an
+ * arbitrary CsMat. */
+gs_status gs_csr_from_host(gs_ctx *ctx, int64_t rows, int64_t cols, int64_t nnz,
+                           const int64_t *indptr, const int64_t *indices,
+                           const double *data, gs_csr **out);
+void gs_csr_destroy(gs_csr *A);
+/* impl LinearOperator for CsMat<f64>::apply (src/linalg/solver.rs:62-73). */
+gs_status gs_csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x_dev,
+                      double *y_dev);
+/* JacobiPreconditioner::new (src/linalg/preconditioner.rs:37-51). */
+gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv_dev);
+
+/* ---- Jacobi-preconditioned CG ----
+ * ConjugateGradient::new().with_max_iterations().with_tolerance()
+ * .with_abs_tolerance().with_preconditioner() (src/linalg/iterative.rs:16-44)
+ * and Solver::solve (src/linalg/solver.rs:108, src/linalg/iterative.rs:123-131). */
+typedef struct {
+  int64_t max_iterations;    /* default 1000 */
+  double tolerance;          /* default 1e-8 (relative to ||b||) */
+  double abs_tolerance;      /* default 1e-12 */
+  int32_t use_preconditioner;/* default 1 (Jacobi) */
+  int32_t precision;         /* gs_precision, default GS_PREC_FP64 */
+  double inner_tolerance;    /* mixed: inner FP32 relative tolerance */
+  int64_t max_inner_iterations; /* mixed: cap per inner solve */
+  int64_t max_outer_iterations; /* mixed: cap on defect-correction steps */
+} gs_cg_opts;
+
+/* SolverStats (src/linalg/solver.rs:5-20) plus mixed-precision counters. */
+typedef struct {
+  int64_t iterations;        /* FP64: as the reference; mixed: total inner */
+  double residual_norm;
+  double relative_residual;
+  int32_t converged;
+  int32_t pad_;
+  double solve_time;         /* seconds, host wall clock */
+  int64_t outer_iterations;  /* mixed: defect-correction steps */
+  int64_t spmv_count;        /* SpMV launches issued */
+} gs_solver_stats;
+
+gs_status gs_cg_opts_default(gs_cg_opts *opts);
+gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
+                      const gs_cg_opts *opts, double *x_dev,
+                      gs_solver_stats *stats);
+
+/* ---- adjoint (new surface; no reference counterpart) ----
+ * Element backward of theta_e = log k_e (thermal) or log E_e (elastic):
+ * grad_e = -lambda_e^T K_e(theta_e) u_e, with u the full solution (Dirichlet
+ * values included) and lambda the adjoint solution (zero on Dirichlet DOFs). */
+gs_status gs_thermal_element_backward(gs_ctx *ctx, gs_mesh *mesh,
+                                      const double *k_per_elem_dev,
+                                      const double *u_dev,
+                                      const double *lambda_dev,
+                                      double *grad_dev);
+gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
+                                      const double *E_per_elem_dev,
+                                      double poisson, const double *u_dev,
+                                      const double *lambda_dev,
+                                      double *grad_dev);
+
+/* ---- measurement hooks (no reference counterpart) ----
+ * Launch the CG's SpMV kernel `reps` times on the ctx stream between two HIP
+ * events and return the average duration per launch in milliseconds. */
+gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
+                       int32_t reps, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

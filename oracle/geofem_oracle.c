@@ -1,0 +1,924 @@
+/*
+ * geofem_oracle.c -- FP64 CPU restatement of geo_simulator's Tet10
+ * assemble-and-solve path.  TEST INFRASTRUCTURE ONLY (see the header).
+ *
+ * Parallel structure mirrors the reference: element kernels run in parallel
+ * (OpenMP here, Rayon par_iter there: src/fem/assembly.rs:77-104,399-410);
+ * the triplet merge, CSR conversion, Dirichlet elimination and CG are serial
+ * (src/fem/assembly.rs:107-116,168-230, src/linalg/iterative.rs:81-110).
+ */
+#include "geofem_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------------ */
+/* Quadrature tables: src/fem/quadrature.rs:13-104 (literal constants).      */
+/* ------------------------------------------------------------------------ */
+#define QA 0.5854101966249685
+#define QB 0.1381966011250105
+static const double Q1_PTS[1][4] = {{0.25, 0.25, 0.25, 0.25}};
+static const double Q1_W[1] = {1.0 / 6.0};
+static const double Q4_PTS[4][4] = {
+    {QA, QB, QB, QB}, {QB, QA, QB, QB}, {QB, QB, QA, QB}, {QB, QB, QB, QA}};
+static const double Q4_W[4] = {1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0, 1.0 / 24.0};
+static const double Q5_PTS[5][4] = {{0.25, 0.25, 0.25, 0.25},
+                                    {1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0, 0.5},
+                                    {1.0 / 6.0, 1.0 / 6.0, 0.5, 1.0 / 6.0},
+                                    {1.0 / 6.0, 0.5, 1.0 / 6.0, 1.0 / 6.0},
+                                    {0.5, 1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0}};
+static const double Q5_W[5] = {-2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0,
+                               3.0 / 40.0};
+#define A1 0.25
+#define A2 0.0714285714285714
+#define B2 0.7857142857142857
+#define A3 0.3994035761667992
+#define B3 0.1005964238332008
+static const double Q11_PTS[11][4] = {
+    {A1, A1, A1, A1}, {A2, A2, A2, B2}, {A2, A2, B2, A2}, {A2, B2, A2, A2},
+    {B2, A2, A2, A2}, {A3, A3, B3, B3}, {A3, B3, A3, B3}, {A3, B3, B3, A3},
+    {B3, A3, A3, B3}, {B3, A3, B3, A3}, {B3, B3, A3, A3}};
+static const double Q11_W[11] = {
+    -0.01315555555555556, 0.007622222222222222, 0.007622222222222222,
+    0.007622222222222222, 0.007622222222222222, 0.024888888888888888,
+    0.024888888888888888, 0.024888888888888888, 0.024888888888888888,
+    0.024888888888888888, 0.024888888888888888};
+
+int or_quad_rule(int npts, const double **pts, const double **weights) {
+  switch (npts) {
+  case 1: *pts = &Q1_PTS[0][0]; *weights = Q1_W; return OR_OK;
+  case 4: *pts = &Q4_PTS[0][0]; *weights = Q4_W; return OR_OK;
+  case 5: *pts = &Q5_PTS[0][0]; *weights = Q5_W; return OR_OK;
+  case 11: *pts = &Q11_PTS[0][0]; *weights = Q11_W; return OR_OK;
+  default: return OR_ERR_INVALID_ARG;
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Tet10 basis: src/fem/basis.rs                                             */
+/* ------------------------------------------------------------------------ */
+
+/* src/fem/basis.rs:56-73 */
+void or_shape_functions(const double L[4], double N[10]) {
+  const double L0 = L[0], L1 = L[1], L2 = L[2], L3 = L[3];
+  N[0] = L0 * (2.0 * L0 - 1.0);
+  N[1] = L1 * (2.0 * L1 - 1.0);
+  N[2] = L2 * (2.0 * L2 - 1.0);
+  N[3] = L3 * (2.0 * L3 - 1.0);
+  N[4] = 4.0 * L0 * L1;
+  N[5] = 4.0 * L1 * L2;
+  N[6] = 4.0 * L2 * L0;
+  N[7] = 4.0 * L0 * L3;
+  N[8] = 4.0 * L1 * L3;
+  N[9] = 4.0 * L2 * L3;
+}
+
+/* src/fem/basis.rs:85-110 */
+void or_shape_derivs_bary(const double L[4], double dN[10][4]) {
+  const double L0 = L[0], L1 = L[1], L2 = L[2], L3 = L[3];
+  memset(dN, 0, sizeof(double) * 40);
+  dN[0][0] = 4.0 * L0 - 1.0;
+  dN[1][1] = 4.0 * L1 - 1.0;
+  dN[2][2] = 4.0 * L2 - 1.0;
+  dN[3][3] = 4.0 * L3 - 1.0;
+  dN[4][0] = 4.0 * L1; dN[4][1] = 4.0 * L0;
+  dN[5][1] = 4.0 * L2; dN[5][2] = 4.0 * L1;
+  dN[6][0] = 4.0 * L2; dN[6][2] = 4.0 * L0;
+  dN[7][0] = 4.0 * L3; dN[7][3] = 4.0 * L0;
+  dN[8][1] = 4.0 * L3; dN[8][3] = 4.0 * L1;
+  dN[9][2] = 4.0 * L3; dN[9][3] = 4.0 * L2;
+}
+
+/* src/fem/basis.rs:196-223: J[r][c] = sum_i x_i[r] * (dN_i/dL_{c+1} - dN_i/dL_0),
+ * accumulated over the 10 nodes in order. */
+void or_jacobian(const double L[4], const double nodes[10][3], double J[3][3]) {
+  double dN[10][4];
+  or_shape_derivs_bary(L, dN);
+  for (int r = 0; r < 3; ++r)
+    for (int c = 0; c < 3; ++c) J[r][c] = 0.0;
+  for (int i = 0; i < 10; ++i) {
+    const double d1 = dN[i][1] - dN[i][0];
+    const double d2 = dN[i][2] - dN[i][0];
+    const double d3 = dN[i][3] - dN[i][0];
+    for (int r = 0; r < 3; ++r) {
+      const double xr = nodes[i][r];
+      J[r][0] += xr * d1;
+      J[r][1] += xr * d2;
+      J[r][2] += xr * d3;
+    }
+  }
+}
+
+/* 3x3 determinant by cofactor expansion along the first row (the textbook
+ * form; nalgebra's exact rounding order is parity unpinned). */
+double or_det3(const double m[3][3]) {
+  const double minor_12_23 = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  const double minor_11_23 = m[1][0] * m[2][2] - m[2][0] * m[1][2];
+  const double minor_11_22 = m[1][0] * m[2][1] - m[2][0] * m[1][1];
+  return m[0][0] * minor_12_23 - m[0][1] * minor_11_23 + m[0][2] * minor_11_22;
+}
+
+/* Adjugate / determinant inverse; fails (returns nonzero) when det == 0,
+ * matching `J.try_inverse().expect("Singular Jacobian")` (src/fem/basis.rs:247). */
+int or_inv3(const double m[3][3], double o[3][3]) {
+  const double minor_12_23 = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  const double minor_11_23 = m[1][0] * m[2][2] - m[2][0] * m[1][2];
+  const double minor_11_22 = m[1][0] * m[2][1] - m[2][0] * m[1][1];
+  const double det =
+      m[0][0] * minor_12_23 - m[0][1] * minor_11_23 + m[0][2] * minor_11_22;
+  if (det == 0.0) return OR_ERR_SINGULAR_JACOBIAN;
+  o[0][0] = minor_12_23 / det;
+  o[0][1] = (m[0][2] * m[2][1] - m[2][2] * m[0][1]) / det;
+  o[0][2] = (m[0][1] * m[1][2] - m[1][1] * m[0][2]) / det;
+  o[1][0] = -minor_11_23 / det;
+  o[1][1] = (m[0][0] * m[2][2] - m[2][0] * m[0][2]) / det;
+  o[1][2] = (m[0][2] * m[1][0] - m[1][2] * m[0][0]) / det;
+  o[2][0] = minor_11_22 / det;
+  o[2][1] = (m[0][1] * m[2][0] - m[2][1] * m[0][0]) / det;
+  o[2][2] = (m[0][0] * m[1][1] - m[1][0] * m[0][1]) / det;
+  return OR_OK;
+}
+
+/* src/fem/basis.rs:236-276: grad N_i = J^{-T} (dN_i/dL1-dN_i/dL0, ...). */
+int or_shape_derivs_cart(const double L[4], const double nodes[10][3],
+                         double dNdx[10][3]) {
+  double dN[10][4], J[3][3], Ji[3][3];
+  or_shape_derivs_bary(L, dN);
+  or_jacobian(L, nodes, J);
+  if (or_inv3(J, Ji) != OR_OK) return OR_ERR_SINGULAR_JACOBIAN;
+  for (int i = 0; i < 10; ++i) {
+    const double v0 = dN[i][1] - dN[i][0];
+    const double v1 = dN[i][2] - dN[i][0];
+    const double v2 = dN[i][3] - dN[i][0];
+    for (int r = 0; r < 3; ++r) /* (J^-T v)[r] = sum_m Jinv[m][r] v[m] */
+      dNdx[i][r] = Ji[0][r] * v0 + Ji[1][r] * v1 + Ji[2][r] * v2;
+  }
+  return OR_OK;
+}
+
+/* src/fem/basis.rs:352-361 */
+double or_element_volume(const double nodes[10][3]) {
+  double vol = 0.0, J[3][3];
+  for (int q = 0; q < 4; ++q) {
+    or_jacobian(Q4_PTS[q], nodes, J);
+    vol += fabs(or_det3(J)) * Q4_W[q];
+  }
+  return vol;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Element kernels                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* src/fem/element.rs:23-61 (4-point rule; any rule for the rule-invariance
+ * check). K[i][j] += (k * grad_dot) * w, w = weight * |det J|. */
+int or_thermal_stiffness_rule(const double nodes[10][3], double k, int npts,
+                              double K[10][10]) {
+  const double *pts, *wts;
+  if (or_quad_rule(npts, &pts, &wts) != OR_OK) return OR_ERR_INVALID_ARG;
+  memset(K, 0, sizeof(double) * 100);
+  for (int q = 0; q < npts; ++q) {
+    const double *L = pts + 4 * q;
+    double dNdx[10][3], J[3][3];
+    if (or_shape_derivs_cart(L, nodes, dNdx) != OR_OK)
+      return OR_ERR_SINGULAR_JACOBIAN;
+    or_jacobian(L, nodes, J);
+    const double w = wts[q] * fabs(or_det3(J));
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 10; ++j) {
+        const double gd = dNdx[i][0] * dNdx[j][0] + dNdx[i][1] * dNdx[j][1] +
+                          dNdx[i][2] * dNdx[j][2];
+        K[i][j] += k * gd * w;
+      }
+  }
+  return OR_OK;
+}
+
+int or_thermal_stiffness(const double nodes[10][3], double k, double K[10][10]) {
+  return or_thermal_stiffness_rule(nodes, k, 4, K);
+}
+
+/* src/fem/element.rs:117-140: f_i += (N_i * Q) * w. */
+int or_thermal_load(const double nodes[10][3], double Q, double fe[10]) {
+  for (int i = 0; i < 10; ++i) fe[i] = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    double N[10], J[3][3];
+    or_shape_functions(Q4_PTS[q], N);
+    or_jacobian(Q4_PTS[q], nodes, J);
+    const double w = Q4_W[q] * fabs(or_det3(J));
+    for (int i = 0; i < 10; ++i) fe[i] += N[i] * Q * w;
+  }
+  return OR_OK;
+}
+
+/* src/mechanics/constitutive.rs:30-41 (validation) and :63-102 (D). */
+int or_constitutive(double E, double nu, double D[6][6]) {
+  if (!(E > 0.0)) return OR_ERR_INVALID_MATERIAL;
+  if (!(nu > -1.0 && nu < 0.5)) return OR_ERR_INVALID_MATERIAL;
+  const double factor = E / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double diag = 1.0 - nu, off = nu, shear = (1.0 - 2.0 * nu) / 2.0;
+  memset(D, 0, sizeof(double) * 36);
+  D[0][0] = diag; D[0][1] = off;  D[0][2] = off;
+  D[1][0] = off;  D[1][1] = diag; D[1][2] = off;
+  D[2][0] = off;  D[2][1] = off;  D[2][2] = diag;
+  D[3][3] = shear; D[4][4] = shear; D[5][5] = shear;
+  for (int a = 0; a < 6; ++a)
+    for (int b = 0; b < 6; ++b) D[a][b] = D[a][b] * factor;
+  return OR_OK;
+}
+
+/* src/mechanics/strain.rs:38-71: Voigt rows [xx,yy,zz,xy,yz,zx]. */
+void or_b_matrix(const double dNdx[10][3], double B[6][30]) {
+  memset(B, 0, sizeof(double) * 180);
+  for (int i = 0; i < 10; ++i) {
+    const int c = 3 * i;
+    const double dx = dNdx[i][0], dy = dNdx[i][1], dz = dNdx[i][2];
+    B[0][c + 0] = dx;
+    B[1][c + 1] = dy;
+    B[2][c + 2] = dz;
+    B[3][c + 0] = dy; B[3][c + 1] = dx;
+    B[4][c + 1] = dz; B[4][c + 2] = dy;
+    B[5][c + 2] = dx; B[5][c + 0] = dz;
+  }
+}
+
+/* src/mechanics/element.rs:33-65: dense DB = D*B, BtDB = B^T*(DB),
+ * K += w * BtDB, products accumulated in ascending inner index. */
+static int elastic_stiffness_D(const double nodes[10][3], const double D[6][6],
+                               double K[30][30]) {
+  memset(K, 0, sizeof(double) * 900);
+  for (int q = 0; q < 4; ++q) {
+    const double *L = Q4_PTS[q];
+    double J[3][3], dNdx[10][3], B[6][30], DB[6][30];
+    or_jacobian(L, nodes, J);
+    const double det = or_det3(J);
+    if (or_shape_derivs_cart(L, nodes, dNdx) != OR_OK)
+      return OR_ERR_SINGULAR_JACOBIAN;
+    or_b_matrix(dNdx, B);
+    const double w = Q4_W[q] * fabs(det);
+    for (int kk = 0; kk < 6; ++kk)
+      for (int c = 0; c < 30; ++c) {
+        double s = D[kk][0] * B[0][c];
+        for (int m = 1; m < 6; ++m) s += D[kk][m] * B[m][c];
+        DB[kk][c] = s;
+      }
+    for (int a = 0; a < 30; ++a)
+      for (int c = 0; c < 30; ++c) {
+        double s = B[0][a] * DB[0][c];
+        for (int kk = 1; kk < 6; ++kk) s += B[kk][a] * DB[kk][c];
+        K[a][c] += w * s;
+      }
+  }
+  return OR_OK;
+}
+
+int or_elastic_stiffness(const double nodes[10][3], double E, double nu,
+                         double K[30][30]) {
+  double D[6][6];
+  int st = or_constitutive(E, nu, D);
+  if (st != OR_OK) return st;
+  return elastic_stiffness_D(nodes, D, K);
+}
+
+/* src/mechanics/body_force.rs:49-84: f[3i+c] += (w * N_i) * (rho g)_c. */
+int or_gravity_load(const double nodes[10][3], double rho, const double g[3],
+                    double fe[30]) {
+  const double bf[3] = {rho * g[0], rho * g[1], rho * g[2]};
+  for (int i = 0; i < 30; ++i) fe[i] = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    double N[10], J[3][3];
+    or_shape_functions(Q4_PTS[q], N);
+    or_jacobian(Q4_PTS[q], nodes, J);
+    const double w = Q4_W[q] * fabs(or_det3(J));
+    for (int i = 0; i < 10; ++i)
+      for (int c = 0; c < 3; ++c) fe[3 * i + c] += w * N[i] * bf[c];
+  }
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Mesh generator: src/mesh_generator_improved.rs:19-169                     */
+/* ------------------------------------------------------------------------ */
+void or_cube_sizes(int64_t nx, int64_t ny, int64_t nz, int64_t *n_nodes,
+                   int64_t *n_elems) {
+  *n_nodes = (2 * nx + 1) * (2 * ny + 1) * (2 * nz + 1);
+  *n_elems = 6 * nx * ny * nz;
+}
+
+typedef struct {
+  uint64_t *keys; /* (lo << 32 | hi) + 1, 0 = empty */
+  int64_t *vals;
+  uint64_t mask;
+} edge_map;
+
+static uint64_t mix64(uint64_t x) {
+  x ^= x >> 33; x *= 0xff51afd7ed558ccdULL;
+  x ^= x >> 33; x *= 0xc4ceb9fe1a85ec53ULL;
+  x ^= x >> 33; return x;
+}
+
+/* get-or-create the midpoint of edge (v1,v2): edge cache of
+ * src/mesh_generator_improved.rs:42-57 (new nodes appended in first-touch
+ * order, coordinates (p1+p2)/2). */
+static int64_t edge_mid(edge_map *em, double *xyz, int64_t *n, int64_t v1,
+                        int64_t v2) {
+  const int64_t lo = v1 < v2 ? v1 : v2, hi = v1 < v2 ? v2 : v1;
+  const uint64_t key = (((uint64_t)lo << 32) | (uint64_t)hi) + 1;
+  uint64_t h = mix64(key) & em->mask;
+  while (em->keys[h] != 0) {
+    if (em->keys[h] == key) return em->vals[h];
+    h = (h + 1) & em->mask;
+  }
+  const int64_t id = (*n)++;
+  xyz[3 * id + 0] = (xyz[3 * v1 + 0] + xyz[3 * v2 + 0]) / 2.0;
+  xyz[3 * id + 1] = (xyz[3 * v1 + 1] + xyz[3 * v2 + 1]) / 2.0;
+  xyz[3 * id + 2] = (xyz[3 * v1 + 2] + xyz[3 * v2 + 2]) / 2.0;
+  em->keys[h] = key;
+  em->vals[h] = id;
+  return id;
+}
+
+int or_generate_cube(int64_t nx, int64_t ny, int64_t nz, double lx, double ly,
+                     double lz, double *xyz, int64_t *conn) {
+  if (nx <= 0 || ny <= 0 || nz <= 0) return OR_ERR_INVALID_ARG;
+  int64_t n_nodes, n_elems;
+  or_cube_sizes(nx, ny, nz, &n_nodes, &n_elems);
+  const double dx = lx / (double)nx, dy = ly / (double)ny, dz = lz / (double)nz;
+  int64_t n = 0;
+  for (int64_t iz = 0; iz <= nz; ++iz)
+    for (int64_t iy = 0; iy <= ny; ++iy)
+      for (int64_t ix = 0; ix <= nx; ++ix) {
+        xyz[3 * n + 0] = (double)ix * dx;
+        xyz[3 * n + 1] = (double)iy * dy;
+        xyz[3 * n + 2] = (double)iz * dz;
+        ++n;
+      }
+  uint64_t cap = 1;
+  while (cap < (uint64_t)(2 * n_nodes)) cap <<= 1;
+  edge_map em;
+  em.keys = (uint64_t *)calloc(cap, sizeof(uint64_t));
+  em.vals = (int64_t *)malloc(cap * sizeof(int64_t));
+  em.mask = cap - 1;
+  if (!em.keys || !em.vals) { free(em.keys); free(em.vals); return OR_ERR_OOM; }
+  const int64_t sx = 1, sy = nx + 1, sz = (nx + 1) * (ny + 1);
+  int64_t e = 0;
+  for (int64_t iz = 0; iz < nz; ++iz)
+    for (int64_t iy = 0; iy < ny; ++iy)
+      for (int64_t ix = 0; ix < nx; ++ix) {
+        const int64_t b = ix * sx + iy * sy + iz * sz;
+        const int64_t v000 = b, v100 = b + sx, v010 = b + sy, v110 = b + sx + sy;
+        const int64_t v001 = b + sz, v101 = b + sx + sz, v011 = b + sy + sz,
+                      v111 = b + sx + sy + sz;
+        /* src/mesh_generator_improved.rs:105-118 */
+        const int64_t tets[6][4] = {{v000, v100, v110, v111},
+                                    {v000, v100, v111, v101},
+                                    {v000, v010, v110, v111},
+                                    {v000, v010, v111, v011},
+                                    {v000, v001, v111, v101},
+                                    {v000, v001, v111, v011}};
+        for (int t = 0; t < 6; ++t) {
+          const int64_t v0 = tets[t][0], v1 = tets[t][1];
+          int64_t v2 = tets[t][2], v3 = tets[t][3];
+          /* orientation fix: src/mesh_generator_improved.rs:139-155 */
+          const double *p0 = xyz + 3 * v0, *p1 = xyz + 3 * v1,
+                       *p2 = xyz + 3 * v2, *p3 = xyz + 3 * v3;
+          const double e1[3] = {p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2]};
+          const double e2[3] = {p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2]};
+          const double e3[3] = {p3[0] - p0[0], p3[1] - p0[1], p3[2] - p0[2]};
+          const double cx = e2[1] * e3[2] - e2[2] * e3[1];
+          const double cy = e2[2] * e3[0] - e2[0] * e3[2];
+          const double cz = e2[0] * e3[1] - e2[1] * e3[0];
+          const double det = e1[0] * cx + e1[1] * cy + e1[2] * cz;
+          if (det < 0.0) { const int64_t tmp = v2; v2 = v3; v3 = tmp; }
+          /* edges in the order e01,e12,e20,e03,e13,e23 (:158-163) */
+          const int64_t e01 = edge_mid(&em, xyz, &n, v0, v1);
+          const int64_t e12 = edge_mid(&em, xyz, &n, v1, v2);
+          const int64_t e20 = edge_mid(&em, xyz, &n, v2, v0);
+          const int64_t e03 = edge_mid(&em, xyz, &n, v0, v3);
+          const int64_t e13 = edge_mid(&em, xyz, &n, v1, v3);
+          const int64_t e23 = edge_mid(&em, xyz, &n, v2, v3);
+          int64_t *c = conn + 10 * e;
+          c[0] = v0; c[1] = v1; c[2] = v2; c[3] = v3;
+          c[4] = e01; c[5] = e12; c[6] = e20; c[7] = e03; c[8] = e13; c[9] = e23;
+          ++e;
+        }
+      }
+  free(em.keys);
+  free(em.vals);
+  return (n == n_nodes && e == n_elems) ? OR_OK : OR_ERR_INVALID_ARG;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Triplets -> CSR with duplicates summed (sprs TriMat::to_csr semantics,    */
+/* src/fem/assembly.rs:57,116,229).  Summation order per slot: triplet       */
+/* insertion order (ascending element index for the assemblers).             */
+/* ------------------------------------------------------------------------ */
+void or_csr_free(or_csr *m) {
+  if (!m) return;
+  free(m->indptr); free(m->indices); free(m->data);
+  m->indptr = NULL; m->indices = NULL; m->data = NULL;
+  m->rows = m->cols = m->nnz = 0;
+}
+
+typedef struct { int64_t col; double val; } colval;
+
+static int cmp_colval(const void *a, const void *b) {
+  const int64_t x = ((const colval *)a)->col, y = ((const colval *)b)->col;
+  return (x > y) - (x < y);
+}
+
+/* trip_row/trip_col as int32 (when small) or int64; accessor by macro */
+static int triplets_to_csr(int64_t rows, int64_t cols, int64_t nt,
+                           const int32_t *ti32, const int32_t *tj32,
+                           const int64_t *ti64, const int64_t *tj64,
+                           const double *tv, or_csr *out) {
+#define TI(t) (ti32 ? (int64_t)ti32[t] : ti64[t])
+#define TJ(t) (tj32 ? (int64_t)tj32[t] : tj64[t])
+  memset(out, 0, sizeof(*out));
+  int64_t *rstart = (int64_t *)calloc((size_t)rows + 1, sizeof(int64_t));
+  int64_t *perm = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+  int64_t *pos = (int64_t *)malloc((size_t)(cols > 0 ? cols : 1) * sizeof(int64_t));
+  colval *acc = NULL;
+  int64_t acc_cap = 0;
+  if (!rstart || !perm || !pos) goto oom;
+  for (int64_t t = 0; t < nt; ++t) {
+    const int64_t i = TI(t), j = TJ(t);
+    if (i < 0 || i >= rows || j < 0 || j >= cols) {
+      free(rstart); free(perm); free(pos);
+      return OR_ERR_INVALID_ARG;
+    }
+    rstart[i + 1]++;
+  }
+  for (int64_t r = 0; r < rows; ++r) rstart[r + 1] += rstart[r];
+  {
+    int64_t *cur = (int64_t *)malloc((size_t)(rows > 0 ? rows : 1) * sizeof(int64_t));
+    if (!cur) goto oom;
+    memcpy(cur, rstart, (size_t)rows * sizeof(int64_t));
+    for (int64_t t = 0; t < nt; ++t) perm[cur[TI(t)]++] = t; /* stable */
+    free(cur);
+  }
+  for (int64_t c = 0; c < cols; ++c) pos[c] = -1;
+  out->rows = rows;
+  out->cols = cols;
+  out->indptr = (int64_t *)malloc((size_t)(rows + 1) * sizeof(int64_t));
+  out->indices = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+  out->data = (double *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(double));
+  if (!out->indptr || !out->indices || !out->data) goto oom;
+  int64_t nnz = 0;
+  out->indptr[0] = 0;
+  for (int64_t r = 0; r < rows; ++r) {
+    const int64_t b = rstart[r], e = rstart[r + 1];
+    if (e - b > acc_cap) {
+      acc_cap = (e - b) * 2;
+      free(acc);
+      acc = (colval *)malloc((size_t)acc_cap * sizeof(colval));
+      if (!acc) goto oom;
+    }
+    int64_t nu = 0;
+    for (int64_t k = b; k < e; ++k) {
+      const int64_t t = perm[k];
+      const int64_t j = TJ(t);
+      if (pos[j] < 0) {
+        pos[j] = nu;
+        acc[nu].col = j;
+        acc[nu].val = tv[t];
+        ++nu;
+      } else {
+        acc[pos[j]].val += tv[t];
+      }
+    }
+    for (int64_t u = 0; u < nu; ++u) pos[acc[u].col] = -1;
+    qsort(acc, (size_t)nu, sizeof(colval), cmp_colval);
+    for (int64_t u = 0; u < nu; ++u) {
+      out->indices[nnz] = acc[u].col;
+      out->data[nnz] = acc[u].val;
+      ++nnz;
+    }
+    out->indptr[r + 1] = nnz;
+  }
+  out->nnz = nnz;
+  if (nnz > 0) {
+    int64_t *ni = (int64_t *)realloc(out->indices, (size_t)nnz * sizeof(int64_t));
+    double *nd = (double *)realloc(out->data, (size_t)nnz * sizeof(double));
+    if (ni) out->indices = ni;
+    if (nd) out->data = nd;
+  }
+  free(rstart); free(perm); free(pos); free(acc);
+  return OR_OK;
+oom:
+  free(rstart); free(perm); free(pos); free(acc);
+  or_csr_free(out);
+  return OR_ERR_OOM;
+#undef TI
+#undef TJ
+}
+
+int or_csr_from_triplets(int64_t rows, int64_t cols, int64_t ntrip,
+                         const int64_t *ti, const int64_t *tj,
+                         const double *tv, or_csr *out) {
+  return triplets_to_csr(rows, cols, ntrip, NULL, NULL, ti, tj, tv, out);
+}
+
+static void gather_nodes(const double *xyz, const int64_t *c, double nodes[10][3]) {
+  for (int a = 0; a < 10; ++a)
+    for (int r = 0; r < 3; ++r) nodes[a][r] = xyz[3 * c[a] + r];
+}
+
+static int check_conn(int64_t n_nodes, int64_t n_elems, const int64_t *conn) {
+  for (int64_t i = 0; i < 10 * n_elems; ++i)
+    if (conn[i] < 0 || conn[i] >= n_nodes) return OR_ERR_INVALID_ARG;
+  return OR_OK;
+}
+
+static void set_threads(int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#else
+  (void)nthreads;
+#endif
+}
+
+/* src/fem/assembly.rs:69-117: parallel element kernels (Rayon map/collect ->
+ * OpenMP loop into an element-ordered array), then a serial element-major
+ * triplet merge and to_csr.  DOF = node id (:95-97). */
+int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double k, const double *k_elem,
+                        int nthreads, or_csr *out) {
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 100 * sizeof(double));
+  if (!Ke) return OR_ERR_OOM;
+  int err = OR_OK;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    const double ke = k_elem ? k_elem[e] : k;
+    if (or_thermal_stiffness(nodes, ke, (double(*)[10])(Ke + 100 * e)) != OR_OK) {
+#pragma omp atomic write
+      err = OR_ERR_SINGULAR_JACOBIAN;
+    }
+  }
+  if (err != OR_OK) { free(Ke); return err; }
+  const int64_t nt = 100 * n_elems;
+  const int small = n_nodes < 2147483647LL;
+  int32_t *ti32 = NULL, *tj32 = NULL;
+  int64_t *ti64 = NULL, *tj64 = NULL;
+  if (small) {
+    ti32 = (int32_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int32_t));
+    tj32 = (int32_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int32_t));
+  } else {
+    ti64 = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+    tj64 = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+  }
+  if ((small && (!ti32 || !tj32)) || (!small && (!ti64 || !tj64))) {
+    free(Ke); free(ti32); free(tj32); free(ti64); free(tj64);
+    return OR_ERR_OOM;
+  }
+  int64_t t = 0;
+  for (int64_t e = 0; e < n_elems; ++e) {
+    const int64_t *c = conn + 10 * e;
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 10; ++j) {
+        if (small) { ti32[t] = (int32_t)c[i]; tj32[t] = (int32_t)c[j]; }
+        else { ti64[t] = c[i]; tj64[t] = c[j]; }
+        ++t;
+      }
+  }
+  err = triplets_to_csr(n_nodes, n_nodes, nt, ti32, tj32, ti64, tj64, Ke, out);
+  free(Ke); free(ti32); free(tj32); free(ti64); free(tj64);
+  return err;
+}
+
+/* src/fem/assembly.rs:389-437: parallel 30x30 element matrices, then serial
+ * scatter with global_dof(node,c) = 3*node + c (src/fem/dof.rs:87-106). */
+int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double E, double nu,
+                        const double *E_elem, int nthreads, or_csr *out) {
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  double D[6][6];
+  if (!E_elem) {
+    const int st = or_constitutive(E, nu, D);
+    if (st != OR_OK) return st;
+  } else {
+    for (int64_t e = 0; e < n_elems; ++e)
+      if (or_constitutive(E_elem[e], nu, D) != OR_OK) return OR_ERR_INVALID_MATERIAL;
+  }
+  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 900 * sizeof(double));
+  if (!Ke) return OR_ERR_OOM;
+  int err = OR_OK;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], De[6][6];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    or_constitutive(E_elem ? E_elem[e] : E, nu, De);
+    if (elastic_stiffness_D(nodes, (const double(*)[6])De,
+                            (double(*)[30])(Ke + 900 * e)) != OR_OK) {
+#pragma omp atomic write
+      err = OR_ERR_SINGULAR_JACOBIAN;
+    }
+  }
+  if (err != OR_OK) { free(Ke); return err; }
+  const int64_t ndof = 3 * n_nodes;
+  const int64_t nt = 900 * n_elems;
+  const int small = ndof < 2147483647LL;
+  int32_t *ti32 = NULL, *tj32 = NULL;
+  int64_t *ti64 = NULL, *tj64 = NULL;
+  double *tv = (double *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(double));
+  if (small) {
+    ti32 = (int32_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int32_t));
+    tj32 = (int32_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int32_t));
+  } else {
+    ti64 = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+    tj64 = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
+  }
+  if (!tv || (small && (!ti32 || !tj32)) || (!small && (!ti64 || !tj64))) {
+    free(Ke); free(tv); free(ti32); free(tj32); free(ti64); free(tj64);
+    return OR_ERR_OOM;
+  }
+  int64_t t = 0;
+  for (int64_t e = 0; e < n_elems; ++e) {
+    const int64_t *c = conn + 10 * e;
+    const double *K = Ke + 900 * e;
+    for (int i = 0; i < 10; ++i)
+      for (int di = 0; di < 3; ++di) {
+        const int64_t gi = 3 * c[i] + di;
+        for (int j = 0; j < 10; ++j)
+          for (int dj = 0; dj < 3; ++dj) {
+            const int64_t gj = 3 * c[j] + dj;
+            if (small) { ti32[t] = (int32_t)gi; tj32[t] = (int32_t)gj; }
+            else { ti64[t] = gi; tj64[t] = gj; }
+            tv[t] = K[(3 * i + di) * 30 + 3 * j + dj];
+            ++t;
+          }
+      }
+  }
+  free(Ke);
+  err = triplets_to_csr(ndof, ndof, nt, ti32, tj32, ti64, tj64, tv, out);
+  free(tv); free(ti32); free(tj32); free(ti64); free(tj64);
+  return err;
+}
+
+/* src/fem/assembly.rs:130-154 (serial, element order). */
+int or_assemble_thermal_load(int64_t n_nodes, const double *xyz,
+                             int64_t n_elems, const int64_t *conn, double Q,
+                             double *f) {
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  for (int64_t i = 0; i < n_nodes; ++i) f[i] = 0.0;
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], fe[10];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    or_thermal_load(nodes, Q, fe);
+    for (int i = 0; i < 10; ++i) f[conn[10 * e + i]] += fe[i];
+  }
+  return OR_OK;
+}
+
+/* src/fem/assembly.rs:906-937 (serial, element order). */
+int or_assemble_gravity_load(int64_t n_nodes, const double *xyz,
+                             int64_t n_elems, const int64_t *conn, double rho,
+                             const double g[3], double *f) {
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  for (int64_t i = 0; i < 3 * n_nodes; ++i) f[i] = 0.0;
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], fe[30];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    or_gravity_load(nodes, rho, g, fe);
+    for (int i = 0; i < 10; ++i)
+      for (int c = 0; c < 3; ++c) f[3 * conn[10 * e + i] + c] += fe[3 * i + c];
+  }
+  return OR_OK;
+}
+
+/* src/fem/assembly.rs:168-230: scaled Dirichlet elimination.
+ * c = max(1, max_i |K_ii|); free rows: f_j -= K_ji g_i over constrained
+ * columns in column order; constrained row -> c e_i, f_i = g_i c; free rows
+ * keep only free columns. */
+int or_apply_dirichlet(const or_csr *K, const double *f, const uint8_t *is_dir,
+                       const double *values, or_csr *Kbc, double *fbc) {
+  memset(Kbc, 0, sizeof(*Kbc));
+  const int64_t n = K->rows;
+  double max_diag = 1.0;
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t k = K->indptr[r]; k < K->indptr[r + 1]; ++k)
+      if (K->indices[k] == r) {
+        const double a = fabs(K->data[k]);
+        if (a > max_diag) max_diag = a;
+      }
+  const double c = max_diag;
+  for (int64_t r = 0; r < n; ++r) fbc[r] = f[r];
+  for (int64_t r = 0; r < n; ++r) {
+    if (is_dir[r]) continue;
+    for (int64_t k = K->indptr[r]; k < K->indptr[r + 1]; ++k) {
+      const int64_t j = K->indices[k];
+      if (is_dir[j]) fbc[r] -= K->data[k] * values[j];
+    }
+  }
+  /* second pass: rows are already sorted/unique, so the rebuilt TriMat's
+   * to_csr is a filtered copy. */
+  int64_t nnz = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    if (is_dir[r]) { nnz += 1; continue; }
+    for (int64_t k = K->indptr[r]; k < K->indptr[r + 1]; ++k)
+      if (!is_dir[K->indices[k]]) ++nnz;
+  }
+  Kbc->rows = n;
+  Kbc->cols = K->cols;
+  Kbc->nnz = nnz;
+  Kbc->indptr = (int64_t *)malloc((size_t)(n + 1) * sizeof(int64_t));
+  Kbc->indices = (int64_t *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int64_t));
+  Kbc->data = (double *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+  if (!Kbc->indptr || !Kbc->indices || !Kbc->data) { or_csr_free(Kbc); return OR_ERR_OOM; }
+  int64_t p = 0;
+  Kbc->indptr[0] = 0;
+  for (int64_t r = 0; r < n; ++r) {
+    if (is_dir[r]) {
+      Kbc->indices[p] = r;
+      Kbc->data[p] = c;
+      ++p;
+      fbc[r] = values[r] * c;
+    } else {
+      for (int64_t k = K->indptr[r]; k < K->indptr[r + 1]; ++k) {
+        const int64_t j = K->indices[k];
+        if (!is_dir[j]) {
+          Kbc->indices[p] = j;
+          Kbc->data[p] = K->data[k];
+          ++p;
+        }
+      }
+    }
+    Kbc->indptr[r + 1] = p;
+  }
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Linear algebra                                                            */
+/* ------------------------------------------------------------------------ */
+
+/* src/linalg/solver.rs:62-73: row-serial sequential sum. */
+void or_spmv(const or_csr *A, const double *x, double *y) {
+  for (int64_t r = 0; r < A->rows; ++r) {
+    double s = 0.0;
+    for (int64_t k = A->indptr[r]; k < A->indptr[r + 1]; ++k)
+      s += A->data[k] * x[A->indices[k]];
+    y[r] = s;
+  }
+}
+
+/* src/linalg/preconditioner.rs:37-51: 1/A_ii if stored and |A_ii| > 1e-14,
+ * else 1. */
+void or_jacobi_diag_inv(const or_csr *A, double *dinv) {
+  for (int64_t r = 0; r < A->rows; ++r) {
+    dinv[r] = 1.0;
+    for (int64_t k = A->indptr[r]; k < A->indptr[r + 1]; ++k)
+      if (A->indices[k] == r) {
+        if (fabs(A->data[k]) > 1e-14) dinv[r] = 1.0 / A->data[k];
+        break;
+      }
+  }
+}
+
+static double norm2(const double *v, int64_t n) {
+  double s = 0.0;
+  for (int64_t i = 0; i < n; ++i) s += v[i] * v[i];
+  return sqrt(s);
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
+/* src/linalg/iterative.rs:46-119 (+ :123-131 for the preconditioner choice),
+ * statement by statement.  The iteration counter excludes the converging
+ * pass; the recursive residual is never recomputed. */
+int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
+                double abs_tol, int use_precond, double *x, or_stats *st) {
+  const int64_t n = A->rows;
+  const double t0 = now_s();
+  const double b_norm = norm2(b, n);
+  memset(st, 0, sizeof(*st));
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  if (b_norm < 1e-25) {
+    st->converged = 1;
+    st->solve_time = now_s() - t0;
+    return OR_OK;
+  }
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *z = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *ap = (double *)malloc((size_t)n * sizeof(double));
+  double *dinv = (double *)malloc((size_t)n * sizeof(double));
+  if (!r || !z || !p || !ap || !dinv) {
+    free(r); free(z); free(p); free(ap); free(dinv);
+    return OR_ERR_OOM;
+  }
+  if (use_precond) or_jacobi_diag_inv(A, dinv);
+  else for (int64_t i = 0; i < n; ++i) dinv[i] = 1.0;
+  for (int64_t i = 0; i < n; ++i) r[i] = b[i];
+  for (int64_t i = 0; i < n; ++i) z[i] = use_precond ? r[i] * dinv[i] : r[i];
+  for (int64_t i = 0; i < n; ++i) p[i] = z[i];
+  double rz = 0.0;
+  for (int64_t i = 0; i < n; ++i) rz += r[i] * z[i];
+  int64_t it = 0;
+  int converged = 0;
+  double final_res = b_norm;
+  while (it < max_iter) {
+    or_spmv(A, p, ap);
+    double pap = 0.0;
+    for (int64_t i = 0; i < n; ++i) pap += p[i] * ap[i];
+    if (fabs(pap) < 1e-30) break;
+    const double alpha = rz / pap;
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * ap[i];
+    }
+    const double rn = norm2(r, n);
+    final_res = rn;
+    if (rn < tol * b_norm || rn < abs_tol) { converged = 1; break; }
+    for (int64_t i = 0; i < n; ++i) z[i] = use_precond ? r[i] * dinv[i] : r[i];
+    double rz_new = 0.0;
+    for (int64_t i = 0; i < n; ++i) rz_new += r[i] * z[i];
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  st->iterations = it;
+  st->residual_norm = final_res;
+  st->relative_residual = b_norm > 1e-20 ? final_res / b_norm : 0.0;
+  st->converged = converged;
+  st->solve_time = now_s() - t0;
+  free(r); free(z); free(p); free(ap); free(dinv);
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint element backward (no reference counterpart; pinned by FD).        */
+/* With theta_e = log k_e (or log E_e), dK_e/dtheta_e = K_e, so              */
+/* dL/dtheta_e = -lambda_e^T K_e u_e with u_e including Dirichlet values.    */
+/* ------------------------------------------------------------------------ */
+int or_thermal_element_backward(int64_t n_nodes, const double *xyz,
+                                int64_t n_elems, const int64_t *conn,
+                                const double *k_elem, const double *u,
+                                const double *lam, double *grad) {
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  int err = OR_OK;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], K[10][10];
+    const int64_t *c = conn + 10 * e;
+    gather_nodes(xyz, c, nodes);
+    if (or_thermal_stiffness(nodes, k_elem[e], K) != OR_OK) {
+#pragma omp atomic write
+      err = OR_ERR_SINGULAR_JACOBIAN;
+      continue;
+    }
+    double s = 0.0;
+    for (int a = 0; a < 10; ++a) {
+      double ka = 0.0;
+      for (int b = 0; b < 10; ++b) ka += K[a][b] * u[c[b]];
+      s += lam[c[a]] * ka;
+    }
+    grad[e] = -s;
+  }
+  return err;
+}
+
+int or_elastic_element_backward(int64_t n_nodes, const double *xyz,
+                                int64_t n_elems, const int64_t *conn,
+                                const double *E_elem, double nu,
+                                const double *u, const double *lam,
+                                double *grad) {
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  int err = OR_OK;
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], K[30][30];
+    const int64_t *c = conn + 10 * e;
+    gather_nodes(xyz, c, nodes);
+    if (or_elastic_stiffness(nodes, E_elem[e], nu, K) != OR_OK) {
+#pragma omp atomic write
+      err = OR_ERR_SINGULAR_JACOBIAN;
+      continue;
+    }
+    double s = 0.0;
+    for (int a = 0; a < 30; ++a) {
+      double ka = 0.0;
+      for (int b = 0; b < 30; ++b) ka += K[a][b] * u[3 * c[b / 3] + b % 3];
+      s += lam[3 * c[a / 3] + a % 3] * ka;
+    }
+    grad[e] = -s;
+  }
+  return err;
+}

@@ -1,0 +1,140 @@
+/*
+ * geofem_oracle.h -- FP64 CPU restatement of geo_simulator's Tet10
+ * assemble-and-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the correctness checker for the
+ * HIP implementation in geo_simulator_amd/.  Only tests/, the smoke check and
+ * the CPU-baseline leg of bench.py may load it.  The product path never links
+ * or calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the reference repository root).  Arithmetic is plain IEEE FP64 in the
+ * reference's operation order, compiled without floating-point contraction
+ * (-ffp-contract=off), because Rust does not fuse multiply-adds.
+ *
+ * Pinning (what checks each function independently of this file):
+ *  - quadrature tables: weights sum to 1/6 and exactness on monomials up to
+ *    each rule's degree, against exact rational integrals (tests).
+ *  - basis/Jacobian: partition of unity, Kronecker delta, linear-field
+ *    reproduction, derivative sums, volumes, curved-element volume (the
+ *    reference's own unit tests) and J == vertex-edge matrix for affine tets.
+ *  - element matrices/loads: exact rational golden vectors (sympy) for the
+ *    reference tet and the six Kuhn tets (tests/golden/).
+ *  - global assembly: scipy COO->CSR duplicate summation, dense brute force,
+ *    K*1 == 0 (thermal).
+ *  - Dirichlet elimination: dense brute force restatement on small meshes,
+ *    closed-form nodal solutions (thermal quadratic profile, confined
+ *    elastic column).
+ *  - CG: the reference's 2x2 KAT, scipy spsolve, iteration-count semantics.
+ *  - element backward (adjoint): central finite differences.
+ * Bit-exact agreement with nalgebra's 3x3 determinant/inverse rounding and
+ * with sprs' duplicate-summation order is parity unpinned (neither crate's
+ * source is available); this oracle fixes its own order (ascending element
+ * index per CSR slot) and documents it.
+ */
+#ifndef GEOFEM_ORACLE_H
+#define GEOFEM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  OR_OK = 0,
+  OR_ERR_INVALID_ARG = 1,
+  OR_ERR_SINGULAR_JACOBIAN = 3,
+  OR_ERR_BAD_DOFS = 4,
+  OR_ERR_INVALID_MATERIAL = 5,
+  OR_ERR_OOM = 8
+};
+
+/* ---- quadrature (src/fem/quadrature.rs:13-104) ---- */
+int or_quad_rule(int npts, const double **pts, const double **weights);
+
+/* ---- Tet10 basis (src/fem/basis.rs) ---- */
+void or_shape_functions(const double L[4], double N[10]);
+void or_shape_derivs_bary(const double L[4], double dN[10][4]);
+void or_jacobian(const double L[4], const double nodes[10][3], double J[3][3]);
+double or_det3(const double J[3][3]);
+int or_inv3(const double J[3][3], double Jinv[3][3]);
+int or_shape_derivs_cart(const double L[4], const double nodes[10][3],
+                         double dNdx[10][3]);
+double or_element_volume(const double nodes[10][3]);
+
+/* ---- element kernels ---- */
+int or_thermal_stiffness(const double nodes[10][3], double k, double Ke[10][10]);
+int or_thermal_stiffness_rule(const double nodes[10][3], double k, int npts,
+                              double Ke[10][10]);
+int or_thermal_load(const double nodes[10][3], double Q, double fe[10]);
+int or_constitutive(double E, double nu, double D[6][6]);
+void or_b_matrix(const double dNdx[10][3], double B[6][30]);
+int or_elastic_stiffness(const double nodes[10][3], double E, double nu,
+                         double Ke[30][30]);
+int or_gravity_load(const double nodes[10][3], double rho, const double g[3],
+                    double fe[30]);
+
+/* ---- mesh generator (src/mesh_generator_improved.rs:19-169) ---- */
+void or_cube_sizes(int64_t nx, int64_t ny, int64_t nz, int64_t *n_nodes,
+                   int64_t *n_elems);
+int or_generate_cube(int64_t nx, int64_t ny, int64_t nz, double lx, double ly,
+                     double lz, double *xyz, int64_t *conn);
+
+/* ---- global assembly (src/fem/assembly.rs) ---- */
+typedef struct {
+  int64_t rows, cols, nnz;
+  int64_t *indptr;  /* rows+1 */
+  int64_t *indices; /* nnz, sorted ascending within a row */
+  double *data;     /* nnz */
+} or_csr;
+
+void or_csr_free(or_csr *m);
+int or_csr_from_triplets(int64_t rows, int64_t cols, int64_t ntrip,
+                         const int64_t *ti, const int64_t *tj,
+                         const double *tv, or_csr *out);
+int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double k, const double *k_elem,
+                        int nthreads, or_csr *out);
+int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double E, double nu,
+                        const double *E_elem, int nthreads, or_csr *out);
+int or_assemble_thermal_load(int64_t n_nodes, const double *xyz,
+                             int64_t n_elems, const int64_t *conn, double Q,
+                             double *f);
+int or_assemble_gravity_load(int64_t n_nodes, const double *xyz,
+                             int64_t n_elems, const int64_t *conn, double rho,
+                             const double g[3], double *f);
+int or_apply_dirichlet(const or_csr *K, const double *f, const uint8_t *is_dir,
+                       const double *values, or_csr *Kbc, double *fbc);
+
+/* ---- linear algebra (src/linalg/) ---- */
+typedef struct {
+  int64_t iterations;
+  double residual_norm;
+  double relative_residual;
+  int32_t converged;
+  int32_t pad_;
+  double solve_time;
+} or_stats;
+
+void or_spmv(const or_csr *A, const double *x, double *y);
+void or_jacobi_diag_inv(const or_csr *A, double *dinv);
+int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
+                double abs_tol, int use_precond, double *x, or_stats *st);
+
+/* ---- adjoint element backward (new surface; pinned by central FD) ---- */
+int or_thermal_element_backward(int64_t n_nodes, const double *xyz,
+                                int64_t n_elems, const int64_t *conn,
+                                const double *k_elem, const double *u,
+                                const double *lam, double *grad);
+int or_elastic_element_backward(int64_t n_nodes, const double *xyz,
+                                int64_t n_elems, const int64_t *conn,
+                                const double *E_elem, double nu,
+                                const double *u, const double *lam,
+                                double *grad);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,427 @@
+"""ctypes front-end of the FP64 CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, always as the checker.  The product package
+(geo_simulator_amd) never imports this module.
+
+Each wrapper names the reference function it stands for; the C file cites the
+reference file:line of every statement block.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+import scipy.sparse as sp
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+
+OR_OK = 0
+OR_ERR_INVALID_ARG = 1
+OR_ERR_SINGULAR_JACOBIAN = 3
+OR_ERR_BAD_DOFS = 4
+OR_ERR_INVALID_MATERIAL = 5
+
+
+class OracleError(RuntimeError):
+    def __init__(self, code: int, what: str):
+        super().__init__(f"{what}: oracle status {code}")
+        self.code = code
+
+
+class _Csr(C.Structure):
+    _fields_ = [("rows", C.c_int64), ("cols", C.c_int64), ("nnz", C.c_int64),
+                ("indptr", C.POINTER(C.c_int64)),
+                ("indices", C.POINTER(C.c_int64)),
+                ("data", C.POINTER(C.c_double))]
+
+
+class _Stats(C.Structure):
+    _fields_ = [("iterations", C.c_int64), ("residual_norm", C.c_double),
+                ("relative_residual", C.c_double), ("converged", C.c_int32),
+                ("pad_", C.c_int32), ("solve_time", C.c_double)]
+
+
+@dataclass
+class SolverStats:
+    iterations: int
+    residual_norm: float
+    relative_residual: float
+    converged: bool
+    solve_time: float
+
+
+def build() -> None:
+    """Compile oracle/liboracle.so with the committed Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        _lib = C.CDLL(_LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+_dp = C.POINTER(C.c_double)
+_i64p = C.POINTER(C.c_int64)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def _declare(L):
+    L.or_quad_rule.argtypes = [C.c_int, C.POINTER(_dp), C.POINTER(_dp)]
+    L.or_shape_functions.argtypes = [_dp, _dp]
+    L.or_shape_derivs_bary.argtypes = [_dp, _dp]
+    L.or_jacobian.argtypes = [_dp, _dp, _dp]
+    L.or_det3.argtypes = [_dp]
+    L.or_det3.restype = C.c_double
+    L.or_inv3.argtypes = [_dp, _dp]
+    L.or_shape_derivs_cart.argtypes = [_dp, _dp, _dp]
+    L.or_element_volume.argtypes = [_dp]
+    L.or_element_volume.restype = C.c_double
+    L.or_thermal_stiffness.argtypes = [_dp, C.c_double, _dp]
+    L.or_thermal_stiffness_rule.argtypes = [_dp, C.c_double, C.c_int, _dp]
+    L.or_thermal_load.argtypes = [_dp, C.c_double, _dp]
+    L.or_constitutive.argtypes = [C.c_double, C.c_double, _dp]
+    L.or_b_matrix.argtypes = [_dp, _dp]
+    L.or_elastic_stiffness.argtypes = [_dp, C.c_double, C.c_double, _dp]
+    L.or_gravity_load.argtypes = [_dp, C.c_double, _dp, _dp]
+    L.or_cube_sizes.argtypes = [C.c_int64] * 3 + [_i64p, _i64p]
+    L.or_generate_cube.argtypes = [C.c_int64] * 3 + [C.c_double] * 3 + [_dp, _i64p]
+    L.or_csr_free.argtypes = [C.POINTER(_Csr)]
+    L.or_csr_from_triplets.argtypes = [C.c_int64, C.c_int64, C.c_int64, _i64p,
+                                       _i64p, _dp, C.POINTER(_Csr)]
+    L.or_assemble_thermal.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                      C.c_double, _dp, C.c_int, C.POINTER(_Csr)]
+    L.or_assemble_elastic.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                      C.c_double, C.c_double, _dp, C.c_int,
+                                      C.POINTER(_Csr)]
+    L.or_assemble_thermal_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                           C.c_double, _dp]
+    L.or_assemble_gravity_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                           C.c_double, _dp, _dp]
+    L.or_apply_dirichlet.argtypes = [C.POINTER(_Csr), _dp, _u8p, _dp,
+                                     C.POINTER(_Csr), _dp]
+    L.or_spmv.argtypes = [C.POINTER(_Csr), _dp, _dp]
+    L.or_jacobi_diag_inv.argtypes = [C.POINTER(_Csr), _dp]
+    L.or_cg_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_double,
+                              C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
+    L.or_thermal_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                              _dp, _dp, _dp, _dp]
+    L.or_elastic_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
+                                              _dp, C.c_double, _dp, _dp, _dp]
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_i64p)
+
+
+def _f64(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def _check(code, what):
+    if code != OR_OK:
+        raise OracleError(code, what)
+
+
+# ---- quadrature / basis -------------------------------------------------
+
+def quad_rule(npts: int):
+    """GaussQuadrature::tet_{1,4,5,11}point (src/fem/quadrature.rs:13-104)."""
+    p, w = _dp(), _dp()
+    _check(lib().or_quad_rule(npts, C.byref(p), C.byref(w)), "quad_rule")
+    pts = np.ctypeslib.as_array(p, shape=(npts, 4)).copy()
+    wts = np.ctypeslib.as_array(w, shape=(npts,)).copy()
+    return pts, wts
+
+
+def shape_functions(L):
+    out = np.zeros(10)
+    lib().or_shape_functions(_d(_f64(L)), _d(out))
+    return out
+
+
+def shape_derivs_bary(L):
+    out = np.zeros((10, 4))
+    lib().or_shape_derivs_bary(_d(_f64(L)), _d(out))
+    return out
+
+
+def jacobian(L, nodes):
+    out = np.zeros((3, 3))
+    lib().or_jacobian(_d(_f64(L)), _d(_f64(nodes, (10, 3))), _d(out))
+    return out
+
+
+def shape_derivs_cart(L, nodes):
+    out = np.zeros((10, 3))
+    _check(lib().or_shape_derivs_cart(_d(_f64(L)), _d(_f64(nodes, (10, 3))),
+                                      _d(out)), "shape_derivs_cart")
+    return out
+
+
+def element_volume(nodes):
+    return lib().or_element_volume(_d(_f64(nodes, (10, 3))))
+
+
+# ---- element kernels ----------------------------------------------------
+
+def thermal_stiffness(nodes, k, npts=4):
+    out = np.zeros((10, 10))
+    _check(lib().or_thermal_stiffness_rule(_d(_f64(nodes, (10, 3))), k, npts,
+                                           _d(out)), "thermal_stiffness")
+    return out
+
+
+def thermal_load(nodes, Q):
+    out = np.zeros(10)
+    _check(lib().or_thermal_load(_d(_f64(nodes, (10, 3))), Q, _d(out)),
+           "thermal_load")
+    return out
+
+
+def constitutive(E, nu):
+    out = np.zeros((6, 6))
+    _check(lib().or_constitutive(E, nu, _d(out)), "constitutive")
+    return out
+
+
+def b_matrix(dNdx):
+    out = np.zeros((6, 30))
+    lib().or_b_matrix(_d(_f64(dNdx, (10, 3))), _d(out))
+    return out
+
+
+def elastic_stiffness(nodes, E, nu):
+    out = np.zeros((30, 30))
+    _check(lib().or_elastic_stiffness(_d(_f64(nodes, (10, 3))), E, nu, _d(out)),
+           "elastic_stiffness")
+    return out
+
+
+def gravity_load(nodes, rho, g):
+    out = np.zeros(30)
+    _check(lib().or_gravity_load(_d(_f64(nodes, (10, 3))), rho, _d(_f64(g)),
+                                 _d(out)), "gravity_load")
+    return out
+
+
+# ---- mesh ---------------------------------------------------------------
+
+def generate_cube(nx, ny, nz, lx, ly, lz):
+    """ImprovedMeshGenerator::generate_cube -> (xyz (n,3) f64, conn (E,10) i64)."""
+    nn, ne = C.c_int64(), C.c_int64()
+    lib().or_cube_sizes(nx, ny, nz, C.byref(nn), C.byref(ne))
+    xyz = np.zeros((nn.value, 3))
+    conn = np.zeros((ne.value, 10), dtype=np.int64)
+    _check(lib().or_generate_cube(nx, ny, nz, lx, ly, lz, _d(xyz), _i(conn)),
+           "generate_cube")
+    return xyz, conn
+
+
+# ---- assembly -----------------------------------------------------------
+
+def _take_csr(m: _Csr) -> sp.csr_matrix:
+    rows, cols, nnz = m.rows, m.cols, m.nnz
+    indptr = np.ctypeslib.as_array(m.indptr, shape=(rows + 1,)).copy()
+    if nnz > 0:
+        indices = np.ctypeslib.as_array(m.indices, shape=(nnz,)).copy()
+        data = np.ctypeslib.as_array(m.data, shape=(nnz,)).copy()
+    else:
+        indices = np.zeros(0, dtype=np.int64)
+        data = np.zeros(0)
+    lib().or_csr_free(C.byref(m))
+    return sp.csr_matrix((data, indices, indptr), shape=(rows, cols))
+
+
+def _give_csr(A: sp.csr_matrix):
+    A = sp.csr_matrix(A)
+    indptr = np.ascontiguousarray(A.indptr, dtype=np.int64)
+    indices = np.ascontiguousarray(A.indices, dtype=np.int64)
+    data = np.ascontiguousarray(A.data, dtype=np.float64)
+    m = _Csr(A.shape[0], A.shape[1], A.nnz, _i(indptr), _i(indices), _d(data))
+    return m, (indptr, indices, data)
+
+
+def csr_from_triplets(rows, cols, ti, tj, tv):
+    ti = np.ascontiguousarray(ti, dtype=np.int64)
+    tj = np.ascontiguousarray(tj, dtype=np.int64)
+    tv = _f64(tv)
+    m = _Csr()
+    _check(lib().or_csr_from_triplets(rows, cols, len(ti), _i(ti), _i(tj), _d(tv),
+                                      C.byref(m)), "csr_from_triplets")
+    return _take_csr(m)
+
+
+def assemble_thermal(xyz, conn, k, k_elem=None, nthreads=0):
+    """Assembler::assemble_thermal_stiffness_parallel (src/fem/assembly.rs:69-117)."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    ke = None if k_elem is None else _f64(k_elem)
+    m = _Csr()
+    _check(lib().or_assemble_thermal(len(xyz), _d(xyz), len(conn), _i(conn), k,
+                                     None if ke is None else _d(ke), nthreads,
+                                     C.byref(m)), "assemble_thermal")
+    return _take_csr(m)
+
+
+def assemble_elastic(xyz, conn, E, nu, E_elem=None, nthreads=0):
+    """Assembler::assemble_elasticity_stiffness_parallel (src/fem/assembly.rs:389-437)."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    ee = None if E_elem is None else _f64(E_elem)
+    m = _Csr()
+    _check(lib().or_assemble_elastic(len(xyz), _d(xyz), len(conn), _i(conn), E,
+                                     nu, None if ee is None else _d(ee), nthreads,
+                                     C.byref(m)), "assemble_elastic")
+    return _take_csr(m)
+
+
+def assemble_thermal_load(xyz, conn, Q):
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    f = np.zeros(len(xyz))
+    _check(lib().or_assemble_thermal_load(len(xyz), _d(xyz), len(conn), _i(conn),
+                                          Q, _d(f)), "assemble_thermal_load")
+    return f
+
+
+def assemble_gravity_load(xyz, conn, rho, g):
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    f = np.zeros(3 * len(xyz))
+    _check(lib().or_assemble_gravity_load(len(xyz), _d(xyz), len(conn), _i(conn),
+                                          rho, _d(_f64(g)), _d(f)),
+           "assemble_gravity_load")
+    return f
+
+
+def apply_dirichlet(K, f, is_dir, values):
+    """Assembler::apply_dirichlet_bcs (src/fem/assembly.rs:168-230)."""
+    m, keep = _give_csr(K)
+    f = _f64(f)
+    mask = np.ascontiguousarray(is_dir, dtype=np.uint8)
+    vals = _f64(values)
+    out = _Csr()
+    fbc = np.zeros(len(f))
+    _check(lib().or_apply_dirichlet(C.byref(m), _d(f), mask.ctypes.data_as(_u8p),
+                                    _d(vals), C.byref(out), _d(fbc)),
+           "apply_dirichlet")
+    del keep
+    return _take_csr(out), fbc
+
+
+# ---- linear algebra -----------------------------------------------------
+
+def spmv(A, x):
+    m, keep = _give_csr(A)
+    x = _f64(x)
+    y = np.zeros(A.shape[0])
+    lib().or_spmv(C.byref(m), _d(x), _d(y))
+    return y
+
+
+def jacobi_diag_inv(A):
+    m, keep = _give_csr(A)
+    d = np.zeros(A.shape[0])
+    lib().or_jacobi_diag_inv(C.byref(m), _d(d))
+    return d
+
+
+def cg_solve(A, b, max_iter=1000, tol=1e-8, abs_tol=1e-12, use_precond=True):
+    """ConjugateGradient::solve (src/linalg/iterative.rs:46-131)."""
+    m, keep = _give_csr(A)
+    b = _f64(b)
+    x = np.zeros(A.shape[0])
+    st = _Stats()
+    _check(lib().or_cg_solve(C.byref(m), _d(b), max_iter, tol, abs_tol,
+                             1 if use_precond else 0, _d(x), C.byref(st)),
+           "cg_solve")
+    return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
+                          bool(st.converged), st.solve_time)
+
+
+# ---- adjoint --------------------------------------------------------------
+
+def thermal_element_backward(xyz, conn, k_elem, u, lam):
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    g = np.zeros(len(conn))
+    _check(lib().or_thermal_element_backward(len(xyz), _d(xyz), len(conn),
+                                             _i(conn), _d(_f64(k_elem)),
+                                             _d(_f64(u)), _d(_f64(lam)), _d(g)),
+           "thermal_element_backward")
+    return g
+
+
+def elastic_element_backward(xyz, conn, E_elem, nu, u, lam):
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    g = np.zeros(len(conn))
+    _check(lib().or_elastic_element_backward(len(xyz), _d(xyz), len(conn),
+                                             _i(conn), _d(_f64(E_elem)), nu,
+                                             _d(_f64(u)), _d(_f64(lam)), _d(g)),
+           "elastic_element_backward")
+    return g
+
+
+# ---- reference-benchmark problem set-ups -----------------------------------
+
+def thermal_dirichlet(xyz, lz, t_bottom=1280.0, t_top=0.0):
+    """BC selection of benchmarks/thermal_1d_improved_mesh.rs:46-52."""
+    z = xyz[:, 2]
+    mask = np.zeros(len(xyz), dtype=np.uint8)
+    vals = np.zeros(len(xyz))
+    bot = np.abs(z) < 1.0
+    top = (~bot) & (np.abs(z - lz) < 1.0)
+    mask[bot] = 1
+    vals[bot] = t_bottom
+    mask[top] = 1
+    vals[top] = t_top
+    return mask, vals
+
+
+def thermal_analytic(z, lz, t_bottom, t_top, k, q):
+    """benchmarks/thermal_1d_improved_mesh.rs:9-13."""
+    return t_bottom + (t_top - t_bottom) * (z / lz) + (q / (2.0 * k)) * z * (lz - z)
+
+
+def elastic_dirichlet(xyz, lx, ly):
+    """Confined column: bottom fixed (all components), u_x = 0 on x faces,
+    u_y = 0 on y faces (BC pattern of benchmarks/elasticity_halfspace_gravity.rs:75-103
+    with the bottom fully fixed)."""
+    n = len(xyz)
+    mask = np.zeros(3 * n, dtype=np.uint8)
+    x, y, z = xyz[:, 0], xyz[:, 1], xyz[:, 2]
+    bot = np.abs(z) < 1.0
+    for c in range(3):
+        mask[3 * np.nonzero(bot)[0] + c] = 1
+    xf = (np.abs(x) < 1.0) | (np.abs(x - lx) < 1.0)
+    yf = (np.abs(y) < 1.0) | (np.abs(y - ly) < 1.0)
+    mask[3 * np.nonzero(xf)[0] + 0] = 1
+    mask[3 * np.nonzero(yf)[0] + 1] = 1
+    return mask, np.zeros(3 * n)
+
+
+def elastic_analytic_uz(z, H, rho, g, E, nu):
+    """benchmarks/elasticity_halfspace_gravity.rs:23-27."""
+    M = E * (1.0 - nu) / ((1.0 + nu) * (1.0 - 2.0 * nu))
+    return -(rho * g / M) * (H * z - z * z / 2.0)

@@ -25,6 +25,8 @@ GS_ERR_OOM = 8
 GS_PREC_FP64 = 0
 GS_PREC_MIXED = 1
 
+GS_SPMV_VECTOR = 4
+
 _STATUS_NAMES = {
     GS_ERR_INVALID_ARG: "GS_ERR_INVALID_ARG", GS_ERR_HIP: "GS_ERR_HIP",
     GS_ERR_SINGULAR_JACOBIAN: "GS_ERR_SINGULAR_JACOBIAN",
@@ -105,6 +107,7 @@ SIGNATURES = {
     "gs_thermal_element_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_elastic_element_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp,
                                               _vp, _vp]),
+    "gs_csr_set_spmv_shape": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
     "gs_time_spmv": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _dp]),
 }
 

@@ -688,6 +688,22 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
   });
 }
 
+gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
+                                int32_t flags) {
+  if (!A) return GS_ERR_INVALID_ARG;
+  if (flags & ~GS_SPMV_VECTOR) return GS_ERR_INVALID_ARG;
+  const bool vec = (flags & GS_SPMV_VECTOR) != 0;
+  const bool ok = lanes == 0 ||
+                  (vec ? (lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32)
+                       : (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8));
+  if (!ok) return GS_ERR_INVALID_ARG;
+  if (workgroups < 0 || workgroups > 16384) return GS_ERR_INVALID_ARG;
+  A->spmv_lanes = lanes;
+  A->spmv_grid = workgroups;
+  A->spmv_flags = flags;
+  return GS_OK;
+}
+
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms) {
   if (!ctx || !A || !avg_ms) return GS_ERR_INVALID_ARG;

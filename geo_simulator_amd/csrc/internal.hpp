@@ -112,7 +112,8 @@ struct gs_csr {
   gs::DArr<float> val32;
   gs::DArr<double> dinv;
   gs::DArr<float> dinv32;
-  int lanes = 0;  // lanes per row used by the SpMV kernels
+  // SpMV launch shape (0 = automatic); see gs_csr_set_spmv_shape
+  int spmv_lanes = 0, spmv_grid = 0, spmv_flags = 0;
 };
 
 namespace gs {

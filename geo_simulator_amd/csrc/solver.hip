@@ -7,10 +7,17 @@
 //     q = A p; pq = p.q; break if |pq| < 1e-30; alpha = rz/pq
 //     x += alpha p; r -= alpha q; stop if ||r|| < tol ||b|| or ||r|| < abs_tol
 //     z = D^-1 r; beta = rz'/rz; p = z + beta p; it += 1
-// Each line group is one kernel; every scalar stays in device memory
-// (CgState) and each reduction finishes in the last workgroup to arrive, in a
-// fixed order, so results are deterministic run to run.  Kernels return at
-// once when the state says stop, which lets the host enqueue iterations in
+// One kernel per line group: K1 q = A p (+ per-workgroup p.q partials),
+// K2 the x/r/z update (+ r.r, r.z partials), K3 the direction update.
+//
+// Reductions use no atomics and no device-scope fences (on this part a
+// device-scope release writes back the XCD's whole L2, which is ruinous when
+// every workgroup does it).  A producer kernel writes one partial per
+// workgroup; every workgroup of the consumer kernel sums all partials in
+// workgroup order -- the same deterministic value everywhere -- and only
+// workgroup 0 records scalars in the state.  r.z is double-buffered by pass
+// parity so no kernel reads a state word that it also writes.  Kernels return
+// at once when the state says stop, which lets the host enqueue passes in
 // batches and read the state back only between batches.
 #include <algorithm>
 #include <chrono>
@@ -23,14 +30,15 @@ namespace gs {
 namespace {
 
 constexpr int BS = 256;
-constexpr int GMAX = 2048;  // max workgroups of the grid-stride CG kernels
+constexpr int GMAX = 2048;       // workgroups of the grid-stride vector kernels
+constexpr int GRID_MAX = 16384;  // workgroups of any SpMV kernel
 
 struct CgState {
-  double rz, alpha, beta, pq;
+  double rz[2];  // r.z at the start of pass k lives in rz[k & 1]
+  double alpha, beta, pq;
   double rnorm, bnorm, tol_abs, abs_tol;
   long long iteration, max_iter;
   int stop, converged;
-  unsigned int cnt[4];
 };
 
 // ---- reductions -----------------------------------------------------------
@@ -40,141 +48,194 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// Block-wide sum of NV values; the result is valid in thread 0.
+// Block-wide sums of NV values, returned in every thread (fixed order).
 template <int NV>
 __device__ __forceinline__ void block_sum(double (&v)[NV]) {
   __shared__ double sh[BS / 64][NV];
   const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
 #pragma unroll
   for (int i = 0; i < NV; ++i) v[i] = wave_sum(v[i]);
+  __syncthreads();
   if (l == 0) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) sh[w][i] = v[i];
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      double s = sh[0][i];
+  for (int i = 0; i < NV; ++i) {
+    double s = sh[0][i];
 #pragma unroll
-      for (int k = 1; k < BS / 64; ++k) s += sh[k][i];
-      v[i] = s;
-    }
+    for (int k = 1; k < BS / 64; ++k) s += sh[k][i];
+    v[i] = s;
   }
-  __syncthreads();
 }
 
-__device__ __forceinline__ void store_agent(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double *p) {
-  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Publish this block's NV partial sums; returns true in every thread of the
-// last block to finish, which then holds the grid totals in tot[] (thread 0
-// after the block reduction; broadcast through shared memory to all).
+// Write this workgroup's NV partial sums (after a block reduction).
 template <int NV>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NV], double *partials,
-                                            unsigned int *counter,
-                                            double (&tot)[NV]) {
-  __shared__ bool amlast;
-  __shared__ double shtot[NV];
+__device__ __forceinline__ void put_partials(double (&v)[NV], double *part) {
   block_sum<NV>(v);
   if (threadIdx.x == 0) {
 #pragma unroll
-    for (int i = 0; i < NV; ++i) store_agent(&partials[blockIdx.x * NV + i], v[i]);
-    __threadfence();
-    const unsigned prev = atomicAdd(counter, 1u);
-    amlast = (prev == gridDim.x - 1);
+    for (int i = 0; i < NV; ++i) part[blockIdx.x * NV + i] = v[i];
   }
-  __syncthreads();
-  if (!amlast) return false;
-  __threadfence();
-  double acc[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) acc[i] = 0.0;
-  for (int b = threadIdx.x; b < (int)gridDim.x; b += BS) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) acc[i] += load_agent(&partials[b * NV + i]);
-  }
-  block_sum<NV>(acc);
-  if (threadIdx.x == 0) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) shtot[i] = acc[i];
-    *counter = 0u;
-  }
-  __syncthreads();
-#pragma unroll
-  for (int i = 0; i < NV; ++i) tot[i] = shtot[i];
-  return true;
 }
 
-// ---- SpMV row kernel: L lanes per row, wave64 -------------------------------
-template <int L, typename T>
-__device__ __forceinline__ T row_dot(const int *__restrict__ rp,
-                                     const int *__restrict__ ci,
-                                     const T *__restrict__ val,
-                                     const T *__restrict__ x, int row, int lane) {
-  const int b = rp[row], e = rp[row + 1];
-  T s = (T)0;
-  for (int k = b + lane; k < e; k += L) s += val[k] * x[ci[k]];
+// Sum of the producer's partials (np workgroups x NV), in every thread.
+template <int NV>
+__device__ __forceinline__ void get_total(const double *__restrict__ part,
+                                          int np, double (&tot)[NV]) {
 #pragma unroll
-  for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-  return s;
+  for (int i = 0; i < NV; ++i) tot[i] = 0.0;
+  for (int b = threadIdx.x; b < np; b += BS) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) tot[i] += part[b * NV + i];
+  }
+  block_sum<NV>(tot);
+}
+
+__device__ __forceinline__ bool leader() {
+  return blockIdx.x == 0 && threadIdx.x == 0;
+}
+
+// ---- SpMV engines ------------------------------------------------------------
+// Both call epi(row, (A x)[row]) once per row from one thread.
+//
+// Stream engine (default): a workgroup takes a block of R = BS/S consecutive
+// rows; all 256 threads load the block's (value, column) pairs in contiguous
+// chunks (fully coalesced, CH/BS independent gathers of x in flight per
+// thread) and park the products in LDS; then S threads per row sum the row's
+// products.  Row blocks are visited grid-stride, so the workgroups resident
+// at any moment work on nearby rows and share the gathered part of x in L2.
+template <typename T> struct StreamChunk { static constexpr int value = 2048; };
+template <> struct StreamChunk<float> { static constexpr int value = 4096; };
+
+template <int S, typename T, class Epi>
+__device__ __forceinline__ void spmv_stream(int n, const int *__restrict__ rp,
+                                            const int *__restrict__ ci,
+                                            const T *__restrict__ val,
+                                            const T *__restrict__ x, Epi &epi) {
+  constexpr int R = BS / S;
+  constexpr int CH = StreamChunk<T>::value;
+  __shared__ T sprod[CH];
+  __shared__ int srp[R + 1];
+  const int t = threadIdx.x;
+  const int lr = t / S, sub = t - (t / S) * S;
+  const int nblk = (n + R - 1) / R;
+  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
+    const int r0 = blk * R;
+    const int nr = min(R, n - r0);
+    for (int i = t; i <= nr; i += BS) srp[i] = rp[r0 + i];
+    __syncthreads();
+    const int k0 = srp[0], k1 = srp[nr];
+    const bool own = lr < nr;
+    const int a = own ? srp[lr] : 0, b = own ? srp[lr + 1] : 0;
+    T acc = (T)0;
+    for (int base = k0; base < k1; base += CH) {
+      const int m = min(CH, k1 - base);
+      int cj[CH / BS];
+      T vj[CH / BS];
+#pragma unroll
+      for (int i = 0; i < CH / BS; ++i) {
+        const int j = t + i * BS;
+        cj[i] = j < m ? ci[base + j] : 0;
+        vj[i] = j < m ? val[base + j] : (T)0;
+      }
+#pragma unroll
+      for (int i = 0; i < CH / BS; ++i) {
+        const int j = t + i * BS;
+        if (j < m) sprod[j] = vj[i] * x[cj[i]];
+      }
+      __syncthreads();
+      const int lo = max(a, base) - base, hi = min(b, base + m) - base;
+      for (int j = lo + sub; j < hi; j += S) acc += sprod[j];
+      __syncthreads();
+    }
+#pragma unroll
+    for (int off = S / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, S);
+    if (own && sub == 0) epi(r0 + lr, acc);
+    __syncthreads();
+  }
+}
+
+// Vector engine: L lanes per row, rows visited grid-stride.
+template <int L, typename T, class Epi>
+__device__ __forceinline__ void spmv_vector(int n, const int *__restrict__ rp,
+                                            const int *__restrict__ ci,
+                                            const T *__restrict__ val,
+                                            const T *__restrict__ x, Epi &epi) {
+  const int lane = threadIdx.x & (L - 1);
+  const int ng = (gridDim.x * BS) / L;
+  for (int row = (blockIdx.x * BS + threadIdx.x) / L; row < n; row += ng) {
+    const int b = rp[row], e = rp[row + 1];
+    T s = (T)0;
+    for (int k = b + lane; k < e; k += L) s += val[k] * x[ci[k]];
+#pragma unroll
+    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
+    if (lane == 0) epi(row, s);
+  }
+}
+
+// ENG = 0: stream with S = W threads per row; ENG = 1: vector with L = W lanes.
+template <int ENG, int W, typename T, class Epi>
+__device__ __forceinline__ void spmv_rows(int n, const int *__restrict__ rp,
+                                          const int *__restrict__ ci,
+                                          const T *__restrict__ val,
+                                          const T *__restrict__ x, Epi &epi) {
+  if constexpr (ENG == 0) spmv_stream<W, T>(n, rp, ci, val, x, epi);
+  else spmv_vector<W, T>(n, rp, ci, val, x, epi);
 }
 
 // y = A x
-template <int L, typename T>
+template <int ENG, int W, typename T>
 __global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ rp,
                                              const int *__restrict__ ci,
                                              const T *__restrict__ val,
                                              const T *__restrict__ x,
                                              T *__restrict__ y) {
-  const int lane = threadIdx.x & (L - 1);
-  const int ng = (gridDim.x * BS) / L;
-  for (int row = (blockIdx.x * BS + threadIdx.x) / L; row < n; row += ng) {
-    const T s = row_dot<L, T>(rp, ci, val, x, row, lane);
-    if (lane == 0) y[row] = s;
-  }
+  auto epi = [&](int row, T s) { y[row] = s; };
+  spmv_rows<ENG, W, T>(n, rp, ci, val, x, epi);
 }
 
-// CG step 1: q = A p, pq = p.q, alpha = rz / pq (breakdown if |pq| < 1e-30).
-template <int L, typename T>
+// K1: q = A p; per-workgroup partials of p.q.
+template <int ENG, int W, typename T>
 __global__ __launch_bounds__(BS) void k_cg_spmv(
     int n, const int *__restrict__ rp, const int *__restrict__ ci,
     const T *__restrict__ val, const T *__restrict__ p, T *__restrict__ q,
-    CgState *__restrict__ st, double *__restrict__ partials) {
+    const CgState *__restrict__ st, double *__restrict__ part) {
   if (st->stop) return;
-  const int lane = threadIdx.x & (L - 1);
-  const int ng = (gridDim.x * BS) / L;
   double acc[1] = {0.0};
-  for (int row = (blockIdx.x * BS + threadIdx.x) / L; row < n; row += ng) {
-    const T s = row_dot<L, T>(rp, ci, val, p, row, lane);
-    if (lane == 0) {
-      q[row] = s;
-      acc[0] += (double)p[row] * (double)s;
-    }
-  }
-  double tot[1];
-  if (grid_reduce<1>(acc, partials, &st->cnt[0], tot) && threadIdx.x == 0) {
-    st->pq = tot[0];
-    if (fabs(tot[0]) < 1e-30) st->stop = 1;
-    else st->alpha = st->rz / tot[0];
-  }
+  auto epi = [&](int row, T s) {
+    q[row] = s;
+    acc[0] += (double)p[row] * (double)s;
+  };
+  spmv_rows<ENG, W, T>(n, rp, ci, val, p, epi);
+  put_partials<1>(acc, part);
 }
 
-// CG step 2: x += alpha p; r -= alpha q; convergence test on ||r||;
-// z = D^-1 r; beta = rz_new / rz; it += 1.
+// K2 (pass `it`): pq = sum of K1 partials; break if |pq| < 1e-30;
+// alpha = rz/pq; x += alpha p; r -= alpha q; z = D^-1 r; partials r.r, r.z.
 template <typename T, bool PREC>
 __global__ __launch_bounds__(BS) void k_cg_update(
-    int n, T *__restrict__ x, T *__restrict__ r, T *__restrict__ z,
-    const T *__restrict__ p, const T *__restrict__ q,
+    int n, long long it, T *__restrict__ x, T *__restrict__ r,
+    T *__restrict__ z, const T *__restrict__ p, const T *__restrict__ q,
     const T *__restrict__ dinv, CgState *__restrict__ st,
-    double *__restrict__ partials) {
+    const double *__restrict__ part_in, int np_in, double *__restrict__ part) {
   if (st->stop) return;
-  const T alpha = (T)st->alpha;
+  double pq[1];
+  get_total<1>(part_in, np_in, pq);
+  if (fabs(pq[0]) < 1e-30) {  // src/linalg/iterative.rs:85
+    if (leader()) {
+      st->pq = pq[0];
+      st->stop = 1;
+    }
+    return;
+  }
+  const double alpha_d = st->rz[it & 1] / pq[0];
+  if (leader()) {
+    st->pq = pq[0];
+    st->alpha = alpha_d;
+  }
+  const T alpha = (T)alpha_d;
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     x[i] += alpha * p[i];
@@ -185,40 +246,45 @@ __global__ __launch_bounds__(BS) void k_cg_update(
     acc[0] += (double)ri * (double)ri;
     acc[1] += (double)ri * (double)zi;
   }
-  double tot[2];
-  if (grid_reduce<2>(acc, partials, &st->cnt[1], tot) && threadIdx.x == 0) {
-    const double rn = sqrt(tot[0]);
-    st->rnorm = rn;
-    if (rn < st->tol_abs || rn < st->abs_tol) {
-      st->converged = 1;
-      st->stop = 1;
-    } else {
-      st->beta = tot[1] / st->rz;
-      st->rz = tot[1];
-      st->iteration += 1;
-      if (st->iteration >= st->max_iter) st->stop = 1;
-    }
-  }
+  put_partials<2>(acc, part);
 }
 
-// CG step 3: p = z + beta p.
+// K3 (pass `it`): ||r|| test; beta = rz'/rz; p = z + beta p; it += 1.
 template <typename T>
 __global__ __launch_bounds__(BS) void k_cg_direction(
-    int n, const T *__restrict__ z, T *__restrict__ p,
-    const CgState *__restrict__ st) {
+    int n, long long it, const T *__restrict__ z, T *__restrict__ p,
+    CgState *__restrict__ st, const double *__restrict__ part_in, int np_in) {
   if (st->stop) return;
-  const T beta = (T)st->beta;
+  double tot[2];
+  get_total<2>(part_in, np_in, tot);
+  const double rn = sqrt(tot[0]);
+  if (rn < st->tol_abs || rn < st->abs_tol) {  // src/linalg/iterative.rs:95-98
+    if (leader()) {
+      st->rnorm = rn;
+      st->converged = 1;
+      st->stop = 1;
+    }
+    return;
+  }
+  const double beta_d = tot[1] / st->rz[it & 1];
+  if (leader()) {
+    st->rnorm = rn;
+    st->beta = beta_d;
+    st->rz[(it + 1) & 1] = tot[1];
+    st->iteration = it + 1;
+    if (it + 1 >= st->max_iter) st->stop = 1;
+  }
+  const T beta = (T)beta_d;
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
     p[i] = z[i] + beta * p[i];
 }
 
-// CG set-up: x = 0, r = b, z = D^-1 r, p = z, rz = r.z, ||b||.
+// Set-up: x = 0, r = b, z = D^-1 r, p = z; partials b.b, r.z.
 template <typename T, bool PREC>
 __global__ __launch_bounds__(BS) void k_cg_init(
     int n, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ r,
     T *__restrict__ z, T *__restrict__ p, const T *__restrict__ dinv,
-    CgState *__restrict__ st, double *__restrict__ partials, long long max_iter,
-    double tol, double abs_tol) {
+    double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     const T bi = b[i];
@@ -230,50 +296,57 @@ __global__ __launch_bounds__(BS) void k_cg_init(
     acc[0] += (double)bi * (double)bi;
     acc[1] += (double)bi * (double)zi;
   }
+  put_partials<2>(acc, part);
+}
+
+// Set-up, single workgroup: ||b||, rz, stopping thresholds, early exits.
+__global__ __launch_bounds__(BS) void k_cg_init_final(
+    CgState *__restrict__ st, const double *__restrict__ part_in, int np_in,
+    long long max_iter, double tol, double abs_tol) {
   double tot[2];
-  if (grid_reduce<2>(acc, partials, &st->cnt[2], tot) && threadIdx.x == 0) {
-    const double bn = sqrt(tot[0]);
-    st->bnorm = bn;
-    st->rnorm = bn;
-    st->rz = tot[1];
-    st->tol_abs = tol * bn;
-    st->abs_tol = abs_tol;
-    st->iteration = 0;
-    st->max_iter = max_iter;
-    st->converged = 0;
-    st->stop = 0;
-    st->alpha = st->beta = st->pq = 0.0;
-    if (bn < 1e-25) {  // src/linalg/iterative.rs:60-68
-      st->stop = 1;
-      st->converged = 1;
-      st->rnorm = 0.0;
-    } else if (max_iter <= 0) {
-      st->stop = 1;
-    }
+  get_total<2>(part_in, np_in, tot);
+  if (threadIdx.x != 0) return;
+  CgState s{};
+  const double bn = sqrt(tot[0]);
+  s.bnorm = bn;
+  s.rnorm = bn;
+  s.rz[0] = tot[1];
+  s.tol_abs = tol * bn;
+  s.abs_tol = abs_tol;
+  s.max_iter = max_iter;
+  if (bn < 1e-25) {  // src/linalg/iterative.rs:60-68
+    s.stop = 1;
+    s.converged = 1;
+    s.rnorm = 0.0;
+  } else if (max_iter <= 0) {
+    s.stop = 1;
   }
+  *st = s;
 }
 
 // ---- mixed-precision helpers ----------------------------------------------
-template <int L>
+// r = b - A x (FP64); per-workgroup partials of r.r
+template <int ENG, int W>
 __global__ __launch_bounds__(BS) void k_residual(
     int n, const int *__restrict__ rp, const int *__restrict__ ci,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, double *__restrict__ r,
-    CgState *__restrict__ st, double *__restrict__ partials) {
-  const int lane = threadIdx.x & (L - 1);
-  const int ng = (gridDim.x * BS) / L;
+    double *__restrict__ part) {
   double acc[1] = {0.0};
-  for (int row = (blockIdx.x * BS + threadIdx.x) / L; row < n; row += ng) {
-    const double s = row_dot<L, double>(rp, ci, val, x, row, lane);
-    if (lane == 0) {
-      const double ri = b[row] - s;
-      r[row] = ri;
-      acc[0] += ri * ri;
-    }
-  }
+  auto epi = [&](int row, double s) {
+    const double ri = b[row] - s;
+    r[row] = ri;
+    acc[0] += ri * ri;
+  };
+  spmv_rows<ENG, W, double>(n, rp, ci, val, x, epi);
+  put_partials<1>(acc, part);
+}
+
+__global__ __launch_bounds__(BS) void k_norm_final(
+    CgState *__restrict__ st, const double *__restrict__ part_in, int np_in) {
   double tot[1];
-  if (grid_reduce<1>(acc, partials, &st->cnt[3], tot) && threadIdx.x == 0)
-    st->rnorm = sqrt(tot[0]);
+  get_total<1>(part_in, np_in, tot);
+  if (threadIdx.x == 0) st->rnorm = sqrt(tot[0]);
 }
 
 __global__ void k_scale_to_f32(int n, const double *__restrict__ r,
@@ -301,10 +374,6 @@ __global__ void k_to_f32(int64_t n, const double *__restrict__ a,
 
 __global__ void k_zero_f64(int n, double *__restrict__ x) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] = 0.0;
-}
-
-__global__ void k_copy_f64(int n, const double *__restrict__ a, double *__restrict__ b) {
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) b[i] = a[i];
 }
 
 __global__ void k_fill_pattern(int n, double *__restrict__ x) {
@@ -335,79 +404,100 @@ __global__ void k_jacobi(int n, const int *__restrict__ rp,
 
 __global__ void k_state_reset(CgState *st) { *st = CgState{}; }
 
-inline unsigned grid_rows(int64_t n, int L) {
-  const int64_t need = (n * L + BS - 1) / BS;
-  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, GMAX));
-}
 inline unsigned grid_vec(int64_t n) {
   const int64_t need = (n + BS - 1) / BS;
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, GMAX));
 }
 
-int choose_lanes(int64_t rows, int64_t nnz) {
-  const double avg = rows > 0 ? (double)nnz / (double)rows : 0.0;
-  if (avg <= 12.0) return 4;
-  if (avg <= 40.0) return 8;
-  if (avg <= 100.0) return 16;
-  return 32;
+// SpMV launch shape: engine, threads (stream) or lanes (vector) per row,
+// workgroups.
+struct Shape {
+  int eng = 0, W = 2, grid = 1;
+};
+
+Shape shape_of(const gs_csr *A) {
+  Shape sh;
+  const int64_t n = A->rows;
+  const double avg = n > 0 ? (double)A->nnz / (double)n : 0.0;
+  sh.eng = (A->spmv_flags & GS_SPMV_VECTOR) ? 1 : 0;
+  if (sh.eng == 0) {
+    // rows per workgroup R = 256 / W: a few thousand nonzeros per row block
+    sh.W = A->spmv_lanes ? A->spmv_lanes : (avg <= 40.0 ? 2 : (avg <= 100.0 ? 4 : 8));
+  } else {
+    sh.W = A->spmv_lanes ? A->spmv_lanes
+                         : (avg <= 12.0 ? 4 : (avg <= 40.0 ? 8 : (avg <= 100.0 ? 16 : 32)));
+  }
+  const int64_t rows_per_pass = BS / sh.W;
+  const int64_t want = A->spmv_grid > 0 ? A->spmv_grid : 2048;
+  const int64_t cap = std::max<int64_t>(1, (n + rows_per_pass - 1) / rows_per_pass);
+  sh.grid = (int)std::min<int64_t>(std::min<int64_t>(want, cap), GRID_MAX);
+  return sh;
 }
 
-// ---- per-precision dispatch ---------------------------------------------------
+#define GS_SPMV_DISPATCH(SH, KERNEL, T, ...)                                        \
+  do {                                                                           \
+    if ((SH).eng == 0) {                                                         \
+      switch ((SH).W) {                                                          \
+      case 1: hipLaunchKernelGGL((KERNEL<0, 1 T>), __VA_ARGS__); break;          \
+      case 2: hipLaunchKernelGGL((KERNEL<0, 2 T>), __VA_ARGS__); break;          \
+      case 4: hipLaunchKernelGGL((KERNEL<0, 4 T>), __VA_ARGS__); break;          \
+      default: hipLaunchKernelGGL((KERNEL<0, 8 T>), __VA_ARGS__); break;         \
+      }                                                                          \
+    } else {                                                                     \
+      switch ((SH).W) {                                                          \
+      case 4: hipLaunchKernelGGL((KERNEL<1, 4 T>), __VA_ARGS__); break;          \
+      case 8: hipLaunchKernelGGL((KERNEL<1, 8 T>), __VA_ARGS__); break;          \
+      case 16: hipLaunchKernelGGL((KERNEL<1, 16 T>), __VA_ARGS__); break;        \
+      default: hipLaunchKernelGGL((KERNEL<1, 32 T>), __VA_ARGS__); break;        \
+      }                                                                          \
+    }                                                                            \
+  } while (0)
+#define GS_COMMA ,
+
 template <typename T>
-void launch_cg_spmv(hipStream_t s, int L, int n, const int *rp, const int *ci,
-                    const T *val, const T *p, T *q, CgState *st, double *part) {
-  const unsigned g = grid_rows(n, L);
-  switch (L) {
-  case 4: hipLaunchKernelGGL((k_cg_spmv<4, T>), g, BS, 0, s, n, rp, ci, val, p, q, st, part); break;
-  case 8: hipLaunchKernelGGL((k_cg_spmv<8, T>), g, BS, 0, s, n, rp, ci, val, p, q, st, part); break;
-  case 16: hipLaunchKernelGGL((k_cg_spmv<16, T>), g, BS, 0, s, n, rp, ci, val, p, q, st, part); break;
-  default: hipLaunchKernelGGL((k_cg_spmv<32, T>), g, BS, 0, s, n, rp, ci, val, p, q, st, part); break;
-  }
+void launch_cg_spmv(hipStream_t s, const Shape &sh, int n, const int *rp,
+                    const int *ci, const T *val, const T *p, T *q,
+                    const CgState *st, double *part) {
+  GS_SPMV_DISPATCH(sh, k_cg_spmv, GS_COMMA T, sh.grid, BS, 0, s, n, rp, ci, val,
+                   p, q, st, part);
 }
 
 template <typename T>
-void launch_spmv(hipStream_t s, int L, int n, const int *rp, const int *ci,
-                 const T *val, const T *x, T *y) {
-  const unsigned g = grid_rows(n, L);
-  switch (L) {
-  case 4: hipLaunchKernelGGL((k_spmv<4, T>), g, BS, 0, s, n, rp, ci, val, x, y); break;
-  case 8: hipLaunchKernelGGL((k_spmv<8, T>), g, BS, 0, s, n, rp, ci, val, x, y); break;
-  case 16: hipLaunchKernelGGL((k_spmv<16, T>), g, BS, 0, s, n, rp, ci, val, x, y); break;
-  default: hipLaunchKernelGGL((k_spmv<32, T>), g, BS, 0, s, n, rp, ci, val, x, y); break;
-  }
+void launch_spmv(hipStream_t s, const Shape &sh, int n, const int *rp,
+                 const int *ci, const T *val, const T *x, T *y) {
+  GS_SPMV_DISPATCH(sh, k_spmv, GS_COMMA T, sh.grid, BS, 0, s, n, rp, ci, val, x, y);
 }
 
-void launch_residual(hipStream_t s, int L, int n, const int *rp, const int *ci,
-                     const double *val, const double *x, const double *b,
-                     double *r, CgState *st, double *part) {
-  const unsigned g = grid_rows(n, L);
-  switch (L) {
-  case 4: hipLaunchKernelGGL((k_residual<4>), g, BS, 0, s, n, rp, ci, val, x, b, r, st, part); break;
-  case 8: hipLaunchKernelGGL((k_residual<8>), g, BS, 0, s, n, rp, ci, val, x, b, r, st, part); break;
-  case 16: hipLaunchKernelGGL((k_residual<16>), g, BS, 0, s, n, rp, ci, val, x, b, r, st, part); break;
-  default: hipLaunchKernelGGL((k_residual<32>), g, BS, 0, s, n, rp, ci, val, x, b, r, st, part); break;
-  }
+// r = b - A x and ||r|| -> st->rnorm
+void launch_residual(hipStream_t s, const Shape &sh, int n, const int *rp,
+                     const int *ci, const double *val, const double *x,
+                     const double *b, double *r, CgState *st, double *part) {
+  GS_SPMV_DISPATCH(sh, k_residual, , sh.grid, BS, 0, s, n, rp, ci, val, x, b, r,
+                   part);
+  hipLaunchKernelGGL(k_norm_final, 1, BS, 0, s, st, part, sh.grid);
 }
 
 // Solver scratch carved from one per-context buffer that only grows, so
 // repeated solves on the same problem size allocate nothing.
 struct Work {
-  double *partials = nullptr;
+  double *part1 = nullptr;  // one partial per SpMV workgroup
+  double *part2 = nullptr;  // two partials per vector-kernel workgroup
   CgState *st = nullptr;
   void *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
 };
 
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// bytes for `k` Work blocks with vectors of `n` elements of `esz` bytes
 size_t work_bytes(int64_t n, size_t esz) {
-  return align256(2 * GMAX * sizeof(double)) + align256(sizeof(CgState)) +
+  return 2 * align256(GRID_MAX * sizeof(double)) + align256(sizeof(CgState)) +
          4 * align256((size_t)std::max<int64_t>(n, 1) * esz);
 }
 
 char *carve_work(char *base, int64_t n, size_t esz, Work &w) {
-  w.partials = (double *)base;
-  base += align256(2 * GMAX * sizeof(double));
+  w.part1 = (double *)base;
+  base += align256(GRID_MAX * sizeof(double));
+  w.part2 = (double *)base;
+  base += align256(GRID_MAX * sizeof(double));
   w.st = (CgState *)base;
   base += align256(sizeof(CgState));
   void **v[4] = {&w.r, &w.z, &w.p, &w.q};
@@ -443,41 +533,41 @@ struct PcgResult {
   long long spmv = 0;
 };
 
-// Device-resident PCG on one precision.  Iterations are enqueued in batches
-// sized from the observed residual decay; the state is read once per batch.
+// Device-resident PCG in one precision.  Passes are enqueued in batches sized
+// from the observed residual decay; the state is read once per batch.
 template <typename T>
-PcgResult pcg(gs_ctx *ctx, int L, int n, const int *rp, const int *ci,
+PcgResult pcg(gs_ctx *ctx, const Shape &sh, int n, const int *rp, const int *ci,
               const T *val, const T *b, T *x, const T *dinv, long long max_iter,
               double tol, double abs_tol, Work &w) {
   hipStream_t s = ctx->stream;
-  struct VecView { T *v; T *get() const { return v; } };
-  const VecView r{(T *)w.r}, z{(T *)w.z}, p{(T *)w.p}, q{(T *)w.q};
+  T *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q;
   CgState *st = w.st;
-  double *part = w.partials;
-  hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, st);
   const unsigned gv = grid_vec(n);
   if (dinv)
-    hipLaunchKernelGGL((k_cg_init<T, true>), gv, BS, 0, s, n, b, x, r.get(),
-                       z.get(), p.get(), dinv, st, part, max_iter, tol, abs_tol);
+    hipLaunchKernelGGL((k_cg_init<T, true>), gv, BS, 0, s, n, b, x, r, z, p,
+                       dinv, w.part2);
   else
-    hipLaunchKernelGGL((k_cg_init<T, false>), gv, BS, 0, s, n, b, x, r.get(),
-                       z.get(), p.get(), dinv, st, part, max_iter, tol, abs_tol);
+    hipLaunchKernelGGL((k_cg_init<T, false>), gv, BS, 0, s, n, b, x, r, z, p,
+                       dinv, w.part2);
+  hipLaunchKernelGGL(k_cg_init_final, 1, BS, 0, s, st, w.part2, (int)gv,
+                     max_iter, tol, abs_tol);
   GS_LAUNCH_CHECK("k_cg_init");
   PcgResult res;
+  long long launched = 0;  // pass index of the next launch
   long long batch = 8;
   double prev_rn = -1.0;
   long long prev_it = 0;
   for (;;) {
-    for (long long i = 0; i < batch; ++i) {
-      launch_cg_spmv<T>(s, L, n, rp, ci, val, p.get(), q.get(), st, part);
+    for (long long i = 0; i < batch; ++i, ++launched) {
+      launch_cg_spmv<T>(s, sh, n, rp, ci, val, p, q, st, w.part1);
       if (dinv)
-        hipLaunchKernelGGL((k_cg_update<T, true>), gv, BS, 0, s, n, x, r.get(),
-                           z.get(), p.get(), q.get(), dinv, st, part);
+        hipLaunchKernelGGL((k_cg_update<T, true>), gv, BS, 0, s, n, launched, x,
+                           r, z, p, q, dinv, st, w.part1, sh.grid, w.part2);
       else
-        hipLaunchKernelGGL((k_cg_update<T, false>), gv, BS, 0, s, n, x, r.get(),
-                           z.get(), p.get(), q.get(), dinv, st, part);
-      hipLaunchKernelGGL((k_cg_direction<T>), gv, BS, 0, s, n, z.get(), p.get(),
-                         st);
+        hipLaunchKernelGGL((k_cg_update<T, false>), gv, BS, 0, s, n, launched, x,
+                           r, z, p, q, dinv, st, w.part1, sh.grid, w.part2);
+      hipLaunchKernelGGL((k_cg_direction<T>), gv, BS, 0, s, n, launched, z, p, st,
+                         w.part2, (int)gv);
     }
     GS_LAUNCH_CHECK("cg batch");
     const CgState h = read_state(ctx, st);
@@ -486,7 +576,7 @@ PcgResult pcg(gs_ctx *ctx, int L, int n, const int *rp, const int *ci,
     res.bnorm = h.bnorm;
     res.converged = h.converged != 0;
     if (h.stop) break;
-    // next batch: iterations to reach the target at the observed decay rate
+    // next batch: passes to reach the target at the observed decay rate
     long long next = 16;
     if (prev_rn > 0.0 && h.iteration > prev_it && h.rnorm > 0.0) {
       const double rate =
@@ -502,7 +592,7 @@ PcgResult pcg(gs_ctx *ctx, int L, int n, const int *rp, const int *ci,
     batch = std::max(4ll, std::min(64ll, next));
     batch = std::min(batch, std::max(1ll, max_iter - h.iteration));
   }
-  // SpMV launches that did work: one per pass (the converging pass included)
+  // SpMV launches that did work: one per pass, the converging pass included
   res.spmv = res.iterations + (res.converged && res.bnorm >= 1e-25 ? 1 : 0);
   return res;
 }
@@ -518,13 +608,9 @@ void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv) {
   GS_LAUNCH_CHECK("k_jacobi");
 }
 
-static int lanes_of(const gs_csr *A) {
-  return A->lanes ? A->lanes : choose_lanes(A->rows, A->nnz);
-}
-
 void csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y) {
   if (A->rows == 0) return;
-  launch_spmv<double>(ctx->stream, lanes_of(A), (int)A->rows, A->row_ptr.get(),
+  launch_spmv<double>(ctx->stream, shape_of(A), (int)A->rows, A->row_ptr.get(),
                       A->col.get(), A->val.get(), x, y);
   GS_LAUNCH_CHECK("k_spmv");
   GS_HIP(hipStreamSynchronize(ctx->stream));
@@ -556,12 +642,12 @@ static void ensure_f32(gs_ctx *ctx, gs_csr *A) {
 
 void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
               const gs_cg_opts &o, double *x, gs_solver_stats *st) {
-  gs_csr *A = const_cast<gs_csr *>(Ac);  // caches only
+  gs_csr *A = const_cast<gs_csr *>(Ac);  // derived caches only
   const auto t0 = std::chrono::steady_clock::now();
   memset(st, 0, sizeof(*st));
   if (A->rows != A->cols) throw Error(GS_ERR_INVALID_ARG, "CG needs a square matrix");
   const int n = (int)A->rows;
-  const int L = lanes_of(A);
+  const Shape sh = shape_of(A);
   hipStream_t s = ctx->stream;
   if (o.use_preconditioner) ensure_dinv(ctx, A);
   const bool mixed = !(o.precision == GS_PREC_FP64 || n == 0);
@@ -575,7 +661,7 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   Work w;
   base = carve_work(base, n, sizeof(double), w);
   if (!mixed) {
-    const PcgResult r = pcg<double>(ctx, L, n, A->row_ptr.get(), A->col.get(),
+    const PcgResult r = pcg<double>(ctx, sh, n, A->row_ptr.get(), A->col.get(),
                                     A->val.get(), b, x,
                                     o.use_preconditioner ? A->dinv.get() : nullptr,
                                     o.max_iterations, o.tolerance,
@@ -593,18 +679,18 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
     ensure_f32(ctx, A);
     Work wi;
     base = carve_work(base, n, sizeof(float), wi);
-    struct { double *p; double *get() const { return p; } } r64{(double *)base};
+    double *r64 = (double *)base;
     base += align256((size_t)std::max(n, 1) * sizeof(double));
-    struct { float *p; float *get() const { return p; } } r32{(float *)base};
+    float *r32 = (float *)base;
     base += align256((size_t)std::max(n, 1) * sizeof(float));
-    struct { float *p; float *get() const { return p; } } d32{(float *)base};
+    float *d32 = (float *)base;
     CgState *S = w.st;
     const unsigned gv = grid_vec(n);
     hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, S);
     hipLaunchKernelGGL(k_zero_f64, gv, BS, 0, s, n, x);
-    // r = b - A*0 = b, computed by the residual kernel for its norm
-    launch_residual(s, L, n, A->row_ptr.get(), A->col.get(), A->val.get(), x,
-                    b, r64.get(), S, w.partials);
+    // r = b - A*0 = b, through the residual kernel for its norm
+    launch_residual(s, sh, n, A->row_ptr.get(), A->col.get(), A->val.get(), x,
+                    b, r64, S, w.part1);
     GS_LAUNCH_CHECK("k_residual");
     CgState h = read_state(ctx, S);
     const double bnorm = h.rnorm;
@@ -624,19 +710,18 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
           break;
         }
         if (inner_total >= o.max_iterations) break;
-        hipLaunchKernelGGL(k_scale_to_f32, gv, BS, 0, s, n, r64.get(), S,
-                           r32.get());
+        hipLaunchKernelGGL(k_scale_to_f32, gv, BS, 0, s, n, r64, S, r32);
         const long long cap = std::min<long long>(
             o.max_inner_iterations > 0 ? o.max_inner_iterations : o.max_iterations,
             o.max_iterations - inner_total);
-        const PcgResult ri = pcg<float>(
-            ctx, L, n, A->row_ptr.get(), A->col.get(), A->val32.get(), r32.get(),
-            d32.get(), dinv32, cap, o.inner_tolerance, 0.0, wi);
+        const PcgResult ri = pcg<float>(ctx, sh, n, A->row_ptr.get(), A->col.get(),
+                                        A->val32.get(), r32, d32, dinv32, cap,
+                                        o.inner_tolerance, 0.0, wi);
         inner_total += ri.iterations + (ri.converged ? 1 : 0);
         spmv += ri.spmv;
-        hipLaunchKernelGGL(k_axpy_f32, gv, BS, 0, s, n, d32.get(), S, x);
-        launch_residual(s, L, n, A->row_ptr.get(), A->col.get(), A->val.get(),
-                        x, b, r64.get(), S, w.partials);
+        hipLaunchKernelGGL(k_axpy_f32, gv, BS, 0, s, n, d32, S, x);
+        launch_residual(s, sh, n, A->row_ptr.get(), A->col.get(), A->val.get(),
+                        x, b, r64, S, w.part1);
         GS_LAUNCH_CHECK("defect correction");
         ++spmv;
         ++outer;
@@ -661,13 +746,12 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
 double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   gs_csr *A = const_cast<gs_csr *>(Ac);
   const int n = (int)A->rows;
-  const int L = lanes_of(A);
+  const Shape sh = shape_of(A);
   hipStream_t s = ctx->stream;
   if (n == 0 || reps <= 0) return 0.0;
   Work w;
   carve_work(ctx_scratch(ctx, work_bytes(n, sizeof(double))), n, sizeof(double), w);
-  CgState *st = w.st;
-  hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, st);
+  hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, w.st);
   hipEvent_t e0, e1;
   GS_HIP(hipEventCreate(&e0));
   GS_HIP(hipEventCreate(&e1));
@@ -675,34 +759,30 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   if (precision == GS_PREC_FP64) {
     double *p = (double *)w.p, *q = (double *)w.q;
     hipLaunchKernelGGL(k_fill_pattern, grid_vec(n), BS, 0, s, n, p);
-    launch_cg_spmv<double>(s, L, n, A->row_ptr.get(), A->col.get(),
-                           A->val.get(), p, q, st, w.partials);
+    launch_cg_spmv<double>(s, sh, n, A->row_ptr.get(), A->col.get(),
+                           A->val.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i)
-      launch_cg_spmv<double>(s, L, n, A->row_ptr.get(), A->col.get(),
-                             A->val.get(), p, q, st, w.partials);
+      launch_cg_spmv<double>(s, sh, n, A->row_ptr.get(), A->col.get(),
+                             A->val.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e1, s));
-    GS_HIP(hipEventSynchronize(e1));
-    GS_HIP(hipEventElapsedTime(&ms, e0, e1));
   } else {
     ensure_f32(ctx, A);
     float *p = (float *)w.p, *q = (float *)w.q;
     hipLaunchKernelGGL(k_fill_pattern32, grid_vec(n), BS, 0, s, n, p);
-    launch_cg_spmv<float>(s, L, n, A->row_ptr.get(), A->col.get(),
-                          A->val32.get(), p, q, st, w.partials);
+    launch_cg_spmv<float>(s, sh, n, A->row_ptr.get(), A->col.get(),
+                          A->val32.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i)
-      launch_cg_spmv<float>(s, L, n, A->row_ptr.get(), A->col.get(),
-                            A->val32.get(), p, q, st, w.partials);
+      launch_cg_spmv<float>(s, sh, n, A->row_ptr.get(), A->col.get(),
+                            A->val32.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e1, s));
-    GS_HIP(hipEventSynchronize(e1));
-    GS_HIP(hipEventElapsedTime(&ms, e0, e1));
   }
   GS_LAUNCH_CHECK("time_spmv");
-  const CgState h = read_state(ctx, st);
+  GS_HIP(hipEventSynchronize(e1));
+  GS_HIP(hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  if (h.stop) throw Error(GS_ERR_INVALID_ARG, "SpMV timing probe hit |p.Ap| < 1e-30");
   return (double)ms / reps;
 }
 

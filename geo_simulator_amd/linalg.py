@@ -100,6 +100,12 @@ class CsrMatrix:
                                                 dptr(d)), "gs_jacobi_diag_inv")
         return d
 
+    def set_spmv_shape(self, lanes: int = 0, workgroups: int = 0, flags: int = 0):
+        """Launch-shape override for this matrix's SpMV (0 = automatic)."""
+        st = lib().gs_csr_set_spmv_shape(self.handle, lanes, workgroups, flags)
+        if st != 0:
+            raise _lib.GeoFemError(st, "gs_csr_set_spmv_shape: invalid shape")
+
     def time_spmv(self, precision: str = "fp64", reps: int = 20) -> float:
         """Average duration (ms) of the CG SpMV kernel over `reps` launches."""
         out = C.c_double()

@@ -189,6 +189,18 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
                                       const double *lambda_dev,
                                       double *grad_dev);
 
+/* ---- SpMV launch shape (no reference counterpart) ----
+ * Override the launch shape of the SpMV kernels (and of the CG built on them)
+ * for this matrix.  Default engine: CSR-stream (a workgroup stages a block of
+ * 256/lanes rows' products in LDS with coalesced loads; lanes = threads per
+ * row in the row sums, 1, 2, 4 or 8).  GS_SPMV_VECTOR selects the
+ * lanes-per-row engine (lanes = 4, 8, 16 or 32).  lanes 0 = chosen from the
+ * mean row length; workgroups 0 = default, at most 16384.  Results depend on
+ * the shape only through floating-point summation order. */
+enum { GS_SPMV_VECTOR = 4 };
+gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
+                                int32_t flags);
+
 /* ---- measurement hooks (no reference counterpart) ----
  * Launch the CG's SpMV kernel `reps` times on the ctx stream between two HIP
  * events and return the average duration per launch in milliseconds. */

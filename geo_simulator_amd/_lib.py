@@ -26,6 +26,7 @@ GS_PREC_FP64 = 0
 GS_PREC_MIXED = 1
 
 GS_SPMV_VECTOR = 4
+GS_SPMV_INDEX32 = 8
 
 _STATUS_NAMES = {
     GS_ERR_INVALID_ARG: "GS_ERR_INVALID_ARG", GS_ERR_HIP: "GS_ERR_HIP",

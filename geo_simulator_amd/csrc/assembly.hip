@@ -18,6 +18,7 @@
 // subtract, as in the reference.
 #pragma clang fp contract(off)
 
+#include <climits>
 #include <cstring>
 
 #include <rocprim/rocprim.hpp>
@@ -36,7 +37,9 @@ inline unsigned nblk(int64_t n, int bs = BS) {
 // ---------------------------------------------------------------------------
 // symbolic phase
 // ---------------------------------------------------------------------------
-__global__ void k_make_keys(int64_t E, const int *__restrict__ conn, int bits,
+// Keys are (row, col) in the internal numbering iperm[caller node].
+__global__ void k_make_keys(int64_t E, const int *__restrict__ conn,
+                            const int *__restrict__ iperm, int bits,
                             uint64_t *__restrict__ keys,
                             int *__restrict__ vals) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -44,8 +47,8 @@ __global__ void k_make_keys(int64_t E, const int *__restrict__ conn, int bits,
   const int64_t e = t / 100;
   const int ab = (int)(t - e * 100);
   const int a = ab / 10, b = ab - 10 * (ab / 10);
-  const uint64_t r = (uint32_t)conn[e * 10 + a];
-  const uint64_t c = (uint32_t)conn[e * 10 + b];
+  const uint64_t r = (uint32_t)iperm[conn[e * 10 + a]];
+  const uint64_t c = (uint32_t)iperm[conn[e * 10 + b]];
   keys[t] = (r << bits) | c;
   vals[t] = (int)t;
 }
@@ -190,11 +193,13 @@ __global__ void k_gather_elastic(int64_t nnz, const int *__restrict__ rp,
 
 // f[r] = sum over (e,a) with conn[e][a] == r, ascending e, of Fe[e][a]:
 // exactly the contributors of the diagonal slot (r,r).
+// Written at node out_map[r] (caller numbering) when out_map is given.
 template <int DPN>
 __global__ void k_gather_load(int64_t n_nodes, const int *__restrict__ diag,
                               const int *__restrict__ cptr,
                               const int *__restrict__ contrib,
                               const double *__restrict__ Fe,
+                              const int *__restrict__ out_map,
                               double *__restrict__ f) {
   const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= n_nodes) return;
@@ -211,8 +216,9 @@ __global__ void k_gather_load(int64_t n_nodes, const int *__restrict__ diag,
       for (int c = 0; c < DPN; ++c) v[c] += Fe[((int64_t)e * 10 + a) * DPN + c];
     }
   }
+  const int64_t o = out_map ? (int64_t)out_map[r] : r;
 #pragma unroll
-  for (int c = 0; c < DPN; ++c) f[r * DPN + c] = v[c];
+  for (int c = 0; c < DPN; ++c) f[o * DPN + c] = v[c];
 }
 
 // ---------------------------------------------------------------------------
@@ -248,12 +254,17 @@ __global__ void k_bc_count(int64_t n, const int *__restrict__ rp,
   if ((threadIdx.x & 63) == 0 && mine != 0) atomicMax(maxbits, mine);
 }
 
+// Pass 2: copy kept entries and lift the RHS.  The reference subtracts
+// K_ji g_i in ascending caller column order; with an internal numbering
+// (dperm) the constrained entries are visited in ascending dperm order by
+// repeated minimum search, so the lift is the same sequence of operations.
 __global__ void k_bc_fill(int64_t n, const int *__restrict__ rp,
                           const int *__restrict__ col,
                           const double *__restrict__ val,
                           const uint8_t *__restrict__ dir,
                           const double *__restrict__ g,
                           const double *__restrict__ f,
+                          const int *__restrict__ dperm,
                           const unsigned long long *__restrict__ maxbits,
                           const int *__restrict__ nrp, int *__restrict__ ncol,
                           double *__restrict__ nval, double *__restrict__ fbc) {
@@ -267,18 +278,66 @@ __global__ void k_bc_fill(int64_t n, const int *__restrict__ rp,
     fbc[r] = g[r] * c;
     return;
   }
+  const int k0 = rp[r], k1 = rp[r + 1];
   double fr = f[r];
-  for (int k = rp[r]; k < rp[r + 1]; ++k) {
+  for (int k = k0; k < k1; ++k) {
     const int j = col[k];
     if (dir[j]) {
-      fr -= val[k] * g[j];
+      if (!dperm) fr -= val[k] * g[j];
     } else {
       ncol[p] = j;
       nval[p] = val[k];
       ++p;
     }
   }
+  if (dperm) {
+    int last = -1;
+    for (;;) {
+      int best = INT_MAX, bk = -1;
+      for (int k = k0; k < k1; ++k) {
+        const int j = col[k];
+        if (dir[j]) {
+          const int oc = dperm[j];
+          if (oc > last && oc < best) {
+            best = oc;
+            bk = k;
+          }
+        }
+      }
+      if (bk < 0) break;
+      fr -= val[bk] * g[col[bk]];
+      last = best;
+    }
+  }
   fbc[r] = fr;
+}
+
+__global__ void k_perm_gather(int64_t n, const double *__restrict__ src,
+                              const int *__restrict__ map,
+                              double *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[map[i]];
+}
+
+__global__ void k_perm_gather_u8(int64_t n, const uint8_t *__restrict__ src,
+                                 const int *__restrict__ map,
+                                 uint8_t *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[map[i]];
+}
+
+__global__ void k_perm_scatter(int64_t n, const double *__restrict__ src,
+                               const int *__restrict__ map,
+                               double *__restrict__ dst) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[map[i]] = src[i];
+}
+
+// DOF permutation for 3 DOFs per node: perm3[3 r + c] = 3 perm[r] + c
+__global__ void k_perm3(int64_t n_nodes, const int *__restrict__ perm,
+                        int *__restrict__ perm3) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 3 * n_nodes) perm3[i] = 3 * perm[i / 3] + (int)(i % 3);
 }
 
 template <class F> void scan_inclusive(gs_ctx *ctx, const int *in, int *out,
@@ -342,7 +401,8 @@ void build_node_pattern(gs_ctx *ctx, gs_mesh *m) {
     auto keys2 = dalloc<uint64_t>(N);
     auto vals = dalloc<int>(N);
     p.contrib = dalloc<int>(N);
-    hipLaunchKernelGGL(k_make_keys, nblk(N), BS, 0, s, E, m->conn.get(), bits,
+    hipLaunchKernelGGL(k_make_keys, nblk(N), BS, 0, s, E, m->conn.get(),
+                       m->iperm.get(), bits,
                        keys.get(), vals.get());
     GS_LAUNCH_CHECK("k_make_keys");
     size_t bytes = 0;
@@ -421,11 +481,11 @@ void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
 }
 
 void gather_thermal_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
-                         const double *Fe, double *f) {
+                         const double *Fe, const int *out_map, double *f) {
   if (n_nodes == 0) return;
   hipLaunchKernelGGL((k_gather_load<1>), nblk(n_nodes), BS, 0, s, n_nodes,
                      p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(),
-                     Fe, f);
+                     Fe, out_map, f);
   GS_LAUNCH_CHECK("k_gather_load<1>");
 }
 
@@ -440,11 +500,11 @@ void gather_elastic(hipStream_t s, const NodePattern &p, int64_t n_nodes,
 }
 
 void gather_gravity_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
-                         const double *Fe, double *f) {
+                         const double *Fe, const int *out_map, double *f) {
   if (n_nodes == 0) return;
   hipLaunchKernelGGL((k_gather_load<3>), nblk(n_nodes), BS, 0, s, n_nodes,
                      p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(),
-                     Fe, f);
+                     Fe, out_map, f);
   GS_LAUNCH_CHECK("k_gather_load<3>");
 }
 
@@ -464,6 +524,7 @@ gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
   out->ctx = ctx;
   out->rows = n;
   out->cols = K->cols;
+  out->dperm = K->dperm;
   out->row_ptr = dalloc<int>(n + 1);
   GS_HIP(hipMemsetAsync(out->row_ptr.get(), 0, sizeof(int), s));
   if (n > 0) {
@@ -479,12 +540,44 @@ gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
   out->val = dalloc<double>(out->nnz);
   if (n > 0) {
     hipLaunchKernelGGL(k_bc_fill, nblk(n), BS, 0, s, n, K->row_ptr.get(),
-                       K->col.get(), K->val.get(), is_dir, g, f, maxbits.get(),
-                       out->row_ptr.get(), out->col.get(), out->val.get(), fbc);
+                       K->col.get(), K->val.get(), is_dir, g, f,
+                       K->dperm.get(), maxbits.get(), out->row_ptr.get(),
+                       out->col.get(), out->val.get(), fbc);
     GS_LAUNCH_CHECK("k_bc_fill");
   }
   GS_HIP(hipStreamSynchronize(s));
   return out.release();
+}
+
+void build_perm3(gs_ctx *ctx, gs_mesh *m) {
+  if (m->perm3) return;
+  m->perm3 = dalloc<int>(3 * m->n_nodes);
+  if (m->n_nodes > 0) {
+    hipLaunchKernelGGL(k_perm3, nblk(3 * m->n_nodes), BS, 0, ctx->stream,
+                       m->n_nodes, m->perm.get(), m->perm3.get());
+    GS_LAUNCH_CHECK("k_perm3");
+  }
+}
+
+void permute_gather(hipStream_t s, int64_t n, const double *src, const int *map,
+                    double *dst) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_perm_gather, nblk(n), BS, 0, s, n, src, map, dst);
+  GS_LAUNCH_CHECK("k_perm_gather");
+}
+
+void permute_gather_u8(hipStream_t s, int64_t n, const uint8_t *src,
+                       const int *map, uint8_t *dst) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_perm_gather_u8, nblk(n), BS, 0, s, n, src, map, dst);
+  GS_LAUNCH_CHECK("k_perm_gather_u8");
+}
+
+void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map,
+                     double *dst) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_perm_scatter, nblk(n), BS, 0, s, n, src, map, dst);
+  GS_LAUNCH_CHECK("k_perm_scatter");
 }
 
 }  // namespace gs

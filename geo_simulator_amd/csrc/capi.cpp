@@ -1,5 +1,6 @@
 // C ABI of libgeofem (include/geofem.h): handle management, argument
 // validation, and the assemble/solve pipelines built from the device kernels.
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -80,9 +81,11 @@ gs_csr *assemble_thermal_K(gs_ctx *ctx, gs_mesh *m, double k,
   check_flags(read_flag_and_clear(ctx, flag.get()));
   auto val = dalloc<double>(p.nnz);
   gather_thermal(s, p, m->n_elems, Ke.get(), val.get());
-  if (with_load) gather_thermal_load(s, p, m->n_nodes, Fe.get(), f_dev);
+  if (with_load) gather_thermal_load(s, p, m->n_nodes, Fe.get(), nullptr, f_dev);
   GS_HIP(hipStreamSynchronize(s));
-  return new_csr(ctx, m->n_nodes, m->n_nodes, p.nnz, p.row_ptr, p.col, val);
+  gs_csr *K = new_csr(ctx, m->n_nodes, m->n_nodes, p.nnz, p.row_ptr, p.col, val);
+  K->dperm = m->perm;
+  return K;
 }
 
 gs_csr *assemble_elastic_K(gs_ctx *ctx, gs_mesh *m, double E, double nu,
@@ -103,16 +106,79 @@ gs_csr *assemble_elastic_K(gs_ctx *ctx, gs_mesh *m, double E, double nu,
   check_flags(read_flag_and_clear(ctx, flag.get()));
   auto val = dalloc<double>(m->dof3.nnz);
   gather_elastic(s, p, m->n_nodes, Ke.get(), val.get());
-  if (with_load) gather_gravity_load(s, p, m->n_nodes, Fe.get(), f_dev);
+  if (with_load) gather_gravity_load(s, p, m->n_nodes, Fe.get(), nullptr, f_dev);
+  build_perm3(ctx, m);
   GS_HIP(hipStreamSynchronize(s));
-  return new_csr(ctx, 3 * m->n_nodes, 3 * m->n_nodes, m->dof3.nnz,
-                 m->dof3.row_ptr, m->dof3.col, val);
+  gs_csr *K = new_csr(ctx, 3 * m->n_nodes, 3 * m->n_nodes, m->dof3.nnz,
+                      m->dof3.row_ptr, m->dof3.col, val);
+  K->dperm = m->perm3;
+  return K;
 }
 
 void check_mesh_dof(const gs_mesh *m, const gs_dofmap *d) {
   require(m && d, GS_ERR_INVALID_ARG, "null mesh or dofmap");
   require(d->n_nodes == m->n_nodes, GS_ERR_INVALID_ARG,
           "dofmap node count differs from the mesh");
+}
+
+// Dirichlet elimination with the caller's numbering at the boundary: f is in
+// caller numbering when f_caller, else already internal; f_bc is returned in
+// caller numbering.  Masks and values are gathered into K's numbering.
+gs_csr *eliminate_caller_rhs(gs_ctx *ctx, const gs_csr *K, const double *f,
+                             bool f_caller, const gs_dofmap *dof, double *f_bc) {
+  hipStream_t s = ctx->stream;
+  const int *dp = K->dperm.get();
+  if (!dp) {
+    return dirichlet_eliminate(ctx, K, f, dof->is_dir.get(), dof->values.get(),
+                               f_bc);
+  }
+  const int64_t n = K->rows;
+  auto dir = dalloc<uint8_t>(n);
+  auto g = dalloc<double>(n);
+  auto fi = dalloc<double>(n);
+  auto fo = dalloc<double>(n);
+  permute_gather_u8(s, n, dof->is_dir.get(), dp, dir.get());
+  permute_gather(s, n, dof->values.get(), dp, g.get());
+  const double *fin = f;
+  if (f_caller) {
+    permute_gather(s, n, f, dp, fi.get());
+    fin = fi.get();
+  }
+  gs_csr *out = dirichlet_eliminate(ctx, K, fin, dir.get(), g.get(), fo.get());
+  permute_scatter(s, n, fo.get(), dp, f_bc);
+  GS_HIP(hipStreamSynchronize(s));
+  return out;
+}
+
+// Locality order of the mesh nodes: lexicographic in the coordinates, the
+// longest bounding-box axis slowest, so a row's neighbours sit within a few
+// coordinate layers and a window of consecutive rows gathers from a small,
+// cache-resident part of the vector.  Ties keep the caller order.  Any
+// non-finite coordinate leaves the identity order.
+std::vector<int> locality_order(const double *xyz, int64_t n) {
+  std::vector<int> order((size_t)n);
+  for (int64_t i = 0; i < n; ++i) order[i] = (int)i;
+  double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  for (int64_t i = 0; i < 3 * n; ++i)
+    if (!std::isfinite(xyz[i])) return order;
+  for (int c = 0; c < 3 && n > 0; ++c) {
+    lo[c] = hi[c] = xyz[c];
+    for (int64_t i = 0; i < n; ++i) {
+      lo[c] = std::min(lo[c], xyz[3 * i + c]);
+      hi[c] = std::max(hi[c], xyz[3 * i + c]);
+    }
+  }
+  int ax[3] = {0, 1, 2};
+  std::stable_sort(ax, ax + 3, [&](int a, int b) { return hi[a] - lo[a] > hi[b] - lo[b]; });
+  std::sort(order.begin(), order.end(), [&](int a, int b) {
+    for (int k = 0; k < 3; ++k) {
+      const double va = xyz[3 * (int64_t)a + ax[k]], vb = xyz[3 * (int64_t)b + ax[k]];
+      if (va < vb) return true;
+      if (va > vb) return false;
+    }
+    return a < b;
+  });
+  return order;
 }
 
 // ---- host mesh generator (src/mesh_generator_improved.rs:19-169) ----------
@@ -333,6 +399,17 @@ gs_status gs_mesh_create(gs_ctx *ctx, const double *xyz, int64_t n_nodes,
     if (n_elems)
       GS_HIP(hipMemcpyAsync(m->conn.get(), c32.data(), n_elems * 10 * sizeof(int),
                             hipMemcpyHostToDevice, ctx->stream));
+    const std::vector<int> perm = locality_order(xyz, n_nodes);
+    std::vector<int> iperm((size_t)n_nodes);
+    for (int64_t i = 0; i < n_nodes; ++i) iperm[perm[i]] = (int)i;
+    m->perm = dalloc<int>(n_nodes);
+    m->iperm = dalloc<int>(n_nodes);
+    if (n_nodes) {
+      GS_HIP(hipMemcpyAsync(m->perm.get(), perm.data(), n_nodes * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+      GS_HIP(hipMemcpyAsync(m->iperm.get(), iperm.data(), n_nodes * sizeof(int),
+                            hipMemcpyHostToDevice, ctx->stream));
+    }
     GS_HIP(hipStreamSynchronize(ctx->stream));
     *out = m.release();
   });
@@ -410,7 +487,7 @@ gs_status gs_assemble_thermal_load(gs_ctx *ctx, gs_mesh *mesh,
     launch_thermal_elements(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(),
                             1.0, nullptr, Q, true, Ke.get(), Fe.get(), flag.get());
     check_flags(read_flag_and_clear(ctx, flag.get()) & ~1);
-    gather_thermal_load(s, mesh->pat, mesh->n_nodes, Fe.get(), f);
+    gather_thermal_load(s, mesh->pat, mesh->n_nodes, Fe.get(), mesh->perm.get(), f);
     GS_HIP(hipStreamSynchronize(s));
   });
 }
@@ -456,7 +533,7 @@ gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                             1.0, 0.25, nullptr, rho, g, true, Ke.get(), Fe.get(),
                             flag.get());
     check_flags(read_flag_and_clear(ctx, flag.get()) & ~1);
-    gather_gravity_load(s, mesh->pat, mesh->n_nodes, Fe.get(), f);
+    gather_gravity_load(s, mesh->pat, mesh->n_nodes, Fe.get(), mesh->perm.get(), f);
     GS_HIP(hipStreamSynchronize(s));
   });
 }
@@ -472,8 +549,7 @@ gs_status gs_apply_dirichlet_bcs(gs_ctx *ctx, const gs_csr *K, const double *f,
     require(K->rows == dof->n_dofs && K->cols == dof->n_dofs, GS_ERR_INVALID_ARG,
             "matrix size differs from the DOF count");
     set_device(ctx);
-    *K_bc = dirichlet_eliminate(ctx, K, f, dof->is_dir.get(), dof->values.get(),
-                                f_bc);
+    *K_bc = eliminate_caller_rhs(ctx, K, f, true, dof, f_bc);
   });
 }
 
@@ -491,8 +567,7 @@ gs_status gs_assemble_thermal_system(gs_ctx *ctx, gs_mesh *mesh,
     auto f = dalloc<double>(mesh->n_nodes);
     std::unique_ptr<gs_csr> K(
         assemble_thermal_K(ctx, mesh, k, k_elem, true, Q, f.get()));
-    *K_bc = dirichlet_eliminate(ctx, K.get(), f.get(), dof->is_dir.get(),
-                                dof->values.get(), f_bc);
+    *K_bc = eliminate_caller_rhs(ctx, K.get(), f.get(), false, dof, f_bc);
   });
 }
 
@@ -517,8 +592,7 @@ gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
     auto f = dalloc<double>(3 * mesh->n_nodes);
     std::unique_ptr<gs_csr> K(
         assemble_elastic_K(ctx, mesh, E, nu, E_elem, true, rho, g, f.get()));
-    *K_bc = dirichlet_eliminate(ctx, K.get(), f.get(), dof->is_dir.get(),
-                                dof->values.get(), f_bc);
+    *K_bc = eliminate_caller_rhs(ctx, K.get(), f.get(), false, dof, f_bc);
   });
 }
 
@@ -536,21 +610,50 @@ gs_status gs_csr_export(gs_ctx *ctx, const gs_csr *A, int64_t *indptr,
   if (!ctx || !A) return GS_ERR_INVALID_ARG;
   return guard(ctx, [&] {
     set_device(ctx);
-    std::vector<int> rp((size_t)A->rows + 1), ci((size_t)A->nnz);
-    GS_HIP(hipMemcpyAsync(rp.data(), A->row_ptr.get(), (A->rows + 1) * sizeof(int),
+    const int64_t rows = A->rows, nnz = A->nnz;
+    std::vector<int> rp((size_t)rows + 1), ci((size_t)nnz);
+    std::vector<double> dv((size_t)nnz);
+    GS_HIP(hipMemcpyAsync(rp.data(), A->row_ptr.get(), (rows + 1) * sizeof(int),
                           hipMemcpyDeviceToHost, ctx->stream));
-    if (A->nnz) {
-      GS_HIP(hipMemcpyAsync(ci.data(), A->col.get(), A->nnz * sizeof(int),
+    if (nnz) {
+      GS_HIP(hipMemcpyAsync(ci.data(), A->col.get(), nnz * sizeof(int),
                             hipMemcpyDeviceToHost, ctx->stream));
-      if (data)
-        GS_HIP(hipMemcpyAsync(data, A->val.get(), A->nnz * sizeof(double),
-                              hipMemcpyDeviceToHost, ctx->stream));
+      GS_HIP(hipMemcpyAsync(dv.data(), A->val.get(), nnz * sizeof(double),
+                            hipMemcpyDeviceToHost, ctx->stream));
     }
+    if (!A->dperm) {
+      GS_HIP(hipStreamSynchronize(ctx->stream));
+      for (int64_t i = 0; indptr && i <= rows; ++i) indptr[i] = rp[i];
+      for (int64_t i = 0; indices && i < nnz; ++i) indices[i] = ci[i];
+      if (data && nnz) memcpy(data, dv.data(), nnz * sizeof(double));
+      return;
+    }
+    // Back to caller numbering: caller row o is internal row inv[o]; its
+    // entries get caller columns dperm[c] and are re-sorted by them.
+    std::vector<int> dp((size_t)rows), inv((size_t)rows);
+    if (rows)
+      GS_HIP(hipMemcpyAsync(dp.data(), A->dperm.get(), rows * sizeof(int),
+                            hipMemcpyDeviceToHost, ctx->stream));
     GS_HIP(hipStreamSynchronize(ctx->stream));
-    if (indptr)
-      for (int64_t i = 0; i <= A->rows; ++i) indptr[i] = rp[i];
-    if (indices)
-      for (int64_t i = 0; i < A->nnz; ++i) indices[i] = ci[i];
+    for (int64_t i = 0; i < rows; ++i) inv[dp[i]] = (int)i;
+    std::vector<std::pair<int, double>> row;
+    int64_t pos = 0;
+    if (indptr) indptr[0] = 0;
+    for (int64_t o = 0; o < rows; ++o) {
+      const int r = inv[o];
+      row.clear();
+      for (int k = rp[r]; k < rp[r + 1]; ++k) row.emplace_back(dp[ci[k]], dv[k]);
+      std::sort(row.begin(), row.end(),
+                [](const std::pair<int, double> &a, const std::pair<int, double> &b) {
+                  return a.first < b.first;
+                });
+      for (const auto &e : row) {
+        if (indices) indices[pos] = e.first;
+        if (data) data[pos] = e.second;
+        ++pos;
+      }
+      if (indptr) indptr[o + 1] = pos;
+    }
   });
 }
 
@@ -615,7 +718,13 @@ gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv) {
   return guard(ctx, [&] {
     require(dinv || A->rows == 0, GS_ERR_INVALID_ARG, "null output");
     set_device(ctx);
-    jacobi_diag_inv(ctx->stream, A, dinv);
+    if (A->dperm) {
+      auto d = dalloc<double>(A->rows);
+      jacobi_diag_inv(ctx->stream, A, d.get());
+      permute_scatter(ctx->stream, A->rows, d.get(), A->dperm.get(), dinv);
+    } else {
+      jacobi_diag_inv(ctx->stream, A, dinv);
+    }
     GS_HIP(hipStreamSynchronize(ctx->stream));
   });
 }
@@ -691,7 +800,7 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags) {
   if (!A) return GS_ERR_INVALID_ARG;
-  if (flags & ~GS_SPMV_VECTOR) return GS_ERR_INVALID_ARG;
+  if (flags & ~(GS_SPMV_VECTOR | GS_SPMV_INDEX32)) return GS_ERR_INVALID_ARG;
   const bool vec = (flags & GS_SPMV_VECTOR) != 0;
   const bool ok = lanes == 0 ||
                   (vec ? (lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32)

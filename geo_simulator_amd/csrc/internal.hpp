@@ -84,8 +84,12 @@ struct gs_ctx {
 struct gs_mesh {
   gs_ctx *ctx = nullptr;
   int64_t n_nodes = 0, n_elems = 0;
-  gs::DArr<double> xyz; // n_nodes * 3 (AoS, as Point3<f64>)
-  gs::DArr<int> conn;   // n_elems * 10
+  gs::DArr<double> xyz; // n_nodes * 3 (AoS, as Point3<f64>), caller numbering
+  gs::DArr<int> conn;   // n_elems * 10, caller numbering
+  // Internal node numbering used by every device matrix built on this mesh
+  // (a locality order, see gs_mesh_create): perm[new] = old, iperm[old] = new.
+  gs::DArr<int> perm, iperm;
+  gs::DArr<int> perm3;  // DOF-level perm for 3 DOFs per node (lazily)
   bool has_pattern = false;
   gs::NodePattern pat;
   bool has_dof3 = false;
@@ -108,7 +112,13 @@ struct gs_csr {
   gs::DArr<int> row_ptr;
   gs::DArr<int> col;
   gs::DArr<double> val;
+  // Row/column numbering: internal index i is caller index dperm[i] (empty:
+  // identity).  Vectors cross the C ABI in caller numbering.
+  gs::DArr<int> dperm;
   // lazily derived, cached (solver)
+  int idx16 = 0;              // 16-bit column offsets: 0 unknown, 1 yes, -1 no
+  gs::DArr<short> off16;      // nnz
+  gs::DArr<int> anchor;       // one per 1024 nonzeros
   gs::DArr<float> val32;
   gs::DArr<double> dinv;
   gs::DArr<float> dinv32;
@@ -141,16 +151,28 @@ void build_node_pattern(gs_ctx *ctx, gs_mesh *m);
 void build_dof3_pattern(gs_ctx *ctx, gs_mesh *m);
 void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
                     const double *Ke, double *val);
+// Loads land at out_map[r] (caller numbering) when out_map is given, else at
+// internal row r.
 void gather_thermal_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
-                         const double *Fe, double *f);
+                         const double *Fe, const int *out_map, double *f);
 void gather_elastic(hipStream_t s, const NodePattern &p, int64_t n_nodes,
                     const double *Ke, double *val);
 void gather_gravity_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
-                         const double *Fe, double *f);
-// Dirichlet elimination of a CSR matrix; returns the new CSR.
+                         const double *Fe, const int *out_map, double *f);
+void build_perm3(gs_ctx *ctx, gs_mesh *m);
+// Dirichlet elimination of a CSR matrix in its internal numbering (f, is_dir,
+// g, fbc internal too); dperm (nullable) gives each column's caller index,
+// which fixes the order of the RHS lift.  Returns the new CSR.
 gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
                             const uint8_t *is_dir, const double *g,
                             double *fbc);
+// dst[i] = src[map[i]]  /  dst[map[i]] = src[i]
+void permute_gather(hipStream_t s, int64_t n, const double *src, const int *map,
+                    double *dst);
+void permute_gather_u8(hipStream_t s, int64_t n, const uint8_t *src,
+                       const int *map, uint8_t *dst);
+void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map,
+                     double *dst);
 
 // ---- solver.hip ----
 void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv);

@@ -25,6 +25,7 @@
 #include <cstring>
 
 #include "internal.hpp"
+#include "spmv.hpp"
 
 namespace gs {
 namespace {
@@ -97,118 +98,31 @@ __device__ __forceinline__ bool leader() {
   return blockIdx.x == 0 && threadIdx.x == 0;
 }
 
-// ---- SpMV engines ------------------------------------------------------------
-// Both call epi(row, (A x)[row]) once per row from one thread.
-//
-// Stream engine (default): a workgroup takes a block of R = BS/S consecutive
-// rows; all 256 threads load the block's (value, column) pairs in contiguous
-// chunks (fully coalesced, CH/BS independent gathers of x in flight per
-// thread) and park the products in LDS; then S threads per row sum the row's
-// products.  Row blocks are visited grid-stride, so the workgroups resident
-// at any moment work on nearby rows and share the gathered part of x in L2.
-template <typename T> struct StreamChunk { static constexpr int value = 2048; };
-template <> struct StreamChunk<float> { static constexpr int value = 4096; };
-
-template <int S, typename T, class Epi>
-__device__ __forceinline__ void spmv_stream(int n, const int *__restrict__ rp,
-                                            const int *__restrict__ ci,
-                                            const T *__restrict__ val,
-                                            const T *__restrict__ x, Epi &epi) {
-  constexpr int R = BS / S;
-  constexpr int CH = StreamChunk<T>::value;
-  __shared__ T sprod[CH];
-  __shared__ int srp[R + 1];
-  const int t = threadIdx.x;
-  const int lr = t / S, sub = t - (t / S) * S;
-  const int nblk = (n + R - 1) / R;
-  for (int blk = blockIdx.x; blk < nblk; blk += gridDim.x) {
-    const int r0 = blk * R;
-    const int nr = min(R, n - r0);
-    for (int i = t; i <= nr; i += BS) srp[i] = rp[r0 + i];
-    __syncthreads();
-    const int k0 = srp[0], k1 = srp[nr];
-    const bool own = lr < nr;
-    const int a = own ? srp[lr] : 0, b = own ? srp[lr + 1] : 0;
-    T acc = (T)0;
-    for (int base = k0; base < k1; base += CH) {
-      const int m = min(CH, k1 - base);
-      int cj[CH / BS];
-      T vj[CH / BS];
-#pragma unroll
-      for (int i = 0; i < CH / BS; ++i) {
-        const int j = t + i * BS;
-        cj[i] = j < m ? ci[base + j] : 0;
-        vj[i] = j < m ? val[base + j] : (T)0;
-      }
-#pragma unroll
-      for (int i = 0; i < CH / BS; ++i) {
-        const int j = t + i * BS;
-        if (j < m) sprod[j] = vj[i] * x[cj[i]];
-      }
-      __syncthreads();
-      const int lo = max(a, base) - base, hi = min(b, base + m) - base;
-      for (int j = lo + sub; j < hi; j += S) acc += sprod[j];
-      __syncthreads();
-    }
-#pragma unroll
-    for (int off = S / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, S);
-    if (own && sub == 0) epi(r0 + lr, acc);
-    __syncthreads();
-  }
-}
-
-// Vector engine: L lanes per row, rows visited grid-stride.
-template <int L, typename T, class Epi>
-__device__ __forceinline__ void spmv_vector(int n, const int *__restrict__ rp,
-                                            const int *__restrict__ ci,
-                                            const T *__restrict__ val,
-                                            const T *__restrict__ x, Epi &epi) {
-  const int lane = threadIdx.x & (L - 1);
-  const int ng = (gridDim.x * BS) / L;
-  for (int row = (blockIdx.x * BS + threadIdx.x) / L; row < n; row += ng) {
-    const int b = rp[row], e = rp[row + 1];
-    T s = (T)0;
-    for (int k = b + lane; k < e; k += L) s += val[k] * x[ci[k]];
-#pragma unroll
-    for (int off = L / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, L);
-    if (lane == 0) epi(row, s);
-  }
-}
-
-// ENG = 0: stream with S = W threads per row; ENG = 1: vector with L = W lanes.
-template <int ENG, int W, typename T, class Epi>
-__device__ __forceinline__ void spmv_rows(int n, const int *__restrict__ rp,
-                                          const int *__restrict__ ci,
-                                          const T *__restrict__ val,
-                                          const T *__restrict__ x, Epi &epi) {
-  if constexpr (ENG == 0) spmv_stream<W, T>(n, rp, ci, val, x, epi);
-  else spmv_vector<W, T>(n, rp, ci, val, x, epi);
-}
-
+// ---- SpMV kernels (engines in spmv.hpp) ------------------------------------
 // y = A x
-template <int ENG, int W, typename T>
+template <int ENG, int W, class C, typename T>
 __global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ rp,
-                                             const int *__restrict__ ci,
+                                             const C cols,
                                              const T *__restrict__ val,
                                              const T *__restrict__ x,
                                              T *__restrict__ y) {
   auto epi = [&](int row, T s) { y[row] = s; };
-  spmv_rows<ENG, W, T>(n, rp, ci, val, x, epi);
+  spmv_rows<ENG, W, T>(n, rp, cols, val, x, epi);
 }
 
 // K1: q = A p; per-workgroup partials of p.q.
-template <int ENG, int W, typename T>
+template <int ENG, int W, class C, typename T>
 __global__ __launch_bounds__(BS) void k_cg_spmv(
-    int n, const int *__restrict__ rp, const int *__restrict__ ci,
-    const T *__restrict__ val, const T *__restrict__ p, T *__restrict__ q,
-    const CgState *__restrict__ st, double *__restrict__ part) {
+    int n, const int *__restrict__ rp, const C cols, const T *__restrict__ val,
+    const T *__restrict__ p, T *__restrict__ q, const CgState *__restrict__ st,
+    double *__restrict__ part) {
   if (st->stop) return;
   double acc[1] = {0.0};
   auto epi = [&](int row, T s) {
     q[row] = s;
     acc[0] += (double)p[row] * (double)s;
   };
-  spmv_rows<ENG, W, T>(n, rp, ci, val, p, epi);
+  spmv_rows<ENG, W, T>(n, rp, cols, val, p, epi);
   put_partials<1>(acc, part);
 }
 
@@ -326,9 +240,9 @@ __global__ __launch_bounds__(BS) void k_cg_init_final(
 
 // ---- mixed-precision helpers ----------------------------------------------
 // r = b - A x (FP64); per-workgroup partials of r.r
-template <int ENG, int W>
+template <int ENG, int W, class C>
 __global__ __launch_bounds__(BS) void k_residual(
-    int n, const int *__restrict__ rp, const int *__restrict__ ci,
+    int n, const int *__restrict__ rp, const C cols,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, double *__restrict__ r,
     double *__restrict__ part) {
@@ -338,7 +252,7 @@ __global__ __launch_bounds__(BS) void k_residual(
     r[row] = ri;
     acc[0] += ri * ri;
   };
-  spmv_rows<ENG, W, double>(n, rp, ci, val, x, epi);
+  spmv_rows<ENG, W, double>(n, rp, cols, val, x, epi);
   put_partials<1>(acc, part);
 }
 
@@ -409,71 +323,110 @@ inline unsigned grid_vec(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(need, GMAX));
 }
 
-// SpMV launch shape: engine, threads (stream) or lanes (vector) per row,
-// workgroups.
-struct Shape {
-  int eng = 0, W = 2, grid = 1;
-};
+// anchor[g] = row holding nonzero g*1024 (binary search in the row pointer)
+__global__ void k_anchor(int n, int ng, const int *__restrict__ rp,
+                         int *__restrict__ anchor) {
+  const int g = blockIdx.x * BS + threadIdx.x;
+  if (g >= ng) return;
+  const int k = g << ANCHOR_SHIFT;
+  int lo = 0, hi = n;  // rp[lo] <= k < rp[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (rp[mid] <= k) lo = mid;
+    else hi = mid;
+  }
+  anchor[g] = lo;
+}
 
-Shape shape_of(const gs_csr *A) {
+__global__ void k_offsets16(int64_t nnz, const int *__restrict__ ci,
+                            const int *__restrict__ anchor,
+                            short *__restrict__ off, int *__restrict__ bad) {
+  for (int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x; k < nnz;
+       k += (int64_t)gridDim.x * BS) {
+    const int d = ci[k] - anchor[k >> ANCHOR_SHIFT];
+    if (d < -32768 || d > 32767) *bad = 1;
+    off[k] = (short)d;
+  }
+}
+
+Shape shape_of(const gs_csr *A, bool fp32) {
   Shape sh;
   const int64_t n = A->rows;
   const double avg = n > 0 ? (double)A->nnz / (double)n : 0.0;
-  sh.eng = (A->spmv_flags & GS_SPMV_VECTOR) ? 1 : 0;
+  const bool forced = A->spmv_lanes != 0 || (A->spmv_flags & GS_SPMV_VECTOR);
+  if (forced) {
+    sh.eng = (A->spmv_flags & GS_SPMV_VECTOR) ? 1 : 0;
+  } else {
+    // measured on MI355X: the stream engine wins in FP64, the vector engine
+    // in FP32
+    sh.eng = fp32 ? 1 : 0;
+  }
+  // Defaults from launch-shape sweeps on MI355X at 1.08M rows x 27 nnz/row:
+  // FP64 stream engine, 4 threads/row, 2048 workgroups; FP32 vector engine,
+  // 4 lanes/row, 8192 workgroups.
   if (sh.eng == 0) {
     // rows per workgroup R = 256 / W: a few thousand nonzeros per row block
-    sh.W = A->spmv_lanes ? A->spmv_lanes : (avg <= 40.0 ? 2 : (avg <= 100.0 ? 4 : 8));
+    sh.W = A->spmv_lanes ? A->spmv_lanes : (avg <= 40.0 ? 4 : 8);
   } else {
     sh.W = A->spmv_lanes ? A->spmv_lanes
-                         : (avg <= 12.0 ? 4 : (avg <= 40.0 ? 8 : (avg <= 100.0 ? 16 : 32)));
+                         : (avg <= 40.0 ? 4 : (avg <= 100.0 ? 8 : 16));
   }
   const int64_t rows_per_pass = BS / sh.W;
-  const int64_t want = A->spmv_grid > 0 ? A->spmv_grid : 2048;
+  const int64_t want = A->spmv_grid > 0 ? A->spmv_grid : (fp32 ? 8192 : 2048);
   const int64_t cap = std::max<int64_t>(1, (n + rows_per_pass - 1) / rows_per_pass);
   sh.grid = (int)std::min<int64_t>(std::min<int64_t>(want, cap), GRID_MAX);
+  sh.idx16 = A->idx16 > 0 && !(A->spmv_flags & GS_SPMV_INDEX32);
   return sh;
 }
 
-#define GS_SPMV_DISPATCH(SH, KERNEL, T, ...)                                        \
-  do {                                                                           \
-    if ((SH).eng == 0) {                                                         \
-      switch ((SH).W) {                                                          \
-      case 1: hipLaunchKernelGGL((KERNEL<0, 1 T>), __VA_ARGS__); break;          \
-      case 2: hipLaunchKernelGGL((KERNEL<0, 2 T>), __VA_ARGS__); break;          \
-      case 4: hipLaunchKernelGGL((KERNEL<0, 4 T>), __VA_ARGS__); break;          \
-      default: hipLaunchKernelGGL((KERNEL<0, 8 T>), __VA_ARGS__); break;         \
-      }                                                                          \
-    } else {                                                                     \
-      switch ((SH).W) {                                                          \
-      case 4: hipLaunchKernelGGL((KERNEL<1, 4 T>), __VA_ARGS__); break;          \
-      case 8: hipLaunchKernelGGL((KERNEL<1, 8 T>), __VA_ARGS__); break;          \
-      case 16: hipLaunchKernelGGL((KERNEL<1, 16 T>), __VA_ARGS__); break;        \
-      default: hipLaunchKernelGGL((KERNEL<1, 32 T>), __VA_ARGS__); break;        \
-      }                                                                          \
-    }                                                                            \
-  } while (0)
-#define GS_COMMA ,
+Cols32 cols32(const gs_csr *A) { return Cols32{A->col.get()}; }
+Cols16 cols16(const gs_csr *A) { return Cols16{A->off16.get(), A->anchor.get()}; }
 
 template <typename T>
-void launch_cg_spmv(hipStream_t s, const Shape &sh, int n, const int *rp,
-                    const int *ci, const T *val, const T *p, T *q,
-                    const CgState *st, double *part) {
-  GS_SPMV_DISPATCH(sh, k_cg_spmv, GS_COMMA T, sh.grid, BS, 0, s, n, rp, ci, val,
-                   p, q, st, part);
+void launch_cg_spmv(hipStream_t s, const Shape &sh, const gs_csr *A,
+                    const T *val, const T *p, T *q, const CgState *st,
+                    double *part) {
+  const int n = (int)A->rows;
+  dispatch_shape(sh, [&](auto eng, auto w) {
+    constexpr int E = decltype(eng)::value, Wv = decltype(w)::value;
+    if (sh.idx16)
+      hipLaunchKernelGGL((k_cg_spmv<E, Wv, Cols16, T>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols16(A), val, p, q, st, part);
+    else
+      hipLaunchKernelGGL((k_cg_spmv<E, Wv, Cols32, T>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols32(A), val, p, q, st, part);
+  });
 }
 
 template <typename T>
-void launch_spmv(hipStream_t s, const Shape &sh, int n, const int *rp,
-                 const int *ci, const T *val, const T *x, T *y) {
-  GS_SPMV_DISPATCH(sh, k_spmv, GS_COMMA T, sh.grid, BS, 0, s, n, rp, ci, val, x, y);
+void launch_spmv(hipStream_t s, const Shape &sh, const gs_csr *A, const T *val,
+                 const T *x, T *y) {
+  const int n = (int)A->rows;
+  dispatch_shape(sh, [&](auto eng, auto w) {
+    constexpr int E = decltype(eng)::value, Wv = decltype(w)::value;
+    if (sh.idx16)
+      hipLaunchKernelGGL((k_spmv<E, Wv, Cols16, T>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols16(A), val, x, y);
+    else
+      hipLaunchKernelGGL((k_spmv<E, Wv, Cols32, T>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols32(A), val, x, y);
+  });
 }
 
 // r = b - A x and ||r|| -> st->rnorm
-void launch_residual(hipStream_t s, const Shape &sh, int n, const int *rp,
-                     const int *ci, const double *val, const double *x,
-                     const double *b, double *r, CgState *st, double *part) {
-  GS_SPMV_DISPATCH(sh, k_residual, , sh.grid, BS, 0, s, n, rp, ci, val, x, b, r,
-                   part);
+void launch_residual(hipStream_t s, const Shape &sh, const gs_csr *A,
+                     const double *x, const double *b, double *r, CgState *st,
+                     double *part) {
+  const int n = (int)A->rows;
+  dispatch_shape(sh, [&](auto eng, auto w) {
+    constexpr int E = decltype(eng)::value, Wv = decltype(w)::value;
+    if (sh.idx16)
+      hipLaunchKernelGGL((k_residual<E, Wv, Cols16>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols16(A), A->val.get(), x, b, r, part);
+    else
+      hipLaunchKernelGGL((k_residual<E, Wv, Cols32>), sh.grid, BS, 0, s, n,
+                         A->row_ptr.get(), cols32(A), A->val.get(), x, b, r, part);
+  });
   hipLaunchKernelGGL(k_norm_final, 1, BS, 0, s, st, part, sh.grid);
 }
 
@@ -536,10 +489,11 @@ struct PcgResult {
 // Device-resident PCG in one precision.  Passes are enqueued in batches sized
 // from the observed residual decay; the state is read once per batch.
 template <typename T>
-PcgResult pcg(gs_ctx *ctx, const Shape &sh, int n, const int *rp, const int *ci,
-              const T *val, const T *b, T *x, const T *dinv, long long max_iter,
-              double tol, double abs_tol, Work &w) {
+PcgResult pcg(gs_ctx *ctx, const Shape &sh, const gs_csr *A, const T *val,
+              const T *b, T *x, const T *dinv, long long max_iter, double tol,
+              double abs_tol, Work &w) {
   hipStream_t s = ctx->stream;
+  const int n = (int)A->rows;
   T *r = (T *)w.r, *z = (T *)w.z, *p = (T *)w.p, *q = (T *)w.q;
   CgState *st = w.st;
   const unsigned gv = grid_vec(n);
@@ -559,7 +513,7 @@ PcgResult pcg(gs_ctx *ctx, const Shape &sh, int n, const int *rp, const int *ci,
   long long prev_it = 0;
   for (;;) {
     for (long long i = 0; i < batch; ++i, ++launched) {
-      launch_cg_spmv<T>(s, sh, n, rp, ci, val, p, q, st, w.part1);
+      launch_cg_spmv<T>(s, sh, A, val, p, q, st, w.part1);
       if (dinv)
         hipLaunchKernelGGL((k_cg_update<T, true>), gv, BS, 0, s, n, launched, x,
                            r, z, p, q, dinv, st, w.part1, sh.grid, w.part2);
@@ -610,16 +564,58 @@ void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv) {
 
 void csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y) {
   if (A->rows == 0) return;
-  launch_spmv<double>(ctx->stream, shape_of(A), (int)A->rows, A->row_ptr.get(),
-                      A->col.get(), A->val.get(), x, y);
+  hipStream_t s = ctx->stream;
+  const int *dp = A->dperm.get();
+  if (!dp) {
+    launch_spmv<double>(s, shape_of(A, false), A, A->val.get(), x, y);
+  } else {
+    const size_t vec64 = align256((size_t)A->rows * sizeof(double));
+    char *base = ctx_scratch(ctx, 2 * vec64);
+    double *xi = (double *)base, *yi = (double *)(base + vec64);
+    permute_gather(s, A->rows, x, dp, xi);
+    launch_spmv<double>(s, shape_of(A, false), A, A->val.get(), xi, yi);
+    permute_scatter(s, A->rows, yi, dp, y);
+  }
   GS_LAUNCH_CHECK("k_spmv");
-  GS_HIP(hipStreamSynchronize(ctx->stream));
+  GS_HIP(hipStreamSynchronize(s));
 }
 
 static void ensure_dinv(gs_ctx *ctx, gs_csr *A) {
   if (!A->dinv) {
     A->dinv = dalloc<double>(A->rows);
     jacobi_diag_inv(ctx->stream, A, A->dinv.get());
+  }
+}
+
+// 16-bit anchored column offsets (see spmv.hpp), built once per matrix;
+// A->idx16 = 1 when every offset fits, -1 when the matrix keeps int32.
+static void ensure_idx16(gs_ctx *ctx, gs_csr *A) {
+  if (A->idx16 != 0) return;
+  if (A->nnz == 0 || A->rows == 0) {
+    A->idx16 = -1;
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  const int ng = (int)((A->nnz + (1 << ANCHOR_SHIFT) - 1) >> ANCHOR_SHIFT);
+  auto anchor = dalloc<int>(ng);
+  auto off = dalloc<short>(A->nnz);
+  auto bad = dalloc<int>(1);
+  GS_HIP(hipMemsetAsync(bad.get(), 0, sizeof(int), s));
+  hipLaunchKernelGGL(k_anchor, (unsigned)((ng + BS - 1) / BS), BS, 0, s,
+                     (int)A->rows, ng, A->row_ptr.get(), anchor.get());
+  hipLaunchKernelGGL(k_offsets16, (unsigned)GMAX, BS, 0, s, A->nnz, A->col.get(),
+                     anchor.get(), off.get(), bad.get());
+  GS_LAUNCH_CHECK("k_offsets16");
+  int h = 0;
+  GS_HIP(hipMemcpyAsync(ctx->pinned, bad.get(), sizeof(int), hipMemcpyDeviceToHost, s));
+  GS_HIP(hipStreamSynchronize(s));
+  memcpy(&h, ctx->pinned, sizeof(int));
+  if (h) {
+    A->idx16 = -1;
+  } else {
+    A->anchor = anchor;
+    A->off16 = off;
+    A->idx16 = 1;
   }
 }
 
@@ -647,22 +643,36 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   memset(st, 0, sizeof(*st));
   if (A->rows != A->cols) throw Error(GS_ERR_INVALID_ARG, "CG needs a square matrix");
   const int n = (int)A->rows;
-  const Shape sh = shape_of(A);
   hipStream_t s = ctx->stream;
   if (o.use_preconditioner) ensure_dinv(ctx, A);
+  ensure_idx16(ctx, A);
+  const Shape sh = shape_of(A, false);
+  const Shape sh32 = shape_of(A, true);
   const bool mixed = !(o.precision == GS_PREC_FP64 || n == 0);
-  // scratch: FP64 work block [+ FP32 inner work block + r64, r32, d32]
-  size_t bytes = work_bytes(n, sizeof(double));
+  const int *dp = A->dperm.get();
+  const size_t vec64 = align256((size_t)std::max(n, 1) * sizeof(double));
+  // scratch: FP64 work block [+ b, x in internal numbering]
+  //          [+ FP32 inner work block + r64, r32, d32]
+  size_t bytes = work_bytes(n, sizeof(double)) + (dp ? 2 * vec64 : 0);
   if (mixed)
-    bytes += work_bytes(n, sizeof(float)) +
-             align256((size_t)std::max(n, 1) * sizeof(double)) +
+    bytes += work_bytes(n, sizeof(float)) + vec64 +
              2 * align256((size_t)std::max(n, 1) * sizeof(float));
   char *base = ctx_scratch(ctx, bytes);
   Work w;
   base = carve_work(base, n, sizeof(double), w);
+  // vectors cross the ABI in caller numbering; the matrix is internal
+  const double *b_user = b;
+  double *x_user = x;
+  if (dp) {
+    double *bi = (double *)base;
+    double *xi = (double *)(base + vec64);
+    base += 2 * vec64;
+    permute_gather(s, n, b_user, dp, bi);
+    b = bi;
+    x = xi;
+  }
   if (!mixed) {
-    const PcgResult r = pcg<double>(ctx, sh, n, A->row_ptr.get(), A->col.get(),
-                                    A->val.get(), b, x,
+    const PcgResult r = pcg<double>(ctx, sh, A, A->val.get(), b, x,
                                     o.use_preconditioner ? A->dinv.get() : nullptr,
                                     o.max_iterations, o.tolerance,
                                     o.abs_tolerance, w);
@@ -689,8 +699,7 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
     hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, S);
     hipLaunchKernelGGL(k_zero_f64, gv, BS, 0, s, n, x);
     // r = b - A*0 = b, through the residual kernel for its norm
-    launch_residual(s, sh, n, A->row_ptr.get(), A->col.get(), A->val.get(), x,
-                    b, r64, S, w.part1);
+    launch_residual(s, sh, A, x, b, r64, S, w.part1);
     GS_LAUNCH_CHECK("k_residual");
     CgState h = read_state(ctx, S);
     const double bnorm = h.rnorm;
@@ -714,14 +723,12 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
         const long long cap = std::min<long long>(
             o.max_inner_iterations > 0 ? o.max_inner_iterations : o.max_iterations,
             o.max_iterations - inner_total);
-        const PcgResult ri = pcg<float>(ctx, sh, n, A->row_ptr.get(), A->col.get(),
-                                        A->val32.get(), r32, d32, dinv32, cap,
-                                        o.inner_tolerance, 0.0, wi);
+        const PcgResult ri = pcg<float>(ctx, sh32, A, A->val32.get(), r32, d32,
+                                        dinv32, cap, o.inner_tolerance, 0.0, wi);
         inner_total += ri.iterations + (ri.converged ? 1 : 0);
         spmv += ri.spmv;
         hipLaunchKernelGGL(k_axpy_f32, gv, BS, 0, s, n, d32, S, x);
-        launch_residual(s, sh, n, A->row_ptr.get(), A->col.get(), A->val.get(),
-                        x, b, r64, S, w.part1);
+        launch_residual(s, sh, A, x, b, r64, S, w.part1);
         GS_LAUNCH_CHECK("defect correction");
         ++spmv;
         ++outer;
@@ -738,6 +745,8 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
     st->outer_iterations = outer;
     st->spmv_count = spmv;
   }
+  if (dp) permute_scatter(s, n, x, dp, x_user);
+  (void)b_user;
   GS_HIP(hipStreamSynchronize(s));
   st->solve_time =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -746,9 +755,11 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
 double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   gs_csr *A = const_cast<gs_csr *>(Ac);
   const int n = (int)A->rows;
-  const Shape sh = shape_of(A);
   hipStream_t s = ctx->stream;
   if (n == 0 || reps <= 0) return 0.0;
+  ensure_idx16(ctx, A);
+  const bool fp32 = precision != GS_PREC_FP64;
+  const Shape sh = shape_of(A, fp32);
   Work w;
   carve_work(ctx_scratch(ctx, work_bytes(n, sizeof(double))), n, sizeof(double), w);
   hipLaunchKernelGGL(k_state_reset, 1, 1, 0, s, w.st);
@@ -756,26 +767,22 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   GS_HIP(hipEventCreate(&e0));
   GS_HIP(hipEventCreate(&e1));
   float ms = 0.f;
-  if (precision == GS_PREC_FP64) {
+  if (!fp32) {
     double *p = (double *)w.p, *q = (double *)w.q;
     hipLaunchKernelGGL(k_fill_pattern, grid_vec(n), BS, 0, s, n, p);
-    launch_cg_spmv<double>(s, sh, n, A->row_ptr.get(), A->col.get(),
-                           A->val.get(), p, q, w.st, w.part1);
+    launch_cg_spmv<double>(s, sh, A, A->val.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i)
-      launch_cg_spmv<double>(s, sh, n, A->row_ptr.get(), A->col.get(),
-                             A->val.get(), p, q, w.st, w.part1);
+      launch_cg_spmv<double>(s, sh, A, A->val.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e1, s));
   } else {
     ensure_f32(ctx, A);
     float *p = (float *)w.p, *q = (float *)w.q;
     hipLaunchKernelGGL(k_fill_pattern32, grid_vec(n), BS, 0, s, n, p);
-    launch_cg_spmv<float>(s, sh, n, A->row_ptr.get(), A->col.get(),
-                          A->val32.get(), p, q, w.st, w.part1);
+    launch_cg_spmv<float>(s, sh, A, A->val32.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e0, s));
     for (int i = 0; i < reps; ++i)
-      launch_cg_spmv<float>(s, sh, n, A->row_ptr.get(), A->col.get(),
-                            A->val32.get(), p, q, w.st, w.part1);
+      launch_cg_spmv<float>(s, sh, A, A->val32.get(), p, q, w.st, w.part1);
     GS_HIP(hipEventRecord(e1, s));
   }
   GS_LAUNCH_CHECK("time_spmv");

@@ -194,10 +194,11 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
  * for this matrix.  Default engine: CSR-stream (a workgroup stages a block of
  * 256/lanes rows' products in LDS with coalesced loads; lanes = threads per
  * row in the row sums, 1, 2, 4 or 8).  GS_SPMV_VECTOR selects the
- * lanes-per-row engine (lanes = 4, 8, 16 or 32).  lanes 0 = chosen from the
+ * lanes-per-row engine (lanes = 4, 8, 16 or 32); GS_SPMV_INDEX32 keeps
+ * 32-bit column indices where 16-bit offsets would fit.  lanes 0 = chosen from the
  * mean row length; workgroups 0 = default, at most 16384.  Results depend on
  * the shape only through floating-point summation order. */
-enum { GS_SPMV_VECTOR = 4 };
+enum { GS_SPMV_VECTOR = 4, GS_SPMV_INDEX32 = 8 };
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags);
 

@@ -111,6 +111,32 @@ def test_elastic_dirichlet_bitexact_and_fused(oracle):
     assert same_csr(Kf.to_scipy(), Kbo) and np.array_equal(ff.cpu().numpy(), fbo)
 
 
+def test_dirichlet_on_uploaded_matrix_bitexact(oracle):
+    # identity-numbered path: apply_dirichlet_bcs on a host CsMat
+    xyz, conn, mesh = mesh_pair(oracle, 3, 3, 2, 1000.0, 1000.0, 600.0, 0.05, 9)
+    mask, vals = oracle.thermal_dirichlet(xyz, 600.0)
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    Ko = oracle.assemble_thermal(xyz, conn, 3.0)
+    fo = oracle.assemble_thermal_load(xyz, conn, 1e-6)
+    Kbo, fbo = oracle.apply_dirichlet(Ko, fo, mask, vals)
+    Kbc, fbc = G.Assembler.apply_dirichlet_bcs(Ko, fo, dof)
+    assert same_csr(Kbc.to_scipy(), Kbo) and np.array_equal(fbc, fbo)
+
+
+def test_jacobi_and_elastic_spmv_caller_numbering(oracle):
+    xyz, conn, mesh = mesh_pair(oracle, 2, 3, 2, 1e6, 1e6, 6e5, 0.03, 10)
+    dof = G.DofManager(mesh.num_nodes(), 3)
+    K = G.Assembler.assemble_elasticity_stiffness_parallel(
+        mesh, dof, G.IsotropicElasticity(1e11, 0.25))
+    Ko = oracle.assemble_elastic(xyz, conn, 1e11, 0.25)
+    d = G.JacobiPreconditioner(K).diag_inv.cpu().numpy()
+    assert np.array_equal(d, oracle.jacobi_diag_inv(Ko))
+    x = np.random.default_rng(2).standard_normal(K.shape[0])
+    y = K.apply(x)
+    assert np.max(np.abs(y - Ko @ x)) <= 1e-13 * np.max(np.abs(Ko) @ np.abs(x))
+
+
 def test_assembly_deterministic():
     mesh = G.ImprovedMeshGenerator.generate_cube(8, 8, 8, 1000.0, 1000.0, 600.0)
     dof = G.DofManager(mesh.num_nodes(), 1)

@@ -98,6 +98,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tight", action="store_true")
     ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (counter "
+                         "calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -185,14 +188,30 @@ def main():
                                rel_l2_vs_closed_form=err)
             log(f"[bench] tight {prec}: {(ta+ts)*1e3:.2f} ms it={st.iterations} relL2={err:.2e}")
 
-    # dominant kernel: the CG SpMV (HIP events on the library's stream)
-    bytes64 = nnz * 12 + n * 20
-    bytes32 = nnz * 8 + n * 12
+    # dominant kernels: the CG SpMV in each precision, timed with HIP events on
+    # the library's own stream (the stream the kernels run on)
+    bytes64 = nnz * 12 + n * 20   # algorithmic: int32 indices, FP64 values/vectors
+    bytes32 = nnz * 8 + n * 12    # algorithmic: int32 indices, FP32 values/vectors
     ms64 = K.time_spmv("fp64", args.spmv_reps)
     ms32 = K.time_spmv("mixed", args.spmv_reps)
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
     log(f"[bench] SpMV fp64 {ms64*1e3:.1f} us {gbs64:.0f} GB/s; fp32 {ms32*1e3:.1f} us {gbs32:.0f} GB/s")
+    if args.copy_probe_mib > 0:
+        m = args.copy_probe_mib * 2 ** 20 // 8
+        src = torch.ones(m, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+    traffic32 = traffic64 = None
+    pmc_path = os.path.join(ROOT, "profiles", f"spmv_{args.config}_pmc.json")
+    if os.path.exists(pmc_path):
+        with open(pmc_path) as fh:
+            pmc = json.load(fh)
+        traffic32 = pmc.get("fp32", {}).get("hbm_bytes")
+        traffic64 = pmc.get("fp64", {}).get("hbm_bytes")
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -233,10 +252,15 @@ def main():
         "symbolic_s": t_symbolic,
         "upload_s": t_upload,
         "spmv": {"fp64_us": ms64 * 1e3, "fp64_GBs": gbs64, "fp64_bytes": bytes64,
-                 "fp32_us": ms32 * 1e3, "fp32_GBs": gbs32, "fp32_bytes": bytes32},
-        "roofline": {"bound": "hbm", "achieved": gbs64, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs64 / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_cg_spmv<8,double> (FP64 CSR SpMV + p.Ap)"},
+                 "fp64_frac": gbs64 / HBM_PEAK_GBS, "fp64_traffic": traffic64,
+                 "fp32_us": ms32 * 1e3, "fp32_GBs": gbs32, "fp32_bytes": bytes32,
+                 "fp32_frac": gbs32 / HBM_PEAK_GBS, "fp32_traffic": traffic32},
+        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": gbs32 / HBM_PEAK_GBS,
+                     "traffic": traffic32,
+                     "kernel": "k_cg_spmv FP32 (CG SpMV + p.Ap partials; dominant "
+                               "kernel of the mixed path)",
+                     "algorithmic_bytes": bytes32},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -340,6 +340,283 @@ __global__ void k_perm3(int64_t n_nodes, const int *__restrict__ perm,
   if (i < 3 * n_nodes) perm3[i] = 3 * perm[i / 3] + (int)(i % 3);
 }
 
+// ---------------------------------------------------------------------------
+// System plans: the Dirichlet elimination (src/fem/assembly.rs:168-230)
+// decided once on the pattern, so the numeric phase writes K_bc directly.
+// ---------------------------------------------------------------------------
+// Per row: kept entries (constrained row: its diagonal only) and entries of
+// the RHS lift (free row, constrained column).
+__global__ void k_plan_count(int64_t n, const int *__restrict__ rp,
+                             const int *__restrict__ col,
+                             const uint8_t *__restrict__ dir,
+                             int *__restrict__ kept, int *__restrict__ lift) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  if (dir[r]) {
+    kept[r] = 1;
+    lift[r] = 0;
+    return;
+  }
+  int a = 0, b = 0;
+  for (int k = rp[r]; k < rp[r + 1]; ++k) {
+    if (dir[col[k]]) ++b;
+    else ++a;
+  }
+  kept[r] = a;
+  lift[r] = b;
+}
+
+// K_bc columns, the slot destinations and the lift list.  Lift entries of a
+// row are numbered in ascending caller column order (the order in which the
+// reference subtracts them), found by repeated minimum search.
+__global__ void k_plan_fill(int64_t n, const int *__restrict__ rp,
+                            const int *__restrict__ col,
+                            const uint8_t *__restrict__ dir,
+                            const int *__restrict__ dperm,
+                            const int *__restrict__ brp, int *__restrict__ bcol,
+                            int *__restrict__ dst, const int *__restrict__ lptr,
+                            int *__restrict__ lcol, int *__restrict__ diag) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int k0 = rp[r], k1 = rp[r + 1];
+  int p = brp[r];
+  if (dir[r]) {
+    bcol[p] = (int)r;
+    diag[r] = p;
+    for (int k = k0; k < k1; ++k) dst[k] = -1;
+    return;
+  }
+  int dg = -1;
+  for (int k = k0; k < k1; ++k) {
+    const int j = col[k];
+    if (!dir[j]) {
+      bcol[p] = j;
+      dst[k] = p;
+      if (j == r) dg = p;
+      ++p;
+    }
+  }
+  diag[r] = dg;
+  int q = lptr[r], last = -1;
+  for (;;) {
+    int best = INT_MAX, bk = -1;
+    for (int k = k0; k < k1; ++k) {
+      const int j = col[k];
+      if (dir[j]) {
+        const int oc = dperm ? dperm[j] : j;
+        if (oc > last && oc < best) {
+          best = oc;
+          bk = k;
+        }
+      }
+    }
+    if (bk < 0) break;
+    dst[bk] = -2 - q;
+    lcol[q] = col[bk];
+    ++q;
+    last = best;
+  }
+}
+
+// Numeric phase, thermal: the segmented sum of k_gather_thermal per node
+// slot, written where the plan sends it.
+__global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
+                                     const int *__restrict__ contrib,
+                                     const double *__restrict__ Ke,
+                                     const int *__restrict__ dst,
+                                     const int *__restrict__ pos,
+                                     double *__restrict__ val,
+                                     double *__restrict__ sval,
+                                     float *__restrict__ sval32,
+                                     double *__restrict__ liftv) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int d = dst[s];
+  if (d == -1) return;
+  const int b0 = cptr[s], b1 = cptr[s + 1];
+  double v = 0.0;
+  for (int k = b0; k < b1; ++k) {
+    const int t = contrib[k];
+    const int e = t / 100;
+    const int ab = t - 100 * e;
+    const int a = ab / 10, b = ab - 10 * (ab / 10);
+    const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
+    const double x = Ke[(int64_t)e * 55 + p];
+    v = (k == b0) ? x : v + x;
+  }
+  if (d >= 0) {
+    val[d] = v;
+    const int e = pos[d];
+    sval[e] = v;
+    sval32[e] = (float)v;
+  } else {
+    liftv[-2 - d] = v;
+  }
+}
+
+// Numeric phase, elastic: the 3x3 block sums of k_gather_elastic per node
+// slot, each DOF entry written where the plan sends it.
+__global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
+                                     const int *__restrict__ slot_row,
+                                     const int *__restrict__ cptr,
+                                     const int *__restrict__ contrib,
+                                     const double *__restrict__ Ke,
+                                     const int *__restrict__ dst,
+                                     const int *__restrict__ pos,
+                                     double *__restrict__ val,
+                                     double *__restrict__ sval,
+                                     float *__restrict__ sval32,
+                                     double *__restrict__ liftv) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nnz) return;
+  const int r = slot_row[s];
+  const int base = rp[r], deg = rp[r + 1] - base;
+  const int j = (int)(s - base);
+  int dd[9];
+  bool any = false;
+#pragma unroll
+  for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+    for (int cj = 0; cj < 3; ++cj) {
+      const int d = dst[9ll * base + 3ll * ci * deg + 3ll * j + cj];
+      dd[3 * ci + cj] = d;
+      any |= d != -1;
+    }
+  if (!any) return;
+  const int b0 = cptr[s], b1 = cptr[s + 1];
+  double v[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) v[m] = 0.0;
+  for (int k = b0; k < b1; ++k) {
+    const int t = contrib[k];
+    const int e = t / 100;
+    const int ab = t - 100 * e;
+    const int a = ab / 10, b = ab - 10 * (ab / 10);
+    const double *Kr = Ke + (int64_t)e * 900 + (3 * a) * 30 + 3 * b;
+#pragma unroll
+    for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+      for (int cj = 0; cj < 3; ++cj) {
+        const double x = Kr[ci * 30 + cj];
+        v[3 * ci + cj] = (k == b0) ? x : v[3 * ci + cj] + x;
+      }
+  }
+#pragma unroll
+  for (int m = 0; m < 9; ++m) {
+    const int d = dd[m];
+    if (d >= 0) {
+      val[d] = v[m];
+      const int e = pos[d];
+      sval[e] = v[m];
+      sval32[e] = (float)v[m];
+    } else if (d <= -2) {
+      liftv[-2 - d] = v[m];
+    }
+  }
+}
+
+// max |K_ii| over all diagonal entries, recomputed from the diagonal node
+// slots in the gather's summation order (bit pattern of a non-negative
+// double: integer max is order independent, hence exact).
+template <int DPN>
+__global__ void k_sys_diagmax(int64_t n_nodes, const int *__restrict__ diag_slot,
+                              const int *__restrict__ cptr,
+                              const int *__restrict__ contrib,
+                              const double *__restrict__ Ke,
+                              unsigned long long *__restrict__ maxbits) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  unsigned long long mine = 0;
+  const int s = r < n_nodes ? diag_slot[r] : -1;
+  if (s >= 0) {
+    double v[DPN];
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) v[c] = 0.0;
+    const int b0 = cptr[s], b1 = cptr[s + 1];
+    for (int k = b0; k < b1; ++k) {
+      const int t = contrib[k];
+      const int e = t / 100;
+      const int a = (t - 100 * e) / 10;  // diagonal slot: b == a
+#pragma unroll
+      for (int c = 0; c < DPN; ++c) {
+        const double x = DPN == 1 ? Ke[(int64_t)e * 55 + tri_index(a, a)]
+                                  : Ke[(int64_t)e * 900 + (3 * a + c) * 30 + 3 * a + c];
+        v[c] = (k == b0) ? x : v[c] + x;
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < DPN; ++c) {
+      const double a = fabs(v[c]);
+      if (a == a) {
+        const unsigned long long bits = (unsigned long long)__double_as_longlong(a);
+        mine = bits > mine ? bits : mine;
+      }
+    }
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off, 64);
+    mine = o > mine ? o : mine;
+  }
+  if ((threadIdx.x & 63) == 0 && mine != 0) atomicMax(maxbits, mine);
+}
+
+// Per DOF row (internal numbering): load (k_gather_load's sum), then either
+// the constrained row (c e_i, f = g c) or the lift f -= K_ji g_i in the
+// plan's order; the RHS lands at caller index dperm[r]; Jacobi inverse
+// diagonal (src/linalg/preconditioner.rs:37-51) in SELL order.
+template <int DPN>
+__global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
+                          const int *__restrict__ cptr,
+                          const int *__restrict__ contrib,
+                          const double *__restrict__ Fe,
+                          const uint8_t *__restrict__ dir,
+                          const double *__restrict__ g,
+                          const int *__restrict__ lptr,
+                          const int *__restrict__ lcol,
+                          const double *__restrict__ liftv,
+                          const int *__restrict__ diag,
+                          const int *__restrict__ pos,
+                          const int *__restrict__ siperm,
+                          const unsigned long long *__restrict__ maxbits,
+                          double *__restrict__ val, double *__restrict__ sval,
+                          float *__restrict__ sval32,
+                          const int *__restrict__ dperm,
+                          double *__restrict__ fbc, double *__restrict__ sdinv,
+                          float *__restrict__ sdinv32) {
+  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const int64_t node = r / DPN;
+  const int c = (int)(r - DPN * node);
+  double f = 0.0;
+  const int s = diag_slot[node];
+  if (s >= 0) {
+    for (int k = cptr[s]; k < cptr[s + 1]; ++k) {
+      const int t = contrib[k];
+      const int e = t / 100;
+      const int a = (t - 100 * e) / 10;
+      f += Fe[((int64_t)e * 10 + a) * DPN + c];
+    }
+  }
+  double dv = 0.0;
+  const int d = diag[r];
+  if (dir[r]) {
+    const double cv = __longlong_as_double((long long)*maxbits);
+    val[d] = cv;
+    const int e = pos[d];
+    sval[e] = cv;
+    sval32[e] = (float)cv;
+    f = g[r] * cv;
+    dv = cv;
+  } else {
+    for (int q = lptr[r]; q < lptr[r + 1]; ++q) f -= liftv[q] * g[lcol[q]];
+    dv = d >= 0 ? val[d] : 0.0;
+  }
+  fbc[dperm ? dperm[r] : r] = f;
+  const double di = fabs(dv) > 1e-14 ? 1.0 / dv : 1.0;
+  const int si = siperm[r];
+  sdinv[si] = di;
+  sdinv32[si] = (float)di;
+}
+
 template <class F> void scan_inclusive(gs_ctx *ctx, const int *in, int *out,
                                        int64_t n, F op) {
   size_t bytes = 0;
@@ -578,6 +855,123 @@ void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map
   if (n == 0) return;
   hipLaunchKernelGGL(k_perm_scatter, nblk(n), BS, 0, s, n, src, map, dst);
   GS_LAUNCH_CHECK("k_perm_scatter");
+}
+
+std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
+                                        const gs_dofmap *d) {
+  for (auto &p : m->plans)
+    if (p->dof_uid == d->uid) return p;
+  hipStream_t s = ctx->stream;
+  const int dpn = d->dpn;
+  if (dpn == 1) {
+    build_node_pattern(ctx, m);
+  } else {
+    build_dof3_pattern(ctx, m);
+    build_perm3(ctx, m);
+  }
+  const int *rp = dpn == 1 ? m->pat.row_ptr.get() : m->dof3.row_ptr.get();
+  const int *col = dpn == 1 ? m->pat.col.get() : m->dof3.col.get();
+  const int64_t nnzK = dpn == 1 ? m->pat.nnz : m->dof3.nnz;
+  const DArr<int> &dperm = dpn == 1 ? m->perm : m->perm3;
+  const int64_t n = d->n_dofs;
+  auto P = std::make_shared<SystemPlan>();
+  P->dof_uid = d->uid;
+  P->dpn = dpn;
+  P->n = n;
+  P->dir = dalloc<uint8_t>(n);
+  P->g = dalloc<double>(n);
+  permute_gather_u8(s, n, d->is_dir.get(), dperm.get(), P->dir.get());
+  permute_gather(s, n, d->values.get(), dperm.get(), P->g.get());
+  P->row_ptr = dalloc<int>(n + 1);
+  P->lift_ptr = dalloc<int>(n + 1);
+  GS_HIP(hipMemsetAsync(P->row_ptr.get(), 0, sizeof(int), s));
+  GS_HIP(hipMemsetAsync(P->lift_ptr.get(), 0, sizeof(int), s));
+  if (n > 0) {
+    auto kept = dalloc<int>(n), lift = dalloc<int>(n);
+    hipLaunchKernelGGL(k_plan_count, nblk(n), BS, 0, s, n, rp, col, P->dir.get(),
+                       kept.get(), lift.get());
+    GS_LAUNCH_CHECK("k_plan_count");
+    scan_inclusive(ctx, kept.get(), P->row_ptr.get() + 1, n, rocprim::plus<int>());
+    scan_inclusive(ctx, lift.get(), P->lift_ptr.get() + 1, n, rocprim::plus<int>());
+    P->nnz = read_int(ctx, P->row_ptr.get() + n);
+    P->n_lift = read_int(ctx, P->lift_ptr.get() + n);
+  }
+  P->col = dalloc<int>(P->nnz);
+  P->dst = dalloc<int>(nnzK);
+  P->lift_col = dalloc<int>(P->n_lift);
+  P->diag = dalloc<int>(n);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_plan_fill, nblk(n), BS, 0, s, n, rp, col, P->dir.get(),
+                       dperm.get(), P->row_ptr.get(), P->col.get(), P->dst.get(),
+                       P->lift_ptr.get(), P->lift_col.get(), P->diag.get());
+    GS_LAUNCH_CHECK("k_plan_fill");
+  }
+  gs_csr pat;
+  pat.ctx = ctx;
+  pat.rows = pat.cols = n;
+  pat.nnz = P->nnz;
+  pat.row_ptr = P->row_ptr;
+  pat.col = P->col;
+  pat.dperm = dperm;
+  P->sell = build_sell_pattern(ctx, &pat, false);
+  GS_HIP(hipStreamSynchronize(s));
+  m->plans.push_back(P);
+  if (m->plans.size() > 4) m->plans.erase(m->plans.begin());
+  return P;
+}
+
+void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
+                   const double *Ke, double *val, double *sval, float *sval32,
+                   double *liftv) {
+  const NodePattern &p = m->pat;
+  if (p.nnz == 0) return;
+  const int *pos = P.sell->pos.get();
+  if (P.dpn == 1)
+    hipLaunchKernelGGL(k_sys_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
+                       p.contrib_ptr.get(), p.contrib.get(), Ke, P.dst.get(), pos,
+                       val, sval, sval32, liftv);
+  else
+    hipLaunchKernelGGL(k_sys_gather_elastic, nblk(p.nnz), BS, 0, s, p.nnz,
+                       p.row_ptr.get(), p.slot_row.get(), p.contrib_ptr.get(),
+                       p.contrib.get(), Ke, P.dst.get(), pos, val, sval, sval32,
+                       liftv);
+  GS_LAUNCH_CHECK("k_sys_gather");
+}
+
+void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
+                   const double *Ke, const double *Fe, double *val, double *sval,
+                   float *sval32, const double *liftv, const int *dperm,
+                   double *fbc, double *sdinv, float *sdinv32) {
+  if (P.n == 0) return;
+  hipStream_t s = ctx->stream;
+  const NodePattern &p = m->pat;
+  auto maxbits = palloc<unsigned long long>(ctx->pool, 1);
+  const double one = 1.0;
+  unsigned long long one_bits;
+  memcpy(&one_bits, &one, sizeof(one));
+  hipLaunchKernelGGL(k_set_u64, 1, 1, 0, s, maxbits.get(), one_bits);
+  const int *sip = P.sell->siperm.get(), *pos = P.sell->pos.get();
+  if (P.dpn == 1) {
+    hipLaunchKernelGGL((k_sys_diagmax<1>), nblk(m->n_nodes), BS, 0, s, m->n_nodes,
+                       p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
+                       maxbits.get());
+    hipLaunchKernelGGL((k_sys_rhs<1>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
+                       p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
+                       P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
+                       P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
+                       dperm, fbc, sdinv, sdinv32);
+  } else {
+    hipLaunchKernelGGL((k_sys_diagmax<3>), nblk(m->n_nodes), BS, 0, s, m->n_nodes,
+                       p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
+                       maxbits.get());
+    hipLaunchKernelGGL((k_sys_rhs<3>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
+                       p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
+                       P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
+                       P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
+                       dperm, fbc, sdinv, sdinv32);
+  }
+  GS_LAUNCH_CHECK("k_sys_rhs");
+  GS_HIP(hipStreamSynchronize(s));
 }
 
 }  // namespace gs

@@ -26,6 +26,7 @@ ARCH = os.environ.get("GEOFEM_ARCH", "gfx950")
 SOURCES = [
     ("elements.hip", ["-ffp-contract=off"]),
     ("assembly.hip", ["-ffp-contract=off"]),
+    ("operator.hip", []),
     ("solver.hip", []),
     ("capi.cpp", []),
 ]

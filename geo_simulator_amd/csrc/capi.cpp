@@ -1,6 +1,7 @@
 // C ABI of libgeofem (include/geofem.h): handle management, argument
 // validation, and the assemble/solve pipelines built from the device kernels.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -112,6 +113,57 @@ gs_csr *assemble_elastic_K(gs_ctx *ctx, gs_mesh *m, double E, double nu,
   gs_csr *K = new_csr(ctx, 3 * m->n_nodes, 3 * m->n_nodes, m->dof3.nnz,
                       m->dof3.row_ptr, m->dof3.col, val);
   K->dperm = m->perm3;
+  return K;
+}
+
+struct SystemParams {
+  bool elastic = false;
+  double k = 0.0, Q = 0.0;
+  const double *k_elem = nullptr;
+  double E = 0.0, nu = 0.0, rho = 0.0, g[3] = {0.0, 0.0, 0.0};
+  const double *E_elem = nullptr;
+};
+
+// Element kernel + the cached system plan's numeric phase: K_bc (CSR and its
+// SELL view, FP64 and FP32), the Jacobi inverse diagonal and f_bc (caller
+// numbering) without materialising the full K.  Results are bit-identical to
+// assembling K, the load, and then eliminating.
+gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
+                        const SystemParams &sp, double *f_bc) {
+  auto P = system_plan(ctx, m, dof);
+  hipStream_t s = ctx->stream;
+  const auto &pool = ctx->pool;
+  const int64_t E = m->n_elems;
+  const int dpn = sp.elastic ? 3 : 1;
+  auto Ke = palloc<double>(pool, E * (sp.elastic ? 900 : 55));
+  auto Fe = palloc<double>(pool, E * 10 * dpn);
+  auto flag = palloc<int>(pool, 1);
+  GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+  if (sp.elastic)
+    launch_elastic_elements(s, E, m->xyz.get(), m->conn.get(), sp.E, sp.nu, sp.E_elem,
+                            sp.rho, sp.g, true, Ke.get(), Fe.get(), flag.get());
+  else
+    launch_thermal_elements(s, E, m->xyz.get(), m->conn.get(), sp.k, sp.k_elem, sp.Q,
+                            true, Ke.get(), Fe.get(), flag.get());
+  check_flags(read_flag_and_clear(ctx, flag.get()));
+  const SellPattern &S = *P->sell;
+  auto val = palloc<double>(pool, P->nnz);
+  auto sval = palloc<double>(pool, S.entries);
+  auto sval32 = palloc<float>(pool, S.entries);
+  auto liftv = palloc<double>(pool, P->n_lift);
+  auto sdinv = palloc<double>(pool, P->n);
+  auto sdinv32 = palloc<float>(pool, P->n);
+  const DArr<int> &dperm = sp.elastic ? m->perm3 : m->perm;
+  system_gather(s, m, *P, Ke.get(), val.get(), sval.get(), sval32.get(), liftv.get());
+  system_finish(ctx, m, *P, Ke.get(), Fe.get(), val.get(), sval.get(), sval32.get(),
+                liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get());
+  gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, val);
+  K->dperm = dperm;
+  K->sell = P->sell;
+  K->sval = sval;
+  K->sval32 = sval32;
+  K->sdinv = sdinv;
+  K->sdinv32 = sdinv32;
   return K;
 }
 
@@ -248,6 +300,7 @@ void gs_ctx_destroy(gs_ctx *ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->ws = DArr<char>();
+  ctx->pool.reset();  // frees the cached blocks; live arrays free their own
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
@@ -435,8 +488,10 @@ gs_status gs_dofmap_create(gs_ctx *ctx, int64_t n_nodes, int dpn,
       vals[i] = (mask[i] && values) ? values[i] : 0.0;
       fixed += mask[i];
     }
+    static std::atomic<uint64_t> next_uid{1};
     std::unique_ptr<gs_dofmap> d(new gs_dofmap());
     d->ctx = ctx;
+    d->uid = next_uid.fetch_add(1);
     d->n_nodes = n_nodes;
     d->dpn = dpn;
     d->n_dofs = nd;
@@ -564,10 +619,11 @@ gs_status gs_assemble_thermal_system(gs_ctx *ctx, gs_mesh *mesh,
     require(dof->dpn == 1, GS_ERR_BAD_DOFS, "thermal assembly needs 1 DOF per node");
     require(f_bc || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null output");
     set_device(ctx);
-    auto f = dalloc<double>(mesh->n_nodes);
-    std::unique_ptr<gs_csr> K(
-        assemble_thermal_K(ctx, mesh, k, k_elem, true, Q, f.get()));
-    *K_bc = eliminate_caller_rhs(ctx, K.get(), f.get(), false, dof, f_bc);
+    SystemParams sp;
+    sp.k = k;
+    sp.k_elem = k_elem;
+    sp.Q = Q;
+    *K_bc = assemble_system(ctx, mesh, dof, sp, f_bc);
   });
 }
 
@@ -589,10 +645,14 @@ gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
       check_material(E, nu);
     }
     set_device(ctx);
-    auto f = dalloc<double>(3 * mesh->n_nodes);
-    std::unique_ptr<gs_csr> K(
-        assemble_elastic_K(ctx, mesh, E, nu, E_elem, true, rho, g, f.get()));
-    *K_bc = eliminate_caller_rhs(ctx, K.get(), f.get(), false, dof, f_bc);
+    SystemParams sp;
+    sp.elastic = true;
+    sp.E = E;
+    sp.nu = nu;
+    sp.E_elem = E_elem;
+    sp.rho = rho;
+    for (int c = 0; c < 3; ++c) sp.g[c] = g[c];
+    *K_bc = assemble_system(ctx, mesh, dof, sp, f_bc);
   });
 }
 
@@ -737,7 +797,7 @@ gs_status gs_cg_opts_default(gs_cg_opts *o) {
   o->abs_tolerance = 1e-12;
   o->use_preconditioner = 1;
   o->precision = GS_PREC_FP64;
-  o->inner_tolerance = 1e-4;
+  o->inner_tolerance = 1e-2;
   o->max_inner_iterations = 0;
   o->max_outer_iterations = 50;
   return GS_OK;
@@ -800,8 +860,11 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags) {
   if (!A) return GS_ERR_INVALID_ARG;
-  if (flags & ~(GS_SPMV_VECTOR | GS_SPMV_INDEX32)) return GS_ERR_INVALID_ARG;
+  if (flags & ~(GS_SPMV_VECTOR | GS_SPMV_INDEX32 | GS_SPMV_CSR))
+    return GS_ERR_INVALID_ARG;
+  const bool csr = (flags & GS_SPMV_CSR) != 0;
   const bool vec = (flags & GS_SPMV_VECTOR) != 0;
+  if (!csr && (vec || lanes != 0)) return GS_ERR_INVALID_ARG;  // SELL: no lanes
   const bool ok = lanes == 0 ||
                   (vec ? (lanes == 4 || lanes == 8 || lanes == 16 || lanes == 32)
                        : (lanes == 1 || lanes == 2 || lanes == 4 || lanes == 8));

@@ -4,9 +4,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
 #include "../../include/geofem.h"
 
@@ -49,6 +52,55 @@ template <class T> DArr<T> dalloc(int64_t n) {
   return a;
 }
 
+// Device blocks kept for reuse by exact byte size (one pool per context), so
+// repeated assemblies and solves of one problem stop calling hipMalloc.  A
+// block returns here when its last DArr releases it; the bytes cached are
+// capped, and the rest is freed.
+struct Pool {
+  std::mutex m;
+  std::multimap<size_t, void *> free;
+  size_t cached = 0;
+  size_t cap = (size_t)16 << 30;
+  ~Pool() {
+    for (auto &kv : free) (void)hipFree(kv.second);
+  }
+};
+
+template <class T> DArr<T> palloc(const std::shared_ptr<Pool> &pool, int64_t n) {
+  const size_t bytes = (size_t)(n > 0 ? n : 1) * sizeof(T);
+  void *ptr = nullptr;
+  {
+    std::lock_guard<std::mutex> g(pool->m);
+    auto it = pool->free.find(bytes);
+    if (it != pool->free.end()) {
+      ptr = it->second;
+      pool->free.erase(it);
+      pool->cached -= bytes;
+    }
+  }
+  if (!ptr) {
+    hipError_t e = hipMalloc(&ptr, bytes);
+    if (e != hipSuccess)
+      throw Error(GS_ERR_OOM, "hipMalloc(" + std::to_string(bytes) +
+                                  " bytes): " + hipGetErrorString(e));
+  }
+  std::weak_ptr<Pool> wp = pool;
+  DArr<T> a;
+  a.p = std::shared_ptr<T>((T *)ptr, [wp, bytes](T *q) {
+    if (auto pl = wp.lock()) {
+      std::lock_guard<std::mutex> g(pl->m);
+      if (pl->cached + bytes <= pl->cap) {
+        pl->free.emplace(bytes, (void *)q);
+        pl->cached += bytes;
+        return;
+      }
+    }
+    (void)hipFree(q);
+  });
+  a.n = n;
+  return a;
+}
+
 // Node-level CSR pattern of a Tet10 mesh plus the slot-sorted contributor
 // list that drives the deterministic segmented-sum assembly.
 struct NodePattern {
@@ -70,6 +122,39 @@ struct DofPattern3 {
   DArr<int> col;     // nnz
 };
 
+// SELL-64 view of a square matrix (layout in sell.hpp): the solver's operator
+// format.  SELL row s is CSR row sperm[s] and caller index vperm[s]; columns
+// are SELL rows.  Shared by every matrix with the same pattern and numbering.
+struct SellPattern {
+  int64_t n = 0, nnz = 0, nslices = 0, nchunks = 0, entries = 0;
+  bool idx16 = true;
+  DArr<int> sperm, siperm, vperm;  // n
+  DArr<int> cb;                    // nslices + 1 chunk offsets
+  DArr<short> off;                 // entries (16-bit variant)
+  DArr<int> col;                   // entries (32-bit variant)
+  DArr<int> pos;                   // nnz: CSR nonzero -> SELL entry
+};
+
+// Everything about a Dirichlet-eliminated system that depends only on the
+// mesh and the DOF map, built once (cached on the mesh): the K_bc pattern in
+// the internal numbering, its SELL view, and for every DOF slot of the full K
+// where its value goes -- a K_bc nonzero, an entry of the RHS lift
+// f_j -= K_ji g_i, or nowhere.  The numeric phase then writes K_bc directly.
+struct SystemPlan {
+  uint64_t dof_uid = 0;
+  int dpn = 1;
+  int64_t n = 0, nnz = 0, n_lift = 0;
+  DArr<int> row_ptr, col;            // K_bc pattern, internal numbering
+  std::shared_ptr<SellPattern> sell; // SELL view of that pattern
+  DArr<int> dst;      // full-K DOF slot -> K_bc nonzero (>= 0), lift entry
+                      // q (as -2 - q), or -1
+  DArr<int> lift_ptr; // n + 1: row r's lift entries, ascending caller column
+  DArr<int> lift_col; // n_lift: internal column of each lift entry
+  DArr<int> diag;     // n: K_bc nonzero holding row r's diagonal (-1: none)
+  DArr<uint8_t> dir;  // n, internal numbering
+  DArr<double> g;     // n, internal numbering
+};
+
 }  // namespace gs
 
 struct gs_ctx {
@@ -79,6 +164,7 @@ struct gs_ctx {
   void *pinned = nullptr;  // small pinned staging buffer (4 KiB)
   gs::DArr<char> ws;       // solver scratch, grows on demand
   size_t ws_bytes = 0;
+  std::shared_ptr<gs::Pool> pool = std::make_shared<gs::Pool>();
 };
 
 struct gs_mesh {
@@ -94,10 +180,12 @@ struct gs_mesh {
   gs::NodePattern pat;
   bool has_dof3 = false;
   gs::DofPattern3 dof3;
+  std::vector<std::shared_ptr<gs::SystemPlan>> plans;  // most recent last
 };
 
 struct gs_dofmap {
   gs_ctx *ctx = nullptr;
+  uint64_t uid = 0;  // unique per created dofmap (keys the system plans)
   int64_t n_nodes = 0;
   int dpn = 1;
   int64_t n_dofs = 0;
@@ -116,12 +204,17 @@ struct gs_csr {
   // identity).  Vectors cross the C ABI in caller numbering.
   gs::DArr<int> dperm;
   // lazily derived, cached (solver)
-  int idx16 = 0;              // 16-bit column offsets: 0 unknown, 1 yes, -1 no
+  int idx16 = 0;              // 16-bit CSR column offsets: 0 unknown, 1 yes, -1 no
   gs::DArr<short> off16;      // nnz
   gs::DArr<int> anchor;       // one per 1024 nonzeros
-  gs::DArr<float> val32;
-  gs::DArr<double> dinv;
-  gs::DArr<float> dinv32;
+  gs::DArr<double> dinv;      // CSR order
+  gs::DArr<float> val32;      // CSR order, FP32 (CSR engine timing only)
+  std::shared_ptr<gs::SellPattern> sell;  // SELL view (square matrices)
+  bool sell_force32 = false;              // the view was built with int32 forced
+  gs::DArr<double> sval;                  // SELL values, FP64
+  gs::DArr<float> sval32;                 // SELL values, FP32 (mixed precision)
+  gs::DArr<double> sdinv;                 // Jacobi inverse diagonal, SELL order
+  gs::DArr<float> sdinv32;
   // SpMV launch shape (0 = automatic); see gs_csr_set_spmv_shape
   int spmv_lanes = 0, spmv_grid = 0, spmv_flags = 0;
 };
@@ -173,6 +266,29 @@ void permute_gather_u8(hipStream_t s, int64_t n, const uint8_t *src,
                        const int *map, uint8_t *dst);
 void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map,
                      double *dst);
+
+// System plans and the fused numeric phase (assembly.hip).
+std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
+                                        const gs_dofmap *d);
+// values of K_bc (CSR and SELL, FP64 and FP32) and the lift values
+void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
+                   const double *Ke, double *val, double *sval, float *sval32,
+                   double *liftv);
+// c = max(1, max |K_ii|), constrained rows, RHS (load + lift, written at
+// caller index dperm[r]) and the Jacobi inverse diagonal in SELL order
+void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
+                   const double *Ke, const double *Fe, double *val, double *sval,
+                   float *sval32, const double *liftv, const int *dperm,
+                   double *fbc, double *sdinv, float *sdinv32);
+
+// ---- operator.hip ----
+std::shared_ptr<SellPattern> build_sell_pattern(gs_ctx *ctx, const gs_csr *A,
+                                                bool force32);
+// sval[pos[k]] = val[k], padding zero
+template <typename T>
+void sell_values(hipStream_t s, const SellPattern &P, const double *val, T *sval);
+void sell_jacobi(hipStream_t s, const SellPattern &P, const gs_csr *A, double *d64,
+                 float *d32);
 
 // ---- solver.hip ----
 void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv);

@@ -1,0 +1,144 @@
+// Sliced-ELL (SELL-64) SpMV engine for gfx950 (included by solver.hip).
+//
+// Layout.  Rows are cut into slices of 64, one wavefront per slice and one
+// lane per row.  A slice stores its rows' entries in chunks of 256: chunk c
+// holds entries 4c..4c+3 of every row, lane l's four entries contiguous at
+// 4l, so each lane fetches 16 (FP32) or 32 (FP64) bytes of values and 8 bytes
+// of indices per load and a wavefront reads one contiguous 1-2 KiB stretch.
+// A slice has as many chunks as its longest row needs; shorter rows are
+// padded with entries whose index is the pad marker (they add nothing, and
+// their values are never used, so they need no clearing).  To
+// keep padding low the rows are sorted by descending length inside windows
+// of 256 consecutive rows; the sort is a symmetric renumbering, so vectors
+// live in the same order and every lane writes its own row of y
+// contiguously.
+//
+// Indices: a 16-bit offset from the slice's first row (column =
+// 64*slice + off, pad = -32768) when every column fits, else int32 columns
+// (pad = -1).
+//
+// Scheduling.  Workgroups take blocks of 4 slices.  The blocks are split in
+// 8 contiguous ranges, one per XCD (workgroup i runs on XCD i mod 8), so the
+// rows an XCD works on at any moment are close together and the gathered
+// part of x stays in that XCD's L2.
+#pragma once
+
+namespace gs {
+namespace {
+
+constexpr int SELL_C = 64;        // rows per slice
+constexpr int SELL_CW = 4;        // entries per lane per chunk
+constexpr int SELL_CHUNK = 256;   // entries per chunk
+constexpr int SELL_WPB = 4;       // slices per workgroup (256 threads)
+constexpr int SELL_SIGMA = 256;   // length-sorting window (rows)
+constexpr int SELL_PAD16 = -32768;
+
+struct SellIdx16 {
+  const short *__restrict__ off;
+  static constexpr bool wide = false;
+};
+struct SellIdx32 {
+  const int *__restrict__ col;
+  static constexpr bool wide = true;
+};
+
+template <typename T> struct SellVec;
+template <> struct SellVec<float> {
+  typedef float t __attribute__((ext_vector_type(4)));
+};
+template <> struct SellVec<double> {
+  typedef double t __attribute__((ext_vector_type(4)));
+};
+
+// One lane's dot product over chunks [c0, c1) of its slice.
+template <typename T, class IDX>
+__device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
+                                       const T *__restrict__ x, int c0, int c1,
+                                       int lane, int base_row) {
+  using V = typename SellVec<T>::t;
+  T acc = (T)0;
+  int c = c0;
+  auto step = [&](const V &v, int j0, int j1, int j2, int j3) {
+    const int cc[4] = {j0, j1, j2, j3};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      // Branch-free: a pad entry loads the slice's first row (a valid,
+      // cached address) and its product is dropped by a select, so neither
+      // its (unwritten) value nor a non-finite x can reach the sum.
+      const bool ok = IDX::wide ? cc[j] >= 0 : cc[j] != SELL_PAD16;
+      const int col = ok ? (IDX::wide ? cc[j] : base_row + cc[j]) : base_row;
+      const T prod = v[j] * x[col];
+      acc += ok ? prod : (T)0;
+    }
+  };
+  for (; c + 2 <= c1; c += 2) {
+    const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
+    const size_t e1 = e0 + SELL_CHUNK;
+    V v0 = *(const V *)(val + e0);
+    V v1 = *(const V *)(val + e1);
+    if constexpr (IDX::wide) {
+      typedef int I4 __attribute__((ext_vector_type(4)));
+      const I4 i0 = *(const I4 *)(idx.col + e0);
+      const I4 i1 = *(const I4 *)(idx.col + e1);
+      step(v0, i0[0], i0[1], i0[2], i0[3]);
+      step(v1, i1[0], i1[1], i1[2], i1[3]);
+    } else {
+      typedef short S4 __attribute__((ext_vector_type(4)));
+      const S4 i0 = *(const S4 *)(idx.off + e0);
+      const S4 i1 = *(const S4 *)(idx.off + e1);
+      step(v0, i0[0], i0[1], i0[2], i0[3]);
+      step(v1, i1[0], i1[1], i1[2], i1[3]);
+    }
+  }
+  if (c < c1) {
+    const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
+    V v0 = *(const V *)(val + e0);
+    if constexpr (IDX::wide) {
+      typedef int I4 __attribute__((ext_vector_type(4)));
+      const I4 i0 = *(const I4 *)(idx.col + e0);
+      step(v0, i0[0], i0[1], i0[2], i0[3]);
+    } else {
+      typedef short S4 __attribute__((ext_vector_type(4)));
+      const S4 i0 = *(const S4 *)(idx.off + e0);
+      step(v0, i0[0], i0[1], i0[2], i0[3]);
+    }
+  }
+  return acc;
+}
+
+// Calls epi(row, (A x)[row]) for every row, from the row's lane.  Every
+// thread of the workgroup returns (no early exit), so callers may follow
+// with workgroup-wide reductions.
+template <typename T, class IDX, class Epi>
+__device__ __forceinline__ void sell_rows(int n, int nslices,
+                                          const int *__restrict__ cb,
+                                          const IDX idx,
+                                          const T *__restrict__ val,
+                                          const T *__restrict__ x, Epi &epi) {
+  const int nb = (nslices + SELL_WPB - 1) / SELL_WPB;
+  const int G = gridDim.x, i = blockIdx.x;
+  int first, stride, last;
+  if (G >= 8) {
+    const int xcd = i & 7, j = i >> 3;
+    const int gx = (G >> 3) + (xcd < (G & 7) ? 1 : 0);
+    first = (int)((int64_t)nb * xcd / 8) + j;
+    last = (int)((int64_t)nb * (xcd + 1) / 8);
+    stride = gx;
+  } else {
+    first = i;
+    last = nb;
+    stride = G;
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int b = first; b < last; b += stride) {
+    const int s = b * SELL_WPB + w;
+    if (s < nslices) {
+      const int row = s * SELL_C + lane;
+      const T acc = sell_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C);
+      if (row < n) epi(row, acc);
+    }
+  }
+}
+
+}  // namespace
+}  // namespace gs

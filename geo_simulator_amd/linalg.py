@@ -163,7 +163,7 @@ class ConjugateGradient:
         self.abs_tolerance = 1e-12
         self.use_preconditioner = True
         self.precision = "fp64"
-        self.inner_tolerance = 1e-4
+        self.inner_tolerance = 1e-2
         self.max_inner_iterations = 0
         self.max_outer_iterations = 50
 

@@ -152,9 +152,11 @@ typedef struct {
   double abs_tolerance;      /* default 1e-12 */
   int32_t use_preconditioner;/* default 1 (Jacobi) */
   int32_t precision;         /* gs_precision, default GS_PREC_FP64 */
-  double inner_tolerance;    /* mixed: inner FP32 relative tolerance */
-  int64_t max_inner_iterations; /* mixed: cap per inner solve */
-  int64_t max_outer_iterations; /* mixed: cap on defect-correction steps */
+  double inner_tolerance;    /* mixed: FP64 residual replacement once the FP32
+                                recursive residual fell by this factor (1e-2) */
+  int64_t max_inner_iterations; /* mixed: cap on FP32 passes between
+                                   replacements (0 = none) */
+  int64_t max_outer_iterations; /* mixed: cap on replacements (0 = 100) */
 } gs_cg_opts;
 
 /* SolverStats (src/linalg/solver.rs:5-20) plus mixed-precision counters. */
@@ -165,7 +167,7 @@ typedef struct {
   int32_t converged;
   int32_t pad_;
   double solve_time;         /* seconds, host wall clock */
-  int64_t outer_iterations;  /* mixed: defect-correction steps */
+  int64_t outer_iterations;  /* mixed: FP64 residual replacements */
   int64_t spmv_count;        /* SpMV launches issued */
 } gs_solver_stats;
 
@@ -190,15 +192,18 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
                                       double *grad_dev);
 
 /* ---- SpMV launch shape (no reference counterpart) ----
- * Override the launch shape of the SpMV kernels (and of the CG built on them)
- * for this matrix.  Default engine: CSR-stream (a workgroup stages a block of
- * 256/lanes rows' products in LDS with coalesced loads; lanes = threads per
- * row in the row sums, 1, 2, 4 or 8).  GS_SPMV_VECTOR selects the
- * lanes-per-row engine (lanes = 4, 8, 16 or 32); GS_SPMV_INDEX32 keeps
- * 32-bit column indices where 16-bit offsets would fit.  lanes 0 = chosen from the
- * mean row length; workgroups 0 = default, at most 16384.  Results depend on
- * the shape only through floating-point summation order. */
-enum { GS_SPMV_VECTOR = 4, GS_SPMV_INDEX32 = 8 };
+ * Override the SpMV engine and launch shape for this matrix.  Default engine
+ * (square matrices; the only one CG runs on): sliced ELL, SELL-64 -- a
+ * wavefront per 64-row slice, a lane per row, rows length-sorted within
+ * windows of 256, 16-bit column offsets where they fit; workgroups 0 = one
+ * per 4 slices, else a persistent grid of that many (at most 16384).
+ * GS_SPMV_INDEX32 keeps 32-bit column indices.  GS_SPMV_CSR selects the CSR
+ * engines for gs_csr_spmv and gs_time_spmv (always used for rectangular
+ * matrices): CSR-stream by default (lanes = threads per row in the row sums,
+ * 1, 2, 4 or 8), GS_SPMV_VECTOR for lanes-per-row (4, 8, 16 or 32); lanes 0 =
+ * chosen from the mean row length.  Results depend on the engine and shape
+ * only through floating-point summation order. */
+enum { GS_SPMV_VECTOR = 4, GS_SPMV_INDEX32 = 8, GS_SPMV_CSR = 16 };
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags);
 

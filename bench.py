@@ -1,19 +1,25 @@
 #!/usr/bin/env python3
-"""Benchmark: Tet10 thermal assemble + Jacobi-PCG on one MI355X.
+"""Benchmark: Tet10 assemble + Jacobi-PCG on one MI355X.
 
-Workload (default): steady -div(k grad T) = Q on a 1000 x 1000 x 600 km box,
+Headline workload: steady -div(k grad T) = Q on a 1000 x 1000 x 600 km box,
 6-tet Kuhn Tet10 mesh at nx=ny=64, nz=32 (786,432 elements, 1,081,665 DOFs),
 k=3, Q=1e-6, T=1280 at z=0 and T=0 at z=600.  Synthetic input (generated
 mesh), resident in HBM before timing.
 
-One step = numeric assembly (element kernels, deterministic CSR gather,
-Dirichlet elimination, load) + Jacobi-PCG to rel-res 1e-8 by the reference's
-criterion ||r|| < 1e-8 ||b||.  Timed for both the plain FP64 path and the
-mixed path (FP32 inner PCG inside FP64 defect correction).  The symbolic
-phase (CSR pattern + contributor lists) runs once per mesh in the warm-up and
-is reported separately.
+One step = numeric assembly (element kernels and the cached system plan's
+fused gather: K_bc values in CSR and SELL layouts, Dirichlet lift, load,
+Jacobi diagonal) + Jacobi-PCG to rel-res 1e-8 by the reference's criterion
+||r|| < 1e-8 ||b||.  Timed for the plain FP64 path and the mixed path (FP64
+flexible CG around an FP32 Jacobi-PCG recursion).  The symbolic phase (CSR
+pattern, contributor lists, system plan, SELL layout) runs once per mesh and
+DOF map in the warm-up and is reported separately.
 
-Prints ONE JSON line.  value = mixed-precision assemble+PCG seconds per step.
+Also reported (same JSON line): the elastic column at nx=ny=48, nz=24
+(1,383,123 DOFs), and the adjoint inversion gradients (thermal log k, elastic
+log E) with finite-difference checks.
+
+Prints ONE JSON line.  value = mixed-precision assemble+PCG seconds per step
+of the headline workload.
 """
 from __future__ import annotations
 
@@ -42,27 +48,9 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def thermal_setup(G, nx, ny, nz):
-    mesh = G.ImprovedMeshGenerator.generate_cube(nx, ny, nz, 1000.0, 1000.0, 600.0)
-    z = mesh.nodes[:, 2]
-    dof = G.DofManager(mesh.num_nodes(), 1)
-    bot = np.nonzero(np.abs(z) < 1.0)[0]
-    top = np.nonzero(np.abs(z - 600.0) < 1.0)[0]
-    dof.set_dirichlet_many(bot, 1280.0)
-    dof.set_dirichlet_many(top, 0.0)
-    exact = 1280.0 * (1 - z / 600.0) + (1e-6 / (2 * 3.0)) * z * (600.0 - z)
-    return mesh, dof, exact
-
-
-def run_step(G, torch, mesh, dof, cg):
-    t0 = time.perf_counter()
-    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+def sync():
+    import torch
     torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    x, st = cg.solve(K, f)
-    torch.cuda.synchronize()
-    t2 = time.perf_counter()
-    return K, f, x, st, t1 - t0, t2 - t1
 
 
 def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
@@ -87,61 +75,60 @@ def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
     return total, t_asm, per_iter
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", default="thermal_64x64x32", choices=sorted(CONFIGS))
-    ap.add_argument("--tol", type=float, default=1e-8)
-    ap.add_argument("--inner-tol", type=float, default=None)
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-tight", action="store_true")
-    ap.add_argument("--spmv-reps", type=int, default=50)
-    ap.add_argument("--copy-probe-mib", type=int, default=0,
-                    help="also run a plain device copy of this size (counter "
-                         "calibration under rocprofv3 --pmc)")
-    args = ap.parse_args()
+def make_cg(G, prec, tol, inner_tol=None, abs_tol=None, max_iter=200000):
+    cg = (G.ConjugateGradient().with_max_iterations(max_iter).with_tolerance(tol)
+          .with_precision(prec))
+    if abs_tol is not None:
+        cg.with_abs_tolerance(abs_tol)
+    if prec == "mixed" and inner_tol is not None:
+        cg.with_inner_tolerance(inner_tol)
+    return cg
 
-    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU: the assemble-and-solve path is built and "
-              "measured on a single MI355X (multi-GPU sharding is a design note "
-              "only); run with --gpus 1", file=sys.stderr)
-        sys.exit(2)
 
+def timed_steps(assemble, cg, steps, warmup):
+    """warm-up, then `steps` timed assemble+solve steps; returns wall/step,
+    mean assembly and solve seconds and the last (K, f, x, stats)."""
+    for _ in range(warmup):
+        K, f = assemble()
+        cg.solve(K, f)
+    sync()
+    asm_t, sol_t, last = [], [], None
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ta = time.perf_counter()
+        K, f = assemble()
+        sync()
+        tb = time.perf_counter()
+        x, st = cg.solve(K, f)
+        sync()
+        asm_t.append(tb - ta)
+        sol_t.append(time.perf_counter() - tb)
+        last = (K, f, x, st)
+    wall = (time.perf_counter() - t0) / steps
+    return wall, float(np.mean(asm_t)), float(np.mean(sol_t)), last
+
+
+def run_headline(G, args, nx, ny, nz):
     import torch
-    import geo_simulator_amd as G
-
-    if not torch.cuda.is_available():
-        print("no HIP device", file=sys.stderr)
-        sys.exit(1)
-    nx, ny, nz = CONFIGS[args.config]
-    log(f"[bench] config {args.config}: generating mesh")
-    mesh, dof, exact = thermal_setup(G, nx, ny, nz)
+    from geo_simulator_amd import workloads as W
+    log(f"[bench] thermal {nx}x{ny}x{nz}: generating mesh")
+    mesh, dof, exact = W.thermal_box(G, nx, ny, nz)
     n = mesh.num_nodes()
     ctx = G.default_context()
-
-    def make_cg(prec, tol):
-        cg = (G.ConjugateGradient().with_max_iterations(100000).with_tolerance(tol)
-              .with_precision(prec))
-        if prec == "mixed" and args.inner_tol is not None:
-            cg.with_inner_tolerance(args.inner_tol)
-        return cg
-
-    # upload + symbolic phase (once per mesh), untimed in the step
-    torch.cuda.synchronize()
+    sync()
     t0 = time.perf_counter()
     mesh.handle(ctx)
     dof.handle(ctx)
-    torch.cuda.synchronize()
+    sync()
     t_upload = time.perf_counter() - t0
+    assemble = lambda: G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
     t0 = time.perf_counter()
-    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
-    torch.cuda.synchronize()
+    K, f = assemble()
+    sync()
     t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
-    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
-    torch.cuda.synchronize()
+    K, f = assemble()
+    sync()
     t_second = time.perf_counter() - t0
     t_symbolic = max(0.0, t_first - t_second)
     nnz = K.nnz()
@@ -149,44 +136,29 @@ def main():
 
     results = {}
     for prec in ("fp64", "mixed"):
-        cg = make_cg(prec, args.tol)
-        for _ in range(args.warmup):
-            run_step(G, torch, mesh, dof, cg)
-        log(f"[bench] {prec}: warm-up done")
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        asm_t, sol_t, last = [], [], None
-        for _ in range(args.steps):
-            K, f, x, st, ta, ts = run_step(G, torch, mesh, dof, cg)
-            asm_t.append(ta)
-            sol_t.append(ts)
-            last = (K, f, x, st)
-        torch.cuda.synchronize()
-        wall = (time.perf_counter() - t0) / args.steps
-        K, f, x, st = last
-        xh = x.cpu().numpy()
-        err = float(np.linalg.norm(xh - exact) / np.linalg.norm(exact))
-        results[prec] = dict(step_s=wall, asm_s=float(np.mean(asm_t)),
-                             solve_s=float(np.mean(sol_t)), iterations=st.iterations,
+        cg = make_cg(G, prec, args.tol, args.inner_tol)
+        wall, ta, ts, (K, f, x, st) = timed_steps(assemble, cg, args.steps, args.warmup)
+        err = float(np.linalg.norm(x.cpu().numpy() - exact) / np.linalg.norm(exact))
+        results[prec] = dict(step_s=wall, asm_s=ta, solve_s=ts, iterations=st.iterations,
                              outer=st.outer_iterations, spmv=st.spmv_count,
                              rel_res=st.relative_residual, converged=st.converged,
                              rel_l2_vs_closed_form=err)
-        log(f"[bench] {prec}: {wall*1e3:.2f} ms/step  asm {np.mean(asm_t)*1e3:.2f} ms  "
-            f"solve {np.mean(sol_t)*1e3:.2f} ms  it={st.iterations} outer={st.outer_iterations} "
+        log(f"[bench] {prec}: {wall*1e3:.2f} ms/step  asm {ta*1e3:.2f} ms  "
+            f"solve {ts*1e3:.2f} ms  it={st.iterations} outer={st.outer_iterations} "
             f"relres={st.relative_residual:.2e} relL2={err:.2e}")
 
     # FP64-equivalent accuracy: both paths to a tight residual
     tight = {}
     if not args.no_tight:
         for prec in ("fp64", "mixed"):
-            cg = make_cg(prec, 1e-12)
-            run_step(G, torch, mesh, dof, cg)
-            K, f, x, st, ta, ts = run_step(G, torch, mesh, dof, cg)
+            cg = make_cg(G, prec, 1e-12, args.inner_tol)
+            wall, ta, ts, (K, f, x, st) = timed_steps(assemble, cg, 1, 1)
             err = float(np.linalg.norm(x.cpu().numpy() - exact) / np.linalg.norm(exact))
-            tight[prec] = dict(step_s=ta + ts, iterations=st.iterations,
+            tight[prec] = dict(step_s=wall, iterations=st.iterations,
                                outer=st.outer_iterations, rel_res=st.relative_residual,
                                rel_l2_vs_closed_form=err)
-            log(f"[bench] tight {prec}: {(ta+ts)*1e3:.2f} ms it={st.iterations} relL2={err:.2e}")
+            log(f"[bench] tight {prec}: {wall*1e3:.2f} ms it={st.iterations} relL2={err:.2e}")
+        tight["speedup_mixed_vs_fp64"] = tight["fp64"]["step_s"] / tight["mixed"]["step_s"]
 
     # dominant kernels: the CG SpMV in each precision, timed with HIP events on
     # the library's own stream (the stream the kernels run on)
@@ -203,7 +175,7 @@ def main():
         dst = torch.empty_like(src)
         for _ in range(3):
             dst.copy_(src)
-        torch.cuda.synchronize()
+        sync()
         del src, dst
     traffic32 = traffic64 = None
     pmc_path = os.path.join(ROOT, "profiles", f"spmv_{args.config}_pmc.json")
@@ -212,6 +184,229 @@ def main():
             pmc = json.load(fh)
         traffic32 = pmc.get("fp32", {}).get("hbm_bytes")
         traffic64 = pmc.get("fp64", {}).get("hbm_bytes")
+    spmv = {"fp64_us": ms64 * 1e3, "fp64_GBs": gbs64, "fp64_bytes": bytes64,
+            "fp64_frac": gbs64 / HBM_PEAK_GBS, "fp64_traffic": traffic64,
+            "fp32_us": ms32 * 1e3, "fp32_GBs": gbs32, "fp32_bytes": bytes32,
+            "fp32_frac": gbs32 / HBM_PEAK_GBS, "fp32_traffic": traffic32}
+    info = dict(n=n, nnz=nnz, elements=mesh.num_elements(), symbolic_s=t_symbolic,
+                upload_s=t_upload)
+    return results, tight, spmv, info
+
+
+def run_elastic(G, args):
+    from geo_simulator_amd import workloads as W
+    nx, ny, nz = 48, 48, 24
+    mesh, dof, uz = W.elastic_box(G, nx, ny, nz)
+    mat = G.IsotropicElasticity(1e11, 0.25)
+    assemble = lambda: G.Assembler.assemble_elasticity_system(mesh, dof, mat, 3300.0,
+                                                              [0.0, 0.0, -9.81])
+    t0 = time.perf_counter()
+    K, f = assemble()
+    sync()
+    t_first = time.perf_counter() - t0
+    out = dict(workload=f"elastic Tet10 gravity column {nx}x{ny}x{nz} (metres, "
+                        f"{mesh.num_elements()} elements, {3 * mesh.num_nodes()} DOFs, "
+                        f"nnz_bc {K.nnz()})", first_assembly_s=t_first)
+    for prec in ("fp64", "mixed"):
+        for tol in (1e-8, 1e-12):
+            cg = make_cg(G, prec, tol, args.inner_tol)
+            wall, ta, ts, (K, f, u, st) = timed_steps(assemble, cg, 2 if tol == 1e-8 else 1, 1)
+            uh = u.cpu().numpy()
+            err = float(np.linalg.norm(uh[2::3] - uz) / np.linalg.norm(uz))
+            key = f"{prec}" if tol == 1e-8 else f"tight_{prec}"
+            out[key] = dict(step_s=wall, asm_s=ta, solve_s=ts, iterations=st.iterations,
+                            outer=st.outer_iterations, rel_res=st.relative_residual,
+                            converged=st.converged, rel_l2_uz_vs_closed_form=err,
+                            max_abs_ux_m=float(np.abs(uh[0::3]).max()))
+            log(f"[bench] elastic {key}: {wall*1e3:.1f} ms/step asm {ta*1e3:.1f} ms "
+                f"it={st.iterations} outer={st.outer_iterations} relL2={err:.2e}")
+    out["speedup_mixed_vs_fp64"] = out["fp64"]["step_s"] / out["mixed"]["step_s"]
+    out["tight_speedup_mixed_vs_fp64"] = (out["tight_fp64"]["step_s"] /
+                                          out["tight_mixed"]["step_s"])
+    nnz, n = K.nnz(), 3 * mesh.num_nodes()
+    b64, b32 = nnz * 12 + n * 20, nnz * 8 + n * 12
+    ms64, ms32 = K.time_spmv("fp64", 20), K.time_spmv("mixed", 20)
+    out["spmv"] = {"fp64_us": ms64 * 1e3, "fp64_GBs": b64 / ms64 / 1e6,
+                   "fp32_us": ms32 * 1e3, "fp32_GBs": b32 / ms32 / 1e6}
+    log(f"[bench] elastic SpMV fp64 {ms64*1e3:.1f} us fp32 {ms32*1e3:.1f} us")
+    return out
+
+
+def fd_central(solve_obs, theta, e_list, h, obs_ref):
+    """Central differences of L = 0.5 ||y(theta) - obs||^2 in theta_e,
+    evaluated as 0.5 sum (y+ - y-)(y+ + y- - 2 obs) / (2h): the same
+    difference quotient, without the cancellation of forming L first.
+    solve_obs returns (y - y0, y0) at the observations."""
+    import torch
+    out = []
+    for e in e_list:
+        tp = theta.clone()
+        tm = theta.clone()
+        tp[e] += h
+        tm[e] -= h
+        dp, y0 = solve_obs(tp)
+        dm, _ = solve_obs(tm)
+        dl = 0.5 * torch.sum((dp - dm) * (2.0 * y0 + dp + dm - 2.0 * obs_ref)).item()
+        out.append(dl / (2.0 * h))
+    return np.array(out)
+
+
+def corrected_solver(G, assemble, y0, cg):
+    """theta -> (y(theta) - y0, y0): the perturbed solution as y0 plus a
+    correction solved from the residual, K(theta) dy = f - K(theta) y0, so
+    the difference of two nearby solutions keeps its full relative accuracy
+    (the correction is solved to 1e-13 of its own size)."""
+    def solve(theta):
+        K, f = assemble(theta)
+        r = f - K.apply(y0)
+        dy, _ = cg.solve(K, r)
+        return dy
+    return solve
+
+
+def run_adjoint(G, args):
+    """Inversion gradients: thermal d L / d log k_e and elastic d L / d log E_e
+    from the adjoint (forward solve, reverse solve with the same SPD operator,
+    element-backward kernel), mixed and FP64 solves, checked by central FD."""
+    import torch
+    from geo_simulator_amd import workloads as W
+    from geo_simulator_amd.adjoint import (ElasticProblem, ThermalProblem, elastic_solve,
+                                           thermal_solve)
+    dev = "cuda"
+    out = {}
+    cases = []
+    if not args.no_adjoint_thermal:
+        cases.append("thermal")
+    if not args.no_adjoint_elastic:
+        cases.append("elastic")
+    for case in cases:
+        if case == "thermal":
+            mesh, dof, theta, theta_true, obs = W.thermal_adjoint_case(G)
+            mk = lambda solver: ThermalProblem(mesh, dof, 1e-6, solver=solver)
+            apply = thermal_solve
+            desc = (f"thermal 64x64x32, log k_e from a Gaussian field (corr 100 km, k in "
+                    f"[1,5]), T observed at {len(obs)} free nodes")
+        else:
+            mesh, dof, theta, theta_true, obs = W.elastic_adjoint_case(G)
+            mk = lambda solver: ElasticProblem(mesh, dof, 0.25, 3300.0, [0.0, 0.0, -9.81],
+                                               solver=solver)
+            apply = elastic_solve
+            desc = (f"elastic 48x48x24, log E_e log-normal (median 1e11 Pa, sigma 0.5), "
+                    f"u_z observed at {len(obs)} top-face nodes")
+        obs_t = torch.tensor(obs, device=dev)
+        fp64 = make_cg(G, "fp64", 1e-12, abs_tol=0.0)
+        mixed = make_cg(G, "mixed", 1e-12, args.inner_tol, abs_tol=0.0)
+        with torch.no_grad():
+            y_obs = apply(torch.tensor(theta_true, device=dev), mk(fp64))[obs_t].clone()
+        res = {"workload": desc, "loss": "0.5 ||y_obs - y||^2", "solve_rel_res": 1e-12}
+        grads = {}
+        for name, solver in (("mixed", mixed), ("fp64", fp64)):
+            prob = mk(solver)
+            for _ in range(2):  # the second run is timed (the first builds the plan)
+                th = torch.tensor(theta, device=dev, requires_grad=True)
+                sync()
+                t0 = time.perf_counter()
+                y = apply(th, prob)
+                L = 0.5 * ((y[obs_t] - y_obs) ** 2).sum()
+                L.backward()
+                sync()
+                dt = time.perf_counter() - t0
+            grads[name] = th.grad.cpu().numpy()
+            res[name] = dict(gradient_s=dt, loss=L.item(),
+                             forward_iterations=prob.forward_stats.iterations,
+                             adjoint_iterations=prob.backward_stats.iterations)
+            log(f"[bench] adjoint {case} {name}: gradient {dt*1e3:.1f} ms "
+                f"(fwd it {prob.forward_stats.iterations}, adj it "
+                f"{prob.backward_stats.iterations})")
+        g, g64 = grads["mixed"], grads["fp64"]
+        res["speedup_mixed_vs_fp64"] = res["fp64"]["gradient_s"] / res["mixed"]["gradient_s"]
+        res["mixed_vs_fp64_max_rel"] = float(np.abs(g - g64).max() / np.abs(g64).max())
+        # central finite differences (FP64 solves, residual-corrected)
+        if args.fd_elems > 0:
+            import torch as _t
+            fdcg = make_cg(G, "fp64", 1e-13, abs_tol=0.0, max_iter=20000)
+            y0_full = apply(_t.tensor(theta, device=dev), mk(fdcg)).detach()
+            if case == "thermal":
+                asm = lambda th: G.Assembler.assemble_thermal_system(
+                    mesh, dof, 1.0, 1e-6, k_per_elem=_t.exp(th).contiguous())
+            else:
+                mat1 = G.IsotropicElasticity(1.0, 0.25)
+                asm = lambda th: G.Assembler.assemble_elasticity_system(
+                    mesh, dof, mat1, 3300.0, [0.0, 0.0, -9.81],
+                    E_per_elem=_t.exp(th).contiguous())
+            corr = corrected_solver(G, asm, y0_full, fdcg)
+            solve_obs = lambda th: (corr(th)[obs_t], y0_full[obs_t])
+            rng = np.random.Generator(np.random.PCG64(77))
+            elems = np.sort(rng.choice(len(g), min(args.fd_elems, len(g)), replace=False))
+            th0 = torch.tensor(theta, device=dev)
+            t0 = time.perf_counter()
+            with torch.no_grad():
+                fd = fd_central(solve_obs, th0, elems, 1e-6, y_obs)
+                v = rng.standard_normal(len(g))
+                vt = torch.tensor(v, device=dev)
+                dp, y0 = solve_obs(th0 + 1e-6 * vt)
+                dm, _ = solve_obs(th0 - 1e-6 * vt)
+                fdv = 0.5 * torch.sum((dp - dm) * (2.0 * y0 + dp + dm - 2.0 * y_obs)).item() / 2e-6
+            scale = np.abs(g).max()
+            rel = np.abs(fd - g[elems]) / np.maximum(np.abs(fd), 1e-3 * scale)
+            rel_pure = np.abs(fd - g[elems]) / np.abs(fd)
+            res["fd"] = dict(h=1e-6, elements=len(elems),
+                             selection="32 elements drawn with PCG64 seed 77 from all",
+                             error="|fd - g| / max(|fd|, 1e-3 max|g|)",
+                             max_rel_err=float(rel.max()), median_rel_err=float(np.median(rel)),
+                             max_pure_rel_err=float(rel_pure.max()),
+                             median_pure_rel_err=float(np.median(rel_pure)),
+                             min_abs_fd=float(np.abs(fd).min()), max_abs_g=float(scale),
+                             direction_rel_err=float(abs(fdv - g @ v) / abs(fdv)),
+                             seconds=time.perf_counter() - t0)
+            log(f"[bench] adjoint {case} FD ({len(elems)} elems): max rel {rel.max():.2e} "
+                f"median {np.median(rel):.2e}; random direction {res['fd']['direction_rel_err']:.2e}")
+        out[case] = res
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--config", default="thermal_64x64x32", choices=sorted(CONFIGS))
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--inner-tol", type=float, default=None)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tight", action="store_true")
+    ap.add_argument("--no-elastic", action="store_true")
+    ap.add_argument("--no-adjoint-thermal", action="store_true")
+    ap.add_argument("--no-adjoint-elastic", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="skip the elastic and adjoint sections")
+    ap.add_argument("--fd-elems", type=int, default=32)
+    ap.add_argument("--spmv-reps", type=int, default=50)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="also run a plain device copy of this size (counter "
+                         "calibration under rocprofv3 --pmc)")
+    args = ap.parse_args()
+    if args.headline_only:
+        args.no_elastic = args.no_adjoint_thermal = args.no_adjoint_elastic = True
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU: the assemble-and-solve path is built and "
+              "measured on a single MI355X (multi-GPU sharding is a design note "
+              "only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import geo_simulator_amd as G
+
+    if not torch.cuda.is_available():
+        print("no HIP device", file=sys.stderr)
+        sys.exit(1)
+    nx, ny, nz = CONFIGS[args.config]
+    results, tight, spmv, info = run_headline(G, args, nx, ny, nz)
+    elastic = None if args.no_elastic else run_elastic(G, args)
+    adjoint = None
+    if not (args.no_adjoint_thermal and args.no_adjoint_elastic):
+        adjoint = run_adjoint(G, args)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -243,24 +438,25 @@ def main():
         "dtype": "f32-inner/f64-outer",
         "data": "synthetic",
         "config": {"workload": f"thermal Tet10 assemble+Jacobi-PCG {nx}x{ny}x{nz} "
-                               f"({mesh.num_elements()} elements, {n} DOFs, nnz_bc {nnz})",
-                   "rel_res": args.tol, "path": "mixed (FP32 inner PCG, FP64 defect correction)"},
+                               f"({info['elements']} elements, {info['n']} DOFs, "
+                               f"nnz_bc {info['nnz']})",
+                   "rel_res": args.tol,
+                   "path": "mixed (FP64 flexible CG around an FP32 Jacobi-PCG recursion)"},
         "fp64": fp64,
         "mixed": mixed,
         "speedup_mixed_vs_fp64": fp64["step_s"] / mixed["step_s"],
         "tight_rel_res_1e-12": tight,
-        "symbolic_s": t_symbolic,
-        "upload_s": t_upload,
-        "spmv": {"fp64_us": ms64 * 1e3, "fp64_GBs": gbs64, "fp64_bytes": bytes64,
-                 "fp64_frac": gbs64 / HBM_PEAK_GBS, "fp64_traffic": traffic64,
-                 "fp32_us": ms32 * 1e3, "fp32_GBs": gbs32, "fp32_bytes": bytes32,
-                 "fp32_frac": gbs32 / HBM_PEAK_GBS, "fp32_traffic": traffic32},
-        "roofline": {"bound": "hbm", "achieved": gbs32, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": gbs32 / HBM_PEAK_GBS,
-                     "traffic": traffic32,
-                     "kernel": "k_cg_spmv FP32 (CG SpMV + p.Ap partials; dominant "
-                               "kernel of the mixed path)",
-                     "algorithmic_bytes": bytes32},
+        "symbolic_s": info["symbolic_s"],
+        "upload_s": info["upload_s"],
+        "spmv": spmv,
+        "roofline": {"bound": "hbm", "achieved": spmv["fp32_GBs"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": spmv["fp32_GBs"] / HBM_PEAK_GBS,
+                     "traffic": spmv["fp32_traffic"],
+                     "kernel": "k_cg_q<float> (SELL-64 CG SpMV + p.Ap partials; "
+                               "dominant kernel of the mixed path)",
+                     "algorithmic_bytes": spmv["fp32_bytes"]},
+        "elastic_48x48x24": elastic,
+        "adjoint": adjoint,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -24,17 +24,16 @@
 // Vectors live in the SELL order of the operator (sell.hpp); the C ABI's
 // vectors are gathered into it and scattered back.
 //
-// Mixed precision: the passes run in FP32 (SELL values, vectors and SpMV;
-// dot products accumulated in FP64) on the system scaled by 1/||b||.  When
-// the recursive FP32 residual has dropped by inner_tolerance since the last
-// replacement, or reaches the target, F2 pauses the solve; the host folds the
-// FP32 correction into the FP64 solution, recomputes r = b - A x in FP64
-// (the stopping test uses this true residual), and continues the FP32
-// recursion from it while keeping the search direction (residual
-// replacement), so Krylov information is not thrown away as in plain
-// defect correction.  One cycle can gain at most about eps32 * cond(A) on
-// the true residual, so replacements come every inner_tolerance (1e-2 by
-// default) and the direction is reset when the two residuals disagree.
+// Mixed precision: FP64 flexible CG (Notay's FCG with one previous
+// direction kept) whose preconditioner is a fresh FP32 Jacobi-PCG solve of the
+// current residual (FP32 SELL values, vectors and SpMV; dot products
+// accumulated in FP64) to relative tolerance inner_tolerance.  Every outer
+// step is a Galerkin step with the exact FP64 operator: residual, solution
+// and search directions stay in FP64, the A-norm error never grows, and the
+// stopping test is the reference's on the FP64 residual (checked against
+// b - A x before returning).  Unlike plain defect correction (x += A32^-1 r),
+// this converges also when eps32 * cond(A) is not small, as for the
+// heterogeneous elastic operators.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -57,12 +56,10 @@ struct CgState {
   double rz[2];  // r.z at the start of pass k lives in rz[k & 1]
   double alpha, beta, pq;
   double rnorm, bnorm, tol_abs, abs_tol, tol;
-  double rrec;   // mixed: recursive FP32 residual norm (scaled by 1/||b||)
-  double rbase;  // mixed: scaled true residual at the last replacement
-  double rtol;   // mixed: replacement factor (inner_tolerance)
-  double scale;  // mixed: 1/||b||
+  double rbase;  // mixed inner: ||r|| (scaled) when the recursion (re)started
+  double rtol;   // mixed inner: pause once ||r|| < rtol * rbase
   long long iteration, max_iter;
-  int stop, converged, pause, mixed, breakdown;
+  int stop, converged, pause, mixed;
 };
 
 __device__ __forceinline__ bool active(const CgState *st, long long it) {
@@ -192,7 +189,7 @@ __global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ rp,
 // leader of the same kernel writes, except `iteration`, which the leader only
 // advances when the pass completes -- so seeing it+1 also means "live".
 
-// K2 (pass it): pq; FP64 breaks if |pq| < 1e-30 (src/linalg/iterative.rs:85);
+// K2 (pass it): pq; break if |pq| < 1e-30 (src/linalg/iterative.rs:85);
 // alpha = rz/pq; x += alpha p; r -= alpha q; z = D^-1 r; partials r.r, r.z.
 template <typename T, bool PREC>
 __global__ __launch_bounds__(FBS) void k_cg_alpha(
@@ -204,20 +201,16 @@ __global__ __launch_bounds__(FBS) void k_cg_alpha(
   double pq[1];
   sum_partials<1>(part_in, np_in, pq);
   const double alpha_d = st->rz[it & 1] / pq[0];
-  // The scaled FP32 system has its own units, where the reference's absolute
-  // 1e-30 threshold means nothing; it breaks down only on a zero or
-  // non-finite step.
+  // The mixed inner solve works on a scaled FP32 system whose units make the
+  // reference's absolute 1e-30 meaningless; it breaks down only on a zero or
+  // non-finite step, and then pauses for the outer step.
   const bool brk = st->mixed ? (pq[0] == 0.0 || !isfinite(alpha_d))
                              : fabs(pq[0]) < 1e-30;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pq = pq[0];
     if (brk) {
-      if (st->mixed) {
-        st->pause = 1;
-        st->breakdown = 1;
-      } else {
-        st->stop = 1;
-      }
+      if (st->mixed) st->pause = 1;
+      else st->stop = 1;
     } else {
       st->alpha = alpha_d;
     }
@@ -241,12 +234,12 @@ __global__ __launch_bounds__(FBS) void k_cg_alpha(
   }
 }
 
-// K3 (pass it).  FP64: the reference's stopping test
-// (src/linalg/iterative.rs:95-98; the converging pass is not counted), then
-// beta = rz'/rz, it += 1 (stop at max_iter), p = z + beta p.  Mixed: always
-// completes the pass, and pauses for a residual replacement when the
-// recursive residual reached the target, dropped by rtol since the last
-// replacement, or the pass budget is spent.
+// K3 (pass it): the reference's stopping test (src/linalg/iterative.rs:95-98;
+// the converging pass is not counted), then beta = rz'/rz, it += 1 (stop at
+// max_iter), p = z + beta p.  As the inner solver of the mixed path it never
+// stops by itself: it completes the pass and pauses for the outer FP64 step
+// once ||r|| fell below rtol * rbase, below the outer target tol, or the pass
+// budget is spent.
 template <typename T>
 __global__ __launch_bounds__(FBS) void k_cg_beta(
     int n, long long it, const T *__restrict__ z, T *__restrict__ p,
@@ -260,20 +253,18 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
   const bool conv = !mixed && (rn < st->tol_abs || rn < st->abs_tol);
   const double beta_d = tot[1] / st->rz[it & 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rnorm = rn;
     if (conv) {
-      st->rnorm = rn;
       st->converged = 1;
       st->stop = 1;
     } else {
-      if (mixed) st->rrec = rn;
-      else st->rnorm = rn;
       st->beta = beta_d;
       st->rz[(it + 1) & 1] = tot[1];
       st->iteration = it + 1;
       if (!mixed) {
         if (it + 1 >= st->max_iter) st->stop = 1;
-      } else if (rn < st->tol || rn * st->bnorm < st->abs_tol ||
-                 rn < st->rtol * st->rbase || it + 1 >= st->max_iter) {
+      } else if (rn < st->rtol * st->rbase || rn < st->tol ||
+                 it + 1 >= st->max_iter) {
         st->pause = 1;
       }
     }
@@ -284,22 +275,22 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
     p[i] = z[i] + beta * p[i];
 }
 
-// FP64 set-up: x = 0, r = b, z = D^-1 r, p = z; partials b.b, r.z.
-template <bool PREC>
+// Set-up: x = 0, r = b, z = D^-1 r, p = z; partials b.b, r.z.
+template <typename T, bool PREC>
 __global__ __launch_bounds__(BS) void k_cg_init(
-    int n, const double *__restrict__ b, double *__restrict__ x,
-    double *__restrict__ r, double *__restrict__ z, double *__restrict__ p,
-    const double *__restrict__ dinv, double *__restrict__ part) {
+    int n, const T *__restrict__ b, T *__restrict__ x, T *__restrict__ r,
+    T *__restrict__ z, T *__restrict__ p, const T *__restrict__ dinv,
+    double *__restrict__ part) {
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    const double bi = b[i];
-    x[i] = 0.0;
+    const T bi = b[i];
+    x[i] = (T)0;
     r[i] = bi;
-    const double zi = PREC ? bi * dinv[i] : bi;
+    const T zi = PREC ? bi * dinv[i] : bi;
     z[i] = zi;
     p[i] = zi;
-    acc[0] += bi * bi;
-    acc[1] += bi * zi;
+    acc[0] += (double)bi * (double)bi;
+    acc[1] += (double)bi * (double)zi;
   }
   put_partials<2>(acc, part);
 }
@@ -315,7 +306,7 @@ __global__ void k_state_setup(CgState *__restrict__ st, long long max_iter,
   *st = s;
 }
 
-// FP64 set-up, single workgroup: ||b||, rz, thresholds, early exits
+// Set-up, single workgroup: ||b||, rz, thresholds, early exits
 // (src/linalg/iterative.rs:60-68).
 __global__ __launch_bounds__(FBS) void k_fin_init(CgState *__restrict__ st,
                                                   const double *__restrict__ part,
@@ -337,12 +328,24 @@ __global__ __launch_bounds__(FBS) void k_fin_init(CgState *__restrict__ st,
   }
 }
 
-// ---- mixed-precision kernels -------------------------------------------------
+// ---- flexible-CG (mixed precision) kernels ----------------------------------
+// Sums of NV partials -> out[0..NV) (single workgroup).
+template <int NV>
+__global__ __launch_bounds__(FBS) void k_fin_sum(const double *__restrict__ part,
+                                                 int np, double *__restrict__ out) {
+  double tot[NV];
+  sum_partials<NV>(part, np, tot);
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < NV; ++i) out[i] = tot[i];
+  }
+}
+
 // x = 0, r = b; partials b.b
-__global__ __launch_bounds__(BS) void k_mx_start(int n, const double *__restrict__ b,
-                                                 double *__restrict__ x,
-                                                 double *__restrict__ r,
-                                                 double *__restrict__ part) {
+__global__ __launch_bounds__(BS) void k_fcg_start(int n, const double *__restrict__ b,
+                                                  double *__restrict__ x,
+                                                  double *__restrict__ r,
+                                                  double *__restrict__ part) {
   double acc[1] = {0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     const double bi = b[i];
@@ -353,71 +356,108 @@ __global__ __launch_bounds__(BS) void k_mx_start(int n, const double *__restrict
   put_partials<1>(acc, part);
 }
 
-// ||r|| from partials; on the first call also ||b||, 1/||b||, thresholds
-__global__ __launch_bounds__(FBS) void k_fin_norm(CgState *__restrict__ st,
-                                                  const double *__restrict__ part,
-                                                  int np, int first) {
-  double tot[1];
-  sum_partials<1>(part, np, tot);
-  if (threadIdx.x != 0) return;
-  const double rn = sqrt(tot[0]);
-  st->rnorm = rn;
-  if (first) {
-    st->bnorm = rn;
-    st->scale = rn > 0.0 ? 1.0 / rn : 0.0;
-    st->tol_abs = st->tol * rn;
-  }
-}
-
-// FP32 recursion (re)started from the FP64 residual: rs = r/||b||,
-// zs = D^-1 rs [and p = zs, xs = 0 on a restart]; partials rs.zs
+// Inner FP32 recursion (re)started from the FP64 residual: rs = r/||b||,
+// zs = D^-1 rs [and p = zs on a restart]; partials rs.zs
 template <bool PREC, bool RESTART>
-__global__ __launch_bounds__(BS) void k_mx_rescale(
-    int n, const double *__restrict__ r, const float *__restrict__ dinv,
-    float *__restrict__ rs, float *__restrict__ zs, float *__restrict__ p,
-    float *__restrict__ xs, const CgState *__restrict__ st,
-    double *__restrict__ part) {
-  const double sc = st->scale;
+__global__ __launch_bounds__(BS) void k_fcg_inner(
+    int n, const double *__restrict__ r, double scale,
+    const float *__restrict__ dinv, float *__restrict__ rs, float *__restrict__ zs,
+    float *__restrict__ p, double *__restrict__ part) {
   double acc[1] = {0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    const float ri = (float)(r[i] * sc);
+    const float ri = (float)(r[i] * scale);
     rs[i] = ri;
     const float zi = PREC ? ri * dinv[i] : ri;
     zs[i] = zi;
-    if (RESTART) {
-      p[i] = zi;
-      xs[i] = 0.0f;
-    }
+    if (RESTART) p[i] = zi;
     acc[0] += (double)ri * (double)zi;
   }
   put_partials<1>(acc, part);
 }
 
-__global__ __launch_bounds__(FBS) void k_fin_rescale(CgState *__restrict__ st,
-                                                     const double *__restrict__ part,
-                                                     int np) {
+// Inner recursion state after a (re)start: rz, rbase; resume passes.
+__global__ __launch_bounds__(FBS) void k_fin_inner(CgState *__restrict__ st,
+                                                   const double *__restrict__ part,
+                                                   int np, double rbase) {
   double tot[1];
   sum_partials<1>(part, np, tot);
   if (threadIdx.x != 0) return;
   st->rz[st->iteration & 1] = tot[0];
-  st->rbase = st->rnorm * st->scale;
-  st->rrec = st->rbase;
+  st->rbase = rbase;
+  st->rnorm = rbase;
   st->pause = 0;
-  st->breakdown = 0;
 }
 
-// x += ||b|| xs (FP64); xs = 0
-__global__ __launch_bounds__(BS) void k_mx_fold(int n, float *__restrict__ xs,
-                                                double *__restrict__ x,
-                                                const CgState *__restrict__ st) {
-  const double bn = st->bnorm;
+// d = ||b|| xs (the inner correction in FP64 and in units of b); xs = 0
+__global__ __launch_bounds__(BS) void k_fcg_expand(int n, float *__restrict__ xs,
+                                                   double scale,
+                                                   double *__restrict__ d) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    x[i] += bn * (double)xs[i];
+    d[i] = scale * (double)xs[i];
     xs[i] = 0.0f;
   }
 }
 
+// w = A d (FP64 SELL); partials d.Ap_prev
+template <class IDX>
+__global__ __launch_bounds__(BS) void k_fcg_spmv(
+    int n, int ns, const int *__restrict__ cb, const IDX idx,
+    const double *__restrict__ val, const double *__restrict__ d,
+    double *__restrict__ w, const double *__restrict__ ap_prev,
+    double *__restrict__ part) {
+  double acc[1] = {0.0};
+  auto epi = [&](int row, double s) {
+    w[row] = s;
+    if (ap_prev) acc[0] += d[row] * ap_prev[row];
+  };
+  sell_rows<double>(n, ns, cb, idx, val, d, epi);
+  put_partials<1>(acc, part);
+}
+
+// p = d - gamma p; ap = w - gamma ap (first step: p = d, ap = w);
+// partials p.r, p.ap
+__global__ __launch_bounds__(BS) void k_fcg_dir(int n, const double *__restrict__ d,
+                                                const double *__restrict__ w,
+                                                const double *__restrict__ r,
+                                                double gamma, int first,
+                                                double *__restrict__ p,
+                                                double *__restrict__ ap,
+                                                double *__restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    const double pi = first ? d[i] : d[i] - gamma * p[i];
+    const double ai = first ? w[i] : w[i] - gamma * ap[i];
+    p[i] = pi;
+    ap[i] = ai;
+    acc[0] += pi * r[i];
+    acc[1] += pi * ai;
+  }
+  put_partials<2>(acc, part);
+}
+
+// x += alpha p; r -= alpha ap; partials r.r
+__global__ __launch_bounds__(BS) void k_fcg_update(int n, double alpha,
+                                                   const double *__restrict__ p,
+                                                   const double *__restrict__ ap,
+                                                   double *__restrict__ x,
+                                                   double *__restrict__ r,
+                                                   double *__restrict__ part) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * ap[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  put_partials<1>(acc, part);
+}
+
 // ---- misc --------------------------------------------------------------------
+template <typename T>
+__global__ void k_zero(int n, T *__restrict__ x) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) x[i] = (T)0;
+}
+
 template <typename T>
 __global__ void k_fill_pattern(int n, T *__restrict__ x) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
@@ -641,39 +681,52 @@ void enqueue_pass(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *z
 }
 
 struct PcgResult {
-  long long iterations = 0, outer = 0, spmv = 0, restarts = 0;
+  long long iterations = 0, outer = 0, spmv = 0;
   double rnorm = 0.0, bnorm = 0.0;
   bool converged = false;
 };
 
-// FP64 Jacobi-PCG with the reference's control flow; b, x in SELL order.
-PcgResult pcg64(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
-                bool precond, long long max_iter, double tol, double abs_tol,
-                char *base, Layout &L) {
+template <typename T> struct PcgWork {
+  double *part1 = nullptr, *part2 = nullptr;
+  CgState *st = nullptr;
+  T *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr;
+};
+
+template <typename T>
+PcgWork<T> carve_pcg(char *base, Layout &L, int64_t n) {
+  PcgWork<T> w;
+  w.part1 = (double *)(base + L.add<double>(GRID_MAX));
+  w.part2 = (double *)(base + L.add<double>(2 * GMAX));
+  w.st = (CgState *)(base + L.add<CgState>(1));
+  w.r = (T *)(base + L.add<T>(n));
+  w.z = (T *)(base + L.add<T>(n));
+  w.p = (T *)(base + L.add<T>(n));
+  w.q = (T *)(base + L.add<T>(n));
+  return w;
+}
+
+// Jacobi-PCG with the reference's control flow, in precision T (dot products
+// in FP64); b, x in SELL order.
+template <typename T>
+PcgResult pcg_core(gs_ctx *ctx, const SellOp &o, const T *val, const T *dinv,
+                   const T *b, T *x, long long max_iter, double tol,
+                   double abs_tol, const PcgWork<T> &w) {
   hipStream_t s = ctx->stream;
-  const SellOp o = sell_op(A);
   const int n = o.n();
-  double *part1 = (double *)(base + L.add<double>(GRID_MAX));
-  double *part2 = (double *)(base + L.add<double>(2 * GMAX));
-  CgState *st = (CgState *)(base + L.add<CgState>(1));
-  double *r = (double *)(base + L.add<double>(n));
-  double *z = (double *)(base + L.add<double>(n));
-  double *p = (double *)(base + L.add<double>(n));
-  double *q = (double *)(base + L.add<double>(n));
-  const double *dinv = precond ? A->sdinv.get() : nullptr;
-  const double *val = A->sval.get();
   const unsigned gv = grid_vec(n);
-  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, st, max_iter, tol, abs_tol, 0, 0.0);
+  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, w.st, max_iter, tol, abs_tol, 0, 0.0);
   if (dinv)
-    hipLaunchKernelGGL((k_cg_init<true>), gv, BS, 0, s, n, b, x, r, z, p, dinv, part2);
+    hipLaunchKernelGGL((k_cg_init<T, true>), gv, BS, 0, s, n, b, x, w.r, w.z, w.p,
+                       dinv, w.part2);
   else
-    hipLaunchKernelGGL((k_cg_init<false>), gv, BS, 0, s, n, b, x, r, z, p, dinv, part2);
-  hipLaunchKernelGGL(k_fin_init, 1, FBS, 0, s, st, part2, (int)gv);
+    hipLaunchKernelGGL((k_cg_init<T, false>), gv, BS, 0, s, n, b, x, w.r, w.z, w.p,
+                       dinv, w.part2);
+  hipLaunchKernelGGL(k_fin_init, 1, FBS, 0, s, w.st, w.part2, (int)gv);
   GS_LAUNCH_CHECK("k_cg_init");
   PcgResult res;
   Batcher bt;
   for (;;) {
-    const CgState h = read_state(ctx, st);
+    const CgState h = read_state(ctx, w.st);
     res.iterations = h.iteration;
     res.rnorm = h.rnorm;
     res.bnorm = h.bnorm;
@@ -683,8 +736,8 @@ PcgResult pcg64(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
                                     std::max(h.tol_abs, h.abs_tol),
                                     max_iter - h.iteration);
     for (long long i = 0; i < batch; ++i)
-      enqueue_pass<double>(s, o, val, x, r, z, p, q, dinv, st, h.iteration + i,
-                           part1, part2);
+      enqueue_pass<T>(s, o, val, x, w.r, w.z, w.p, w.q, dinv, w.st, h.iteration + i,
+                      w.part1, w.part2);
     GS_LAUNCH_CHECK("cg batch");
   }
   // one SpMV per pass, the converging (uncounted) pass included
@@ -692,113 +745,192 @@ PcgResult pcg64(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
   return res;
 }
 
-// FP32 Jacobi-PCG with FP64 residual replacement; b, x in SELL order.
-PcgResult pcg_mixed(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
-                    bool precond, const gs_cg_opts &opt, char *base, Layout &L) {
+// Reads n doubles written by the finalize kernels.
+void read_dots(gs_ctx *ctx, const double *dev, double *out, int n) {
+  GS_HIP(hipMemcpyAsync(ctx->pinned, dev, n * sizeof(double), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  memcpy(out, ctx->pinned, n * sizeof(double));
+}
+
+// Mixed precision: FP64 flexible CG, FCG(1) (Notay, SIAM J. Sci. Comput. 22,
+// 2000), whose preconditioner is an FP32 Jacobi-PCG recursion on the system
+// scaled by 1/||b||.
+//   r = b, x = 0; start the FP32 recursion at rs = r/||b||
+//   repeat until ||r|| < tol ||b|| or < abs_tol:
+//     run FP32 passes until the recursion's residual fell by inner_tolerance
+//     d     = ||b|| xs,  xs = 0                     (the FP32 correction)
+//     w     = A d                                   (FP64)
+//     gamma = (d, A p) / (p, A p)      [0 on the first step]
+//     p     = d - gamma p,  Ap = w - gamma Ap
+//     alpha = (p, r) / (p, Ap)
+//     x    += alpha p,      r -= alpha Ap           (FP64)
+//     continue the FP32 recursion from rs = r/||b|| keeping its direction
+//     when the outer step agreed with it (alpha near 1, the new residual near
+//     the recursion's own); otherwise restart it (p = z)
+// Keeping the direction preserves the Krylov information of the FP32 solve
+// across outer steps; the FP64 line search keeps every step safe when the
+// FP32 operator's rounding (eps32 * cond(A)) makes the recursion drift.  The
+// loop ends on the true residual b - A x.
+PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
+                  bool precond, const gs_cg_opts &opt, char *base, Layout &L) {
   hipStream_t s = ctx->stream;
   const SellOp o = sell_op(A);
   const int n = o.n();
-  double *part1 = (double *)(base + L.add<double>(GRID_MAX));
-  double *part2 = (double *)(base + L.add<double>(2 * GMAX));
-  CgState *st = (CgState *)(base + L.add<CgState>(1));
-  double *r64 = (double *)(base + L.add<double>(n));
+  const unsigned gv = grid_vec(n);
+  double *r = (double *)(base + L.add<double>(n));
+  double *d = (double *)(base + L.add<double>(n));
+  double *wv = (double *)(base + L.add<double>(n));
+  double *p = (double *)(base + L.add<double>(n));
+  double *ap = (double *)(base + L.add<double>(n));
+  double *dots = (double *)(base + L.add<double>(8));
   float *xs = (float *)(base + L.add<float>(n));
-  float *rs = (float *)(base + L.add<float>(n));
-  float *zs = (float *)(base + L.add<float>(n));
-  float *p = (float *)(base + L.add<float>(n));
-  float *q = (float *)(base + L.add<float>(n));
-  const float *dinv = precond ? A->sdinv32.get() : nullptr;
+  const PcgWork<float> iw = carve_pcg<float>(base, L, n);
+  const float *dinv32 = precond ? A->sdinv32.get() : nullptr;
   const float *val32 = A->sval32.get();
   const double *val = A->sval.get();
-  const unsigned gv = grid_vec(n);
-  const long long max_iter = opt.max_iterations;
-  const long long max_outer =
-      opt.max_outer_iterations > 0 ? opt.max_outer_iterations : 100;
-  const long long inner_cap = opt.max_inner_iterations;
-  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, st, max_iter, opt.tolerance,
-                     opt.abs_tolerance, 1, opt.inner_tolerance);
-  hipLaunchKernelGGL(k_mx_start, gv, BS, 0, s, n, b, x, r64, part1);
-  hipLaunchKernelGGL(k_fin_norm, 1, FBS, 0, s, st, part1, (int)gv, 1);
-  GS_LAUNCH_CHECK("k_mx_start");
-  CgState h = read_state(ctx, st);
   PcgResult res;
-  res.bnorm = h.bnorm;
-  res.rnorm = h.bnorm;
-  if (h.bnorm < 1e-25) {  // src/linalg/iterative.rs:60-68
+  double h[2];
+  hipLaunchKernelGGL(k_fcg_start, gv, BS, 0, s, n, b, x, r, iw.part2);
+  hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
+  GS_LAUNCH_CHECK("k_fcg_start");
+  read_dots(ctx, dots, h, 1);
+  const double bnorm = std::sqrt(h[0]);
+  res.bnorm = bnorm;
+  res.rnorm = bnorm;
+  if (bnorm < 1e-25) {  // src/linalg/iterative.rs:60-68
     res.converged = true;
     res.rnorm = 0.0;
     return res;
   }
+  const long long max_iter = opt.max_iterations;
   if (max_iter <= 0) return res;
-  auto rescale = [&](bool restart) {
-    if (dinv) {
+  const long long max_outer =
+      opt.max_outer_iterations > 0 ? opt.max_outer_iterations : 100;
+  const double target = std::max(opt.tolerance * bnorm, opt.abs_tolerance);
+  const double scale = 1.0 / bnorm;
+  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, iw.st, max_iter, target * scale,
+                     0.0, 1, opt.inner_tolerance);
+  hipLaunchKernelGGL((k_zero<float>), gv, BS, 0, s, n, xs);
+  auto start_inner = [&](bool restart, double rn) {
+    if (dinv32) {
       if (restart)
-        hipLaunchKernelGGL((k_mx_rescale<true, true>), gv, BS, 0, s, n, r64, dinv,
-                           rs, zs, p, xs, st, part2);
+        hipLaunchKernelGGL((k_fcg_inner<true, true>), gv, BS, 0, s, n, r, scale,
+                           dinv32, iw.r, iw.z, iw.p, iw.part2);
       else
-        hipLaunchKernelGGL((k_mx_rescale<true, false>), gv, BS, 0, s, n, r64, dinv,
-                           rs, zs, p, xs, st, part2);
+        hipLaunchKernelGGL((k_fcg_inner<true, false>), gv, BS, 0, s, n, r, scale,
+                           dinv32, iw.r, iw.z, iw.p, iw.part2);
     } else {
       if (restart)
-        hipLaunchKernelGGL((k_mx_rescale<false, true>), gv, BS, 0, s, n, r64, dinv,
-                           rs, zs, p, xs, st, part2);
+        hipLaunchKernelGGL((k_fcg_inner<false, true>), gv, BS, 0, s, n, r, scale,
+                           dinv32, iw.r, iw.z, iw.p, iw.part2);
       else
-        hipLaunchKernelGGL((k_mx_rescale<false, false>), gv, BS, 0, s, n, r64, dinv,
-                           rs, zs, p, xs, st, part2);
+        hipLaunchKernelGGL((k_fcg_inner<false, false>), gv, BS, 0, s, n, r, scale,
+                           dinv32, iw.r, iw.z, iw.p, iw.part2);
     }
-    hipLaunchKernelGGL(k_fin_rescale, 1, FBS, 0, s, st, part2, (int)gv);
-    GS_LAUNCH_CHECK("k_mx_rescale");
+    hipLaunchKernelGGL(k_fin_inner, 1, FBS, 0, s, iw.st, iw.part2, (int)gv, rn * scale);
+    GS_LAUNCH_CHECK("k_fcg_inner");
   };
-  rescale(true);
-  h = read_state(ctx, st);
+  start_inner(true, bnorm);
+  double rn = bnorm, pap = 0.0;
+  bool first = true, checked = false;
   Batcher bt;
-  long long last_rep = 0;  // pass index of the last replacement
-  int breakdowns = 0;
-  for (;;) {
-    bool replace = h.pause != 0;
-    if (!replace && inner_cap > 0 && h.iteration - last_rep >= inner_cap) replace = true;
-    if (replace) {
-      hipLaunchKernelGGL(k_mx_fold, gv, BS, 0, s, n, xs, x, st);
-      launch_residual(s, o, val, x, b, r64, part1);
-      hipLaunchKernelGGL(k_fin_norm, 1, FBS, 0, s, st, part1, o.grid, 0);
-      GS_LAUNCH_CHECK("residual replacement");
-      h = read_state(ctx, st);
-      ++res.outer;
+  CgState hs = read_state(ctx, iw.st);
+  // A cycle whose recursion has not halved its best residual within
+  // max(200, 2 x the longest cycle so far) passes has stalled on a stale
+  // direction: it is cut short and the recursion restarts.
+  long long longest = 0, cycle_start = 0;
+  while (res.outer < max_outer) {
+    // FP32 passes until the recursion pauses
+    bool stalled = false;
+    double best = hs.rnorm;
+    long long best_at = hs.iteration;
+    while (!hs.pause && !hs.stop) {
+      const long long window = std::max(200ll, 2 * longest);
+      if (hs.rnorm < 0.5 * best) {
+        best = hs.rnorm;
+        best_at = hs.iteration;
+      } else if (hs.iteration - best_at >= window) {
+        stalled = true;
+        break;
+      }
+      const double tgt = std::max(hs.rtol * hs.rbase, hs.tol);
+      const long long batch =
+          bt.next(hs.rnorm, hs.iteration, tgt,
+                  std::min(max_iter - hs.iteration, best_at + window - hs.iteration));
+      for (long long i = 0; i < batch; ++i)
+        enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
+                            hs.iteration + i, iw.part1, iw.part2);
+      GS_LAUNCH_CHECK("cg batch");
+      const long long before = hs.iteration;
+      hs = read_state(ctx, iw.st);
+      res.spmv += hs.iteration - before;
+    }
+    longest = std::max(longest, hs.iteration - cycle_start);
+    res.iterations = hs.iteration;
+    const double predicted = hs.rnorm * bnorm;  // the recursion's residual
+    // outer FP64 step along the FP32 correction
+    hipLaunchKernelGGL(k_fcg_expand, gv, BS, 0, s, n, xs, bnorm, d);
+    with_idx(*o.P, [&](auto idx) {
+      hipLaunchKernelGGL((k_fcg_spmv<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+                         o.P->cb.get(), idx, val, d, wv, first ? nullptr : ap,
+                         iw.part1);
+    });
+    ++res.spmv;
+    double gamma = 0.0;
+    if (!first) {
+      hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part1, o.grid, dots);
+      read_dots(ctx, dots, h, 1);
+      gamma = h[0] / pap;
+    }
+    hipLaunchKernelGGL(k_fcg_dir, gv, BS, 0, s, n, d, wv, r, gamma, first ? 1 : 0, p,
+                       ap, iw.part2);
+    hipLaunchKernelGGL((k_fin_sum<2>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
+    GS_LAUNCH_CHECK("k_fcg_dir");
+    read_dots(ctx, dots, h, 2);
+    if (!(h[1] > 0.0) || !std::isfinite(h[0] / h[1])) break;  // no descent left
+    const double alpha = h[0] / h[1];
+    pap = h[1];
+    first = false;
+    hipLaunchKernelGGL(k_fcg_update, gv, BS, 0, s, n, alpha, p, ap, x, r, iw.part2);
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
+    GS_LAUNCH_CHECK("k_fcg_update");
+    read_dots(ctx, dots, h, 1);
+    rn = std::sqrt(h[0]);
+    ++res.outer;
+    checked = false;
+    if (rn < target) {
+      // confirm on the true residual r = b - A x
+      launch_residual(s, o, val, x, b, r, iw.part1);
+      hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part1, o.grid, dots);
+      GS_LAUNCH_CHECK("k_sell_residual");
+      read_dots(ctx, dots, h, 1);
       ++res.spmv;
-      res.rnorm = h.rnorm;
-      res.iterations = h.iteration;
-      if (h.rnorm < h.tol_abs || h.rnorm < h.abs_tol) {
+      rn = std::sqrt(h[0]);
+      checked = true;
+      if (rn < target) {
         res.converged = true;
         break;
       }
-      if (h.iteration >= max_iter || res.outer >= max_outer) break;
-      if (h.breakdown) {
-        if (++breakdowns >= 2) break;
-      } else {
-        breakdowns = 0;
-      }
-      // Keep the search direction only while the true residual agrees with
-      // the recursive one; a large gap (the FP32 operator's rounding limits
-      // what one cycle can gain) would make the next step length
-      // inconsistent with p, so the recursion restarts from p = z instead.
-      const double gap = h.rnorm * h.scale / std::max(h.rrec, 1e-300);
-      rescale(h.breakdown != 0 || gap > 2.0);
-      if (gap > 2.0) ++res.restarts;
-      last_rep = h.iteration;
-      h = read_state(ctx, st);
     }
-    long long left = max_iter - h.iteration;
-    if (inner_cap > 0) left = std::min(left, inner_cap - (h.iteration - last_rep));
-    const double target = std::max({h.tol, h.rtol * h.rbase, h.abs_tol / h.bnorm});
-    const long long batch = bt.next(h.rrec, h.iteration, target, left);
-    for (long long i = 0; i < batch; ++i)
-      enqueue_pass<float>(s, o, val32, xs, rs, zs, p, q, dinv, st, h.iteration + i,
-                          part1, part2);
-    GS_LAUNCH_CHECK("cg batch");
-    const long long before = h.iteration;
-    h = read_state(ctx, st);
-    res.spmv += h.iteration - before + (h.breakdown ? 1 : 0);
+    if (hs.iteration >= max_iter) break;
+    const bool agree = !stalled && std::fabs(alpha - 1.0) < 0.25 &&
+                       rn < 2.0 * predicted && !(hs.pq == 0.0);
+    start_inner(!agree, rn);
+    bt = Batcher{};
+    hs = read_state(ctx, iw.st);
+    cycle_start = hs.iteration;
   }
+  if (!checked) {  // report the true residual
+    launch_residual(s, o, val, x, b, r, iw.part1);
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part1, o.grid, dots);
+    GS_LAUNCH_CHECK("k_sell_residual");
+    read_dots(ctx, dots, h, 1);
+    ++res.spmv;
+    rn = std::sqrt(h[0]);
+    res.converged = rn < target;
+  }
+  res.rnorm = rn;
   return res;
 }
 
@@ -929,27 +1061,29 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   ensure_sell(ctx, A, mixed, precond);
   Layout L;
   const size_t ob = L.add<double>(n), ox = L.add<double>(n);
-  // solver scratch follows; size it with a dry layout of the same pieces
+  // size the scratch with a dry run of the same layout
   Layout dry = L;
-  dry.add<double>(GRID_MAX);
-  dry.add<double>(2 * GMAX);
-  dry.add<CgState>(1);
   if (mixed) {
-    dry.add<double>(n);
-    for (int i = 0; i < 5; ++i) dry.add<float>(n);
+    for (int i = 0; i < 5; ++i) dry.add<double>(n);
+    dry.add<double>(8);
+    dry.add<float>(n);
+    carve_pcg<float>(nullptr, dry, n);
   } else {
-    for (int i = 0; i < 4; ++i) dry.add<double>(n);
+    carve_pcg<double>(nullptr, dry, n);
   }
   char *base = ctx_scratch(ctx, dry.bytes);
   double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
   const int *vp = A->sell->vperm.get();
   permute_gather(s, n, b, vp, bi);
   PcgResult r;
-  if (!mixed)
-    r = pcg64(ctx, A, bi, xi, precond, o.max_iterations, o.tolerance,
-              o.abs_tolerance, base, L);
-  else
-    r = pcg_mixed(ctx, A, bi, xi, precond, o, base, L);
+  if (!mixed) {
+    const PcgWork<double> w = carve_pcg<double>(base, L, n);
+    r = pcg_core<double>(ctx, sell_op(A), A->sval.get(),
+                         precond ? A->sdinv.get() : nullptr, bi, xi,
+                         o.max_iterations, o.tolerance, o.abs_tolerance, w);
+  } else {
+    r = pcg_fcg(ctx, A, bi, xi, precond, o, base, L);
+  }
   permute_scatter(s, n, xi, vp, x);
   GS_LAUNCH_CHECK("cg_solve");
   GS_HIP(hipStreamSynchronize(s));

@@ -1,0 +1,157 @@
+"""GPU tests of the SELL-64 operator, the system-plan cache and the
+mixed-precision solver (beyond the oracle parity suite).
+
+SpMV results are compared with scipy in FP64 (summation order differs, so
+within 1e-13 of the row-wise absolute sums); CG solutions with a direct
+solve or the FP64 path.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import geo_simulator_amd as G
+from geo_simulator_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def spd_random(n, density, seed, wide=False):
+    """SPD matrix with row lengths from 1 to well above 64: diagonally
+    dominant sum of a random sparse matrix and its transpose, plus a few
+    dense rows; `wide` places columns far from the diagonal."""
+    rng = np.random.default_rng(seed)
+    k = int(density * n * n)
+    rows = rng.integers(0, n, k)
+    cols = rng.integers(0, n, k)
+    if not wide:
+        cols = np.clip(rows + rng.integers(-200, 201, k), 0, n - 1)
+    dense_rows = rng.choice(n, 3, replace=False)
+    rows = np.concatenate([rows, np.repeat(dense_rows, 150)])
+    cols = np.concatenate([cols] + [rng.choice(n, 150, replace=False) for _ in dense_rows])
+    B = sp.coo_matrix((rng.uniform(0.1, 1.0, len(rows)), (rows, cols)), shape=(n, n)).tocsr()
+    A = (B + B.T).tocsr()
+    A = A + sp.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)
+    A = sp.csr_matrix(A)
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+def check_spmv(M, A):
+    x = np.random.default_rng(3).standard_normal(A.shape[1])
+    y = M.apply(x)
+    ref = A @ x
+    bound = 1e-13 * (abs(A) @ np.abs(x))
+    assert np.all(np.abs(y - ref) <= bound + 1e-300)
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_sell_spmv_and_cg_vs_scipy(wide):
+    """Banded (16-bit SELL offsets) and scattered columns over 70,000 rows
+    (offsets beyond 16 bits: int32 SELL columns)."""
+    n, dens = (5000, 0.002) if not wide else (70000, 4e-5)
+    A = spd_random(n, dens, 11, wide=wide)
+    lens = np.diff(A.indptr)
+    assert lens.max() > 64 and lens.min() >= 1
+    if wide:
+        r = np.repeat(np.arange(n), lens)
+        assert np.abs(A.indices - r).max() > 40000
+    M = G.CsrMatrix.from_scipy(A)
+    check_spmv(M, A)
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    sol = {}
+    for prec in ("fp64", "mixed"):
+        x, st = (G.ConjugateGradient().with_precision(prec).with_tolerance(1e-12)
+                 .with_abs_tolerance(0.0).with_max_iterations(20000).solve(M, b))
+        assert st.converged, prec
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 2e-12, prec
+        sol[prec] = x
+    if not wide:
+        xr = spla.spsolve(A.tocsc(), b)
+        assert np.linalg.norm(sol["fp64"] - xr) / np.linalg.norm(xr) < 1e-10
+    assert np.linalg.norm(sol["mixed"] - sol["fp64"]) / np.linalg.norm(sol["fp64"]) < 1e-10
+
+
+def test_spmv_engines_and_index_widths():
+    A = spd_random(3000, 0.003, 21)
+    M = G.CsrMatrix.from_scipy(A)
+    for flags in (0, G._lib.GS_SPMV_INDEX32, G._lib.GS_SPMV_CSR,
+                  G._lib.GS_SPMV_CSR | G._lib.GS_SPMV_VECTOR,
+                  G._lib.GS_SPMV_CSR | G._lib.GS_SPMV_INDEX32):
+        M.set_spmv_shape(0, 0, flags)
+        check_spmv(M, A)
+    M.set_spmv_shape(0, 96, 0)  # persistent SELL grid
+    check_spmv(M, A)
+    with pytest.raises(G.GeoFemError):
+        M.set_spmv_shape(4, 0, 0)  # lanes belong to the CSR engines
+    # rectangular matrices run on the CSR engine
+    rng = np.random.default_rng(4)
+    R = sp.coo_matrix((rng.standard_normal(7700), (rng.integers(0, 700, 7700),
+                                                  rng.integers(0, 1100, 7700))),
+                      shape=(700, 1100)).tocsr()
+    check_spmv(G.CsrMatrix.from_scipy(R), R)
+
+
+def test_time_spmv_reports_positive():
+    A = spd_random(2000, 0.004, 31)
+    M = G.CsrMatrix.from_scipy(A)
+    assert M.time_spmv("fp64", 3) > 0.0 and M.time_spmv("mixed", 3) > 0.0
+
+
+def test_system_plans_per_dofmap():
+    """Two DOF maps on one mesh: each fused system equals the separate
+    assemble / load / eliminate path bit for bit, also when alternated."""
+    mesh = G.ImprovedMeshGenerator.generate_cube(5, 4, 3, 1000.0, 1000.0, 600.0)
+    z = mesh.nodes[:, 2]
+    d1 = G.DofManager(mesh.num_nodes(), 1)
+    d1.set_dirichlet_many(np.nonzero(np.abs(z) < 1.0)[0], 1280.0)
+    d2 = G.DofManager(mesh.num_nodes(), 1)
+    d2.set_dirichlet_many(np.nonzero(np.abs(z - 600.0) < 1.0)[0], 5.0)
+    d2.set_dirichlet_many(np.array([3, 17]), -2.0)
+    for dof in (d1, d2, d1, d2):
+        K = G.Assembler.assemble_thermal_stiffness_parallel(mesh, dof, 2.5)
+        f = G.Assembler.assemble_thermal_load(mesh, dof, 1e-6)
+        Kb, fb = G.Assembler.apply_dirichlet_bcs(K, f, dof)
+        Ks, fs = G.Assembler.assemble_thermal_system(mesh, dof, 2.5, 1e-6)
+        A, B = Kb.to_scipy(), Ks.to_scipy()
+        assert np.array_equal(A.indptr, B.indptr) and np.array_equal(A.indices, B.indices)
+        assert np.array_equal(A.data, B.data)
+        assert np.array_equal(np.asarray(fb), fs.cpu().numpy())
+
+
+def test_mixed_heterogeneous_elastic_matches_fp64():
+    """Log-normal Young's modulus (sigma 1): the FP32 operator's rounding
+    is far from negligible; the mixed path must still converge to the FP64
+    solution."""
+    mesh, dof, _ = W.elastic_box(G, 6, 6, 4)
+    rng = np.random.default_rng(7)
+    E = np.exp(np.log(1e11) + 1.0 * rng.standard_normal(mesh.num_elements()))
+    import torch
+    Et = torch.tensor(E, device="cuda")
+    K, f = G.Assembler.assemble_elasticity_system(mesh, dof, G.IsotropicElasticity(1.0, 0.25),
+                                                  3300.0, [0, 0, -9.81], E_per_elem=Et)
+    sol = {}
+    for prec in ("fp64", "mixed"):
+        x, st = (G.ConjugateGradient().with_precision(prec).with_tolerance(1e-12)
+                 .with_abs_tolerance(0.0).with_max_iterations(100000).solve(K, f))
+        assert st.converged, prec
+        assert st.relative_residual < 1e-12
+        sol[prec] = x.cpu().numpy()
+    err = np.linalg.norm(sol["mixed"] - sol["fp64"]) / np.linalg.norm(sol["fp64"])
+    assert err < 1e-9
+
+
+def test_mixed_reports_true_residual_and_counts():
+    mesh, dof, exact = W.thermal_box(G, 8, 8, 6)
+    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+    x, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-10)
+             .with_abs_tolerance(0.0).solve(K, f))
+    Ks = K.to_scipy()
+    fh = f.cpu().numpy()
+    xh = x.cpu().numpy()
+    true = np.linalg.norm(fh - Ks @ xh) / np.linalg.norm(fh)
+    assert st.converged and true < 1e-10
+    assert abs(true - st.relative_residual) <= 1e-3 * true + 1e-16
+    assert st.outer_iterations >= 1 and st.spmv_count >= st.iterations
+    assert np.linalg.norm(xh - exact) / np.linalg.norm(exact) < 1e-8

@@ -1,5 +1,5 @@
-// Jacobi-preconditioned CG on gfx950 (FP64, and FP32 with FP64 residual
-// replacement).
+// Jacobi-preconditioned CG on gfx950: FP64, and mixed precision (FP64
+// flexible CG around an FP32 recursion).
 //
 // Control flow is the reference's ConjugateGradient (src/linalg/
 // iterative.rs:46-119) statement for statement:
@@ -847,6 +847,9 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
     long long best_at = hs.iteration;
     while (!hs.pause && !hs.stop) {
       const long long window = std::max(200ll, 2 * longest);
+      if (opt.max_inner_iterations > 0 &&
+          hs.iteration - cycle_start >= opt.max_inner_iterations)
+        break;  // the caller's cap on passes per cycle
       if (hs.rnorm < 0.5 * best) {
         best = hs.rnorm;
         best_at = hs.iteration;
@@ -855,9 +858,10 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
         break;
       }
       const double tgt = std::max(hs.rtol * hs.rbase, hs.tol);
-      const long long batch =
-          bt.next(hs.rnorm, hs.iteration, tgt,
-                  std::min(max_iter - hs.iteration, best_at + window - hs.iteration));
+      long long left = std::min(max_iter - hs.iteration, best_at + window - hs.iteration);
+      if (opt.max_inner_iterations > 0)
+        left = std::min(left, cycle_start + opt.max_inner_iterations - hs.iteration);
+      const long long batch = bt.next(hs.rnorm, hs.iteration, tgt, left);
       for (long long i = 0; i < batch; ++i)
         enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
                             hs.iteration + i, iw.part1, iw.part2);

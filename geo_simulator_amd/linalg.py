@@ -151,9 +151,11 @@ class ConjugateGradient:
     """Builder and solver with the reference's defaults
     (max_iterations 1000, tolerance 1e-8, abs_tolerance 1e-12, Jacobi on).
 
-    Extensions: `with_precision("mixed")` runs FP32 inner PCG inside an FP64
-    defect-correction loop (residual, update and RHS in FP64), stopping on the
-    same FP64 criterion ||b - A x|| < tol ||b|| (or < abs_tol)."""
+    Extensions: `with_precision("mixed")` runs FP64 flexible CG around an
+    FP32 Jacobi-PCG recursion (residual, solution and search directions in
+    FP64; the FP32 recursion reduces its residual by `inner_tolerance` between
+    FP64 steps), stopping on the same FP64 criterion ||b - A x|| < tol ||b||
+    (or < abs_tol)."""
 
     name = "ConjugateGradient"
 
@@ -199,6 +201,10 @@ class ConjugateGradient:
 
     def with_max_outer_iterations(self, n):
         self.max_outer_iterations = int(n)
+        return self
+
+    def with_max_inner_iterations(self, n):
+        self.max_inner_iterations = int(n)
         return self
 
     def opts(self) -> _lib.CgOpts:

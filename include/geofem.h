@@ -152,11 +152,12 @@ typedef struct {
   double abs_tolerance;      /* default 1e-12 */
   int32_t use_preconditioner;/* default 1 (Jacobi) */
   int32_t precision;         /* gs_precision, default GS_PREC_FP64 */
-  double inner_tolerance;    /* mixed: FP64 residual replacement once the FP32
-                                recursive residual fell by this factor (1e-2) */
-  int64_t max_inner_iterations; /* mixed: cap on FP32 passes between
-                                   replacements (0 = none) */
-  int64_t max_outer_iterations; /* mixed: cap on replacements (0 = 100) */
+  double inner_tolerance;    /* mixed: an FP64 flexible-CG step each time the
+                                FP32 recursion's residual fell by this
+                                factor (default 1e-2) */
+  int64_t max_inner_iterations; /* mixed: cap on FP32 passes between FP64
+                                   steps (0 = none) */
+  int64_t max_outer_iterations; /* mixed: cap on FP64 steps (0 = 100) */
 } gs_cg_opts;
 
 /* SolverStats (src/linalg/solver.rs:5-20) plus mixed-precision counters. */
@@ -167,7 +168,7 @@ typedef struct {
   int32_t converged;
   int32_t pad_;
   double solve_time;         /* seconds, host wall clock */
-  int64_t outer_iterations;  /* mixed: FP64 residual replacements */
+  int64_t outer_iterations;  /* mixed: FP64 flexible-CG steps */
   int64_t spmv_count;        /* SpMV launches issued */
 } gs_solver_stats;
 

@@ -155,3 +155,27 @@ def test_mixed_reports_true_residual_and_counts():
     assert abs(true - st.relative_residual) <= 1e-3 * true + 1e-16
     assert st.outer_iterations >= 1 and st.spmv_count >= st.iterations
     assert np.linalg.norm(xh - exact) / np.linalg.norm(exact) < 1e-8
+
+
+def test_mixed_inner_cap_and_outer_cap():
+    """max_inner_iterations bounds the FP32 passes between FP64 steps (more
+    outer steps, same answer); max_outer_iterations stops the solve
+    unconverged with the true residual reported."""
+    mesh, dof, exact = W.thermal_box(G, 8, 8, 6)
+    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+    base = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-10)
+            .with_abs_tolerance(0.0))
+    x0, s0 = base.solve(K, f)
+    capped = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-10)
+              .with_abs_tolerance(0.0).with_max_inner_iterations(5)
+              .with_max_outer_iterations(1000))
+    x1, s1 = capped.solve(K, f)
+    assert s0.converged and s1.converged
+    assert s1.outer_iterations > s0.outer_iterations
+    xa, xb = x0.cpu().numpy(), x1.cpu().numpy()
+    assert np.linalg.norm(xa - xb) / np.linalg.norm(xa) < 1e-8
+    one = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-14)
+           .with_abs_tolerance(0.0).with_max_inner_iterations(3)
+           .with_max_outer_iterations(1))
+    _, s2 = one.solve(K, f)
+    assert not s2.converged and s2.outer_iterations == 1 and s2.relative_residual > 1e-14

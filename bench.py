@@ -361,7 +361,44 @@ def run_adjoint(G, args):
                              seconds=time.perf_counter() - t0)
             log(f"[bench] adjoint {case} FD ({len(elems)} elems): max rel {rel.max():.2e} "
                 f"median {np.median(rel):.2e}; random direction {res['fd']['direction_rel_err']:.2e}")
+        if args.oracle_adjoint and case == "thermal":
+            res["oracle_check"] = oracle_thermal_adjoint(theta, theta_true, obs, g, g64)
         out[case] = res
+    return out
+
+
+def oracle_thermal_adjoint(theta, theta_true, obs, g_mixed, g_fp64):
+    """The whole thermal inversion gradient recomputed by the oracle (C
+    restatement, serial CG as in the reference) on the host: observation
+    solve, forward solve, adjoint solve, element backward."""
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    threads = min(os.cpu_count() or 1, 16)
+    xyz, conn = O.generate_cube(64, 64, 32, 1000.0, 1000.0, 600.0)
+    mask, vals = O.thermal_dirichlet(xyz, 600.0)
+    f = O.assemble_thermal_load(xyz, conn, 1e-6)
+
+    def solve(k, rhs=None):
+        K = O.assemble_thermal(xyz, conn, 1.0, k_elem=k, nthreads=threads)
+        Kb, fb = O.apply_dirichlet(K, f, mask, vals)
+        x, st = O.cg_solve(Kb, fb if rhs is None else rhs, 100000, 1e-12, 0.0)
+        return Kb, x, st
+
+    k_true, k = np.exp(theta_true), np.exp(theta)
+    _, t_true, _ = solve(k_true)
+    Kb, T, st_f = solve(k)
+    rhs = np.zeros(len(T))
+    rhs[obs] = T[obs] - t_true[obs]
+    rhs[mask == 1] = 0.0
+    lam, st_a = O.cg_solve(Kb, rhs, 100000, 1e-12, 0.0)
+    g_or = O.thermal_element_backward(xyz, conn, k, T, lam)
+    scale = np.abs(g_or).max()
+    out = dict(seconds=time.perf_counter() - t0, threads_assembly=threads,
+               forward_iterations=st_f.iterations, adjoint_iterations=st_a.iterations,
+               mixed_vs_oracle_max_rel=float(np.abs(g_mixed - g_or).max() / scale),
+               fp64_vs_oracle_max_rel=float(np.abs(g_fp64 - g_or).max() / scale))
+    log(f"[bench] oracle thermal adjoint ({out['seconds']:.0f} s): mixed vs oracle "
+        f"{out['mixed_vs_oracle_max_rel']:.2e}, fp64 vs oracle {out['fp64_vs_oracle_max_rel']:.2e}")
     return out
 
 
@@ -381,6 +418,9 @@ def main():
     ap.add_argument("--headline-only", action="store_true",
                     help="skip the elastic and adjoint sections")
     ap.add_argument("--fd-elems", type=int, default=32)
+    ap.add_argument("--oracle-adjoint", action="store_true",
+                    help="also recompute the thermal inversion gradient with the CPU "
+                         "oracle (about a minute of host time)")
     ap.add_argument("--spmv-reps", type=int, default=50)
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="also run a plain device copy of this size (counter "

@@ -174,12 +174,12 @@ __global__ void k_gather_elastic(int64_t nnz, const int *__restrict__ rp,
     const int e = t / 100;
     const int ab = t - 100 * e;
     const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const double *Kr = Ke + (int64_t)e * 900 + (3 * a) * 30 + 3 * b;
+    const double *Kr = Ke + (int64_t)e * 900 + (a * 10 + b) * 9;
 #pragma unroll
     for (int ci = 0; ci < 3; ++ci)
 #pragma unroll
       for (int cj = 0; cj < 3; ++cj) {
-        const double x = Kr[ci * 30 + cj];
+        const double x = Kr[ci * 3 + cj];
         v[3 * ci + cj] = (k == b0) ? x : v[3 * ci + cj] + x;
       }
   }
@@ -492,12 +492,12 @@ __global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
     const int e = t / 100;
     const int ab = t - 100 * e;
     const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const double *Kr = Ke + (int64_t)e * 900 + (3 * a) * 30 + 3 * b;
+    const double *Kr = Ke + (int64_t)e * 900 + (a * 10 + b) * 9;
 #pragma unroll
     for (int ci = 0; ci < 3; ++ci)
 #pragma unroll
       for (int cj = 0; cj < 3; ++cj) {
-        const double x = Kr[ci * 30 + cj];
+        const double x = Kr[ci * 3 + cj];
         v[3 * ci + cj] = (k == b0) ? x : v[3 * ci + cj] + x;
       }
   }
@@ -539,7 +539,7 @@ __global__ void k_sys_diagmax(int64_t n_nodes, const int *__restrict__ diag_slot
 #pragma unroll
       for (int c = 0; c < DPN; ++c) {
         const double x = DPN == 1 ? Ke[(int64_t)e * 55 + tri_index(a, a)]
-                                  : Ke[(int64_t)e * 900 + (3 * a + c) * 30 + 3 * a + c];
+                                  : Ke[(int64_t)e * 900 + (a * 10 + a) * 9 + c * 3 + c];
         v[c] = (k == b0) ? x : v[c] + x;
       }
     }

@@ -199,8 +199,17 @@ __global__ __launch_bounds__(128) void k_thermal_elements(
 }
 
 // ---------------------------------------------------------------------------
-// Elasticity: 32 lanes per element (two elements per wavefront); lane a < 30
-// produces row a of the 30x30 K_e and, optionally, f_e[a].
+// Elasticity: 32 lanes per element, four elements per 128-thread workgroup.
+//
+//   1. lanes 0..29 stage the ten node coordinates in LDS; lanes 0..3 form J,
+//      det and J^-1 at quadrature point q = lane; lanes then form the forty
+//      gradients grad N_i(q) (the same expression as `gradients`), all into
+//      LDS, so each is computed once per element instead of once per lane;
+//   2. lane a < 30 accumulates row a of the 30x30 K_e and f_e[a];
+//   3. the rows go through LDS and leave in block-major order
+//      Ke[e*900 + (i*10 + j)*9 + ci*3 + cj] for entry (3i+ci, 3j+cj): every
+//      store of the workgroup is contiguous, and the gathers read each 3x3
+//      node block as 72 contiguous bytes.
 //
 // Row a = 3i+ca of B^T (D B) only meets the three Voigt rows k where B[k][a]
 // is structurally nonzero (ca=0: k=0,3,5; ca=1: 1,3,4; ca=2: 2,4,5) and
@@ -208,69 +217,110 @@ __global__ __launch_bounds__(128) void k_thermal_elements(
 // performs exactly the nonzero products of the dense reference product in
 // the same order; the dropped terms are exact zeros.
 // ---------------------------------------------------------------------------
+constexpr int EL_PER_WG = 4;
+
 template <bool PER_ELEM, bool WITH_LOAD>
 __global__ __launch_bounds__(128) void k_elastic_elements(
     int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
     double Emod, double nu, const double *__restrict__ E_elem, double bfx,
     double bfy, double bfz, double *__restrict__ Ke, double *__restrict__ Fe,
     int *__restrict__ flag) {
-  const int64_t gt = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t e = gt >> 5;
-  const int a = (int)(gt & 31);
-  if (e >= E || a >= 30) return;
-  const int ia = a / 3, ca = a - 3 * (a / 3);
-  double X[10][3];
-  load_nodes(xyz, conn, e, X);
-  const double Ee = PER_ELEM ? E_elem[e] : Emod;
-  int bad = (Ee > 0.0) ? 0 : FLAG_MATERIAL;
-  // D (src/mechanics/constitutive.rs:63-102): unscaled entries times factor.
-  const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
-  const double Dd = (1.0 - nu) * factor;
-  const double Do = nu * factor;
-  const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
-  double K[30];
+  __shared__ double sX[EL_PER_WG][10][3];
+  __shared__ double sJi[EL_PER_WG][4][3][3];
+  __shared__ double sW[EL_PER_WG][4];
+  __shared__ double sG[EL_PER_WG][4][10][3];
+  __shared__ double sK[EL_PER_WG][900];
+  const int le = threadIdx.x >> 5;  // element within the workgroup
+  const int a = threadIdx.x & 31;   // lane within the element
+  const int64_t e = (int64_t)blockIdx.x * EL_PER_WG + le;
+  const bool live = e < E;
+  int bad = 0;
+  if (live && a < 30) {
+    const int node = a / 3, r = a - 3 * (a / 3);
+    sX[le][node][r] = xyz[(int64_t)conn[e * 10 + node] * 3 + r];
+  }
+  __syncthreads();
+  if (live && a < 4) {
+    double X[10][3], J[3][3], Ji[3][3];
 #pragma unroll
-  for (int b = 0; b < 30; ++b) K[b] = 0.0;
-  double f = 0.0;
-  const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
+    for (int i = 0; i < 10; ++i)
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    double J[3][3], Ji[3][3], G[10][3];
-    jacobian(q, X, J);
+      for (int r = 0; r < 3; ++r) X[i][r] = sX[le][i][r];
+    jacobian(a, X, J);
     const double det = inverse3(J, Ji);
     bad |= det_flags(det);
-    gradients(q, Ji, G);
-    const double w = QW * fabs(det);
-    // B[k][a] for the three structural Voigt rows of column a
-    const double gi0 = G[ia][0], gi1 = G[ia][1], gi2 = G[ia][2];
-    const double Bk1 = ca == 0 ? gi0 : (ca == 1 ? gi1 : gi2);  // k1 = ca
-    const double Bk2 = ca == 0 ? gi1 : (ca == 1 ? gi0 : gi1);  // k2 = 3,3,4
-    const double Bk3 = ca == 0 ? gi2 : (ca == 1 ? gi2 : gi0);  // k3 = 5,4,5
 #pragma unroll
-    for (int j = 0; j < 10; ++j) {
-      const double g0 = G[j][0], g1 = G[j][1], g2 = G[j][2];
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-      for (int cb = 0; cb < 3; ++cb) {
-        const double gjc = cb == 0 ? g0 : (cb == 1 ? g1 : g2);
-        // (D B)[k1][b] = D[ca][cb] * B[cb][b]
-        const double db1 = (ca == cb ? Dd : Do) * gjc;
-        // (D B)[k][b] = Ds * B[k][b] for shear rows k = 3, 4, 5
-        const double B3 = cb == 0 ? g1 : (cb == 1 ? g0 : 0.0);
-        const double B4 = cb == 0 ? 0.0 : (cb == 1 ? g2 : g1);
-        const double B5 = cb == 0 ? g2 : (cb == 1 ? 0.0 : g0);
-        const double db2 = Ds * (ca == 2 ? B4 : B3);
-        const double db3 = Ds * (ca == 1 ? B4 : B5);
-        const double s = Bk1 * db1 + Bk2 * db2 + Bk3 * db3;
-        K[3 * j + cb] += w * s;
-      }
-    }
-    if (WITH_LOAD) f += w * kT.N[q][ia] * bfa;
+      for (int c = 0; c < 3; ++c) sJi[le][a][r][c] = Ji[r][c];
+    sW[le][a] = QW * fabs(det);
   }
-  if (bad && a == 0) atomicOr(flag, bad);
-  double *out = Ke + e * 900 + a * 30;
+  __syncthreads();
+  if (live) {
+    for (int t = a; t < 40; t += 32) {
+      const int q = t / 10, i = t - 10 * (t / 10);
+      const double v0 = kT.dl[q][i][0], v1 = kT.dl[q][i][1], v2 = kT.dl[q][i][2];
 #pragma unroll
-  for (int b = 0; b < 30; ++b) out[b] = K[b];
-  if (WITH_LOAD) Fe[e * 30 + a] = f;
+      for (int r = 0; r < 3; ++r)
+        sG[le][q][i][r] = sJi[le][q][0][r] * v0 + sJi[le][q][1][r] * v1 +
+                          sJi[le][q][2][r] * v2;
+    }
+  }
+  __syncthreads();
+  if (live && a < 30) {
+    const int ia = a / 3, ca = a - 3 * (a / 3);
+    const double Ee = PER_ELEM ? E_elem[e] : Emod;
+    if (!(Ee > 0.0)) bad |= FLAG_MATERIAL;
+    // D (src/mechanics/constitutive.rs:63-102): unscaled entries times factor.
+    const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    const double Dd = (1.0 - nu) * factor;
+    const double Do = nu * factor;
+    const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+    double K[30];
+#pragma unroll
+    for (int b = 0; b < 30; ++b) K[b] = 0.0;
+    double f = 0.0;
+    const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const double w = sW[le][q];
+      // B[k][a] for the three structural Voigt rows of column a
+      const double gi0 = sG[le][q][ia][0], gi1 = sG[le][q][ia][1], gi2 = sG[le][q][ia][2];
+      const double Bk1 = ca == 0 ? gi0 : (ca == 1 ? gi1 : gi2);  // k1 = ca
+      const double Bk2 = ca == 0 ? gi1 : (ca == 1 ? gi0 : gi1);  // k2 = 3,3,4
+      const double Bk3 = ca == 0 ? gi2 : (ca == 1 ? gi2 : gi0);  // k3 = 5,4,5
+#pragma unroll
+      for (int j = 0; j < 10; ++j) {
+        const double g0 = sG[le][q][j][0], g1 = sG[le][q][j][1], g2 = sG[le][q][j][2];
+#pragma unroll
+        for (int cb = 0; cb < 3; ++cb) {
+          const double gjc = cb == 0 ? g0 : (cb == 1 ? g1 : g2);
+          // (D B)[k1][b] = D[ca][cb] * B[cb][b]
+          const double db1 = (ca == cb ? Dd : Do) * gjc;
+          // (D B)[k][b] = Ds * B[k][b] for shear rows k = 3, 4, 5
+          const double B3 = cb == 0 ? g1 : (cb == 1 ? g0 : 0.0);
+          const double B4 = cb == 0 ? 0.0 : (cb == 1 ? g2 : g1);
+          const double B5 = cb == 0 ? g2 : (cb == 1 ? 0.0 : g0);
+          const double db2 = Ds * (ca == 2 ? B4 : B3);
+          const double db3 = Ds * (ca == 1 ? B4 : B5);
+          const double s = Bk1 * db1 + Bk2 * db2 + Bk3 * db3;
+          K[3 * j + cb] += w * s;
+        }
+      }
+      if (WITH_LOAD) f += w * kT.N[q][ia] * bfa;
+    }
+#pragma unroll
+    for (int j = 0; j < 10; ++j)
+#pragma unroll
+      for (int cb = 0; cb < 3; ++cb) sK[le][(ia * 10 + j) * 9 + ca * 3 + cb] = K[3 * j + cb];
+    if (WITH_LOAD) Fe[e * 30 + a] = f;
+  }
+  if (bad) atomicOr(flag, bad);
+  __syncthreads();
+  if (live) {
+    double *out = Ke + e * 900;
+    for (int t = a; t < 900; t += 32) out[t] = sK[le][t];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -408,7 +458,7 @@ void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const double *E_elem, double rho, const double g3[3],
                              bool with_load, double *Ke, double *Fe, int *flag) {
   if (E == 0) return;
-  const unsigned g = blocks_for(E * 32, 128);
+  const unsigned g = blocks_for(E, EL_PER_WG);
   const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
   if (E_elem) {
     if (with_load)

@@ -229,6 +229,16 @@ def run_elastic(G, args):
     out["spmv"] = {"fp64_us": ms64 * 1e3, "fp64_GBs": b64 / ms64 / 1e6,
                    "fp32_us": ms32 * 1e3, "fp32_GBs": b32 / ms32 / 1e6}
     log(f"[bench] elastic SpMV fp64 {ms64*1e3:.1f} us fp32 {ms32*1e3:.1f} us")
+    # element kernel alone: K_e (900 doubles), F_e (30) and the connectivity
+    # per element, each node's coordinates once
+    eb = mesh.num_elements() * (900 * 8 + 30 * 8 + 40) + mesh.num_nodes() * 24
+    kv = {"algorithmic_bytes": eb}
+    for v in ("valu", "mfma"):
+        ms, d = G.time_elastic_elements(mesh, mat, None, v, reps=20)
+        kv[v] = {"ms": ms, "GBs": eb / ms / 1e6, "rel_diff_vs_valu": d}
+        log(f"[bench] elastic element kernel {v}: {ms:.3f} ms ({eb / ms / 1e6:.0f} GB/s) "
+            f"max rel diff {d:.1e}")
+    out["element_kernel"] = kv
     return out
 
 

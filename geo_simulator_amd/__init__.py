@@ -17,6 +17,7 @@ from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
 from .dof import DofManager
 from .linalg import ConjugateGradient, CsrMatrix, JacobiPreconditioner, SolverStats, norm
 from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
+                       time_elastic_elements,
                        thermal_element_backward)
 from .runtime import Context, default_context
 
@@ -24,7 +25,8 @@ __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "CsrMatrix", "JacobiPreconditioner",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
-    "thermal_element_backward", "elastic_element_backward", "Context",
+    "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
+    "Context",
     "default_context",
 ]
 

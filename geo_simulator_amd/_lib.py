@@ -111,6 +111,8 @@ SIGNATURES = {
                                               _vp, _vp]),
     "gs_csr_set_spmv_shape": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
     "gs_time_spmv": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _dp]),
+    "gs_time_elastic_elements": (C.c_int, [_vp, _vp, C.c_double, C.c_double, _vp,
+                                           C.c_int32, C.c_int32, _dp, _dp]),
 }
 
 _lib = None
@@ -125,6 +127,9 @@ def lib():
                 f"{LIB_PATH} is missing: build it with "
                 "`python -m geo_simulator_amd.csrc.build` (there is no "
                 "non-HIP fallback)")
+        # torch's HIP runtime first: the library then binds to the same
+        # runtime instance instead of loading a second one beside it
+        import torch  # noqa: F401
         L = C.CDLL(LIB_PATH)
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(L, name)

@@ -203,3 +203,27 @@ def elastic_element_backward(mesh, E_per_elem, poisson, u, lam, ctx=None):
                                                 dptr(ud), dptr(ld), dptr(g)),
               "elastic_element_backward")
     return g
+
+
+ELEMENT_VARIANTS = {"valu": 0, "mfma": 1}
+
+
+def time_elastic_elements(mesh, material, E_per_elem=None, variant="valu", reps=10,
+                          ctx=None):
+    """Average time (ms) of one launch of the elastic element kernel (K_e and
+    gravity load of every element) and the largest K_e entry difference from
+    the VALU kernel relative to the largest entry -> (ms, rel_diff).
+    variant "valu" is the kernel assembly uses; "mfma" evaluates
+    K_e = sum_q w_q B_q^T D B_q as one 30x24 by 24x30 product on the matrix
+    cores (v_mfma_f64_16x16x4f64)."""
+    ctx = ctx or default_context()
+    if variant not in ELEMENT_VARIANTS:
+        raise ValueError(f"unknown element kernel variant {variant!r}")
+    ee = _elem_array(ctx, E_per_elem, mesh.num_elements())
+    ms, rel = C.c_double(), C.c_double()
+    ctx.sync_torch()
+    ctx.check(lib().gs_time_elastic_elements(
+        ctx.handle, mesh.handle(ctx), material.youngs_modulus, material.poisson_ratio,
+        dptr(ee), ELEMENT_VARIANTS[variant], int(reps), C.byref(ms), C.byref(rel)),
+        "time_elastic_elements")
+    return ms.value, rel.value

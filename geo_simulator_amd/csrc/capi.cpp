@@ -273,6 +273,12 @@ struct EdgeTable {
 
 extern "C" {
 
+namespace {
+// message of the last failed gs_ctx_create on this thread (there is no
+// context to hold it)
+thread_local std::string create_error;
+}  // namespace
+
 gs_status gs_ctx_create(int device, gs_ctx **out) {
   if (!out) return GS_ERR_INVALID_ARG;
   *out = nullptr;
@@ -288,9 +294,11 @@ gs_status gs_ctx_create(int device, gs_ctx **out) {
     GS_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
   });
   if (st != GS_OK) {
+    create_error = "gs_ctx_create: " + ctx->err;
     gs_ctx_destroy(ctx);
     return st;
   }
+  create_error.clear();
   *out = ctx;
   return GS_OK;
 }
@@ -307,7 +315,7 @@ void gs_ctx_destroy(gs_ctx *ctx) {
 }
 
 const char *gs_last_error(const gs_ctx *ctx) {
-  return ctx ? ctx->err.c_str() : "null context";
+  return ctx ? ctx->err.c_str() : create_error.c_str();
 }
 
 gs_status gs_ctx_synchronize(gs_ctx *ctx) {
@@ -874,6 +882,67 @@ gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
   A->spmv_grid = workgroups;
   A->spmv_flags = flags;
   return GS_OK;
+}
+
+gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
+                                   double nu, const double *E_elem,
+                                   int32_t variant, int32_t reps, double *avg_ms,
+                                   double *max_rel_diff) {
+  if (!ctx || !mesh || !avg_ms || !max_rel_diff) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(variant == 0 || variant == 1, GS_ERR_INVALID_ARG, "unknown kernel variant");
+    require(reps >= 1, GS_ERR_INVALID_ARG, "reps must be positive");
+    if (E_elem) {
+      require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
+              "Poisson's ratio must be in (-1, 0.5) for physical stability");
+    } else {
+      check_material(E, nu);
+    }
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    const int64_t ne = mesh->n_elems;
+    const auto &pool = ctx->pool;
+    auto ref = palloc<double>(pool, ne * 900);
+    auto Ke = palloc<double>(pool, ne * 900);
+    auto Fe = palloc<double>(pool, ne * 30);
+    auto flag = palloc<int>(pool, 1);
+    auto bits = palloc<unsigned long long>(pool, 2);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    GS_HIP(hipMemsetAsync(bits.get(), 0, 2 * sizeof(unsigned long long), s));
+    const double g[3] = {0.0, 0.0, -9.81};
+    auto run = [&](int v, double *out) {
+      if (v == 0)
+        launch_elastic_elements(s, ne, mesh->xyz.get(), mesh->conn.get(), E, nu, E_elem,
+                                3300.0, g, true, out, Fe.get(), flag.get());
+      else
+        launch_elastic_elements_mfma(s, ne, mesh->xyz.get(), mesh->conn.get(), E, nu,
+                                     E_elem, 3300.0, g, true, out, Fe.get(), flag.get());
+    };
+    run(0, ref.get());
+    run(variant, Ke.get());
+    hipEvent_t e0, e1;
+    GS_HIP(hipEventCreate(&e0));
+    GS_HIP(hipEventCreate(&e1));
+    GS_HIP(hipEventRecord(e0, s));
+    for (int i = 0; i < reps; ++i) run(variant, Ke.get());
+    GS_HIP(hipEventRecord(e1, s));
+    GS_HIP(hipEventSynchronize(e1));
+    float ms = 0.f;
+    GS_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    check_flags(read_flag_and_clear(ctx, flag.get()));
+    launch_maxdiff(s, ne * 900, Ke.get(), ref.get(), bits.get());
+    unsigned long long hb[2] = {0, 0};
+    GS_HIP(hipMemcpyAsync(ctx->pinned, bits.get(), sizeof(hb), hipMemcpyDeviceToHost, s));
+    GS_HIP(hipStreamSynchronize(s));
+    memcpy(hb, ctx->pinned, sizeof(hb));
+    double md, mb;
+    memcpy(&md, &hb[0], sizeof(double));
+    memcpy(&mb, &hb[1], sizeof(double));
+    *avg_ms = (double)ms / reps;
+    *max_rel_diff = mb > 0.0 ? md / mb : md;
+  });
 }
 
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,

@@ -324,6 +324,176 @@ __global__ __launch_bounds__(128) void k_elastic_elements(
 }
 
 // ---------------------------------------------------------------------------
+// Elasticity on the matrix cores (timing variant; not bit-identical).
+//
+// K_e = sum_q w_q B_q^T (D B_q) is one product of a 30x24 and a 24x30 matrix,
+// the 24 inner indices being (q, Voigt row v): A[a][6q+v] = w_q B_q[v][a],
+// C[6q+v][b] = (D B_q)[v][b].  A wavefront per element forms it with
+// v_mfma_f64_16x16x4f64 on a 2x2 grid of 16x16 output tiles (rows and
+// columns 30, 31 are zero padding), six k-steps each.  Lane l supplies
+// A[16I + l%16][4s + l/16] and C[4s + l/16][16J + l%16] and holds
+// K[16I + 4t + l/16][16J + l%16] in element t of its accumulator.  The
+// geometry (J^-1, w, gradients) is shared through LDS as in the VALU kernel;
+// the matrix cores sum in their own order, so entries differ from the VALU
+// kernel's by rounding only.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+
+// B_q[v][a] for a = 3i+c (src/mechanics/strain.rs:38-71 Voigt layout)
+__device__ __forceinline__ double b_entry(int v, int c, double g0, double g1, double g2) {
+  const double gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
+  switch (v) {
+  case 0: case 1: case 2: return v == c ? gc : 0.0;
+  case 3: return c == 0 ? g1 : (c == 1 ? g0 : 0.0);
+  case 4: return c == 0 ? 0.0 : (c == 1 ? g2 : g1);
+  default: return c == 0 ? g2 : (c == 1 ? 0.0 : g0);
+  }
+}
+
+template <bool PER_ELEM, bool WITH_LOAD>
+__global__ __launch_bounds__(256) void k_elastic_elements_mfma(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    double Emod, double nu, const double *__restrict__ E_elem, double bfx,
+    double bfy, double bfz, double *__restrict__ Ke, double *__restrict__ Fe,
+    int *__restrict__ flag) {
+  __shared__ double sX[EL_PER_WG][10][3];
+  __shared__ double sJi[EL_PER_WG][4][3][3];
+  __shared__ double sW[EL_PER_WG][4];
+  __shared__ double sG[EL_PER_WG][4][10][3];
+  __shared__ double sK[EL_PER_WG][900];
+  const int le = threadIdx.x >> 6;  // element (wavefront) within the workgroup
+  const int l = threadIdx.x & 63;   // lane
+  const int64_t e = (int64_t)blockIdx.x * EL_PER_WG + le;
+  const bool live = e < E;
+  int bad = 0;
+  if (live && l < 30) {
+    const int node = l / 3, r = l - 3 * (l / 3);
+    sX[le][node][r] = xyz[(int64_t)conn[e * 10 + node] * 3 + r];
+  }
+  __syncthreads();
+  if (live && l < 4) {
+    double X[10][3], J[3][3], Ji[3][3];
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) X[i][r] = sX[le][i][r];
+    jacobian(l, X, J);
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) sJi[le][l][r][c] = Ji[r][c];
+    sW[le][l] = QW * fabs(det);
+  }
+  __syncthreads();
+  if (live && l < 40) {
+    const int q = l / 10, i = l - 10 * (l / 10);
+    const double v0 = kT.dl[q][i][0], v1 = kT.dl[q][i][1], v2 = kT.dl[q][i][2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+      sG[le][q][i][r] =
+          sJi[le][q][0][r] * v0 + sJi[le][q][1][r] * v1 + sJi[le][q][2][r] * v2;
+  }
+  __syncthreads();
+  const double Ee = live ? (PER_ELEM ? E_elem[e] : Emod) : 1.0;
+  if (live && l == 0 && !(Ee > 0.0)) bad |= FLAG_MATERIAL;
+  const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+  const double Dd = (1.0 - nu) * factor;
+  const double Do = nu * factor;
+  const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+  if (live) {
+    // operands: a_op[I][s] = A[16I + l%16][4s + l/16], c_op[J][s] = C[4s + l/16][16J + l%16]
+    double a_op[2][6], c_op[2][6];
+#pragma unroll
+    for (int s6 = 0; s6 < 6; ++s6) {
+      const int kk = 4 * s6 + (l >> 4);
+      const int q = kk / 6, v = kk - 6 * (kk / 6);
+      const double w = sW[le][q];
+#pragma unroll
+      for (int T = 0; T < 2; ++T) {
+        const int a = 16 * T + (l & 15);
+        double av = 0.0, cv = 0.0;
+        if (a < 30) {
+          const int i = a / 3, c = a - 3 * (a / 3);
+          const double g0 = sG[le][q][i][0], g1 = sG[le][q][i][1], g2 = sG[le][q][i][2];
+          const double bv = b_entry(v, c, g0, g1, g2);
+          av = w * bv;
+          if (v < 3) {
+            const double gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
+            cv = (v == c ? Dd : Do) * gc;
+          } else {
+            cv = Ds * bv;
+          }
+        }
+        a_op[T][s6] = av;
+        c_op[T][s6] = cv;
+      }
+    }
+    dbl4 acc[2][2];
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int J = 0; J < 2; ++J) {
+        acc[I][J] = (dbl4){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int s6 = 0; s6 < 6; ++s6)
+          acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op[I][s6], c_op[J][s6],
+                                                           acc[I][J], 0, 0, 0);
+      }
+#pragma unroll
+    for (int I = 0; I < 2; ++I)
+#pragma unroll
+      for (int J = 0; J < 2; ++J)
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          const int row = 16 * I + 4 * t + (l >> 4), col = 16 * J + (l & 15);
+          if (row < 30 && col < 30)
+            sK[le][((row / 3) * 10 + col / 3) * 9 + (row % 3) * 3 + col % 3] = acc[I][J][t];
+        }
+    if (WITH_LOAD && l < 30) {
+      const int ia = l / 3, ca = l - 3 * (l / 3);
+      const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
+      double f = 0.0;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) f += sW[le][q] * kT.N[q][ia] * bfa;
+      Fe[e * 30 + l] = f;
+    }
+  }
+  if (bad) atomicOr(flag, bad);
+  __syncthreads();
+  if (live) {
+    double *out = Ke + e * 900;
+    for (int t = l; t < 900; t += 64) out[t] = sK[le][t];
+  }
+}
+
+// max |a - b| and max |b| as non-negative double bit patterns (integer max is
+// order independent)
+__global__ void k_maxdiff(int64_t n, const double *__restrict__ a,
+                          const double *__restrict__ b,
+                          unsigned long long *__restrict__ out) {
+  unsigned long long md = 0, mb = 0;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double d = fabs(a[i] - b[i]), v = fabs(b[i]);
+    const unsigned long long bd = (unsigned long long)__double_as_longlong(d == d ? d : __builtin_inf());
+    const unsigned long long bv = (unsigned long long)__double_as_longlong(v);
+    md = bd > md ? bd : md;
+    mb = bv > mb ? bv : mb;
+  }
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long od = __shfl_xor(md, off, 64), ob = __shfl_xor(mb, off, 64);
+    md = od > md ? od : md;
+    mb = ob > mb ? ob : mb;
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out, md);
+    atomicMax(out + 1, mb);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint element backward: grad_e = -lambda_e^T K_e u_e, evaluated per
 // quadrature point as -sum_q w_q k (G^T lambda).(G^T u) (thermal) and
 // -sum_q w_q (B lambda)^T D (B u) (elastic), one thread per element.
@@ -492,6 +662,34 @@ void launch_elastic_backward(hipStream_t s, int64_t E, const double *xyz,
   hipLaunchKernelGGL(k_elastic_backward, blocks_for(E, 128), 128, 0, s, E, xyz,
                      conn, E_elem, nu, u, lam, grad, flag);
   GS_LAUNCH_CHECK("k_elastic_backward");
+}
+
+void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
+                                  const int *conn, double Emod, double nu,
+                                  const double *E_elem, double rho, const double g3[3],
+                                  bool with_load, double *Ke, double *Fe, int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E, EL_PER_WG);
+  const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
+  if (E_elem) {
+    if (with_load)
+      hipLaunchKernelGGL((k_elastic_elements_mfma<true, true>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_elastic_elements_mfma<true, false>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+  } else {
+    if (with_load)
+      hipLaunchKernelGGL((k_elastic_elements_mfma<false, true>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_elastic_elements_mfma<false, false>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+  }
+  GS_LAUNCH_CHECK("k_elastic_elements_mfma");
+}
+
+void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,
+                    unsigned long long *out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_maxdiff, 2048, 256, 0, s, n, a, b, out);
+  GS_LAUNCH_CHECK("k_maxdiff");
 }
 
 }  // namespace gs

@@ -230,6 +230,15 @@ void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double Emod, double nu,
                              const double *E_elem, double rho, const double g[3],
                              bool with_load, double *Ke, double *Fe, int *flag);
+// Matrix-core variant of the elastic element kernel (timing; entries equal
+// the VALU kernel's up to rounding).
+void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
+                                  const int *conn, double Emod, double nu,
+                                  const double *E_elem, double rho, const double g[3],
+                                  bool with_load, double *Ke, double *Fe, int *flag);
+// out[0] = bits of max |a - b|, out[1] = bits of max |b| (out preset by caller)
+void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,
+                    unsigned long long *out);
 void launch_thermal_backward(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, const double *k_elem,
                              const double *u, const double *lam, double *grad,

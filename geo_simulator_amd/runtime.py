@@ -31,7 +31,10 @@ class Context:
             raise RuntimeError("no HIP device is visible: libgeofem needs a GPU")
         self.device = torch.cuda.current_device() if device is None else int(device)
         h = C.c_void_p()
-        check(lib().gs_ctx_create(self.device, C.byref(h)), None, "gs_ctx_create")
+        st = lib().gs_ctx_create(self.device, C.byref(h))
+        if st != _lib.GS_OK:
+            raw = lib().gs_last_error(None)
+            raise _lib.GeoFemError(st, raw.decode() if raw else "gs_ctx_create")
         self.handle = h
         self._lock = threading.Lock()
 

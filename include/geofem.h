@@ -51,6 +51,8 @@ typedef struct gs_csr gs_csr;
 /* ---- context / memory (no reference counterpart: device plumbing) ---- */
 gs_status gs_ctx_create(int device, gs_ctx **out);
 void gs_ctx_destroy(gs_ctx *ctx);
+/* Message of the context's last failed call; with ctx NULL, of the calling
+ * thread's last failed gs_ctx_create. */
 const char *gs_last_error(const gs_ctx *ctx);
 gs_status gs_ctx_synchronize(gs_ctx *ctx);
 gs_status gs_device_alloc(gs_ctx *ctx, size_t bytes, void **out_dev);
@@ -213,6 +215,14 @@ gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
  * events and return the average duration per launch in milliseconds. */
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms);
+/* Time the elastic element kernel (K_e and the gravity load of every
+ * element; variant 0: VALU, the one assembly uses; variant 1: matrix cores,
+ * v_mfma_f64_16x16x4f64) over `reps` launches, and report the largest entry
+ * difference against the VALU kernel relative to the largest entry. */
+gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
+                                   double poisson, const double *E_per_elem_dev,
+                                   int32_t variant, int32_t reps, double *avg_ms,
+                                   double *max_rel_diff);
 
 #ifdef __cplusplus
 }

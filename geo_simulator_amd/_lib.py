@@ -52,7 +52,8 @@ class CgOpts(C.Structure):
                 ("abs_tolerance", C.c_double), ("use_preconditioner", C.c_int32),
                 ("precision", C.c_int32), ("inner_tolerance", C.c_double),
                 ("max_inner_iterations", C.c_int64),
-                ("max_outer_iterations", C.c_int64)]
+                ("max_outer_iterations", C.c_int64),
+                ("relative_breakdown", C.c_int32), ("reserved_", C.c_int32)]
 
 
 class SolverStatsC(C.Structure):

@@ -12,8 +12,16 @@ log E.  The eliminated operator is SPD and symmetric (free block plus a
 scaled identity on constrained rows), so the reverse solve reuses it; u_e
 carries the Dirichlet values, which accounts for the theta-dependence of the
 lifted right-hand side.  The loads do not depend on theta.
+
+The reverse solve runs on dL/du scaled to unit norm, with the scale-free
+breakdown test (p.Ap == 0 or a non-finite step), and scales lambda back:
+CG is invariant under that scaling except for its absolute thresholds (the
+FP64 path's |p.Ap| < 1e-30 and abs_tol), which would otherwise depend on the
+units of the loss; abs_tol therefore acts relative to ||dL/du||.
 """
 from __future__ import annotations
+
+import copy
 
 import torch
 
@@ -21,6 +29,21 @@ from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        thermal_element_backward)
 from .linalg import ConjugateGradient
 from .runtime import default_context
+
+
+def _reverse_solve(problem, K, grad_u):
+    """lambda = K^-1 (dL/du with Dirichlet DOFs zeroed), solved on the
+    unit-norm right-hand side."""
+    g = grad_u.detach().to(dtype=torch.float64).contiguous().clone()
+    g[problem.mask] = 0.0
+    s = float(torch.linalg.vector_norm(g))
+    if s == 0.0 or s != s:
+        problem.backward_stats = None
+        return g if s == 0.0 else torch.full_like(g, float("nan"))
+    solver = copy.copy(problem.solver).with_relative_breakdown(True)
+    lam, stats = solver.solve(K, g / s)
+    problem.backward_stats = stats
+    return lam * s
 
 
 class ThermalProblem:
@@ -79,10 +102,7 @@ class ThermalSolve(torch.autograd.Function):
     def backward(ctx, grad_T):
         problem = ctx.problem
         k, T = ctx.saved_tensors
-        g = grad_T.detach().to(dtype=torch.float64).contiguous().clone()
-        g[problem.mask] = 0.0
-        lam, stats = problem.solver.solve(ctx.K, g)
-        problem.backward_stats = stats
+        lam = _reverse_solve(problem, ctx.K, grad_T)
         grad = thermal_element_backward(problem.mesh, k, T, lam, ctx=problem.ctx)
         return grad.to(ctx.in_dtype), None
 
@@ -110,10 +130,7 @@ class ElasticSolve(torch.autograd.Function):
     def backward(ctx, grad_u):
         problem = ctx.problem
         E, u = ctx.saved_tensors
-        g = grad_u.detach().to(dtype=torch.float64).contiguous().clone()
-        g[problem.mask] = 0.0
-        lam, stats = problem.solver.solve(ctx.K, g)
-        problem.backward_stats = stats
+        lam = _reverse_solve(problem, ctx.K, grad_u)
         grad = elastic_element_backward(problem.mesh, E, problem.poisson, u, lam,
                                         ctx=problem.ctx)
         return grad.to(ctx.in_dtype), None

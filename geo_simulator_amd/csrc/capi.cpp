@@ -808,6 +808,8 @@ gs_status gs_cg_opts_default(gs_cg_opts *o) {
   o->inner_tolerance = 1e-2;
   o->max_inner_iterations = 0;
   o->max_outer_iterations = 50;
+  o->relative_breakdown = 0;
+  o->reserved_ = 0;
   return GS_OK;
 }
 

@@ -51,6 +51,7 @@ constexpr int FBS = 1024;        // finalize kernels: one workgroup
 constexpr int GMAX = 2048;       // workgroups of the grid-stride vector kernels
 constexpr int VGMAX = 256;       // workgroups (of FBS threads) of a pass's vector kernels
 constexpr int GRID_MAX = 16384;  // workgroups of any SpMV kernel
+constexpr int UNR = 4;           // rows per thread per trip in a pass's vector kernels
 
 struct CgState {
   double rz[2];  // r.z at the start of pass k lives in rz[k & 1]
@@ -60,6 +61,7 @@ struct CgState {
   double rtol;   // mixed inner: pause once ||r|| < rtol * rbase
   long long iteration, max_iter;
   int stop, converged, pause, mixed;
+  int brk_rel;   // break down only on p.Ap == 0 or a non-finite step
 };
 
 __device__ __forceinline__ bool active(const CgState *st, long long it) {
@@ -203,9 +205,10 @@ __global__ __launch_bounds__(FBS) void k_cg_alpha(
   const double alpha_d = st->rz[it & 1] / pq[0];
   // The mixed inner solve works on a scaled FP32 system whose units make the
   // reference's absolute 1e-30 meaningless; it breaks down only on a zero or
-  // non-finite step, and then pauses for the outer step.
-  const bool brk = st->mixed ? (pq[0] == 0.0 || !isfinite(alpha_d))
-                             : fabs(pq[0]) < 1e-30;
+  // non-finite step, and then pauses for the outer step.  A solve asked for
+  // the same scale-free test (brk_rel) stops there.
+  const bool brk = (st->mixed || st->brk_rel) ? (pq[0] == 0.0 || !isfinite(alpha_d))
+                                              : fabs(pq[0]) < 1e-30;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     st->pq = pq[0];
     if (brk) {
@@ -218,14 +221,36 @@ __global__ __launch_bounds__(FBS) void k_cg_alpha(
   if (brk) return;
   const T alpha = (T)alpha_d;
   double acc[2] = {0.0, 0.0};
-  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
-    x[i] += alpha * p[i];
-    const T ri = r[i] - alpha * q[i];
-    r[i] = ri;
-    const T zi = PREC ? ri * dinv[i] : ri;
-    z[i] = zi;
-    acc[0] += (double)ri * (double)ri;
-    acc[1] += (double)ri * (double)zi;
+  // UNR rows per thread per trip, all loads issued before the first use
+  // (rows, and so the partial sums, in the same order as a plain loop)
+  const long long stride = (long long)gridDim.x * FBS;
+  for (long long i0 = (long long)blockIdx.x * FBS + threadIdx.x; i0 < n;
+       i0 += UNR * stride) {
+    T pv[UNR], qv[UNR], xv[UNR], rv[UNR], dv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        pv[u] = p[i];
+        qv[u] = q[i];
+        xv[u] = x[i];
+        rv[u] = r[i];
+        if (PREC) dv[u] = dinv[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        x[i] = xv[u] + alpha * pv[u];
+        const T ri = rv[u] - alpha * qv[u];
+        r[i] = ri;
+        const T zi = PREC ? ri * dv[u] : ri;
+        z[i] = zi;
+        acc[0] += (double)ri * (double)ri;
+        acc[1] += (double)ri * (double)zi;
+      }
+    }
   }
   block_sum<2, FBS>(acc);
   if (threadIdx.x == 0) {
@@ -271,8 +296,24 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
   }
   if (conv) return;
   const T beta = (T)beta_d;
-  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS)
-    p[i] = z[i] + beta * p[i];
+  const long long stride = (long long)gridDim.x * FBS;
+  for (long long i0 = (long long)blockIdx.x * FBS + threadIdx.x; i0 < n;
+       i0 += UNR * stride) {
+    T zv[UNR], pv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        zv[u] = z[i];
+        pv[u] = p[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) p[i] = zv[u] + beta * pv[u];
+    }
+  }
 }
 
 // Set-up: x = 0, r = b, z = D^-1 r, p = z; partials b.b, r.z.
@@ -296,8 +337,10 @@ __global__ __launch_bounds__(BS) void k_cg_init(
 }
 
 __global__ void k_state_setup(CgState *__restrict__ st, long long max_iter,
-                              double tol, double abs_tol, int mixed, double rtol) {
+                              double tol, double abs_tol, int mixed, double rtol,
+                              int brk_rel) {
   CgState s{};
+  s.brk_rel = brk_rel;
   s.max_iter = max_iter;
   s.tol = tol;
   s.abs_tol = abs_tol;
@@ -710,11 +753,12 @@ PcgWork<T> carve_pcg(char *base, Layout &L, int64_t n) {
 template <typename T>
 PcgResult pcg_core(gs_ctx *ctx, const SellOp &o, const T *val, const T *dinv,
                    const T *b, T *x, long long max_iter, double tol,
-                   double abs_tol, const PcgWork<T> &w) {
+                   double abs_tol, bool brk_rel, const PcgWork<T> &w) {
   hipStream_t s = ctx->stream;
   const int n = o.n();
   const unsigned gv = grid_vec(n);
-  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, w.st, max_iter, tol, abs_tol, 0, 0.0);
+  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, w.st, max_iter, tol, abs_tol, 0, 0.0,
+                     brk_rel ? 1 : 0);
   if (dinv)
     hipLaunchKernelGGL((k_cg_init<T, true>), gv, BS, 0, s, n, b, x, w.r, w.z, w.p,
                        dinv, w.part2);
@@ -810,7 +854,7 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
   const double target = std::max(opt.tolerance * bnorm, opt.abs_tolerance);
   const double scale = 1.0 / bnorm;
   hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, iw.st, max_iter, target * scale,
-                     0.0, 1, opt.inner_tolerance);
+                     0.0, 1, opt.inner_tolerance, 0);
   hipLaunchKernelGGL((k_zero<float>), gv, BS, 0, s, n, xs);
   auto start_inner = [&](bool restart, double rn) {
     if (dinv32) {
@@ -1084,7 +1128,8 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
     const PcgWork<double> w = carve_pcg<double>(base, L, n);
     r = pcg_core<double>(ctx, sell_op(A), A->sval.get(),
                          precond ? A->sdinv.get() : nullptr, bi, xi,
-                         o.max_iterations, o.tolerance, o.abs_tolerance, w);
+                         o.max_iterations, o.tolerance, o.abs_tolerance,
+                         o.relative_breakdown != 0, w);
   } else {
     r = pcg_fcg(ctx, A, bi, xi, precond, o, base, L);
   }
@@ -1123,7 +1168,7 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   double *part = (double *)(base + opart);
   CgState *st = (CgState *)(base + ost);
   void *p = base + op, *q = base + oq;
-  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, st, 1ll << 62, 0.0, 0.0, 0, 0.0);
+  hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, st, 1ll << 62, 0.0, 0.0, 0, 0.0, 0);
   hipEvent_t e0, e1;
   GS_HIP(hipEventCreate(&e0));
   GS_HIP(hipEventCreate(&e1));

@@ -168,6 +168,7 @@ class ConjugateGradient:
         self.inner_tolerance = 1e-2
         self.max_inner_iterations = 0
         self.max_outer_iterations = 50
+        self.relative_breakdown = False
 
     @classmethod
     def new(cls):
@@ -207,6 +208,13 @@ class ConjugateGradient:
         self.max_inner_iterations = int(n)
         return self
 
+    def with_relative_breakdown(self, on=True):
+        """FP64: stop on p.Ap == 0 or a non-finite step instead of the
+        reference's |p.Ap| < 1e-30, whose meaning depends on the units of b
+        (the mixed path always uses the scale-free test)."""
+        self.relative_breakdown = bool(on)
+        return self
+
     def opts(self) -> _lib.CgOpts:
         o = _lib.CgOpts()
         lib().gs_cg_opts_default(C.byref(o))
@@ -218,6 +226,7 @@ class ConjugateGradient:
         o.inner_tolerance = self.inner_tolerance
         o.max_inner_iterations = self.max_inner_iterations
         o.max_outer_iterations = self.max_outer_iterations
+        o.relative_breakdown = 1 if self.relative_breakdown else 0
         return o
 
     def solve(self, A, b):

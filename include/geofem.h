@@ -159,7 +159,14 @@ typedef struct {
                                 factor (default 1e-2) */
   int64_t max_inner_iterations; /* mixed: cap on FP32 passes between FP64
                                    steps (0 = none) */
-  int64_t max_outer_iterations; /* mixed: cap on FP64 steps (0 = 100) */
+  int64_t max_outer_iterations; /* mixed: cap on FP64 steps (default 50;
+                                   0 = 100) */
+  int32_t relative_breakdown; /* FP64: stop on p.Ap == 0 or a non-finite step
+                                 instead of the reference's |p.Ap| < 1e-30,
+                                 whose meaning depends on the units of b
+                                 (no reference counterpart; for reverse
+                                 solves); default 0.  Mixed always uses it. */
+  int32_t reserved_;          /* 0 */
 } gs_cg_opts;
 
 /* SolverStats (src/linalg/solver.rs:5-20) plus mixed-precision counters. */

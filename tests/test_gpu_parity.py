@@ -302,6 +302,29 @@ def test_thermal_autograd_vs_fd_and_oracle(oracle):
             assert abs(fd - grad[e]) <= 1e-5 * max(abs(fd), 1e-3 * np.abs(grad).max())
 
 
+def test_reverse_solve_independent_of_loss_units(oracle):
+    """A loss scaled by 1e-40 gives the gradient scaled by 1e-40: the reverse
+    solve runs on the unit-norm right-hand side, so the FP64 path's absolute
+    breakdown threshold (|p.Ap| < 1e-30) does not stop it early."""
+    import torch
+    from geo_simulator_amd.adjoint import ThermalProblem, thermal_solve
+    xyz, conn, mesh, dof, mask, vals = thermal_problem(oracle, 3)
+    rng = np.random.default_rng(23)
+    free = np.nonzero(mask == 0)[0]
+    obs = torch.tensor(rng.choice(free, 10, replace=False), device="cuda")
+    d = torch.tensor(rng.uniform(300, 900, 10), device="cuda")
+    prob = ThermalProblem(mesh, dof, 1e-6)
+    grads = []
+    for scale in (1.0, 1e-40):
+        lk = torch.full((len(conn),), np.log(3.0), dtype=torch.float64, device="cuda",
+                        requires_grad=True)
+        T = thermal_solve(lk, prob)
+        (scale * 0.5 * ((T[obs] - d) ** 2).sum()).backward()
+        assert prob.backward_stats.converged
+        grads.append(lk.grad.cpu().numpy() / scale)
+    assert np.max(np.abs(grads[1] - grads[0])) <= 1e-12 * np.max(np.abs(grads[0]))
+
+
 def test_elastic_autograd_vs_oracle_adjoint(oracle):
     import torch
     from geo_simulator_amd.adjoint import ElasticProblem, elastic_solve

@@ -34,6 +34,48 @@ inline unsigned nblk(int64_t n, int bs = BS) {
   return (unsigned)((n + bs - 1) / bs);
 }
 
+// Block b of a one-item-per-thread kernel, mapped so that each XCD works on
+// a contiguous range: workgroup i runs on XCD i % 8, and XCD x takes the
+// x-th eighth of the blocks in order.  Nearby slots and rows read nearby
+// element matrices, so an XCD's L2 holds its own share of them instead of
+// every XCD fetching the whole stream.
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t G = gridDim.x, i = blockIdx.x;
+  if (G < 8) return i;
+  const int64_t x = i & 7, q = G >> 3, rem = G & 7;
+  return q * x + (x < rem ? x : rem) + (i >> 3);
+}
+__device__ __forceinline__ int64_t xcd_item() { return xcd_block() * BS + threadIdx.x; }
+
+// Grid of a max-reduction kernel: a grid-stride loop over at most this many
+// workgroups, each folding its maximum into one device-wide atomic (atomics
+// on one address serialise; one per wavefront over a million rows cost
+// about 0.3 ms).
+constexpr int64_t MAX_GRID = 1024;
+inline unsigned nblk_max(int64_t n) {
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(n), MAX_GRID));
+}
+
+// Largest of the workgroup's (BS threads) values, folded into *out with one
+// atomic; bit patterns of non-negative doubles, so the integer maximum is
+// the floating-point one and the result is order independent.
+__device__ __forceinline__ void wg_max_atomic(unsigned long long mine,
+                                              unsigned long long *out) {
+  __shared__ unsigned long long sm[BS / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off, 64);
+    mine = o > mine ? o : mine;
+  }
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sm[0];
+#pragma unroll
+    for (int k = 1; k < BS / 64; ++k) m = sm[k] > m ? sm[k] : m;
+    if (m != 0) atomicMax(out, m);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // symbolic phase
 // ---------------------------------------------------------------------------
@@ -138,7 +180,7 @@ __global__ void k_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
                                  const int *__restrict__ contrib,
                                  const double *__restrict__ Ke,
                                  double *__restrict__ val) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_item();
   if (s >= nnz) return;
   const int b0 = cptr[s], b1 = cptr[s + 1];
   double v = 0.0;
@@ -160,7 +202,7 @@ __global__ void k_gather_elastic(int64_t nnz, const int *__restrict__ rp,
                                  const int *__restrict__ contrib,
                                  const double *__restrict__ Ke,
                                  double *__restrict__ val) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_item();
   if (s >= nnz) return;
   const int r = slot_row[s];
   const int base = rp[r], deg = rp[r + 1] - base;
@@ -232,26 +274,24 @@ __global__ void k_bc_count(int64_t n, const int *__restrict__ rp,
                            const uint8_t *__restrict__ dir,
                            int *__restrict__ cnt,
                            unsigned long long *__restrict__ maxbits) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long mine = 0;
-  if (r < n) {
+  for (int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x; r < n;
+       r += (int64_t)gridDim.x * BS) {
     int c = 0;
     for (int k = rp[r]; k < rp[r + 1]; ++k) {
       const int j = col[k];
       if (!dir[j]) ++c;
       if (j == r) {
         const double a = fabs(val[k]);
-        if (a == a) mine = (unsigned long long)__double_as_longlong(a);
+        if (a == a) {
+          const unsigned long long bits = (unsigned long long)__double_as_longlong(a);
+          mine = bits > mine ? bits : mine;
+        }
       }
     }
     cnt[r] = dir[r] ? 1 : c;
   }
-  // wave-level max, then one atomic per wavefront
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(mine, off, 64);
-    mine = o > mine ? o : mine;
-  }
-  if ((threadIdx.x & 63) == 0 && mine != 0) atomicMax(maxbits, mine);
+  wg_max_atomic(mine, maxbits);
 }
 
 // Pass 2: copy kept entries and lift the RHS.  The reference subtracts
@@ -429,7 +469,7 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
                                      double *__restrict__ sval,
                                      float *__restrict__ sval32,
                                      double *__restrict__ liftv) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_item();
   if (s >= nnz) return;
   const int d = dst[s];
   if (d == -1) return;
@@ -467,7 +507,7 @@ __global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
                                      double *__restrict__ sval,
                                      float *__restrict__ sval32,
                                      double *__restrict__ liftv) {
-  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t s = xcd_item();
   if (s >= nnz) return;
   const int r = slot_row[s];
   const int base = rp[r], deg = rp[r + 1] - base;
@@ -524,10 +564,11 @@ __global__ void k_sys_diagmax(int64_t n_nodes, const int *__restrict__ diag_slot
                               const int *__restrict__ contrib,
                               const double *__restrict__ Ke,
                               unsigned long long *__restrict__ maxbits) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   unsigned long long mine = 0;
-  const int s = r < n_nodes ? diag_slot[r] : -1;
-  if (s >= 0) {
+  for (int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x; r < n_nodes;
+       r += (int64_t)gridDim.x * BS) {
+    const int s = diag_slot[r];
+    if (s < 0) continue;
     double v[DPN];
 #pragma unroll
     for (int c = 0; c < DPN; ++c) v[c] = 0.0;
@@ -552,11 +593,7 @@ __global__ void k_sys_diagmax(int64_t n_nodes, const int *__restrict__ diag_slot
       }
     }
   }
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned long long o = __shfl_xor(mine, off, 64);
-    mine = o > mine ? o : mine;
-  }
-  if ((threadIdx.x & 63) == 0 && mine != 0) atomicMax(maxbits, mine);
+  wg_max_atomic(mine, maxbits);
 }
 
 // Per DOF row (internal numbering): load (k_gather_load's sum), then either
@@ -582,7 +619,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
                           const int *__restrict__ dperm,
                           double *__restrict__ fbc, double *__restrict__ sdinv,
                           float *__restrict__ sdinv32) {
-  const int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t r = xcd_item();
   if (r >= n) return;
   const int64_t node = r / DPN;
   const int c = (int)(r - DPN * node);
@@ -805,7 +842,7 @@ gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
   out->row_ptr = dalloc<int>(n + 1);
   GS_HIP(hipMemsetAsync(out->row_ptr.get(), 0, sizeof(int), s));
   if (n > 0) {
-    hipLaunchKernelGGL(k_bc_count, nblk(n), BS, 0, s, n, K->row_ptr.get(),
+    hipLaunchKernelGGL(k_bc_count, nblk_max(n), BS, 0, s, n, K->row_ptr.get(),
                        K->col.get(), K->val.get(), is_dir, cnt.get(),
                        maxbits.get());
     GS_LAUNCH_CHECK("k_bc_count");
@@ -952,7 +989,7 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
   hipLaunchKernelGGL(k_set_u64, 1, 1, 0, s, maxbits.get(), one_bits);
   const int *sip = P.sell->siperm.get(), *pos = P.sell->pos.get();
   if (P.dpn == 1) {
-    hipLaunchKernelGGL((k_sys_diagmax<1>), nblk(m->n_nodes), BS, 0, s, m->n_nodes,
+    hipLaunchKernelGGL((k_sys_diagmax<1>), nblk_max(m->n_nodes), BS, 0, s, m->n_nodes,
                        p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
                        maxbits.get());
     hipLaunchKernelGGL((k_sys_rhs<1>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
@@ -961,7 +998,7 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
                        P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
                        dperm, fbc, sdinv, sdinv32);
   } else {
-    hipLaunchKernelGGL((k_sys_diagmax<3>), nblk(m->n_nodes), BS, 0, s, m->n_nodes,
+    hipLaunchKernelGGL((k_sys_diagmax<3>), nblk_max(m->n_nodes), BS, 0, s, m->n_nodes,
                        p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
                        maxbits.get());
     hipLaunchKernelGGL((k_sys_rhs<3>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),

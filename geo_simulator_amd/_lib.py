@@ -107,6 +107,8 @@ SIGNATURES = {
     "gs_cg_opts_default": (C.c_int, [C.POINTER(CgOpts)]),
     "gs_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                               C.POINTER(SolverStatsC)]),
+    "gs_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
+                                    C.POINTER(SolverStatsC)]),
     "gs_thermal_element_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_elastic_element_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp,
                                               _vp, _vp]),

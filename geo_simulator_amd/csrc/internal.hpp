@@ -304,6 +304,8 @@ void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv);
 void csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y);
 void cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
               const gs_cg_opts &o, double *x, gs_solver_stats *st);
+void bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
+                    const gs_cg_opts &o, double *x, gs_solver_stats *st);
 double time_spmv(gs_ctx *ctx, const gs_csr *A, int precision, int reps);
 
 int read_flag_and_clear(gs_ctx *ctx, int *flag_dev);

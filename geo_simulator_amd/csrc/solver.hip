@@ -982,6 +982,331 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
   return res;
 }
 
+// ---- BiCGSTAB (src/linalg/iterative.rs:177-291) -------------------------------
+// The reference's loop, one pass (iteration) per five launches:
+//   KP  end of the previous pass (final_res = ||r||, it += 1, stop on
+//       ||r|| < tol ||b|| or < abs_tol, on |omega| < 1e-40) and the loop top
+//       (it < max_iter, rho = r^.r, |rho| < 1e-40); p = r + beta (p - omega v)
+//       (p = r on the first pass), p^ = D^-1 p
+//   KV  v = A p^ (SELL), partials r^.v
+//   KS  alpha = rho / r^.v (|r^.v| < 1e-40 stops), s = r - alpha v, s^ = D^-1 s,
+//       partials s.s
+//   KT  t = A s^ (SELL), partials t.t, t.s
+//   KX  stop on ||s|| < tol ||b|| or < abs_tol after x += alpha p^ (the pass is
+//       not counted); |t.t| < 1e-40 stops; omega = t.s / t.t;
+//       x += alpha p^ + omega s^, r = s - omega t; partials r.r, r^.r
+// Every workgroup of a kernel takes the decisions from the same partials;
+// workgroup 0 records them.  A kernel runs only when its predecessor of the
+// same pass completed without stopping, which that predecessor's leader
+// records in its own word mark[k] (read by later kernels only).
+struct BgState {
+  double rho[2];  // rho of pass it in rho[it & 1]
+  double alpha, omega, bnorm, tol_abs, abs_tol, final_res;
+  long long iteration, max_iter;
+  long long mark[5];  // KP: mark[0], KS: mark[2], KX: mark[4]
+  int stop, converged;
+  int on_s;           // the solve ended on the intermediate residual s
+};
+
+__global__ void k_bg_setup(BgState *__restrict__ st, long long max_iter, double tol,
+                           double abs_tol) {
+  BgState s{};
+  s.rho[0] = s.rho[1] = 1.0;
+  s.alpha = s.omega = 1.0;
+  s.max_iter = max_iter;
+  s.tol_abs = tol;  // times ||b|| in k_bg_fin_init
+  s.abs_tol = abs_tol;
+  for (int k = 0; k < 5; ++k) s.mark[k] = -2;
+  *st = s;
+}
+
+// x = 0, r = r^ = b, v = p = 0; partials b.b (twice: ||b||^2 and rho)
+__global__ __launch_bounds__(BS) void k_bg_init(int n, const double *__restrict__ b,
+                                                double *__restrict__ x, double *__restrict__ r,
+                                                double *__restrict__ rh, double *__restrict__ v,
+                                                double *__restrict__ p,
+                                                double *__restrict__ part) {
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    const double bi = b[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    rh[i] = bi;
+    v[i] = 0.0;
+    p[i] = 0.0;
+    acc[0] += bi * bi;
+    acc[1] += bi * bi;
+  }
+  put_partials<2>(acc, part);
+}
+
+// ||b|| and the early exit for b = 0 (src/linalg/iterative.rs:185-189)
+__global__ __launch_bounds__(FBS) void k_bg_fin_init(BgState *__restrict__ st,
+                                                     const double *__restrict__ part,
+                                                     int np) {
+  double tot[2];
+  sum_partials<2>(part, np, tot);
+  if (threadIdx.x != 0) return;
+  const double bn = sqrt(tot[0]);
+  st->bnorm = bn;
+  st->final_res = bn;
+  st->tol_abs *= bn;
+  if (bn < 1e-25) {
+    st->stop = 1;
+    st->converged = 1;
+    st->final_res = 0.0;
+  } else {
+    st->mark[4] = -1;  // pass 0 may start
+  }
+}
+
+template <bool PREC>
+__global__ __launch_bounds__(FBS) void k_bg_dir(
+    int n, long long it, const double *__restrict__ r, double *__restrict__ p,
+    const double *__restrict__ v, const double *__restrict__ dinv,
+    double *__restrict__ ph, BgState *__restrict__ st,
+    const double *__restrict__ part_in, int np_in) {
+  if (st->mark[4] != it - 1) return;
+  double tot[2];  // (r.r, r^.r)
+  sum_partials<2>(part_in, np_in, tot);
+  bool conv = false, stop = false;
+  const double res = sqrt(tot[0]);
+  if (it > 0) {
+    conv = res < st->tol_abs || res < st->abs_tol;
+    stop = conv || fabs(st->omega) < 1e-40;
+  }
+  stop = stop || it >= st->max_iter;
+  const double rho = tot[1];
+  stop = stop || fabs(rho) < 1e-40;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (it > 0) st->final_res = res;
+    st->iteration = it;
+    if (conv) st->converged = 1;
+    if (stop) {
+      st->stop = 1;
+    } else {
+      st->rho[it & 1] = rho;
+      st->mark[0] = it;
+    }
+  }
+  if (stop) return;
+  const double omega = st->omega;
+  const double beta = it == 0 ? 0.0 : (rho / st->rho[(it - 1) & 1]) * (st->alpha / omega);
+  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
+    const double pi = it == 0 ? r[i] : r[i] + beta * (p[i] - omega * v[i]);
+    p[i] = pi;
+    ph[i] = PREC ? pi * dinv[i] : pi;
+  }
+}
+
+// KV / KT: y = A x (SELL) with partials of (w.y) [and y.y when NV == 2:
+// partials (y.y, y.w)]
+template <int NV, class IDX>
+__global__ __launch_bounds__(BS) void k_bg_spmv(
+    int n, int ns, const int *__restrict__ cb, const IDX idx, const double *__restrict__ val,
+    const double *__restrict__ xin, double *__restrict__ y, const double *__restrict__ w,
+    const BgState *__restrict__ st, int mk, long long it, double *__restrict__ part) {
+  if (st->mark[mk] != it) return;
+  double acc[NV];
+#pragma unroll
+  for (int k = 0; k < NV; ++k) acc[k] = 0.0;
+  auto epi = [&](int row, double sv) {
+    y[row] = sv;
+    if (NV == 1) {
+      acc[0] += w[row] * sv;
+    } else {
+      acc[0] += sv * sv;
+      acc[NV - 1] += sv * w[row];
+    }
+  };
+  sell_rows<double>(n, ns, cb, idx, val, xin, epi);
+  put_partials<NV>(acc, part);
+}
+
+template <bool PREC>
+__global__ __launch_bounds__(FBS) void k_bg_s(
+    int n, long long it, const double *__restrict__ r, const double *__restrict__ v,
+    const double *__restrict__ dinv, double *__restrict__ sv, double *__restrict__ sh,
+    BgState *__restrict__ st, const double *__restrict__ part_in, int np_in,
+    double *__restrict__ part) {
+  if (st->mark[0] != it) return;
+  double tot[1];
+  sum_partials<1>(part_in, np_in, tot);
+  const bool stop = fabs(tot[0]) < 1e-40;
+  const double alpha = st->rho[it & 1] / tot[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (stop) {
+      st->stop = 1;
+    } else {
+      st->alpha = alpha;
+      st->mark[2] = it;
+    }
+  }
+  if (stop) return;
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
+    const double si = r[i] - alpha * v[i];
+    sv[i] = si;
+    sh[i] = PREC ? si * dinv[i] : si;
+    acc[0] += si * si;
+  }
+  block_sum<1, FBS>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+__global__ __launch_bounds__(FBS) void k_bg_x(
+    int n, long long it, double *__restrict__ x, double *__restrict__ r,
+    const double *__restrict__ rh, const double *__restrict__ ph,
+    const double *__restrict__ sv, const double *__restrict__ sh,
+    const double *__restrict__ t, BgState *__restrict__ st,
+    const double *__restrict__ part_s, int np_s, const double *__restrict__ part_t,
+    int np_t, double *__restrict__ part) {
+  if (st->mark[2] != it) return;
+  double ss[1], tt[2];  // s.s; (t.t, t.s)
+  sum_partials<1>(part_s, np_s, ss);
+  sum_partials<2>(part_t, np_t, tt);
+  const double alpha = st->alpha;
+  const double s_norm = sqrt(ss[0]);
+  const bool sconv = s_norm < st->tol_abs || s_norm < st->abs_tol;
+  const bool brk = !sconv && fabs(tt[0]) < 1e-40;
+  const double omega = tt[1] / tt[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (sconv) {
+      st->final_res = s_norm;
+      st->converged = 1;
+      st->on_s = 1;
+      st->stop = 1;
+    } else if (brk) {
+      st->stop = 1;
+    } else {
+      st->omega = omega;
+      st->mark[4] = it;
+    }
+  }
+  if (brk) return;
+  if (sconv) {
+    for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS)
+      x[i] += alpha * ph[i];
+    return;
+  }
+  double acc[2] = {0.0, 0.0};
+  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
+    x[i] += alpha * ph[i] + omega * sh[i];
+    const double ri = sv[i] - omega * t[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+    acc[1] += rh[i] * ri;
+  }
+  block_sum<2, FBS>(acc);
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x] = acc[0];
+    part[2 * blockIdx.x + 1] = acc[1];
+  }
+}
+
+BgState read_bg_state(gs_ctx *ctx, const BgState *st) {
+  static_assert(sizeof(BgState) <= 4096, "state exceeds the pinned staging buffer");
+  BgState h;
+  GS_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(BgState), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  memcpy(&h, ctx->pinned, sizeof(BgState));
+  return h;
+}
+
+struct BgWork {
+  double *r, *rh, *v, *p, *ph, *s, *sh, *t, *part_a, *part_b, *part_c;
+  BgState *st;
+};
+
+BgWork carve_bg(char *base, Layout &L, int64_t n) {
+  BgWork w;
+  w.r = (double *)(base + L.add<double>(n));
+  w.rh = (double *)(base + L.add<double>(n));
+  w.v = (double *)(base + L.add<double>(n));
+  w.p = (double *)(base + L.add<double>(n));
+  w.ph = (double *)(base + L.add<double>(n));
+  w.s = (double *)(base + L.add<double>(n));
+  w.sh = (double *)(base + L.add<double>(n));
+  w.t = (double *)(base + L.add<double>(n));
+  w.part_a = (double *)(base + L.add<double>(2 * GRID_MAX));  // SpMV partials
+  w.part_b = (double *)(base + L.add<double>(2 * GMAX));      // KS partials
+  w.part_c = (double *)(base + L.add<double>(2 * GMAX));      // init / KX partials
+  w.st = (BgState *)(base + L.add<BgState>(1));
+  return w;
+}
+
+// KP (and, unless only_dir, KV KS KT KX) of pass it.
+void enqueue_bg_pass(hipStream_t s, const SellOp &o, const double *val,
+                     const double *dinv, double *x, const BgWork &w, long long it,
+                     int np_first, bool only_dir) {
+  const int n = o.n();
+  const unsigned gf = grid_fat(n);
+  const int np_c = it == 0 ? np_first : (int)gf;
+  if (dinv)
+    hipLaunchKernelGGL((k_bg_dir<true>), gf, FBS, 0, s, n, it, w.r, w.p, w.v, dinv,
+                       w.ph, w.st, w.part_c, np_c);
+  else
+    hipLaunchKernelGGL((k_bg_dir<false>), gf, FBS, 0, s, n, it, w.r, w.p, w.v, dinv,
+                       w.ph, w.st, w.part_c, np_c);
+  if (only_dir) return;
+  with_idx(*o.P, [&](auto idx) {
+    hipLaunchKernelGGL((k_bg_spmv<1, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+                       o.P->cb.get(), idx, val, (const double *)w.ph, w.v,
+                       (const double *)w.rh, (const BgState *)w.st, 0, it, w.part_a);
+  });
+  if (dinv)
+    hipLaunchKernelGGL((k_bg_s<true>), gf, FBS, 0, s, n, it, w.r, w.v, dinv, w.s, w.sh,
+                       w.st, w.part_a, o.grid, w.part_b);
+  else
+    hipLaunchKernelGGL((k_bg_s<false>), gf, FBS, 0, s, n, it, w.r, w.v, dinv, w.s, w.sh,
+                       w.st, w.part_a, o.grid, w.part_b);
+  with_idx(*o.P, [&](auto idx) {
+    hipLaunchKernelGGL((k_bg_spmv<2, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+                       o.P->cb.get(), idx, val, (const double *)w.sh, w.t,
+                       (const double *)w.s, (const BgState *)w.st, 2, it, w.part_a);
+  });
+  hipLaunchKernelGGL(k_bg_x, gf, FBS, 0, s, n, it, x, w.r, w.rh, w.ph, w.s, w.sh, w.t,
+                     w.st, w.part_b, (int)gf, w.part_a, o.grid, w.part_c);
+}
+
+// BiCGSTAB on the SELL operator; b, x in SELL order.
+PcgResult bicgstab_core(gs_ctx *ctx, const SellOp &o, const double *val,
+                        const double *dinv, const double *b, double *x,
+                        long long max_iter, double tol, double abs_tol,
+                        const BgWork &w) {
+  hipStream_t s = ctx->stream;
+  const int n = o.n();
+  const unsigned gv = grid_vec(n);
+  hipLaunchKernelGGL(k_bg_setup, 1, 1, 0, s, w.st, max_iter, tol, abs_tol);
+  hipLaunchKernelGGL(k_bg_init, gv, BS, 0, s, n, b, x, w.r, w.rh, w.v, w.p, w.part_c);
+  hipLaunchKernelGGL(k_bg_fin_init, 1, FBS, 0, s, w.st, w.part_c, (int)gv);
+  GS_LAUNCH_CHECK("k_bg_init");
+  PcgResult res;
+  BgState h = read_bg_state(ctx, w.st);
+  res.bnorm = h.bnorm;
+  long long it = 0;
+  Batcher bt;
+  while (!h.stop) {
+    if (it >= max_iter) {  // the last pass's end and the loop-top test
+      enqueue_bg_pass(s, o, val, dinv, x, w, it, (int)gv, true);
+    } else {
+      const long long batch = bt.next(h.final_res, it, std::max(h.tol_abs, h.abs_tol),
+                                      max_iter - it);
+      for (long long i = 0; i < batch; ++i)
+        enqueue_bg_pass(s, o, val, dinv, x, w, it + i, (int)gv, false);
+      it += batch;
+    }
+    GS_LAUNCH_CHECK("bicgstab batch");
+    h = read_bg_state(ctx, w.st);
+  }
+  res.iterations = h.iteration;
+  res.rnorm = h.final_res;
+  res.converged = h.converged != 0;
+  res.spmv = 2 * h.iteration + (h.on_s ? 2 : 0);
+  return res;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1141,6 +1466,50 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   st->relative_residual = r.bnorm > 1e-20 ? r.rnorm / r.bnorm : 0.0;
   st->converged = r.converged;
   st->outer_iterations = r.outer;
+  st->spmv_count = r.spmv;
+  st->solve_time =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// BiCGSTAB::solve (src/linalg/iterative.rs:283-291): Jacobi when
+// use_preconditioner, FP64 only.
+void bicgstab_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
+                    const gs_cg_opts &o, double *x, gs_solver_stats *st) {
+  gs_csr *A = const_cast<gs_csr *>(Ac);  // derived caches only
+  const auto t0 = std::chrono::steady_clock::now();
+  memset(st, 0, sizeof(*st));
+  if (A->rows != A->cols) throw Error(GS_ERR_INVALID_ARG, "BiCGSTAB needs a square matrix");
+  if (o.precision != GS_PREC_FP64)
+    throw Error(GS_ERR_INVALID_ARG, "BiCGSTAB runs in FP64 only");
+  if (A->spmv_flags & GS_SPMV_CSR)
+    throw Error(GS_ERR_INVALID_ARG, "BiCGSTAB runs on the SELL engine only");
+  const int64_t n = A->rows;
+  if (n == 0) {
+    st->converged = 1;
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  const bool precond = o.use_preconditioner != 0;
+  ensure_sell(ctx, A, false, precond);
+  Layout L;
+  const size_t ob = L.add<double>(n), ox = L.add<double>(n);
+  Layout dry = L;
+  carve_bg(nullptr, dry, n);
+  char *base = ctx_scratch(ctx, dry.bytes);
+  double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
+  const BgWork w = carve_bg(base, L, n);
+  const int *vp = A->sell->vperm.get();
+  permute_gather(s, n, b, vp, bi);
+  const PcgResult r = bicgstab_core(ctx, sell_op(A), A->sval.get(),
+                                    precond ? A->sdinv.get() : nullptr, bi, xi,
+                                    o.max_iterations, o.tolerance, o.abs_tolerance, w);
+  permute_scatter(s, n, xi, vp, x);
+  GS_LAUNCH_CHECK("bicgstab_solve");
+  GS_HIP(hipStreamSynchronize(s));
+  st->iterations = r.iterations;
+  st->residual_norm = r.rnorm;
+  st->relative_residual = r.bnorm > 0.0 ? r.rnorm / r.bnorm : 0.0;
+  st->converged = r.converged;
   st->spmv_count = r.spmv;
   st->solve_time =
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();

@@ -231,24 +231,7 @@ class ConjugateGradient:
 
     def solve(self, A, b):
         """Solver::solve (src/linalg/iterative.rs:123-131) -> (x, SolverStats)."""
-        if not isinstance(A, CsrMatrix):
-            A = CsrMatrix.from_scipy(A)
-        ctx = A.ctx
-        host = not _is_torch(b)
-        bd = ctx.to_device(b)
-        if bd.numel() != A.shape[0]:
-            raise ValueError("right-hand side length differs from the matrix rows")
-        x = ctx.empty(A.shape[0])
-        st = _lib.SolverStatsC()
-        o = self.opts()
-        ctx.sync_torch()
-        ctx.check(lib().gs_cg_solve(ctx.handle, A.handle, dptr(bd), C.byref(o),
-                                    dptr(x), C.byref(st)), "gs_cg_solve")
-        stats = SolverStats(int(st.iterations), float(st.residual_norm),
-                            float(st.relative_residual), bool(st.converged),
-                            float(st.solve_time), int(st.outer_iterations),
-                            int(st.spmv_count))
-        return (x.cpu().numpy() if host else x), stats
+        return _krylov_solve("gs_cg_solve", self.opts(), A, b)
 
     # Solver trait accessors (src/linalg/solver.rs:122-135)
     def set_tolerance(self, t):
@@ -261,3 +244,85 @@ class ConjugateGradient:
 def norm(v) -> float:
     """SolverUtils::norm (src/linalg/solver.rs:153-155)."""
     return float(np.sqrt(np.sum(np.asarray(v, dtype=np.float64) ** 2)))
+
+
+def _krylov_solve(entry, o, A, b):
+    """Run the C ABI solver `entry` -> (x, SolverStats); x is a numpy array
+    for a host right-hand side, else a device tensor."""
+    if not isinstance(A, CsrMatrix):
+        A = CsrMatrix.from_scipy(A)
+    ctx = A.ctx
+    host = not _is_torch(b)
+    bd = ctx.to_device(b)
+    if bd.numel() != A.shape[0]:
+        raise ValueError("right-hand side length differs from the matrix rows")
+    x = ctx.empty(A.shape[0])
+    st = _lib.SolverStatsC()
+    ctx.sync_torch()
+    ctx.check(getattr(lib(), entry)(ctx.handle, A.handle, dptr(bd), C.byref(o),
+                                    dptr(x), C.byref(st)), entry)
+    stats = SolverStats(int(st.iterations), float(st.residual_norm),
+                        float(st.relative_residual), bool(st.converged),
+                        float(st.solve_time), int(st.outer_iterations),
+                        int(st.spmv_count))
+    return (x.cpu().numpy() if host else x), stats
+
+
+class BiCGSTAB:
+    """BiCGSTAB for general square matrices with the reference's builder and
+    defaults (max_iterations 1000, tolerance 1e-8, abs_tolerance 1e-12,
+    Jacobi on; src/linalg/iterative.rs:149-176), FP64 on the SELL operator."""
+
+    name = "BiCGSTAB"
+
+    def __init__(self):
+        self.max_iterations = 1000
+        self.tolerance = 1e-8
+        self.abs_tolerance = 1e-12
+        self.use_preconditioner = True
+        self.verbose = False
+
+    @classmethod
+    def new(cls):
+        return cls()
+
+    def with_max_iterations(self, n):
+        self.max_iterations = int(n)
+        return self
+
+    def with_tolerance(self, t):
+        self.tolerance = float(t)
+        return self
+
+    def with_abs_tolerance(self, t):
+        self.abs_tolerance = float(t)
+        return self
+
+    def with_preconditioner(self, use):
+        self.use_preconditioner = bool(use)
+        return self
+
+    def with_verbose(self, verbose):
+        """Accepted for the reference's builder; progress is not printed."""
+        self.verbose = bool(verbose)
+        return self
+
+    def opts(self) -> _lib.CgOpts:
+        o = _lib.CgOpts()
+        lib().gs_cg_opts_default(C.byref(o))
+        o.max_iterations = self.max_iterations
+        o.tolerance = self.tolerance
+        o.abs_tolerance = self.abs_tolerance
+        o.use_preconditioner = 1 if self.use_preconditioner else 0
+        return o
+
+    def solve(self, A, b):
+        """Solver::solve (src/linalg/iterative.rs:283-291) -> (x, SolverStats)."""
+        return _krylov_solve("gs_bicgstab_solve", self.opts(), A, b)
+
+    # Solver trait accessors (src/linalg/iterative.rs:293-299)
+    def set_tolerance(self, t):
+        self.tolerance = float(t)
+
+    def set_abs_tolerance(self, t):
+        self.abs_tolerance = float(t)

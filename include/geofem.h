@@ -185,6 +185,16 @@ gs_status gs_cg_opts_default(gs_cg_opts *opts);
 gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
                       const gs_cg_opts *opts, double *x_dev,
                       gs_solver_stats *stats);
+/* BiCGSTAB::new().with_max_iterations().with_tolerance().with_abs_tolerance()
+ * .with_preconditioner() and Solver::solve (src/linalg/iterative.rs:149-291),
+ * for general (nonsymmetric) square matrices: the same loop, stopping tests
+ * and iteration count (a pass that ends on the intermediate residual s is
+ * not counted).  Uses max_iterations, tolerance, abs_tolerance and
+ * use_preconditioner of opts (the reference's defaults: 1000, 1e-8, 1e-12,
+ * Jacobi); precision must be GS_PREC_FP64. */
+gs_status gs_bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
+                            const gs_cg_opts *opts, double *x_dev,
+                            gs_solver_stats *stats);
 
 /* ---- adjoint (new surface; no reference counterpart) ----
  * Element backward of theta_e = log k_e (thermal) or log E_e (elastic):

@@ -863,6 +863,100 @@ int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
   return OR_OK;
 }
 
+/* src/linalg/iterative.rs:177-280 (BiCGSTAB::solve_with_operator_internal,
+ * + :283-291 for the preconditioner choice), statement by statement: the
+ * shadow residual is b, the iteration counter excludes a pass that ends on
+ * the intermediate residual s, and the stopping tests are absolute
+ * thresholds (1e-40) on rho, r_hat.v, t.t and omega as there. */
+int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,
+                      double tol, double abs_tol, int use_precond, double *x,
+                      or_stats *st) {
+  const int64_t n = A->rows;
+  const double t0 = now_s();
+  const double b_norm = norm2(b, n);
+  memset(st, 0, sizeof(*st));
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  if (b_norm < 1e-25) {
+    st->converged = 1;
+    st->solve_time = now_s() - t0;
+    return OR_OK;
+  }
+  const size_t bytes = (size_t)(n > 0 ? n : 1) * sizeof(double);
+  double *r = (double *)malloc(bytes), *rh = (double *)malloc(bytes);
+  double *v = (double *)malloc(bytes), *p = (double *)malloc(bytes);
+  double *ph = (double *)malloc(bytes), *s = (double *)malloc(bytes);
+  double *sh = (double *)malloc(bytes), *t = (double *)malloc(bytes);
+  double *dinv = (double *)malloc(bytes);
+  if (!r || !rh || !v || !p || !ph || !s || !sh || !t || !dinv) {
+    free(r); free(rh); free(v); free(p); free(ph); free(s); free(sh); free(t);
+    free(dinv);
+    return OR_ERR_OOM;
+  }
+  if (use_precond) or_jacobi_diag_inv(A, dinv);
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = b[i];
+    rh[i] = b[i];
+    v[i] = 0.0;
+    p[i] = 0.0;
+  }
+  double rho = 1.0, alpha = 1.0, omega = 1.0;
+  int64_t it = 0;
+  int converged = 0;
+  double final_res = b_norm;
+  while (it < max_iter) {
+    const double rho_prev = rho;
+    rho = 0.0;
+    for (int64_t i = 0; i < n; ++i) rho += rh[i] * r[i];
+    if (fabs(rho) < 1e-40) break;
+    if (it == 0) {
+      for (int64_t i = 0; i < n; ++i) p[i] = r[i];
+    } else {
+      const double beta = (rho / rho_prev) * (alpha / omega);
+      for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * v[i]);
+    }
+    for (int64_t i = 0; i < n; ++i) ph[i] = use_precond ? p[i] * dinv[i] : p[i];
+    or_spmv(A, ph, v);
+    double rhat_v = 0.0;
+    for (int64_t i = 0; i < n; ++i) rhat_v += rh[i] * v[i];
+    if (fabs(rhat_v) < 1e-40) break;
+    alpha = rho / rhat_v;
+    for (int64_t i = 0; i < n; ++i) s[i] = r[i] - alpha * v[i];
+    const double s_norm = norm2(s, n);
+    if (s_norm < tol * b_norm || s_norm < abs_tol) {
+      for (int64_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
+      final_res = s_norm;
+      converged = 1;
+      break;
+    }
+    for (int64_t i = 0; i < n; ++i) sh[i] = use_precond ? s[i] * dinv[i] : s[i];
+    or_spmv(A, sh, t);
+    double t_t = 0.0, t_s = 0.0;
+    for (int64_t i = 0; i < n; ++i) t_t += t[i] * t[i];
+    for (int64_t i = 0; i < n; ++i) t_s += t[i] * s[i];
+    if (fabs(t_t) < 1e-40) break;
+    omega = t_s / t_t;
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += alpha * ph[i] + omega * sh[i];
+      r[i] = s[i] - omega * t[i];
+    }
+    final_res = norm2(r, n);
+    ++it;
+    if (final_res < tol * b_norm || final_res < abs_tol) {
+      converged = 1;
+      break;
+    }
+    if (fabs(omega) < 1e-40) break;
+  }
+  st->iterations = it;
+  st->residual_norm = final_res;
+  st->relative_residual = final_res / b_norm;
+  st->converged = converged;
+  st->solve_time = now_s() - t0;
+  free(r); free(rh); free(v); free(p); free(ph); free(s); free(sh); free(t);
+  free(dinv);
+  return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint element backward (no reference counterpart; pinned by FD).        */
 /* With theta_e = log k_e (or log E_e), dK_e/dtheta_e = K_e, so              */

@@ -122,6 +122,9 @@ void or_spmv(const or_csr *A, const double *x, double *y);
 void or_jacobi_diag_inv(const or_csr *A, double *dinv);
 int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
                 double abs_tol, int use_precond, double *x, or_stats *st);
+int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,
+                      double tol, double abs_tol, int use_precond, double *x,
+                      or_stats *st);
 
 /* ---- adjoint element backward (new surface; pinned by central FD) ---- */
 int or_thermal_element_backward(int64_t n_nodes, const double *xyz,

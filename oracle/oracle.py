@@ -116,6 +116,8 @@ def _declare(L):
     L.or_jacobi_diag_inv.argtypes = [C.POINTER(_Csr), _dp]
     L.or_cg_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_double,
                               C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
+    L.or_bicgstab_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_double,
+                                    C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
     L.or_thermal_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                               _dp, _dp, _dp, _dp]
     L.or_elastic_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
@@ -355,6 +357,19 @@ def cg_solve(A, b, max_iter=1000, tol=1e-8, abs_tol=1e-12, use_precond=True):
     _check(lib().or_cg_solve(C.byref(m), _d(b), max_iter, tol, abs_tol,
                              1 if use_precond else 0, _d(x), C.byref(st)),
            "cg_solve")
+    return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
+                          bool(st.converged), st.solve_time)
+
+
+def bicgstab_solve(A, b, max_iter=1000, tol=1e-8, abs_tol=1e-12, use_precond=True):
+    """BiCGSTAB::solve (src/linalg/iterative.rs:177-291)."""
+    m, keep = _give_csr(A)
+    b = _f64(b)
+    x = np.zeros(A.shape[0])
+    st = _Stats()
+    _check(lib().or_bicgstab_solve(C.byref(m), _d(b), max_iter, tol, abs_tol,
+                                   1 if use_precond else 0, _d(x), C.byref(st)),
+           "bicgstab_solve")
     return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
                           bool(st.converged), st.solve_time)
 

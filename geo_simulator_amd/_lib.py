@@ -28,6 +28,8 @@ GS_PREC_MIXED = 1
 GS_SPMV_VECTOR = 4
 GS_SPMV_INDEX32 = 8
 GS_SPMV_CSR = 16
+GS_SPMV_SCALAR = 32
+GS_SPMV_BLOCK3 = 64
 
 _STATUS_NAMES = {
     GS_ERR_INVALID_ARG: "GS_ERR_INVALID_ARG", GS_ERR_HIP: "GS_ERR_HIP",

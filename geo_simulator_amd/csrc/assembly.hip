@@ -950,7 +950,7 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
   pat.row_ptr = P->row_ptr;
   pat.col = P->col;
   pat.dperm = dperm;
-  P->sell = build_sell_pattern(ctx, &pat, false);
+  P->sell = build_sell_pattern(ctx, &pat, false, P->dpn == 3 ? 3 : 1);
   GS_HIP(hipStreamSynchronize(s));
   m->plans.push_back(P);
   if (m->plans.size() > 4) m->plans.erase(m->plans.begin());

@@ -148,8 +148,9 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   check_flags(read_flag_and_clear(ctx, flag.get()));
   const SellPattern &S = *P->sell;
   auto val = palloc<double>(pool, P->nnz);
-  auto sval = palloc<double>(pool, S.entries);
-  auto sval32 = palloc<float>(pool, S.entries);
+  auto sval = palloc<double>(pool, S.nvals);
+  auto sval32 = palloc<float>(pool, S.nvals);
+  sell_zero_holes(s, S, sval.get(), sval32.get());
   auto liftv = palloc<double>(pool, P->n_lift);
   auto sdinv = palloc<double>(pool, P->n);
   auto sdinv32 = palloc<float>(pool, P->n);
@@ -159,6 +160,7 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
                 liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get());
   gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, val);
   K->dperm = dperm;
+  K->block = dpn;
   K->sell = P->sell;
   K->sval = sval;
   K->sval32 = sval32;
@@ -572,6 +574,7 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
     }
     set_device(ctx);
     *out = assemble_elastic_K(ctx, mesh, E, nu, E_elem, false, 0.0, nullptr, nullptr);
+    (*out)->block = 3;
   });
 }
 
@@ -613,6 +616,7 @@ gs_status gs_apply_dirichlet_bcs(gs_ctx *ctx, const gs_csr *K, const double *f,
             "matrix size differs from the DOF count");
     set_device(ctx);
     *K_bc = eliminate_caller_rhs(ctx, K, f, true, dof, f_bc);
+    (*K_bc)->block = K->block;
   });
 }
 
@@ -883,8 +887,10 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags) {
   if (!A) return GS_ERR_INVALID_ARG;
-  if (flags & ~(GS_SPMV_VECTOR | GS_SPMV_INDEX32 | GS_SPMV_CSR))
+  if (flags & ~(GS_SPMV_VECTOR | GS_SPMV_INDEX32 | GS_SPMV_CSR | GS_SPMV_SCALAR |
+                GS_SPMV_BLOCK3))
     return GS_ERR_INVALID_ARG;
+  if ((flags & GS_SPMV_SCALAR) && (flags & GS_SPMV_BLOCK3)) return GS_ERR_INVALID_ARG;
   const bool csr = (flags & GS_SPMV_CSR) != 0;
   const bool vec = (flags & GS_SPMV_VECTOR) != 0;
   if (!csr && (vec || lanes != 0)) return GS_ERR_INVALID_ARG;  // SELL: no lanes

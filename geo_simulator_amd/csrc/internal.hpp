@@ -125,14 +125,20 @@ struct DofPattern3 {
 // SELL-64 view of a square matrix (layout in sell.hpp): the solver's operator
 // format.  SELL row s is CSR row sperm[s] and caller index vperm[s]; columns
 // are SELL rows.  Shared by every matrix with the same pattern and numbering.
+// bs = 3: the block variant for 3-DOF-per-node operators -- slices of 64
+// block rows (3 DOF rows each), one 3x3 block per lane per chunk, one index
+// per block; DOF row 3s+c of the SELL order is block row s, component c.
 struct SellPattern {
-  int64_t n = 0, nnz = 0, nslices = 0, nchunks = 0, entries = 0;
+  int bs = 1;                      // 1: scalar SELL-64; 3: 3x3-block SELL-64
+  int64_t n = 0, nnz = 0, nslices = 0, nchunks = 0;
+  int64_t entries = 0;             // index slots (scalar: also the value count)
+  int64_t nvals = 0;               // values (bs = 3: 9 per index slot)
   bool idx16 = true;
-  DArr<int> sperm, siperm, vperm;  // n
+  DArr<int> sperm, siperm, vperm;  // n (DOF rows)
   DArr<int> cb;                    // nslices + 1 chunk offsets
   DArr<short> off;                 // entries (16-bit variant)
   DArr<int> col;                   // entries (32-bit variant)
-  DArr<int> pos;                   // nnz: CSR nonzero -> SELL entry
+  DArr<int> pos;                   // nnz: CSR nonzero -> SELL value
 };
 
 // Everything about a Dirichlet-eliminated system that depends only on the
@@ -211,6 +217,7 @@ struct gs_csr {
   gs::DArr<float> val32;      // CSR order, FP32 (CSR engine timing only)
   std::shared_ptr<gs::SellPattern> sell;  // SELL view (square matrices)
   bool sell_force32 = false;              // the view was built with int32 forced
+  int block = 1;  // 3: rows and columns come in node triples (elasticity)
   gs::DArr<double> sval;                  // SELL values, FP64
   gs::DArr<float> sval32;                 // SELL values, FP32 (mixed precision)
   gs::DArr<double> sdinv;                 // Jacobi inverse diagonal, SELL order
@@ -291,8 +298,12 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
                    double *fbc, double *sdinv, float *sdinv32);
 
 // ---- operator.hip ----
+// block = 3 builds the 3x3-block view (rows must be a multiple of 3; a block
+// is stored when any of its 9 entries is, the others as zeros).
 std::shared_ptr<SellPattern> build_sell_pattern(gs_ctx *ctx, const gs_csr *A,
-                                                bool force32);
+                                                bool force32, int block = 1);
+// zero values of a block view that no CSR nonzero writes (bs = 1: nothing)
+void sell_zero_holes(hipStream_t s, const SellPattern &P, double *sval, float *sval32);
 // sval[pos[k]] = val[k], padding zero
 template <typename T>
 void sell_values(hipStream_t s, const SellPattern &P, const double *val, T *sval);

@@ -17,6 +17,14 @@
 // 64*slice + off, pad = -32768) when every column fits, else int32 columns
 // (pad = -1).
 //
+// Block variant (3-DOF-per-node operators).  Slices of 64 block rows (3 DOF
+// rows each), one lane per block row.  Chunk c holds one 3x3 block per lane:
+// the block's column index at c*64 + lane and its 9 values component-major,
+// value m of lane l at (c*9 + m)*64 + l, so every load of a wavefront is one
+// contiguous stretch.  A block is stored when any of its entries is in the
+// matrix; the missing ones are zeros.  One index per 9 values instead of one
+// per value.  DOF row 3s+c of the SELL order is block row s, component c.
+//
 // Scheduling.  Workgroups take blocks of 4 slices.  The blocks are split in
 // 8 contiguous ranges, one per XCD (workgroup i runs on XCD i mod 8), so the
 // rows an XCD works on at any moment are close together and the gathered
@@ -36,10 +44,23 @@ constexpr int SELL_PAD16 = -32768;
 struct SellIdx16 {
   const short *__restrict__ off;
   static constexpr bool wide = false;
+  static constexpr int bs = 1;
 };
 struct SellIdx32 {
   const int *__restrict__ col;
   static constexpr bool wide = true;
+  static constexpr int bs = 1;
+};
+// block variant: the same index streams, one entry per 3x3 block
+struct BSellIdx16 {
+  const short *__restrict__ off;
+  static constexpr bool wide = false;
+  static constexpr int bs = 3;
+};
+struct BSellIdx32 {
+  const int *__restrict__ col;
+  static constexpr bool wide = true;
+  static constexpr int bs = 3;
 };
 
 template <typename T> struct SellVec;
@@ -106,6 +127,57 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
   return acc;
 }
 
+// One lane's three dot products (block row base_row + lane) over chunks
+// [c0, c1), two chunks in flight.  Each row sums its entries in ascending
+// column order, as the scalar engine does.
+template <typename T, class IDX>
+__device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val,
+                                        const T *__restrict__ x, int c0, int c1,
+                                        int lane, int base_row, T (&acc)[3]) {
+  acc[0] = acc[1] = acc[2] = (T)0;
+  auto load = [&](int c, int &col, bool &ok, T (&v)[9]) {
+    const size_t e = (size_t)c * SELL_C + lane;
+    if constexpr (IDX::wide) {
+      const int j = idx.col[e];
+      ok = j >= 0;
+      col = ok ? j : base_row;
+    } else {
+      const int j = idx.off[e];
+      ok = j != SELL_PAD16;
+      col = ok ? base_row + j : base_row;
+    }
+#pragma unroll
+    for (int m = 0; m < 9; ++m) v[m] = val[((size_t)c * 9 + m) * SELL_C + lane];
+  };
+  auto fma3 = [&](int col, bool ok, const T (&v)[9]) {
+    const T x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      const T p0 = v[3 * r] * x0, p1 = v[3 * r + 1] * x1, p2 = v[3 * r + 2] * x2;
+      acc[r] += ok ? p0 : (T)0;
+      acc[r] += ok ? p1 : (T)0;
+      acc[r] += ok ? p2 : (T)0;
+    }
+  };
+  int c = c0;
+  for (; c + 2 <= c1; c += 2) {
+    int ca, cb2;
+    bool oa, ob;
+    T va[9], vb[9];
+    load(c, ca, oa, va);
+    load(c + 1, cb2, ob, vb);
+    fma3(ca, oa, va);
+    fma3(cb2, ob, vb);
+  }
+  if (c < c1) {
+    int ca;
+    bool oa;
+    T va[9];
+    load(c, ca, oa, va);
+    fma3(ca, oa, va);
+  }
+}
+
 // Calls epi(row, (A x)[row]) for every row, from the row's lane.  Every
 // thread of the workgroup returns (no early exit), so callers may follow
 // with workgroup-wide reductions.
@@ -133,9 +205,18 @@ __device__ __forceinline__ void sell_rows(int n, int nslices,
   for (int b = first; b < last; b += stride) {
     const int s = b * SELL_WPB + w;
     if (s < nslices) {
-      const int row = s * SELL_C + lane;
-      const T acc = sell_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C);
-      if (row < n) epi(row, acc);
+      if constexpr (IDX::bs == 3) {
+        const int brow = s * SELL_C + lane;
+        T acc[3];
+        b3_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C, acc);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+          if (3 * brow + r < n) epi(3 * brow + r, acc[r]);
+      } else {
+        const int row = s * SELL_C + lane;
+        const T acc = sell_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C);
+        if (row < n) epi(row, acc);
+      }
     }
   }
 }

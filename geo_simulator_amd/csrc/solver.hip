@@ -580,8 +580,13 @@ SellOp sell_op(const gs_csr *A) {
 
 // Calls f(idx) with the pattern's index reader.
 template <class F> void with_idx(const SellPattern &P, F &&f) {
-  if (P.idx16) f(SellIdx16{P.off.get()});
-  else f(SellIdx32{P.col.get()});
+  if (P.bs == 3) {
+    if (P.idx16) f(BSellIdx16{P.off.get()});
+    else f(BSellIdx32{P.col.get()});
+  } else {
+    if (P.idx16) f(SellIdx16{P.off.get()});
+    else f(SellIdx32{P.col.get()});
+  }
 }
 
 template <typename T>
@@ -1319,22 +1324,36 @@ void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv) {
 }
 
 // The SELL view and its value/diagonal arrays, built once per matrix.
+// Block size of the SELL view: the 3x3-block view for 3-DOF-per-node
+// operators (or when asked for), the scalar one otherwise or when forced.
+static int sell_block(const gs_csr *A) {
+  if (A->spmv_flags & GS_SPMV_SCALAR) return 1;
+  const bool want = A->block == 3 || (A->spmv_flags & GS_SPMV_BLOCK3);
+  return want && A->rows > 0 && A->rows % 3 == 0 ? 3 : 1;
+}
+
 static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
   const bool force32 = (A->spmv_flags & GS_SPMV_INDEX32) != 0;
+  const int block = sell_block(A);
   hipStream_t s = ctx->stream;
-  if (!A->sell || A->sell_force32 != force32) {
-    A->sell = build_sell_pattern(ctx, A, force32);
+  if (!A->sell || A->sell_force32 != force32 || A->sell->bs != block) {
+    const bool reorder = !A->sell || A->sell->bs != block;
+    A->sell = build_sell_pattern(ctx, A, force32, block);
     A->sell_force32 = force32;
     A->sval = DArr<double>();
     A->sval32 = DArr<float>();
+    if (reorder) {  // the SELL order of the rows changed
+      A->sdinv = DArr<double>();
+      A->sdinv32 = DArr<float>();
+    }
   }
   const SellPattern &P = *A->sell;
   if (!A->sval) {
-    A->sval = dalloc<double>(P.entries);
+    A->sval = dalloc<double>(P.nvals);
     sell_values<double>(s, P, A->val.get(), A->sval.get());
   }
   if (fp32 && !A->sval32) {
-    A->sval32 = dalloc<float>(P.entries);
+    A->sval32 = dalloc<float>(P.nvals);
     sell_values<float>(s, P, A->val.get(), A->sval32.get());
   }
   if (precond && (!A->sdinv || A->sdinv.n != P.n)) {

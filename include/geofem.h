@@ -221,9 +221,21 @@ gs_status gs_elastic_element_backward(gs_ctx *ctx, gs_mesh *mesh,
  * engines for gs_csr_spmv and gs_time_spmv (always used for rectangular
  * matrices): CSR-stream by default (lanes = threads per row in the row sums,
  * 1, 2, 4 or 8), GS_SPMV_VECTOR for lanes-per-row (4, 8, 16 or 32); lanes 0 =
- * chosen from the mean row length.  Results depend on the engine and shape
- * only through floating-point summation order. */
-enum { GS_SPMV_VECTOR = 4, GS_SPMV_INDEX32 = 8, GS_SPMV_CSR = 16 };
+ * chosen from the mean row length.  Operators with 3 DOFs per node (the
+ * elasticity assemblies and their Dirichlet eliminations) use the 3x3-block
+ * SELL view: slices of 64 node rows, one index per 3x3 block (a block is
+ * stored when any of its entries is; the others as zeros).  GS_SPMV_SCALAR
+ * keeps the scalar view for them; GS_SPMV_BLOCK3 asks for the block view for
+ * any square matrix whose order is a multiple of 3.  Results depend on the
+ * engine and shape only through floating-point summation order (each row
+ * still sums its entries in ascending column order on both SELL views). */
+enum {
+  GS_SPMV_VECTOR = 4,
+  GS_SPMV_INDEX32 = 8,
+  GS_SPMV_CSR = 16,
+  GS_SPMV_SCALAR = 32,
+  GS_SPMV_BLOCK3 = 64
+};
 gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
                                 int32_t flags);
 

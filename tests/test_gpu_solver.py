@@ -179,3 +179,93 @@ def test_mixed_inner_cap_and_outer_cap():
            .with_max_outer_iterations(1))
     _, s2 = one.solve(K, f)
     assert not s2.converged and s2.outer_iterations == 1 and s2.relative_residual > 1e-14
+
+
+def block_random(nb, seed, wide=False, holes=0.3):
+    """Square matrix of order 3 nb with 3x3-block structure, some entries of
+    the stored blocks missing (holes), block rows of 1 to about 40 blocks."""
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for I in range(nb):
+        k = int(rng.integers(1, 40))
+        if wide:
+            J = rng.choice(nb, k, replace=False)
+        else:
+            J = np.unique(np.clip(I + rng.integers(-300, 301, k), 0, nb - 1))
+        J = np.union1d(J, [I])
+        for j in J:
+            for a in range(3):
+                for b in range(3):
+                    if j != I and rng.random() < holes:
+                        continue
+                    rows.append(3 * I + a)
+                    cols.append(3 * j + b)
+    vals = rng.uniform(-1.0, 1.0, len(rows))
+    B = sp.coo_matrix((vals, (rows, cols)), shape=(3 * nb, 3 * nb)).tocsr()
+    A = B + B.T
+    A = A + sp.diags(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)
+    A = sp.csr_matrix(A)
+    A.sum_duplicates()
+    A.sort_indices()
+    return A
+
+
+@pytest.mark.parametrize("wide", [False, True])
+def test_block_sell_spmv_and_cg(wide):
+    """3x3-block SELL view (asked for with GS_SPMV_BLOCK3) against scipy and
+    against the scalar view: the same products per row in the same order."""
+    nb = 3000 if not wide else 24000
+    A = block_random(nb, 17, wide=wide)
+    M = G.CsrMatrix.from_scipy(A)
+    M.set_spmv_shape(0, 0, G._lib.GS_SPMV_BLOCK3)
+    check_spmv(M, A)
+    x = np.random.default_rng(2).standard_normal(A.shape[0])
+    yb = M.apply(x)
+    M.set_spmv_shape(0, 0, G._lib.GS_SPMV_SCALAR)
+    ys = M.apply(x)
+    assert np.max(np.abs(yb - ys)) <= 1e-15 * np.max(np.abs(ys))
+    b = np.random.default_rng(5).standard_normal(A.shape[0])
+    sol = {}
+    for flags in (G._lib.GS_SPMV_BLOCK3, G._lib.GS_SPMV_SCALAR):
+        M.set_spmv_shape(0, 0, flags)
+        for prec in ("fp64", "mixed"):
+            xs, st = (G.ConjugateGradient().with_precision(prec).with_tolerance(1e-12)
+                      .with_abs_tolerance(0.0).with_max_iterations(20000).solve(M, b))
+            assert st.converged, (flags, prec)
+            assert np.linalg.norm(b - A @ xs) / np.linalg.norm(b) < 2e-12
+            sol[(flags, prec)] = (xs, st.iterations)
+    xb, ib = sol[(G._lib.GS_SPMV_BLOCK3, "fp64")]
+    xs_, is_ = sol[(G._lib.GS_SPMV_SCALAR, "fp64")]
+    assert abs(ib - is_) <= 2
+    assert np.linalg.norm(xb - xs_) / np.linalg.norm(xs_) < 1e-10
+    with pytest.raises(G.GeoFemError):
+        M.set_spmv_shape(0, 0, G._lib.GS_SPMV_BLOCK3 | G._lib.GS_SPMV_SCALAR)
+
+
+def test_elastic_system_block_view_matches_scalar():
+    """The elastic system uses the block view by default; SpMV, FP64 CG and
+    the mixed solve agree with the scalar view."""
+    import torch
+    mesh, dof, uz = W.elastic_box(G, 6, 5, 4)
+    mat = G.IsotropicElasticity(1e11, 0.25)
+    K, f = G.Assembler.assemble_elasticity_system(mesh, dof, mat, 3300.0, [0, 0, -9.81])
+    Ks = K.to_scipy()
+    x = torch.tensor(np.random.default_rng(4).standard_normal(Ks.shape[0]), device="cuda")
+    yb = K.apply(x).cpu().numpy()
+    ref = Ks @ x.cpu().numpy()
+    assert np.all(np.abs(yb - ref) <= 1e-13 * (abs(Ks) @ np.abs(x.cpu().numpy())) + 1e-300)
+    res = {}
+    for flags in (0, G._lib.GS_SPMV_SCALAR):
+        K.set_spmv_shape(0, 0, flags)
+        for prec in ("fp64", "mixed"):
+            u, st = (G.ConjugateGradient().with_precision(prec).with_tolerance(1e-12)
+                     .with_abs_tolerance(0.0).with_max_iterations(20000).solve(K, f))
+            assert st.converged
+            res[(flags, prec)] = (u.cpu().numpy(), st.iterations)
+    u0, i0 = res[(0, "fp64")]
+    u1, i1 = res[(G._lib.GS_SPMV_SCALAR, "fp64")]
+    assert abs(i0 - i1) <= 2
+    assert np.linalg.norm(u0 - u1) / np.linalg.norm(u1) < 1e-10
+    assert np.linalg.norm(u0[2::3] - uz) / np.linalg.norm(uz) < 1e-8
+    um, _ = res[(0, "mixed")]
+    assert np.linalg.norm(um - u1) / np.linalg.norm(u1) < 1e-9

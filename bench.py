@@ -75,7 +75,10 @@ def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
     t0 = time.perf_counter()
     _, sb = O.bicgstab_solve(Kbc, fbc, max_iter=5, tol=1e-8)
     bicg_per_iter = (time.perf_counter() - t0) / max(1, sb.iterations)
-    return total, t_asm, per_iter, bicg_per_iter
+    t0 = time.perf_counter()
+    _, sg = O.gmres_solve(Kbc, fbc, max_iter=10, restart=30, tol=1e-8)
+    gm_per_step = (time.perf_counter() - t0) / max(1, sg.iterations)
+    return total, t_asm, per_iter, bicg_per_iter, gm_per_step
 
 
 def make_cg(G, prec, tol, inner_tol=None, abs_tol=None, max_iter=200000):
@@ -212,6 +215,25 @@ def run_headline(G, args, nx, ny, nz):
             note="solve only, on the assembled system of the fp64 step")
         log(f"[bench] BiCGSTAB fp64: solve {ms:.2f} ms it={st.iterations} "
             f"({ms / max(1, st.iterations) * 1e3:.1f} us/pass) relL2={err:.2e}")
+        # GMRES(30), Jacobi, on the same system
+        gm = G.GMRES().with_restart(30).with_tolerance(args.tol).with_max_iterations(20000)
+        gm.solve(K, f)
+        sync()
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            x, st = gm.solve(K, f)
+            sync()
+            ts.append(time.perf_counter() - t0)
+        err = float(np.linalg.norm(x.cpu().numpy() - exact) / np.linalg.norm(exact))
+        ms = float(np.median(ts)) * 1e3
+        results["gmres30_fp64"] = dict(
+            solve_s=ms * 1e-3, iterations=st.iterations, spmv=st.spmv_count,
+            rel_res=st.relative_residual, converged=st.converged,
+            rel_l2_vs_closed_form=err, ms_per_step=ms / max(1, st.iterations),
+            note="solve only, restart 30, on the assembled system of the fp64 step")
+        log(f"[bench] GMRES(30) fp64: solve {ms:.2f} ms it={st.iterations} "
+            f"({ms / max(1, st.iterations) * 1e3:.1f} us/step) relL2={err:.2e}")
     info = dict(n=n, nnz=nnz, elements=mesh.num_elements(), symbolic_s=t_symbolic,
                 upload_s=t_upload)
     return results, tight, spmv, info
@@ -470,7 +492,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tight", action="store_true")
     ap.add_argument("--no-elastic", action="store_true")
-    ap.add_argument("--no-bicgstab", action="store_true")
+    ap.add_argument("--no-bicgstab", action="store_true",
+                    help="skip the BiCGSTAB and GMRES timings")
     ap.add_argument("--no-adjoint-thermal", action="store_true")
     ap.add_argument("--no-adjoint-elastic", action="store_true")
     ap.add_argument("--headline-only", action="store_true",
@@ -510,7 +533,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline (oracle, {threads} threads for the element loop)")
-        total, t_asm, per_iter, bicg_per_iter = cpu_baseline(
+        total, t_asm, per_iter, bicg_per_iter, gm_per_step = cpu_baseline(
             nx, ny, nz, results["fp64"]["iterations"], threads)
         cpu = {"value": total, "unit": "s", "cores": threads, "kind": "port",
                "sample": (f"oracle C restatement on the same {nx}x{ny}x{nz} workload: "
@@ -519,7 +542,8 @@ def main():
                           f"reference), serial Jacobi-CG timed over 25 iterations "
                           f"({per_iter*1e3:.1f} ms/iter) and extrapolated to the GPU "
                           f"FP64 iteration count {results['fp64']['iterations']}"),
-               "bicgstab_s_per_pass": bicg_per_iter}
+               "bicgstab_s_per_pass": bicg_per_iter,
+               "gmres30_s_per_step": gm_per_step}
         log(f"[bench] CPU baseline {total:.1f} s (asm {t_asm:.1f} s, {per_iter*1e3:.1f} ms/iter)")
 
     mixed, fp64 = results["mixed"], results["fp64"]

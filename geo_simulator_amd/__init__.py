@@ -15,7 +15,7 @@ library (there is no CPU fallback).
 from ._lib import GeoFemError, LIB_PATH
 from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
 from .dof import DofManager
-from .linalg import (BiCGSTAB, ConjugateGradient, CsrMatrix, JacobiPreconditioner,
+from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, JacobiPreconditioner,
                      SolverStats, norm)
 from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        time_elastic_elements,
@@ -24,7 +24,7 @@ from .runtime import Context, default_context
 
 __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
-    "DofManager", "ConjugateGradient", "BiCGSTAB", "CsrMatrix", "JacobiPreconditioner",
+    "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context",

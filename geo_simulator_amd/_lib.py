@@ -111,6 +111,8 @@ SIGNATURES = {
                               C.POINTER(SolverStatsC)]),
     "gs_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                                     C.POINTER(SolverStatsC)]),
+    "gs_gmres_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), C.c_int32, _vp,
+                                 C.POINTER(SolverStatsC)]),
     "gs_thermal_element_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),
     "gs_elastic_element_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp,
                                               _vp, _vp]),

@@ -847,6 +847,20 @@ gs_status gs_bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
   });
 }
 
+gs_status gs_gmres_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
+                         const gs_cg_opts *opts, int32_t restart, double *x,
+                         gs_solver_stats *st) {
+  if (!ctx || !A || !st) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((b && x) || A->rows == 0, GS_ERR_INVALID_ARG, "null vector");
+    gs_cg_opts o;
+    gs_cg_opts_default(&o);
+    if (opts) o = *opts;
+    set_device(ctx);
+    gmres_solve(ctx, A, b, o, restart, x, st);
+  });
+}
+
 gs_status gs_thermal_element_backward(gs_ctx *ctx, gs_mesh *mesh,
                                       const double *k_elem, const double *u,
                                       const double *lam, double *grad) {

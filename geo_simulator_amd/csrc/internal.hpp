@@ -317,6 +317,8 @@ void cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
               const gs_cg_opts &o, double *x, gs_solver_stats *st);
 void bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
                     const gs_cg_opts &o, double *x, gs_solver_stats *st);
+void gmres_solve(gs_ctx *ctx, const gs_csr *A, const double *b, const gs_cg_opts &o,
+                 int restart, double *x, gs_solver_stats *st);
 double time_spmv(gs_ctx *ctx, const gs_csr *A, int precision, int reps);
 
 int read_flag_and_clear(gs_ctx *ctx, int *flag_dev);

@@ -38,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <vector>
 
 #include "internal.hpp"
 #include "sell.hpp"
@@ -1312,6 +1313,246 @@ PcgResult bicgstab_core(gs_ctx *ctx, const SellOp &o, const double *val,
   return res;
 }
 
+// ---- GMRES (src/linalg/iterative.rs:313-486) ----------------------------------
+// Restarted GMRES(m), right-preconditioned, modified Gram-Schmidt, Givens
+// rotations.  The vectors stay on the device: per inner step z = M v_j,
+// w = A z (SELL, partials v_0.w), one kernel per Gram-Schmidt projection
+// (sums the previous partials itself: h_ij, w -= h_ij v_i, partials of the
+// next projection), and a normalisation kernel (h_{j+1,j}, v_{j+1}).  The
+// host reads the new Hessenberg column and does the rotations, the residual
+// estimate and the back substitution in the reference's order.
+
+// y = A x (SELL); partials of u.y
+template <class IDX>
+__global__ __launch_bounds__(BS) void k_spmv_dot(int n, int ns, const int *__restrict__ cb,
+                                                 const IDX idx, const double *__restrict__ val,
+                                                 const double *__restrict__ x,
+                                                 double *__restrict__ y,
+                                                 const double *__restrict__ u,
+                                                 double *__restrict__ part) {
+  double acc[1] = {0.0};
+  auto epi = [&](int row, double sv) {
+    y[row] = sv;
+    acc[0] += u[row] * sv;
+  };
+  sell_rows<double>(n, ns, cb, idx, val, x, epi);
+  put_partials<1>(acc, part);
+}
+
+// dst = src / s (the reference divides)
+__global__ __launch_bounds__(BS) void k_div(int n, const double *__restrict__ src, double s,
+                                            double *__restrict__ dst) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) dst[i] = src[i] / s;
+}
+
+// z = D^-1 v
+__global__ __launch_bounds__(BS) void k_precond(int n, const double *__restrict__ v,
+                                                const double *__restrict__ dinv,
+                                                double *__restrict__ z) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) z[i] = v[i] * dinv[i];
+}
+
+// Projection i of step j: h = sum of the partials of v_i.w; H[i] = h;
+// w -= h v_i; partials of next.w (next = v_{i+1}, or w itself after the last
+// projection, for ||w||).
+__global__ __launch_bounds__(FBS) void k_gm_project(int n, const double *__restrict__ vi,
+                                                    const double *__restrict__ next,
+                                                    double *__restrict__ w,
+                                                    double *__restrict__ hcol, int i,
+                                                    const double *__restrict__ part_in,
+                                                    int np_in, double *__restrict__ part) {
+  double tot[1];
+  sum_partials<1>(part_in, np_in, tot);
+  const double h = tot[0];
+  if (blockIdx.x == 0 && threadIdx.x == 0) hcol[i] = h;
+  double acc[1] = {0.0};
+  for (int l = blockIdx.x * FBS + threadIdx.x; l < n; l += gridDim.x * FBS) {
+    const double wl = w[l] - h * vi[l];
+    w[l] = wl;
+    acc[0] += (next ? next[l] : wl) * wl;
+  }
+  block_sum<1, FBS>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// H[j+1][j] = ||w||; v_{j+1} = w / H[j+1][j] when above 1e-40
+__global__ __launch_bounds__(FBS) void k_gm_normalize(int n, const double *__restrict__ w,
+                                                      double *__restrict__ vnext,
+                                                      double *__restrict__ hcol, int j,
+                                                      const double *__restrict__ part_in,
+                                                      int np_in) {
+  double tot[1];
+  sum_partials<1>(part_in, np_in, tot);
+  const double hn = sqrt(tot[0]);
+  if (blockIdx.x == 0 && threadIdx.x == 0) hcol[j + 1] = hn;
+  if (!(fabs(hn) > 1e-40)) return;
+  for (int l = blockIdx.x * FBS + threadIdx.x; l < n; l += gridDim.x * FBS)
+    vnext[l] = w[l] / hn;
+}
+
+// x += M dy, dy = sum_j V_j y_j (accumulated from 0 in j order)
+template <bool PREC>
+__global__ __launch_bounds__(BS) void k_gm_update(int n, int k, const double *__restrict__ V,
+                                                  const double *__restrict__ y,
+                                                  const double *__restrict__ dinv,
+                                                  double *__restrict__ x) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    double dy = 0.0;
+    for (int j = 0; j < k; ++j) dy += V[(size_t)j * n + i] * y[j];
+    x[i] += PREC ? dy * dinv[i] : dy;
+  }
+}
+
+// src/linalg/iterative.rs:463-473
+void givens(double a, double b, double &c, double &s, double &rho) {
+  if (std::fabs(b) < 1e-40) {
+    c = 1.0; s = 0.0; rho = a;
+  } else if (std::fabs(b) > std::fabs(a)) {
+    const double tau = a / b;
+    s = 1.0 / std::sqrt(1.0 + tau * tau);
+    c = s * tau;
+    rho = b * std::sqrt(1.0 + tau * tau);
+  } else {
+    const double tau = b / a;
+    c = 1.0 / std::sqrt(1.0 + tau * tau);
+    s = c * tau;
+    rho = a * std::sqrt(1.0 + tau * tau);
+  }
+}
+
+PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const double *dinv,
+                     const double *b, double *x, long long max_iter, int m, double tol,
+                     double abs_tol, char *base, Layout &L) {
+  hipStream_t s = ctx->stream;
+  const int n = o.n();
+  const unsigned gv = grid_vec(n), gf = grid_fat(n);
+  double *r = (double *)(base + L.add<double>(n));
+  double *w = (double *)(base + L.add<double>(n));
+  double *z = (double *)(base + L.add<double>(n));
+  double *V = (double *)(base + L.add<double>((int64_t)n * (m + 1)));
+  double *hcol = (double *)(base + L.add<double>(m + 2));
+  double *ydev = (double *)(base + L.add<double>(m + 1));
+  double *part1 = (double *)(base + L.add<double>(GRID_MAX));
+  double *part2 = (double *)(base + L.add<double>(GMAX));
+  double *dots = (double *)(base + L.add<double>(8));
+  PcgResult res;
+  double h1[1];
+  hipLaunchKernelGGL(k_fcg_start, gv, BS, 0, s, n, b, x, r, part2);  // x = 0, ||b||
+  hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, part2, (int)gv, dots);
+  GS_LAUNCH_CHECK("k_fcg_start");
+  read_dots(ctx, dots, h1, 1);
+  const double b_norm = std::sqrt(h1[0]);
+  res.bnorm = b_norm;
+  res.rnorm = b_norm;
+  if (b_norm < 1e-25) {  // src/linalg/iterative.rs:346-348
+    res.converged = true;
+    res.rnorm = 0.0;
+    return res;
+  }
+  const double target = tol * b_norm;
+  auto done = [&](double v) { return v < target || v < abs_tol; };
+  std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), col(m + 2);
+  auto HH = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
+  long long total = 0;
+  double final_res = b_norm;
+  bool converged = false;
+  while (total < max_iter) {
+    launch_residual(s, o, val, x, b, r, part1);  // r = b - A x
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, part1, o.grid, dots);
+    GS_LAUNCH_CHECK("gmres residual");
+    read_dots(ctx, dots, h1, 1);
+    ++res.spmv;
+    const double r_norm = std::sqrt(h1[0]);
+    final_res = r_norm;
+    if (done(r_norm)) {
+      converged = true;
+      break;
+    }
+    std::fill(H.begin(), H.end(), 0.0);
+    std::fill(g.begin(), g.end(), 0.0);
+    std::fill(cs.begin(), cs.end(), 0.0);
+    std::fill(sn.begin(), sn.end(), 0.0);
+    hipLaunchKernelGGL(k_div, gv, BS, 0, s, n, (const double *)r, r_norm, V);
+    g[0] = r_norm;
+    int k = 0;
+    for (int j = 0; j < m; ++j) {
+      if (total >= max_iter) break;
+      const double *vj = V + (size_t)j * n;
+      const double *zin = vj;
+      if (dinv) {
+        hipLaunchKernelGGL(k_precond, gv, BS, 0, s, n, vj, dinv, z);
+        zin = z;
+      }
+      with_idx(*o.P, [&](auto idx) {
+        hipLaunchKernelGGL((k_spmv_dot<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+                           o.P->cb.get(), idx, val, zin, w, (const double *)V, part1);
+      });
+      ++res.spmv;
+      // projections i = 0..j; the first sums the SpMV's partials
+      for (int i = 0; i <= j; ++i) {
+        const double *nxt = i < j ? V + (size_t)(i + 1) * n : nullptr;
+        const double *pin = i == 0 ? part1 : part2 + (i & 1 ? 0 : GMAX / 2);
+        const int npin = i == 0 ? o.grid : (int)gf;
+        double *pout = part2 + (i & 1 ? GMAX / 2 : 0);
+        hipLaunchKernelGGL(k_gm_project, gf, FBS, 0, s, n, V + (size_t)i * n, nxt, w, hcol,
+                           i, pin, npin, pout);
+      }
+      hipLaunchKernelGGL(k_gm_normalize, gf, FBS, 0, s, n, (const double *)w,
+                         V + (size_t)(j + 1) * n, hcol, j,
+                         (const double *)(part2 + (j & 1 ? GMAX / 2 : 0)), (int)gf);
+      GS_LAUNCH_CHECK("gmres step");
+      read_dots(ctx, hcol, col.data(), j + 2);
+      for (int i = 0; i <= j + 1; ++i) HH(i, j) = col[i];
+      for (int i = 0; i < j; ++i) {
+        const double temp = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
+        HH(i + 1, j) = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
+        HH(i, j) = temp;
+      }
+      double c, sv, rho;
+      givens(HH(j, j), HH(j + 1, j), c, sv, rho);
+      cs[j] = c;
+      sn[j] = sv;
+      HH(j, j) = rho;
+      HH(j + 1, j) = 0.0;
+      const double temp = cs[j] * g[j];
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = temp;
+      k = j + 1;
+      ++total;
+      final_res = std::fabs(g[k]);
+      if (done(final_res)) break;
+    }
+    std::fill(y.begin(), y.end(), 0.0);
+    for (int i = k - 1; i >= 0; --i) {
+      double sum = 0.0;
+      for (int j = i + 1; j < k; ++j) sum += HH(i, j) * y[j];
+      if (std::fabs(HH(i, i)) < 1e-40) break;
+      y[i] = (g[i] - sum) / HH(i, i);
+    }
+    if (k > 0) {
+      double *stage = (double *)((char *)ctx->pinned + 2048);
+      memcpy(stage, y.data(), sizeof(double) * k);
+      GS_HIP(hipMemcpyAsync(ydev, stage, sizeof(double) * k, hipMemcpyHostToDevice, s));
+      if (dinv)
+        hipLaunchKernelGGL((k_gm_update<true>), gv, BS, 0, s, n, k, (const double *)V,
+                           (const double *)ydev, dinv, x);
+      else
+        hipLaunchKernelGGL((k_gm_update<false>), gv, BS, 0, s, n, k, (const double *)V,
+                           (const double *)ydev, dinv, x);
+      GS_LAUNCH_CHECK("k_gm_update");
+    }
+    if (done(final_res)) {
+      converged = true;
+      break;
+    }
+  }
+  GS_HIP(hipStreamSynchronize(s));
+  res.iterations = total;
+  res.rnorm = final_res;
+  res.converged = converged;
+  return res;
+}
+
 }  // namespace
 
 // ---------------------------------------------------------------------------
@@ -1524,6 +1765,57 @@ void bicgstab_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
                                     o.max_iterations, o.tolerance, o.abs_tolerance, w);
   permute_scatter(s, n, xi, vp, x);
   GS_LAUNCH_CHECK("bicgstab_solve");
+  GS_HIP(hipStreamSynchronize(s));
+  st->iterations = r.iterations;
+  st->residual_norm = r.rnorm;
+  st->relative_residual = r.bnorm > 0.0 ? r.rnorm / r.bnorm : 0.0;
+  st->converged = r.converged;
+  st->spmv_count = r.spmv;
+  st->solve_time =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// GMRES::solve (src/linalg/iterative.rs:476-486): Jacobi when
+// use_preconditioner, FP64 only; restart m in [1, 200].
+void gmres_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b, const gs_cg_opts &o,
+                 int restart, double *x, gs_solver_stats *st) {
+  gs_csr *A = const_cast<gs_csr *>(Ac);  // derived caches only
+  const auto t0 = std::chrono::steady_clock::now();
+  memset(st, 0, sizeof(*st));
+  if (A->rows != A->cols) throw Error(GS_ERR_INVALID_ARG, "GMRES needs a square matrix");
+  if (o.precision != GS_PREC_FP64) throw Error(GS_ERR_INVALID_ARG, "GMRES runs in FP64 only");
+  if (restart < 1 || restart > 200)
+    throw Error(GS_ERR_INVALID_ARG, "GMRES restart must be in [1, 200]");
+  if (A->spmv_flags & GS_SPMV_CSR)
+    throw Error(GS_ERR_INVALID_ARG, "GMRES runs on the SELL engine only");
+  const int64_t n = A->rows;
+  if (n == 0) {
+    st->converged = 1;
+    return;
+  }
+  hipStream_t s = ctx->stream;
+  const bool precond = o.use_preconditioner != 0;
+  ensure_sell(ctx, A, false, precond);
+  Layout L;
+  const size_t ob = L.add<double>(n), ox = L.add<double>(n);
+  Layout dry = L;
+  for (int i = 0; i < 3; ++i) dry.add<double>(n);
+  dry.add<double>(n * (int64_t)(restart + 1));
+  dry.add<double>(restart + 2);
+  dry.add<double>(restart + 1);
+  dry.add<double>(GRID_MAX);
+  dry.add<double>(GMAX);
+  dry.add<double>(8);
+  char *base = ctx_scratch(ctx, dry.bytes);
+  double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
+  const int *vp = A->sell->vperm.get();
+  permute_gather(s, n, b, vp, bi);
+  const PcgResult r = gmres_core(ctx, sell_op(A), A->sval.get(),
+                                 precond ? A->sdinv.get() : nullptr, bi, xi,
+                                 o.max_iterations, restart, o.tolerance, o.abs_tolerance,
+                                 base, L);
+  permute_scatter(s, n, xi, vp, x);
+  GS_LAUNCH_CHECK("gmres_solve");
   GS_HIP(hipStreamSynchronize(s));
   st->iterations = r.iterations;
   st->residual_norm = r.rnorm;

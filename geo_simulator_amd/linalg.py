@@ -246,7 +246,7 @@ def norm(v) -> float:
     return float(np.sqrt(np.sum(np.asarray(v, dtype=np.float64) ** 2)))
 
 
-def _krylov_solve(entry, o, A, b):
+def _krylov_solve(entry, o, A, b, *extra):
     """Run the C ABI solver `entry` -> (x, SolverStats); x is a numpy array
     for a host right-hand side, else a device tensor."""
     if not isinstance(A, CsrMatrix):
@@ -260,7 +260,7 @@ def _krylov_solve(entry, o, A, b):
     st = _lib.SolverStatsC()
     ctx.sync_torch()
     ctx.check(getattr(lib(), entry)(ctx.handle, A.handle, dptr(bd), C.byref(o),
-                                    dptr(x), C.byref(st)), entry)
+                                    *extra, dptr(x), C.byref(st)), entry)
     stats = SolverStats(int(st.iterations), float(st.residual_norm),
                         float(st.relative_residual), bool(st.converged),
                         float(st.solve_time), int(st.outer_iterations),
@@ -326,3 +326,24 @@ class BiCGSTAB:
 
     def set_abs_tolerance(self, t):
         self.abs_tolerance = float(t)
+
+
+class GMRES(BiCGSTAB):
+    """Restarted GMRES with the reference's builder and defaults (restart 10,
+    max_iterations 1000, tolerance 1e-8, abs_tolerance 1e-12, Jacobi on;
+    src/linalg/iterative.rs:302-331), FP64 on the SELL operator; iterations
+    count inner steps over all cycles."""
+
+    name = "GMRES"
+
+    def __init__(self):
+        super().__init__()
+        self.restart = 10
+
+    def with_restart(self, m):
+        self.restart = int(m)
+        return self
+
+    def solve(self, A, b):
+        """Solver::solve (src/linalg/iterative.rs:476-486) -> (x, SolverStats)."""
+        return _krylov_solve("gs_gmres_solve", self.opts(), A, b, int(self.restart))

@@ -195,6 +195,17 @@ gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
 gs_status gs_bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
                             const gs_cg_opts *opts, double *x_dev,
                             gs_solver_stats *stats);
+/* GMRES::new().with_restart().with_max_iterations().with_tolerance()
+ * .with_abs_tolerance().with_preconditioner() and Solver::solve
+ * (src/linalg/iterative.rs:302-486): restarted GMRES(restart), right Jacobi
+ * preconditioning, modified Gram-Schmidt, Givens rotations, each cycle from
+ * the true residual; iterations count inner steps over all cycles.  Uses
+ * max_iterations, tolerance, abs_tolerance and use_preconditioner of opts
+ * (the reference's defaults: 1000, 1e-8, 1e-12, Jacobi; restart 10);
+ * restart in [1, 200] (the reference never advances with 0); FP64. */
+gs_status gs_gmres_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
+                         const gs_cg_opts *opts, int32_t restart, double *x_dev,
+                         gs_solver_stats *stats);
 
 /* ---- adjoint (new surface; no reference counterpart) ----
  * Element backward of theta_e = log k_e (thermal) or log E_e (elastic):

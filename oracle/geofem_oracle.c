@@ -957,6 +957,133 @@ int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,
   return OR_OK;
 }
 
+/* src/linalg/iterative.rs:463-473: a Givens rotation zeroing b against a. */
+static void givens(double a, double b, double *c, double *s, double *rho) {
+  if (fabs(b) < 1e-40) {
+    *c = 1.0; *s = 0.0; *rho = a;
+  } else if (fabs(b) > fabs(a)) {
+    const double tau = a / b;
+    *s = 1.0 / sqrt(1.0 + tau * tau);
+    *c = *s * tau;
+    *rho = b * sqrt(1.0 + tau * tau);
+  } else {
+    const double tau = b / a;
+    *c = 1.0 / sqrt(1.0 + tau * tau);
+    *s = *c * tau;
+    *rho = a * sqrt(1.0 + tau * tau);
+  }
+}
+
+/* src/linalg/iterative.rs:333-460 (GMRES::solve_with_operator_internal,
+ * + :476-486 for the preconditioner choice), statement by statement:
+ * restarted GMRES(m) with right preconditioning, modified Gram-Schmidt,
+ * Givens rotations; the iteration counter counts inner steps over all
+ * cycles; each cycle starts from the true residual b - A x. */
+int or_gmres_solve(const or_csr *A, const double *b, int64_t max_iter,
+                   int64_t restart, double tol, double abs_tol, int use_precond,
+                   double *x, or_stats *st) {
+  const int64_t n = A->rows, m = restart;
+  const double t0 = now_s();
+  const double b_norm = norm2(b, n);
+  memset(st, 0, sizeof(*st));
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  if (b_norm < 1e-25) {
+    st->converged = 1;
+    st->solve_time = now_s() - t0;
+    return OR_OK;
+  }
+  if (m < 1) return OR_ERR_INVALID_ARG; /* restart 0 never advances (the reference loops) */
+  const size_t nb = (size_t)(n > 0 ? n : 1) * sizeof(double);
+  double *r = (double *)malloc(nb), *w = (double *)malloc(nb), *z = (double *)malloc(nb);
+  double *dy = (double *)malloc(nb), *dinv = (double *)malloc(nb);
+  double *V = (double *)malloc(nb * (size_t)(m + 1));
+  double *H = (double *)malloc(sizeof(double) * (size_t)((m + 1) * m));
+  double *g = (double *)malloc(sizeof(double) * (size_t)(m + 1));
+  double *cs = (double *)malloc(sizeof(double) * (size_t)m);
+  double *sn = (double *)malloc(sizeof(double) * (size_t)m);
+  double *y = (double *)malloc(sizeof(double) * (size_t)m);
+  if (!r || !w || !z || !dy || !dinv || !V || !H || !g || !cs || !sn || !y) {
+    free(r); free(w); free(z); free(dy); free(dinv); free(V); free(H); free(g);
+    free(cs); free(sn); free(y);
+    return OR_ERR_OOM;
+  }
+#define HH(i, j) H[(i) * m + (j)]
+  if (use_precond) or_jacobi_diag_inv(A, dinv);
+  int64_t total = 0;
+  int converged = 0;
+  double final_res = b_norm;
+  while (total < max_iter) {
+    or_spmv(A, x, r); /* compute_residual: r = b - A x */
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    const double r_norm = norm2(r, n);
+    final_res = r_norm;
+    if (r_norm < tol * b_norm || r_norm < abs_tol) { converged = 1; break; }
+    memset(V, 0, nb * (size_t)(m + 1));
+    memset(H, 0, sizeof(double) * (size_t)((m + 1) * m));
+    for (int64_t i = 0; i < n; ++i) V[i] = r[i] / r_norm;
+    for (int64_t i = 0; i <= m; ++i) g[i] = 0.0;
+    g[0] = r_norm;
+    for (int64_t i = 0; i < m; ++i) { cs[i] = 0.0; sn[i] = 0.0; }
+    int64_t k = 0;
+    for (int64_t j = 0; j < m; ++j) {
+      if (total >= max_iter) break;
+      const double *vj = V + (size_t)j * n;
+      for (int64_t i = 0; i < n; ++i) z[i] = use_precond ? vj[i] * dinv[i] : vj[i];
+      or_spmv(A, z, w);
+      for (int64_t i = 0; i <= j; ++i) {
+        const double *vi = V + (size_t)i * n;
+        double h = 0.0;
+        for (int64_t l = 0; l < n; ++l) h += vi[l] * w[l];
+        HH(i, j) = h;
+        for (int64_t l = 0; l < n; ++l) w[l] -= h * vi[l];
+      }
+      HH(j + 1, j) = norm2(w, n);
+      if (fabs(HH(j + 1, j)) > 1e-40) {
+        double *vn = V + (size_t)(j + 1) * n;
+        for (int64_t l = 0; l < n; ++l) vn[l] = w[l] / HH(j + 1, j);
+      }
+      for (int64_t i = 0; i < j; ++i) {
+        const double temp = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
+        HH(i + 1, j) = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
+        HH(i, j) = temp;
+      }
+      double c, sv, rho;
+      givens(HH(j, j), HH(j + 1, j), &c, &sv, &rho);
+      cs[j] = c; sn[j] = sv; HH(j, j) = rho; HH(j + 1, j) = 0.0;
+      const double temp = cs[j] * g[j];
+      g[j + 1] = -sn[j] * g[j];
+      g[j] = temp;
+      k = j + 1;
+      ++total;
+      final_res = fabs(g[k]);
+      if (final_res < tol * b_norm || final_res < abs_tol) break;
+    }
+    for (int64_t i = 0; i < k; ++i) y[i] = 0.0;
+    for (int64_t i = k - 1; i >= 0; --i) {
+      double sum = 0.0;
+      for (int64_t j = i + 1; j < k; ++j) sum += HH(i, j) * y[j];
+      if (fabs(HH(i, i)) < 1e-40) break;
+      y[i] = (g[i] - sum) / HH(i, i);
+    }
+    for (int64_t i = 0; i < n; ++i) dy[i] = 0.0;
+    for (int64_t j = 0; j < k; ++j) {
+      const double *vj = V + (size_t)j * n;
+      for (int64_t i = 0; i < n; ++i) dy[i] += vj[i] * y[j];
+    }
+    for (int64_t i = 0; i < n; ++i) x[i] += use_precond ? dy[i] * dinv[i] : dy[i];
+    if (final_res < tol * b_norm || final_res < abs_tol) { converged = 1; break; }
+  }
+#undef HH
+  st->iterations = total;
+  st->residual_norm = final_res;
+  st->relative_residual = final_res / b_norm;
+  st->converged = converged;
+  st->solve_time = now_s() - t0;
+  free(r); free(w); free(z); free(dy); free(dinv); free(V); free(H); free(g);
+  free(cs); free(sn); free(y);
+  return OR_OK;
+}
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint element backward (no reference counterpart; pinned by FD).        */
 /* With theta_e = log k_e (or log E_e), dK_e/dtheta_e = K_e, so              */

@@ -125,6 +125,9 @@ int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
 int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,
                       double tol, double abs_tol, int use_precond, double *x,
                       or_stats *st);
+int or_gmres_solve(const or_csr *A, const double *b, int64_t max_iter,
+                   int64_t restart, double tol, double abs_tol, int use_precond,
+                   double *x, or_stats *st);
 
 /* ---- adjoint element backward (new surface; pinned by central FD) ---- */
 int or_thermal_element_backward(int64_t n_nodes, const double *xyz,

@@ -118,6 +118,8 @@ def _declare(L):
                               C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
     L.or_bicgstab_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_double,
                                     C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
+    L.or_gmres_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_int64, C.c_double,
+                                 C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
     L.or_thermal_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                               _dp, _dp, _dp, _dp]
     L.or_elastic_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
@@ -370,6 +372,19 @@ def bicgstab_solve(A, b, max_iter=1000, tol=1e-8, abs_tol=1e-12, use_precond=Tru
     _check(lib().or_bicgstab_solve(C.byref(m), _d(b), max_iter, tol, abs_tol,
                                    1 if use_precond else 0, _d(x), C.byref(st)),
            "bicgstab_solve")
+    return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
+                          bool(st.converged), st.solve_time)
+
+
+def gmres_solve(A, b, max_iter=1000, restart=10, tol=1e-8, abs_tol=1e-12, use_precond=True):
+    """GMRES::solve (src/linalg/iterative.rs:313-486)."""
+    m, keep = _give_csr(A)
+    b = _f64(b)
+    x = np.zeros(A.shape[0])
+    st = _Stats()
+    _check(lib().or_gmres_solve(C.byref(m), _d(b), max_iter, restart, tol, abs_tol,
+                                1 if use_precond else 0, _d(x), C.byref(st)),
+           "gmres_solve")
     return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
                           bool(st.converged), st.solve_time)
 

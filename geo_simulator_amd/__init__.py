@@ -21,13 +21,15 @@ from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        time_elastic_elements,
                        thermal_element_backward)
 from .runtime import Context, default_context
+from .fields import FieldData, ScalarField, VectorField
+from .vtk import VtkWriter
 
 __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
-    "Context",
+    "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",
     "default_context",
 ]
 

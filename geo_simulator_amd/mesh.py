@@ -13,6 +13,8 @@ import ctypes as C
 
 import numpy as np
 
+from .fields import FieldData
+
 from ._lib import GeoFemError, check, lib
 
 
@@ -36,6 +38,8 @@ class Mesh:
     def __init__(self, nodes, elements):
         self.nodes = np.ascontiguousarray(nodes, dtype=np.float64).reshape(-1, 3)
         self.elements = np.ascontiguousarray(elements, dtype=np.int64).reshape(-1, 10)
+        self.field_data = FieldData()  # nodal fields (src/mesh/geometry.rs:44)
+        self.cell_data = FieldData()   # element fields (src/mesh/geometry.rs:45)
         self._handles = {}
 
     # reference accessors

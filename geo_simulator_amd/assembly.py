@@ -92,6 +92,21 @@ class Assembler:
             dptr(f)), "assemble_thermal_load")
         return _out(ctx, f, device)
 
+    @staticmethod
+    def assemble_thermal_mass(mesh, dof_mgr, density, specific_heat,
+                              rho_cp_per_elem=None, ctx=None):
+        """Global thermal mass from ElementMatrix::thermal_mass
+        (src/fem/element.rs:74-105, 5-point rule), scattered like the
+        stiffness; rho_cp_per_elem overrides density * specific_heat."""
+        ctx = ctx or default_context()
+        rc = _elem_array(ctx, rho_cp_per_elem, mesh.num_elements())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_thermal_mass(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), float(density),
+            float(specific_heat), dptr(rc), C.byref(h)), "assemble_thermal_mass")
+        return CsrMatrix(ctx, h)
+
     # ---- elasticity -----------------------------------------------------------
     @staticmethod
     def assemble_elasticity_stiffness_parallel(mesh, dof_mgr, material,

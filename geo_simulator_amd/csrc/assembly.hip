@@ -176,6 +176,10 @@ __device__ __forceinline__ int tri_index(int a, int b) {
   return 10 * a - (a * (a - 1)) / 2 + (b - a);
 }
 
+// FULL: element matrices with all 100 entries (Ke[e*100 + a*10 + b], the
+// thermal mass, symmetric only to rounding as in the reference); otherwise
+// the packed upper triangle of a bitwise-symmetric K_e.
+template <bool FULL>
 __global__ void k_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
                                  const int *__restrict__ contrib,
                                  const double *__restrict__ Ke,
@@ -189,8 +193,8 @@ __global__ void k_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
     const int e = t / 100;
     const int ab = t - 100 * e;
     const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
-    const double x = Ke[(int64_t)e * 55 + p];
+    const double x = FULL ? Ke[(int64_t)e * 100 + ab]
+                          : Ke[(int64_t)e * 55 + (a <= b ? tri_index(a, b) : tri_index(b, a))];
     v = (k == b0) ? x : v + x;
   }
   val[s] = v;
@@ -789,9 +793,17 @@ void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
                     const double *Ke, double *val) {
   (void)E;
   if (p.nnz == 0) return;
-  hipLaunchKernelGGL(k_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
+  hipLaunchKernelGGL((k_gather_thermal<false>), nblk(p.nnz), BS, 0, s, p.nnz,
                      p.contrib_ptr.get(), p.contrib.get(), Ke, val);
   GS_LAUNCH_CHECK("k_gather_thermal");
+}
+
+void gather_thermal_full(hipStream_t s, const NodePattern &p, const double *Me,
+                         double *val) {
+  if (p.nnz == 0) return;
+  hipLaunchKernelGGL((k_gather_thermal<true>), nblk(p.nnz), BS, 0, s, p.nnz,
+                     p.contrib_ptr.get(), p.contrib.get(), Me, val);
+  GS_LAUNCH_CHECK("k_gather_thermal_full");
 }
 
 void gather_thermal_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,

@@ -578,6 +578,29 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
   });
 }
 
+gs_status gs_assemble_thermal_mass(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                   double density, double specific_heat,
+                                   const double *rho_cp_elem, gs_csr **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    set_device(ctx);
+    build_node_pattern(ctx, mesh);
+    const NodePattern &p = mesh->pat;
+    hipStream_t s = ctx->stream;
+    auto Me = dalloc<double>(mesh->n_elems * 100);
+    launch_thermal_mass(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(),
+                        density * specific_heat, rho_cp_elem, Me.get());
+    auto val = dalloc<double>(p.nnz);
+    gather_thermal_full(s, p, Me.get(), val.get());
+    GS_HIP(hipStreamSynchronize(s));
+    gs_csr *M = new_csr(ctx, mesh->n_nodes, mesh->n_nodes, p.nnz, p.row_ptr, p.col, val);
+    M->dperm = mesh->perm;
+    *out = M;
+  });
+}
+
 gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double rho,
                                    const double g[3], double *f) {

@@ -78,6 +78,36 @@ constexpr Tables make_tables() {
 
 __device__ constexpr Tables kT = make_tables();
 
+// 5-point rule (src/fem/quadrature.rs:44-59, literal constants) for the
+// thermal mass matrix
+constexpr double Q5B = 1.0 / 6.0;
+struct Tables5 {
+  double dl[5][10][3];
+  double N[5][10];
+  double w[5];
+};
+constexpr Tables5 make_tables5() {
+  Tables5 t{};
+  const double P[5][4] = {{0.25, 0.25, 0.25, 0.25},
+                          {Q5B, Q5B, Q5B, 0.5},
+                          {Q5B, Q5B, 0.5, Q5B},
+                          {Q5B, 0.5, Q5B, Q5B},
+                          {0.5, Q5B, Q5B, Q5B}};
+  const double W[5] = {-2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0};
+  for (int q = 0; q < 5; ++q) {
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = dndl(i, 0, P[q]);
+      t.dl[q][i][0] = dndl(i, 1, P[q]) - d0;
+      t.dl[q][i][1] = dndl(i, 2, P[q]) - d0;
+      t.dl[q][i][2] = dndl(i, 3, P[q]) - d0;
+      t.N[q][i] = shape(i, P[q]);
+    }
+    t.w[q] = W[q];
+  }
+  return t;
+}
+__device__ constexpr Tables5 kT5 = make_tables5();
+
 enum : int { FLAG_SINGULAR = 1, FLAG_NONFINITE = 2, FLAG_MATERIAL = 4 };
 
 // J (src/fem/basis.rs:196-223), accumulated over the ten nodes in order.
@@ -494,6 +524,53 @@ __global__ void k_maxdiff(int64_t n, const double *__restrict__ a,
 }
 
 // ---------------------------------------------------------------------------
+// Thermal mass (src/fem/element.rs:74-105): M_ij = sum_q rho_cp N_i N_j w_q
+// |det J_q| over the 5-point rule, all 100 entries in the reference's order
+// ((rho_cp N_i) N_j) w, one thread per element, Me[e*100 + i*10 + j].
+// The reference does not invert J here, so nothing is flagged.
+template <bool PER_ELEM>
+__global__ __launch_bounds__(128) void k_thermal_mass(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    double rho_cp, const double *__restrict__ rc_elem, double *__restrict__ Me) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double X[10][3];
+  load_nodes(xyz, conn, e, X);
+  const double rc = PER_ELEM ? rc_elem[e] : rho_cp;
+  double w[5];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    double J[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) J[r][0] = J[r][1] = J[r][2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const double d1 = kT5.dl[q][i][0], d2 = kT5.dl[q][i][1], d3 = kT5.dl[q][i][2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        J[r][0] += X[i][r] * d1;
+        J[r][1] += X[i][r] * d2;
+        J[r][2] += X[i][r] * d3;
+      }
+    }
+    const double m12_23 = J[1][1] * J[2][2] - J[2][1] * J[1][2];
+    const double m11_23 = J[1][0] * J[2][2] - J[2][0] * J[1][2];
+    const double m11_22 = J[1][0] * J[2][1] - J[2][0] * J[1][1];
+    const double det = J[0][0] * m12_23 - J[0][1] * m11_23 + J[0][2] * m11_22;
+    w[q] = kT5.w[q] * fabs(det);
+  }
+  double *out = Me + e * 100;
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int j = 0; j < 10; ++j) {
+      double m = 0.0;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) m += rc * kT5.N[q][i] * kT5.N[q][j] * w[q];
+      out[i * 10 + j] = m;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint element backward: grad_e = -lambda_e^T K_e u_e, evaluated per
 // quadrature point as -sum_q w_q k (G^T lambda).(G^T u) (thermal) and
 // -sum_q w_q (B lambda)^T D (B u) (elastic), one thread per element.
@@ -683,6 +760,18 @@ void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
       hipLaunchKernelGGL((k_elastic_elements_mfma<false, false>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
   }
   GS_LAUNCH_CHECK("k_elastic_elements_mfma");
+}
+
+void launch_thermal_mass(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                         double rho_cp, const double *rc_elem, double *Me) {
+  if (E == 0) return;
+  if (rc_elem)
+    hipLaunchKernelGGL((k_thermal_mass<true>), blocks_for(E, 128), 128, 0, s, E, xyz, conn,
+                       rho_cp, rc_elem, Me);
+  else
+    hipLaunchKernelGGL((k_thermal_mass<false>), blocks_for(E, 128), 128, 0, s, E, xyz, conn,
+                       rho_cp, rc_elem, Me);
+  GS_LAUNCH_CHECK("k_thermal_mass");
 }
 
 void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,

@@ -237,6 +237,8 @@ void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double Emod, double nu,
                              const double *E_elem, double rho, const double g[3],
                              bool with_load, double *Ke, double *Fe, int *flag);
+void launch_thermal_mass(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                         double rho_cp, const double *rc_elem, double *Me);
 // Matrix-core variant of the elastic element kernel (timing; entries equal
 // the VALU kernel's up to rounding).
 void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
@@ -262,6 +264,9 @@ void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
                     const double *Ke, double *val);
 // Loads land at out_map[r] (caller numbering) when out_map is given, else at
 // internal row r.
+// all-100-entry element matrices (thermal mass)
+void gather_thermal_full(hipStream_t s, const NodePattern &p, const double *Me,
+                         double *val);
 void gather_thermal_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
                          const double *Fe, const int *out_map, double *f);
 void gather_elastic(hipStream_t s, const NodePattern &p, int64_t n_nodes,

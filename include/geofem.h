@@ -103,6 +103,15 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
                                            const double *E_per_elem_dev,
                                            gs_csr **out);
 /* Assembler::assemble_gravity_load (src/fem/assembly.rs:906-937). */
+/* ElementMatrix::thermal_mass (src/fem/element.rs:74-105: M_e = int rho c_p
+ * N^T N dV with the 5-point rule) for every element, scattered like the
+ * stiffness (DOF = node id; the reference has the element routine only).
+ * rho_cp_per_elem_dev (nullable) replaces density * specific_heat per element.
+ * All 100 entries of each M_e are summed (M is symmetric to rounding, as the
+ * reference's). */
+gs_status gs_assemble_thermal_mass(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                   double density, double specific_heat,
+                                   const double *rho_cp_per_elem_dev, gs_csr **out);
 gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double density,
                                    const double gravity[3], double *f_dev);

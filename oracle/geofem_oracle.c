@@ -203,6 +203,26 @@ int or_thermal_stiffness(const double nodes[10][3], double k, double K[10][10]) 
   return or_thermal_stiffness_rule(nodes, k, 4, K);
 }
 
+/* src/fem/element.rs:74-105: M_ij += rho_cp * N_i * N_j * w over the 5-point
+ * rule, w = w_q |det J|, rho_cp = density * specific_heat (no inverse of J,
+ * so no singular-Jacobian panic). */
+void or_thermal_mass(const double nodes[10][3], double density, double specific_heat,
+                     double M[10][10]) {
+  const double *pts, *wts;
+  or_quad_rule(5, &pts, &wts);
+  const double rho_cp = density * specific_heat;
+  memset(M, 0, sizeof(double) * 100);
+  for (int q = 0; q < 5; ++q) {
+    const double *L = pts + 4 * q;
+    double N[10], J[3][3];
+    or_shape_functions(L, N);
+    or_jacobian(L, nodes, J);
+    const double w = wts[q] * fabs(or_det3(J));
+    for (int i = 0; i < 10; ++i)
+      for (int j = 0; j < 10; ++j) M[i][j] += rho_cp * N[i] * N[j] * w;
+  }
+}
+
 /* src/fem/element.rs:117-140: f_i += (N_i * Q) * w. */
 int or_thermal_load(const double nodes[10][3], double Q, double fe[10]) {
   for (int i = 0; i < 10; ++i) fe[i] = 0.0;
@@ -546,6 +566,9 @@ static void set_threads(int nthreads) {
 /* src/fem/assembly.rs:69-117: parallel element kernels (Rayon map/collect ->
  * OpenMP loop into an element-ordered array), then a serial element-major
  * triplet merge and to_csr.  DOF = node id (:95-97). */
+static int scatter_scalar(int64_t n_nodes, int64_t n_elems, const int64_t *conn,
+                          const double *Ke, or_csr *out);
+
 int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
                         const int64_t *conn, double k, const double *k_elem,
                         int nthreads, or_csr *out) {
@@ -566,6 +589,16 @@ int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
     }
   }
   if (err != OR_OK) { free(Ke); return err; }
+  err = scatter_scalar(n_nodes, n_elems, conn, Ke, out);
+  free(Ke);
+  return err;
+}
+
+/* The serial triplet scatter of src/fem/assembly.rs:107-116 (DOF = node id)
+ * for 10x10 element matrices Ke[e][10][10]. */
+static int scatter_scalar(int64_t n_nodes, int64_t n_elems, const int64_t *conn,
+                          const double *Ke, or_csr *out) {
+  int err;
   const int64_t nt = 100 * n_elems;
   const int small = n_nodes < 2147483647LL;
   int32_t *ti32 = NULL, *tj32 = NULL;
@@ -578,7 +611,7 @@ int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
     tj64 = (int64_t *)malloc((size_t)(nt > 0 ? nt : 1) * sizeof(int64_t));
   }
   if ((small && (!ti32 || !tj32)) || (!small && (!ti64 || !tj64))) {
-    free(Ke); free(ti32); free(tj32); free(ti64); free(tj64);
+    free(ti32); free(tj32); free(ti64); free(tj64);
     return OR_ERR_OOM;
   }
   int64_t t = 0;
@@ -592,7 +625,31 @@ int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
       }
   }
   err = triplets_to_csr(n_nodes, n_nodes, nt, ti32, tj32, ti64, tj64, Ke, out);
-  free(Ke); free(ti32); free(tj32); free(ti64); free(tj64);
+  free(ti32); free(tj32); free(ti64); free(tj64);
+  return err;
+}
+
+/* Global thermal mass: element.rs:74-105 per element (parallel), scattered
+ * as the stiffness is (no reference counterpart at the global level). */
+int or_assemble_thermal_mass(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                             const int64_t *conn, double density, double specific_heat,
+                             const double *rho_cp_elem, int nthreads, or_csr *out) {
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  double *Me = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 100 * sizeof(double));
+  if (!Me) return OR_ERR_OOM;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    if (rho_cp_elem)
+      or_thermal_mass(nodes, rho_cp_elem[e], 1.0, (double(*)[10])(Me + 100 * e));
+    else
+      or_thermal_mass(nodes, density, specific_heat, (double(*)[10])(Me + 100 * e));
+  }
+  const int err = scatter_scalar(n_nodes, n_elems, conn, Me, out);
+  free(Me);
   return err;
 }
 

@@ -99,6 +99,11 @@ int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
 int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
                         const int64_t *conn, double E, double nu,
                         const double *E_elem, int nthreads, or_csr *out);
+void or_thermal_mass(const double nodes[10][3], double density, double specific_heat,
+                     double M[10][10]);
+int or_assemble_thermal_mass(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                             const int64_t *conn, double density, double specific_heat,
+                             const double *rho_cp_elem, int nthreads, or_csr *out);
 int or_assemble_thermal_load(int64_t n_nodes, const double *xyz,
                              int64_t n_elems, const int64_t *conn, double Q,
                              double *f);

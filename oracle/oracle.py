@@ -108,6 +108,10 @@ def _declare(L):
                                       C.POINTER(_Csr)]
     L.or_assemble_thermal_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                            C.c_double, _dp]
+    L.or_thermal_mass.argtypes = [_dp, C.c_double, C.c_double, _dp]
+    L.or_thermal_mass.restype = None
+    L.or_assemble_thermal_mass.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_double,
+                                           C.c_double, _dp, C.c_int, C.POINTER(_Csr)]
     L.or_assemble_gravity_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                            C.c_double, _dp, _dp]
     L.or_apply_dirichlet.argtypes = [C.POINTER(_Csr), _dp, _u8p, _dp,
@@ -202,6 +206,13 @@ def thermal_load(nodes, Q):
     return out
 
 
+def thermal_mass(nodes, density, specific_heat):
+    """ElementMatrix::thermal_mass (src/fem/element.rs:74-105)."""
+    out = np.zeros((10, 10))
+    lib().or_thermal_mass(_d(_f64(nodes, (10, 3))), density, specific_heat, _d(out))
+    return out
+
+
 def constitutive(E, nu):
     out = np.zeros((6, 6))
     _check(lib().or_constitutive(E, nu, _d(out)), "constitutive")
@@ -284,6 +295,18 @@ def assemble_thermal(xyz, conn, k, k_elem=None, nthreads=0):
     _check(lib().or_assemble_thermal(len(xyz), _d(xyz), len(conn), _i(conn), k,
                                      None if ke is None else _d(ke), nthreads,
                                      C.byref(m)), "assemble_thermal")
+    return _take_csr(m)
+
+
+def assemble_thermal_mass(xyz, conn, density, specific_heat, rho_cp_elem=None, nthreads=0):
+    """Global thermal mass: element.rs:74-105 scattered like the stiffness."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    re = None if rho_cp_elem is None else _f64(rho_cp_elem)
+    m = _Csr()
+    _check(lib().or_assemble_thermal_mass(len(xyz), _d(xyz), len(conn), _i(conn), density,
+                                          specific_heat, None if re is None else _d(re),
+                                          nthreads, C.byref(m)), "assemble_thermal_mass")
     return _take_csr(m)
 
 

@@ -53,6 +53,20 @@ def test_thermal_assembly_per_element_k_bitexact(oracle):
     assert same_csr(K.to_scipy(), oracle.assemble_thermal(xyz, conn, 0.0, k_elem=k))
 
 
+@pytest.mark.parametrize("perturb", [0.0, 0.08])
+def test_thermal_mass_bitexact(oracle, perturb):
+    """5-point thermal mass (src/fem/element.rs:74-105), all 100 entries per
+    element, uniform and per-element rho c_p, straight and curved elements."""
+    xyz, conn, mesh = mesh_pair(oracle, 5, 4, 3, 1000.0, 800.0, 600.0, perturb, 6)
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    M = G.Assembler.assemble_thermal_mass(mesh, dof, 3300.0, 1250.0)
+    assert same_csr(M.to_scipy(), oracle.assemble_thermal_mass(xyz, conn, 3300.0, 1250.0))
+    rc = np.exp(np.random.default_rng(5).normal(15.0, 0.3, len(conn)))
+    M2 = G.Assembler.assemble_thermal_mass(mesh, dof, 0.0, 0.0, rho_cp_per_elem=rc)
+    assert same_csr(M2.to_scipy(), oracle.assemble_thermal_mass(xyz, conn, 0.0, 0.0,
+                                                                rho_cp_elem=rc))
+
+
 @pytest.mark.parametrize("perturb", [0.0, 0.06])
 def test_elastic_assembly_bitexact(oracle, perturb):
     xyz, conn, mesh = mesh_pair(oracle, 2, 3, 2, 1e6, 1e6, 6e5, perturb, 3)

@@ -22,6 +22,7 @@ from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        thermal_element_backward)
 from .runtime import Context, default_context
 from .fields import FieldData, ScalarField, VectorField
+from .boundary import BoundaryConditions, BoundaryFace, NeumannBC, Tet10Face
 from .vtk import VtkWriter
 
 __all__ = [
@@ -30,6 +31,7 @@ __all__ = [
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",
+    "BoundaryConditions", "BoundaryFace", "NeumannBC", "Tet10Face",
     "default_context",
 ]
 

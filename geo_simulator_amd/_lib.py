@@ -95,6 +95,8 @@ SIGNATURES = {
                                                    C.c_double, _vp, _pp]),
     "gs_assemble_thermal_mass": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp,
                                            _pp]),
+    "gs_apply_neumann_bcs": (C.c_int, [_vp, _vp, _i64, _i64p, C.POINTER(C.c_int32), _dp,
+                                       _vp]),
     "gs_assemble_gravity_load": (C.c_int, [_vp, _vp, _vp, C.c_double, _dp, _vp]),
     "gs_apply_dirichlet_bcs": (C.c_int, [_vp, _vp, _vp, _vp, _pp, _vp]),
     "gs_assemble_thermal_system": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp,

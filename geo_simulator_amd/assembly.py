@@ -107,6 +107,53 @@ class Assembler:
             float(specific_heat), dptr(rc), C.byref(h)), "assemble_thermal_mass")
         return CsrMatrix(ctx, h)
 
+    @staticmethod
+    def apply_neumann_bcs(mesh, f, bcs, ctx=None):
+        """src/fem/assembly.rs:248-273: f += the flux of every Neumann face of
+        `bcs` (a BoundaryConditions), in place on a numpy array or a device
+        tensor; the face integrals run on the GPU."""
+        ctx = ctx or default_context()
+        faces = list(bcs.neumann_faces())
+        if not faces:
+            return f
+        el = np.array([fc.element_id for fc, _ in faces], dtype=np.int64)
+        lf = np.array([fc.local_face_id for fc, _ in faces], dtype=np.int32)
+        fl = np.array([bc.flux for _, bc in faces], dtype=np.float64)
+        import torch
+        on_device = isinstance(f, torch.Tensor)
+        fd = f if on_device and f.is_cuda and f.dtype == torch.float64 and f.is_contiguous() \
+            else ctx.to_device(f)
+        ctx.sync_torch()
+        ctx.check(lib().gs_apply_neumann_bcs(
+            ctx.handle, mesh.handle(ctx), len(el), el.ctypes.data_as(C.POINTER(C.c_int64)),
+            lf.ctypes.data_as(C.POINTER(C.c_int32)), fl.ctypes.data_as(C.POINTER(C.c_double)),
+            dptr(fd)), "apply_neumann_bcs")
+        if fd is not f:
+            if on_device:
+                f.copy_(fd)
+            else:
+                f[...] = fd.cpu().numpy()
+        return f
+
+    @staticmethod
+    def find_boundary_faces(mesh, condition):
+        """src/fem/assembly.rs:285-309: (element_id, local_face_id) of every
+        element face whose six nodes all satisfy `condition` (called with an
+        (n, 3) array of points, or with one point at a time when it does not
+        accept arrays), in element then face order."""
+        from .boundary import FACE_NODES
+        pts = mesh.nodes
+        try:
+            on = np.asarray(condition(pts), dtype=bool).reshape(-1)
+            if on.shape[0] != pts.shape[0]:
+                raise ValueError
+        except Exception:
+            on = np.array([bool(condition(p)) for p in pts])
+        fn = mesh.elements[:, FACE_NODES]          # (E, 4, 6)
+        hit = on[fn].all(axis=2)                   # (E, 4)
+        e, lf = np.nonzero(hit)
+        return list(zip(e.tolist(), lf.tolist()))
+
     # ---- elasticity -----------------------------------------------------------
     @staticmethod
     def assemble_elasticity_stiffness_parallel(mesh, dof_mgr, material,

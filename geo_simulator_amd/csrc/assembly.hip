@@ -658,6 +658,25 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
   sdinv32[si] = (float)di;
 }
 
+__global__ void k_iota(int64_t n, int *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) out[k] = (int)k;
+}
+
+// f[node] += contributions, each node's in face order (the contributions
+// are sorted stably by node; a run head adds its run in order).
+__global__ void k_neumann_add(int64_t N, const int *__restrict__ node,
+                              const int *__restrict__ idx,
+                              const double *__restrict__ contrib, double *__restrict__ f) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= N) return;
+  const int nd = node[k];
+  if (k > 0 && node[k - 1] == nd) return;
+  double acc = f[nd];
+  for (int64_t j = k; j < N && node[j] == nd; ++j) acc += contrib[idx[j]];
+  f[nd] = acc;
+}
+
 template <class F> void scan_inclusive(gs_ctx *ctx, const int *in, int *out,
                                        int64_t n, F op) {
   size_t bytes = 0;
@@ -796,6 +815,31 @@ void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
   hipLaunchKernelGGL((k_gather_thermal<false>), nblk(p.nnz), BS, 0, s, p.nnz,
                      p.contrib_ptr.get(), p.contrib.get(), Ke, val);
   GS_LAUNCH_CHECK("k_gather_thermal");
+}
+
+void apply_neumann(gs_ctx *ctx, const gs_mesh *m, int64_t F, const int64_t *elem,
+                   const int *lface, const double *flux, double *f) {
+  if (F == 0) return;
+  hipStream_t s = ctx->stream;
+  const int64_t N = 6 * F;
+  auto contrib = dalloc<double>(N);
+  auto node = dalloc<int>(N), node2 = dalloc<int>(N);
+  auto idx = dalloc<int>(N), idx2 = dalloc<int>(N);
+  launch_face_flux(s, F, m->xyz.get(), m->conn.get(), elem, lface, flux, contrib.get(),
+                   node.get());
+  hipLaunchKernelGGL(k_iota, nblk(N), BS, 0, s, N, idx.get());
+  int bits = 1;
+  while (bits < 31 && (int64_t(1) << bits) < m->n_nodes) ++bits;
+  size_t bytes = 0;
+  GS_HIP(rocprim::radix_sort_pairs(nullptr, bytes, node.get(), node2.get(), idx.get(),
+                                   idx2.get(), (size_t)N, 0u, (unsigned)bits, s));
+  auto tmp = dalloc<char>((int64_t)bytes);
+  GS_HIP(rocprim::radix_sort_pairs(tmp.get(), bytes, node.get(), node2.get(), idx.get(),
+                                   idx2.get(), (size_t)N, 0u, (unsigned)bits, s));
+  hipLaunchKernelGGL(k_neumann_add, nblk(N), BS, 0, s, N, node2.get(), idx2.get(),
+                     contrib.get(), f);
+  GS_LAUNCH_CHECK("k_neumann_add");
+  GS_HIP(hipStreamSynchronize(s));
 }
 
 void gather_thermal_full(hipStream_t s, const NodePattern &p, const double *Me,

@@ -601,6 +601,35 @@ gs_status gs_assemble_thermal_mass(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *
   });
 }
 
+gs_status gs_apply_neumann_bcs(gs_ctx *ctx, gs_mesh *mesh, int64_t n_faces,
+                               const int64_t *element_ids, const int32_t *local_faces,
+                               const double *fluxes, double *f) {
+  if (!ctx || !mesh) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(n_faces >= 0, GS_ERR_INVALID_ARG, "negative face count");
+    if (n_faces == 0) return;
+    require(element_ids && local_faces && fluxes && f, GS_ERR_INVALID_ARG, "null argument");
+    for (int64_t k = 0; k < n_faces; ++k) {
+      require(local_faces[k] >= 0 && local_faces[k] <= 3, GS_ERR_INVALID_ARG,
+              "Invalid face index for tet10");
+      require(element_ids[k] >= 0 && element_ids[k] < mesh->n_elems, GS_ERR_INVALID_ARG,
+              "face element id out of range");
+    }
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    auto el = dalloc<int64_t>(n_faces);
+    auto lf = dalloc<int>(n_faces);
+    auto fl = dalloc<double>(n_faces);
+    GS_HIP(hipMemcpyAsync(el.get(), element_ids, n_faces * sizeof(int64_t),
+                          hipMemcpyHostToDevice, s));
+    GS_HIP(hipMemcpyAsync(lf.get(), local_faces, n_faces * sizeof(int32_t),
+                          hipMemcpyHostToDevice, s));
+    GS_HIP(hipMemcpyAsync(fl.get(), fluxes, n_faces * sizeof(double), hipMemcpyHostToDevice,
+                          s));
+    apply_neumann(ctx, mesh, n_faces, el.get(), lf.get(), fl.get(), f);
+  });
+}
+
 gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double rho,
                                    const double g[3], double *f) {

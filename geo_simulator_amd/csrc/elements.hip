@@ -571,6 +571,68 @@ __global__ __launch_bounds__(128) void k_thermal_mass(
 }
 
 // ---------------------------------------------------------------------------
+// Neumann faces (src/fem/boundary.rs:72-253): the 6 nodal contributions of
+// a constant flux over a quadratic triangle face, 3-point rule, one thread
+// per face; contrib[6k + i] goes to node[6k + i].
+__constant__ int kFaceNodes[4][6] = {{1, 2, 3, 5, 9, 8},
+                                     {0, 2, 3, 6, 9, 7},
+                                     {0, 1, 3, 4, 8, 7},
+                                     {0, 1, 2, 4, 5, 6}};
+
+__global__ __launch_bounds__(128) void k_face_flux(
+    int64_t F, const double *__restrict__ xyz, const int *__restrict__ conn,
+    const int64_t *__restrict__ elem, const int *__restrict__ lface,
+    const double *__restrict__ flux, double *__restrict__ contrib,
+    int *__restrict__ node) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= F) return;
+  const int64_t e = elem[k];
+  const int lf = lface[k];
+  double c[6][3];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const int nd = conn[e * 10 + kFaceNodes[lf][i]];
+    node[6 * k + i] = nd;
+#pragma unroll
+    for (int r = 0; r < 3; ++r) c[i][r] = xyz[(int64_t)nd * 3 + r];
+  }
+  const double qp[3][3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                           {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                           {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+  const double fl = flux[k];
+  double out[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+  for (int q = 0; q < 3; ++q) {
+    const double l1 = qp[q][0], l2 = qp[q][1], w = qp[q][2];
+    const double l3 = 1.0 - l1 - l2;
+    const double n[6] = {l1 * (2.0 * l1 - 1.0), l2 * (2.0 * l2 - 1.0), l3 * (2.0 * l3 - 1.0),
+                         4.0 * l1 * l2,         4.0 * l2 * l3,         4.0 * l3 * l1};
+    const double d[6][2] = {{4.0 * l1 - 1.0, 0.0},
+                            {0.0, 4.0 * l2 - 1.0},
+                            {-4.0 * l3 + 1.0, -4.0 * l3 + 1.0},
+                            {4.0 * l2, 4.0 * l1},
+                            {-4.0 * l2, 4.0 * (l3 - l2)},
+                            {4.0 * (l3 - l1), -4.0 * l1}};
+    double t1[3] = {0.0, 0.0, 0.0}, t2[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int i = 0; i < 6; ++i)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        t1[r] += c[i][r] * d[i][0];
+        t2[r] += c[i][r] * d[i][1];
+      }
+    const double x = t1[1] * t2[2] - t1[2] * t2[1];
+    const double y = t1[2] * t2[0] - t1[0] * t2[2];
+    const double z = t1[0] * t2[1] - t1[1] * t2[0];
+    const double det = sqrt(x * x + y * y + z * z);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) out[i] += w * n[i] * fl * det;
+  }
+#pragma unroll
+  for (int i = 0; i < 6; ++i) contrib[6 * k + i] = out[i];
+}
+
+// ---------------------------------------------------------------------------
 // Adjoint element backward: grad_e = -lambda_e^T K_e u_e, evaluated per
 // quadrature point as -sum_q w_q k (G^T lambda).(G^T u) (thermal) and
 // -sum_q w_q (B lambda)^T D (B u) (elastic), one thread per element.
@@ -772,6 +834,15 @@ void launch_thermal_mass(hipStream_t s, int64_t E, const double *xyz, const int 
     hipLaunchKernelGGL((k_thermal_mass<false>), blocks_for(E, 128), 128, 0, s, E, xyz, conn,
                        rho_cp, rc_elem, Me);
   GS_LAUNCH_CHECK("k_thermal_mass");
+}
+
+void launch_face_flux(hipStream_t s, int64_t F, const double *xyz, const int *conn,
+                      const int64_t *elem, const int *lface, const double *flux,
+                      double *contrib, int *node) {
+  if (F == 0) return;
+  hipLaunchKernelGGL(k_face_flux, blocks_for(F, 128), 128, 0, s, F, xyz, conn, elem, lface,
+                     flux, contrib, node);
+  GS_LAUNCH_CHECK("k_face_flux");
 }
 
 void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,

@@ -237,6 +237,9 @@ void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double Emod, double nu,
                              const double *E_elem, double rho, const double g[3],
                              bool with_load, double *Ke, double *Fe, int *flag);
+void launch_face_flux(hipStream_t s, int64_t F, const double *xyz, const int *conn,
+                      const int64_t *elem, const int *lface, const double *flux,
+                      double *contrib, int *node);
 void launch_thermal_mass(hipStream_t s, int64_t E, const double *xyz, const int *conn,
                          double rho_cp, const double *rc_elem, double *Me);
 // Matrix-core variant of the elastic element kernel (timing; entries equal
@@ -264,6 +267,9 @@ void gather_thermal(hipStream_t s, const NodePattern &p, int64_t E,
                     const double *Ke, double *val);
 // Loads land at out_map[r] (caller numbering) when out_map is given, else at
 // internal row r.
+// f += Neumann face fluxes (faces given on the device), deterministic order
+void apply_neumann(gs_ctx *ctx, const gs_mesh *m, int64_t F, const int64_t *elem,
+                   const int *lface, const double *flux, double *f);
 // all-100-entry element matrices (thermal mass)
 void gather_thermal_full(hipStream_t s, const NodePattern &p, const double *Me,
                          double *val);

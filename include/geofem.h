@@ -112,6 +112,16 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
 gs_status gs_assemble_thermal_mass(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
                                    double density, double specific_heat,
                                    const double *rho_cp_per_elem_dev, gs_csr **out);
+/* Assembler::apply_neumann_bcs (src/fem/assembly.rs:248-273) with
+ * Tet10Face::from_element_face / integrate_flux (src/fem/boundary.rs:72-253):
+ * f_dev[node] += the constant flux over face local_faces[k] (0..3) of element
+ * element_ids[k], 6 quadratic-triangle nodes, 3-point rule.  Face lists are
+ * host arrays; contributions to one node are added in face order (the
+ * reference iterates a hash map, whose order varies).  An invalid face index
+ * (the reference panics) or element id returns GS_ERR_INVALID_ARG. */
+gs_status gs_apply_neumann_bcs(gs_ctx *ctx, gs_mesh *mesh, int64_t n_faces,
+                               const int64_t *element_ids, const int32_t *local_faces,
+                               const double *fluxes, double *f_dev);
 gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double density,
                                    const double gravity[3], double *f_dev);

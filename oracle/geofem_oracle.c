@@ -818,6 +818,103 @@ int or_apply_dirichlet(const or_csr *K, const double *f, const uint8_t *is_dir,
 }
 
 /* ------------------------------------------------------------------------ */
+/* Neumann faces: src/fem/boundary.rs, src/fem/assembly.rs:248-273          */
+/* ------------------------------------------------------------------------ */
+
+/* Tet10Face::from_element_face (src/fem/boundary.rs:72-115): element-local
+ * node of face node k (vertices, then edges 01, 12, 20 of the triangle). */
+static const int FACE_NODES[4][6] = {{1, 2, 3, 5, 9, 8},
+                                     {0, 2, 3, 6, 9, 7},
+                                     {0, 1, 3, 4, 8, 7},
+                                     {0, 1, 2, 4, 5, 6}};
+
+int or_face_nodes(int local_face, int out[6]) {
+  if (local_face < 0 || local_face > 3) return OR_ERR_INVALID_ARG;
+  for (int k = 0; k < 6; ++k) out[k] = FACE_NODES[local_face][k];
+  return OR_OK;
+}
+
+/* dx/dL1 and dx/dL2 of the quadratic triangle at (l1, l2)
+ * (src/fem/boundary.rs:214-225 derivative table) */
+static void face_tangents(const double c[6][3], double l1, double l2, double t1[3],
+                          double t2[3]) {
+  const double l3 = 1.0 - l1 - l2;
+  const double d[6][2] = {{4.0 * l1 - 1.0, 0.0},
+                          {0.0, 4.0 * l2 - 1.0},
+                          {-4.0 * l3 + 1.0, -4.0 * l3 + 1.0},
+                          {4.0 * l2, 4.0 * l1},
+                          {-4.0 * l2, 4.0 * (l3 - l2)},
+                          {4.0 * (l3 - l1), -4.0 * l1}};
+  for (int r = 0; r < 3; ++r) t1[r] = t2[r] = 0.0;
+  for (int i = 0; i < 6; ++i)
+    for (int r = 0; r < 3; ++r) {
+      t1[r] += c[i][r] * d[i][0];
+      t2[r] += c[i][r] * d[i][1];
+    }
+}
+
+static double cross_norm(const double a[3], const double b[3]) {
+  const double x = a[1] * b[2] - a[2] * b[1];
+  const double y = a[2] * b[0] - a[0] * b[2];
+  const double z = a[0] * b[1] - a[1] * b[0];
+  return sqrt(x * x + y * y + z * z);
+}
+
+static const double FACE_QP[3][3] = {{1.0 / 6.0, 1.0 / 6.0, 1.0 / 6.0},
+                                     {2.0 / 3.0, 1.0 / 6.0, 1.0 / 6.0},
+                                     {1.0 / 6.0, 2.0 / 3.0, 1.0 / 6.0}};
+
+/* Tet10Face::area (src/fem/boundary.rs:118-182): 3-point rule of |dx/dL1 x dx/dL2|. */
+double or_face_area(const double c[6][3]) {
+  double area = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    double t1[3], t2[3];
+    face_tangents(c, FACE_QP[q][0], FACE_QP[q][1], t1, t2);
+    area += FACE_QP[q][2] * cross_norm(t1, t2);
+  }
+  return area;
+}
+
+/* Tet10Face::integrate_flux (src/fem/boundary.rs:187-253):
+ * f_i += w N_i flux |dx/dL1 x dx/dL2| over the 3-point rule. */
+void or_face_flux(const double c[6][3], double flux, double out[6]) {
+  for (int i = 0; i < 6; ++i) out[i] = 0.0;
+  for (int q = 0; q < 3; ++q) {
+    const double l1 = FACE_QP[q][0], l2 = FACE_QP[q][1], w = FACE_QP[q][2];
+    const double l3 = 1.0 - l1 - l2;
+    const double n[6] = {l1 * (2.0 * l1 - 1.0), l2 * (2.0 * l2 - 1.0), l3 * (2.0 * l3 - 1.0),
+                         4.0 * l1 * l2,         4.0 * l2 * l3,         4.0 * l3 * l1};
+    double t1[3], t2[3];
+    face_tangents(c, l1, l2, t1, t2);
+    const double det = cross_norm(t1, t2);
+    for (int i = 0; i < 6; ++i) out[i] += w * n[i] * flux * det;
+  }
+}
+
+/* Assembler::apply_neumann_bcs (src/fem/assembly.rs:248-273): f[node] +=
+ * flux contribution, face by face in the given order (the reference walks a
+ * hash map, whose order is not fixed). */
+int or_apply_neumann(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                     const int64_t *conn, int64_t n_faces, const int64_t *elem_ids,
+                     const int32_t *local_faces, const double *fluxes, double *f) {
+  for (int64_t k = 0; k < n_faces; ++k) {
+    if (elem_ids[k] < 0 || elem_ids[k] >= n_elems) return OR_ERR_INVALID_ARG;
+    if (local_faces[k] < 0 || local_faces[k] > 3) return OR_ERR_INVALID_ARG;
+    const int64_t *c = conn + 10 * elem_ids[k];
+    double coords[6][3], contrib[6];
+    int64_t nodes[6];
+    for (int i = 0; i < 6; ++i) {
+      nodes[i] = c[FACE_NODES[local_faces[k]][i]];
+      if (nodes[i] < 0 || nodes[i] >= n_nodes) return OR_ERR_INVALID_ARG;
+      for (int r = 0; r < 3; ++r) coords[i][r] = xyz[3 * nodes[i] + r];
+    }
+    or_face_flux(coords, fluxes[k], contrib);
+    for (int i = 0; i < 6; ++i) f[nodes[i]] += contrib[i];
+  }
+  return OR_OK;
+}
+
+/* ------------------------------------------------------------------------ */
 /* Linear algebra                                                            */
 /* ------------------------------------------------------------------------ */
 

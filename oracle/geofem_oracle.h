@@ -113,6 +113,14 @@ int or_assemble_gravity_load(int64_t n_nodes, const double *xyz,
 int or_apply_dirichlet(const or_csr *K, const double *f, const uint8_t *is_dir,
                        const double *values, or_csr *Kbc, double *fbc);
 
+/* ---- Neumann faces (src/fem/boundary.rs, src/fem/assembly.rs:248-273) ---- */
+int or_face_nodes(int local_face, int out[6]);
+double or_face_area(const double c[6][3]);
+void or_face_flux(const double c[6][3], double flux, double out[6]);
+int or_apply_neumann(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                     const int64_t *conn, int64_t n_faces, const int64_t *elem_ids,
+                     const int32_t *local_faces, const double *fluxes, double *f);
+
 /* ---- linear algebra (src/linalg/) ---- */
 typedef struct {
   int64_t iterations;

@@ -108,6 +108,13 @@ def _declare(L):
                                       C.POINTER(_Csr)]
     L.or_assemble_thermal_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                            C.c_double, _dp]
+    L.or_face_nodes.argtypes = [C.c_int, C.POINTER(C.c_int)]
+    L.or_face_area.argtypes = [_dp]
+    L.or_face_area.restype = C.c_double
+    L.or_face_flux.argtypes = [_dp, C.c_double, _dp]
+    L.or_face_flux.restype = None
+    L.or_apply_neumann.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_int64, _i64p,
+                                   C.POINTER(C.c_int32), _dp, _dp]
     L.or_thermal_mass.argtypes = [_dp, C.c_double, C.c_double, _dp]
     L.or_thermal_mass.restype = None
     L.or_assemble_thermal_mass.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_double,
@@ -211,6 +218,37 @@ def thermal_mass(nodes, density, specific_heat):
     out = np.zeros((10, 10))
     lib().or_thermal_mass(_d(_f64(nodes, (10, 3))), density, specific_heat, _d(out))
     return out
+
+
+def face_nodes(local_face):
+    """Tet10Face::from_element_face (src/fem/boundary.rs:72-115), element-local."""
+    out = (C.c_int * 6)()
+    _check(lib().or_face_nodes(local_face, out), "face_nodes")
+    return np.array(list(out))
+
+
+def face_area(coords):
+    return lib().or_face_area(_d(_f64(coords, (6, 3))))
+
+
+def face_flux(coords, flux):
+    out = np.zeros(6)
+    lib().or_face_flux(_d(_f64(coords, (6, 3))), flux, _d(out))
+    return out
+
+
+def apply_neumann(xyz, conn, elem_ids, local_faces, fluxes, f):
+    """Assembler::apply_neumann_bcs (src/fem/assembly.rs:248-273), faces in order."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    f = np.array(f, dtype=np.float64)
+    el = np.ascontiguousarray(elem_ids, dtype=np.int64)
+    lf = np.ascontiguousarray(local_faces, dtype=np.int32)
+    fl = _f64(fluxes)
+    _check(lib().or_apply_neumann(len(xyz), _d(xyz), len(conn), _i(conn), len(el), _i(el),
+                                  lf.ctypes.data_as(C.POINTER(C.c_int32)), _d(fl), _d(f)),
+           "apply_neumann")
+    return f
 
 
 def constitutive(E, nu):

@@ -67,6 +67,43 @@ def test_thermal_mass_bitexact(oracle, perturb):
                                                                 rho_cp_elem=rc))
 
 
+@pytest.mark.parametrize("perturb", [0.0, 0.08])
+def test_neumann_faces_bitexact(oracle, perturb):
+    """Neumann fluxes on the top and one side (src/fem/assembly.rs:248-309):
+    the GPU face integrals and the per-node sums in face order equal the
+    oracle's sequential loop bit for bit, on flat and curved faces."""
+    import torch
+    xyz, conn, mesh = mesh_pair(oracle, 4, 3, 3, 1000.0, 800.0, 600.0, perturb, 7)
+    # faces found on the flat mesh; the perturbed one curves them
+    flat = G.Mesh(*oracle.generate_cube(4, 3, 3, 1000.0, 800.0, 600.0))
+    top = G.Assembler.find_boundary_faces(flat, lambda p: np.abs(p[:, 2] - 600.0) < 1.0)
+    side = G.Assembler.find_boundary_faces(flat, lambda p: abs(p[0]) < 1.0)  # per point
+    assert len(top) == 2 * 4 * 3 and len(side) == 2 * 3 * 3
+    bcs = G.BoundaryConditions()
+    rng = np.random.default_rng(3)
+    for e, lf in top + side:
+        bcs.add_neumann(G.BoundaryFace(e, lf), float(rng.uniform(0.01, 0.1)))
+    faces = list(bcs.neumann_faces())
+    el = [fc.element_id for fc, _ in faces]
+    lfs = [fc.local_face_id for fc, _ in faces]
+    fl = [bc.flux for _, bc in faces]
+    f0 = rng.standard_normal(mesh.num_nodes())
+    want = oracle.apply_neumann(xyz, conn, el, lfs, fl, f0)
+    got = G.Assembler.apply_neumann_bcs(mesh, f0.copy(), bcs)
+    assert np.array_equal(got, want)
+    fd = torch.tensor(f0, device="cuda")
+    G.Assembler.apply_neumann_bcs(mesh, fd, bcs)  # in place on the device
+    assert np.array_equal(fd.cpu().numpy(), want)
+    bad = G.BoundaryConditions()
+    bad.add_neumann(G.BoundaryFace(0, 4), 1.0)
+    with pytest.raises(G.GeoFemError):
+        G.Assembler.apply_neumann_bcs(mesh, f0.copy(), bad)
+    with pytest.raises(ValueError):
+        G.Tet10Face.from_element_face(conn[0], 4)
+    assert np.array_equal(G.Tet10Face.from_element_face(conn[0], 0).nodes,
+                          conn[0][oracle.face_nodes(0)])
+
+
 @pytest.mark.parametrize("perturb", [0.0, 0.06])
 def test_elastic_assembly_bitexact(oracle, perturb):
     xyz, conn, mesh = mesh_pair(oracle, 2, 3, 2, 1e6, 1e6, 6e5, perturb, 3)

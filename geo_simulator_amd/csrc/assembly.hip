@@ -47,6 +47,12 @@ __device__ __forceinline__ int64_t xcd_block() {
 }
 __device__ __forceinline__ int64_t xcd_item() { return xcd_block() * BS + threadIdx.x; }
 
+// |v| as the ordered bit pattern of a non-negative double (0 for NaN)
+__device__ __forceinline__ unsigned long long abs_bits(double v) {
+  const double a = fabs(v);
+  return a == a ? (unsigned long long)__double_as_longlong(a) : 0ull;
+}
+
 // Grid of a max-reduction kernel: a grid-stride loop over at most this many
 // workgroups, each folding its maximum into one device-wide atomic (atomics
 // on one address serialise; one per wavefront over a million rows cost
@@ -55,6 +61,26 @@ constexpr int64_t MAX_GRID = 1024;
 inline unsigned nblk_max(int64_t n) {
   return (unsigned)std::max<int64_t>(1, std::min<int64_t>(nblk(n), MAX_GRID));
 }
+
+// Largest of the workgroup's (BS threads) values -> part[blockIdx.x] (0 when
+// none); every thread of the workgroup must call it.
+__device__ __forceinline__ void wg_max_store(unsigned long long mine,
+                                             unsigned long long *part) {
+  __shared__ unsigned long long sm[BS / 64];
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off, 64);
+    mine = o > mine ? o : mine;
+  }
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = sm[0];
+#pragma unroll
+    for (int k = 1; k < BS / 64; ++k) m = sm[k] > m ? sm[k] : m;
+    part[blockIdx.x] = m;
+  }
+}
+
 
 // Largest of the workgroup's (BS threads) values, folded into *out with one
 // atomic; bit patterns of non-negative doubles, so the integer maximum is
@@ -74,6 +100,17 @@ __device__ __forceinline__ void wg_max_atomic(unsigned long long mine,
     for (int k = 1; k < BS / 64; ++k) m = sm[k] > m ? sm[k] : m;
     if (m != 0) atomicMax(out, m);
   }
+}
+
+// max of np partial maxima folded into *out (a grid-stride pass, one
+// atomic per workgroup)
+__global__ __launch_bounds__(BS) void k_max_partials(int64_t np,
+                                                    const unsigned long long *__restrict__ part,
+                                                    unsigned long long *__restrict__ out) {
+  unsigned long long mine = 0;
+  for (int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x; i < np; i += (int64_t)gridDim.x * BS)
+    mine = part[i] > mine ? part[i] : mine;
+  wg_max_atomic(mine, out);
 }
 
 // ---------------------------------------------------------------------------
@@ -472,47 +509,48 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
                                      double *__restrict__ val,
                                      double *__restrict__ sval,
                                      float *__restrict__ sval32,
-                                     double *__restrict__ liftv) {
+                                     double *__restrict__ liftv,
+                                     unsigned long long *__restrict__ dmax) {
   const int64_t s = xcd_item();
-  if (s >= nnz) return;
-  const int d = dst[s];
-  if (d == -1) return;
-  const int b0 = cptr[s], b1 = cptr[s + 1];
-  double v = 0.0;
-  for (int k = b0; k < b1; ++k) {
-    const int t = contrib[k];
-    const int e = t / 100;
-    const int ab = t - 100 * e;
-    const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
-    const double x = Ke[(int64_t)e * 55 + p];
-    v = (k == b0) ? x : v + x;
+  unsigned long long mine = 0;
+  if (s < nnz) {
+    const int d = dst[s];
+    const int b0 = cptr[s], b1 = cptr[s + 1];
+    const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
+    const bool diag = ab0 / 10 == ab0 % 10;  // a diagonal slot: local a == b
+    if (d != -1 || diag) {
+      double v = 0.0;
+      for (int k = b0; k < b1; ++k) {
+        const int t = contrib[k];
+        const int e = t / 100;
+        const int ab = t - 100 * e;
+        const int a = ab / 10, b = ab - 10 * (ab / 10);
+        const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
+        const double x = Ke[(int64_t)e * 55 + p];
+        v = (k == b0) ? x : v + x;
+      }
+      if (diag) mine = abs_bits(v);
+      if (d >= 0) {
+        val[d] = v;
+        const int e = pos[d];
+        sval[e] = v;
+        sval32[e] = (float)v;
+      } else if (d <= -2) {
+        liftv[-2 - d] = v;
+      }
+    }
   }
-  if (d >= 0) {
-    val[d] = v;
-    const int e = pos[d];
-    sval[e] = v;
-    sval32[e] = (float)v;
-  } else {
-    liftv[-2 - d] = v;
-  }
+  wg_max_store(mine, dmax);
 }
 
-// Numeric phase, elastic: the 3x3 block sums of k_gather_elastic per node
-// slot, each DOF entry written where the plan sends it.
-__global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
-                                     const int *__restrict__ slot_row,
-                                     const int *__restrict__ cptr,
-                                     const int *__restrict__ contrib,
-                                     const double *__restrict__ Ke,
-                                     const int *__restrict__ dst,
-                                     const int *__restrict__ pos,
-                                     double *__restrict__ val,
-                                     double *__restrict__ sval,
-                                     float *__restrict__ sval32,
-                                     double *__restrict__ liftv) {
-  const int64_t s = xcd_item();
-  if (s >= nnz) return;
+// One node slot of the elastic system gather (the 3x3 block); mine gets the
+// largest |K_ii| when the slot is a diagonal one.
+__device__ __forceinline__ void gather_elastic_slot(
+    int64_t s, const int *__restrict__ rp, const int *__restrict__ slot_row,
+    const int *__restrict__ cptr, const int *__restrict__ contrib,
+    const double *__restrict__ Ke, const int *__restrict__ dst,
+    const int *__restrict__ pos, double *__restrict__ val, double *__restrict__ sval,
+    float *__restrict__ sval32, double *__restrict__ liftv, unsigned long long &mine) {
   const int r = slot_row[s];
   const int base = rp[r], deg = rp[r + 1] - base;
   const int j = (int)(s - base);
@@ -526,8 +564,10 @@ __global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
       dd[3 * ci + cj] = d;
       any |= d != -1;
     }
-  if (!any) return;
   const int b0 = cptr[s], b1 = cptr[s + 1];
+  const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
+  const bool diag = ab0 / 10 == ab0 % 10;
+  if (!any && !diag) return;
   double v[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) v[m] = 0.0;
@@ -545,6 +585,13 @@ __global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
         v[3 * ci + cj] = (k == b0) ? x : v[3 * ci + cj] + x;
       }
   }
+  if (diag) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const unsigned long long b = abs_bits(v[4 * c]);
+      mine = b > mine ? b : mine;
+    }
+  }
 #pragma unroll
   for (int m = 0; m < 9; ++m) {
     const int d = dd[m];
@@ -559,46 +606,28 @@ __global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
   }
 }
 
-// max |K_ii| over all diagonal entries, recomputed from the diagonal node
-// slots in the gather's summation order (bit pattern of a non-negative
-// double: integer max is order independent, hence exact).
-template <int DPN>
-__global__ void k_sys_diagmax(int64_t n_nodes, const int *__restrict__ diag_slot,
-                              const int *__restrict__ cptr,
-                              const int *__restrict__ contrib,
-                              const double *__restrict__ Ke,
-                              unsigned long long *__restrict__ maxbits) {
+// Numeric phase, elastic: the 3x3 block sums of k_gather_elastic per node
+// slot, each DOF entry written where the plan sends it.
+__global__ void k_sys_gather_elastic(int64_t nnz, const int *__restrict__ rp,
+                                     const int *__restrict__ slot_row,
+                                     const int *__restrict__ cptr,
+                                     const int *__restrict__ contrib,
+                                     const double *__restrict__ Ke,
+                                     const int *__restrict__ dst,
+                                     const int *__restrict__ pos,
+                                     double *__restrict__ val,
+                                     double *__restrict__ sval,
+                                     float *__restrict__ sval32,
+                                     double *__restrict__ liftv,
+                                     unsigned long long *__restrict__ dmax) {
+  const int64_t s = xcd_item();
   unsigned long long mine = 0;
-  for (int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x; r < n_nodes;
-       r += (int64_t)gridDim.x * BS) {
-    const int s = diag_slot[r];
-    if (s < 0) continue;
-    double v[DPN];
-#pragma unroll
-    for (int c = 0; c < DPN; ++c) v[c] = 0.0;
-    const int b0 = cptr[s], b1 = cptr[s + 1];
-    for (int k = b0; k < b1; ++k) {
-      const int t = contrib[k];
-      const int e = t / 100;
-      const int a = (t - 100 * e) / 10;  // diagonal slot: b == a
-#pragma unroll
-      for (int c = 0; c < DPN; ++c) {
-        const double x = DPN == 1 ? Ke[(int64_t)e * 55 + tri_index(a, a)]
-                                  : Ke[(int64_t)e * 900 + (a * 10 + a) * 9 + c * 3 + c];
-        v[c] = (k == b0) ? x : v[c] + x;
-      }
-    }
-#pragma unroll
-    for (int c = 0; c < DPN; ++c) {
-      const double a = fabs(v[c]);
-      if (a == a) {
-        const unsigned long long bits = (unsigned long long)__double_as_longlong(a);
-        mine = bits > mine ? bits : mine;
-      }
-    }
-  }
-  wg_max_atomic(mine, maxbits);
+  if (s < nnz) gather_elastic_slot(s, rp, slot_row, cptr, contrib, Ke, dst, pos, val, sval,
+                                   sval32, liftv, mine);
+  wg_max_store(mine, dmax);
 }
+
+
 
 // Per DOF row (internal numbering): load (k_gather_load's sum), then either
 // the constrained row (c e_i, f = g c) or the lift f -= K_ji g_i in the
@@ -1013,28 +1042,31 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
   return P;
 }
 
+int64_t system_gather_blocks(const gs_mesh *m) { return nblk(m->pat.nnz); }
+
 void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
                    const double *Ke, double *val, double *sval, float *sval32,
-                   double *liftv) {
+                   double *liftv, unsigned long long *dmax) {
   const NodePattern &p = m->pat;
   if (p.nnz == 0) return;
   const int *pos = P.sell->pos.get();
   if (P.dpn == 1)
     hipLaunchKernelGGL(k_sys_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
                        p.contrib_ptr.get(), p.contrib.get(), Ke, P.dst.get(), pos,
-                       val, sval, sval32, liftv);
+                       val, sval, sval32, liftv, dmax);
   else
     hipLaunchKernelGGL(k_sys_gather_elastic, nblk(p.nnz), BS, 0, s, p.nnz,
                        p.row_ptr.get(), p.slot_row.get(), p.contrib_ptr.get(),
                        p.contrib.get(), Ke, P.dst.get(), pos, val, sval, sval32,
-                       liftv);
+                       liftv, dmax);
   GS_LAUNCH_CHECK("k_sys_gather");
 }
 
 void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
                    const double *Ke, const double *Fe, double *val, double *sval,
                    float *sval32, const double *liftv, const int *dperm,
-                   double *fbc, double *sdinv, float *sdinv32) {
+                   double *fbc, double *sdinv, float *sdinv32,
+                   const unsigned long long *dmax) {
   if (P.n == 0) return;
   hipStream_t s = ctx->stream;
   const NodePattern &p = m->pat;
@@ -1042,21 +1074,21 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
   const double one = 1.0;
   unsigned long long one_bits;
   memcpy(&one_bits, &one, sizeof(one));
+  // c = max(1, max |K_ii|): the gather left one maximum per workgroup
   hipLaunchKernelGGL(k_set_u64, 1, 1, 0, s, maxbits.get(), one_bits);
+  if (p.nnz > 0) {
+    const int64_t np = nblk(p.nnz);
+    hipLaunchKernelGGL(k_max_partials, (unsigned)std::min<int64_t>(256, nblk(np)), BS, 0, s,
+                       np, dmax, maxbits.get());
+  }
   const int *sip = P.sell->siperm.get(), *pos = P.sell->pos.get();
   if (P.dpn == 1) {
-    hipLaunchKernelGGL((k_sys_diagmax<1>), nblk_max(m->n_nodes), BS, 0, s, m->n_nodes,
-                       p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
-                       maxbits.get());
     hipLaunchKernelGGL((k_sys_rhs<1>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
                        p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
                        P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
                        P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
                        dperm, fbc, sdinv, sdinv32);
   } else {
-    hipLaunchKernelGGL((k_sys_diagmax<3>), nblk_max(m->n_nodes), BS, 0, s, m->n_nodes,
-                       p.diag_slot.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
-                       maxbits.get());
     hipLaunchKernelGGL((k_sys_rhs<3>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
                        p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
                        P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,

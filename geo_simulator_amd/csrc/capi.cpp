@@ -155,9 +155,11 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   auto sdinv = palloc<double>(pool, P->n);
   auto sdinv32 = palloc<float>(pool, P->n);
   const DArr<int> &dperm = sp.elastic ? m->perm3 : m->perm;
-  system_gather(s, m, *P, Ke.get(), val.get(), sval.get(), sval32.get(), liftv.get());
+  auto dmax = palloc<unsigned long long>(pool, std::max<int64_t>(1, system_gather_blocks(m)));
+  system_gather(s, m, *P, Ke.get(), val.get(), sval.get(), sval32.get(), liftv.get(),
+                dmax.get());
   system_finish(ctx, m, *P, Ke.get(), Fe.get(), val.get(), sval.get(), sval32.get(),
-                liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get());
+                liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get(), dmax.get());
   gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, val);
   K->dperm = dperm;
   K->block = dpn;

@@ -298,15 +298,18 @@ void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map
 std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
                                         const gs_dofmap *d);
 // values of K_bc (CSR and SELL, FP64 and FP32) and the lift values
+// workgroups of the system gather (entries of its diagonal-maximum array)
+int64_t system_gather_blocks(const gs_mesh *m);
 void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
                    const double *Ke, double *val, double *sval, float *sval32,
-                   double *liftv);
+                   double *liftv, unsigned long long *dmax);
 // c = max(1, max |K_ii|), constrained rows, RHS (load + lift, written at
 // caller index dperm[r]) and the Jacobi inverse diagonal in SELL order
 void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
                    const double *Ke, const double *Fe, double *val, double *sval,
                    float *sval32, const double *liftv, const int *dperm,
-                   double *fbc, double *sdinv, float *sdinv32);
+                   double *fbc, double *sdinv, float *sdinv32,
+                   const unsigned long long *dmax);
 
 // ---- operator.hip ----
 // block = 3 builds the 3x3-block view (rows must be a multiple of 3; a block

@@ -569,6 +569,8 @@ def main():
         "mixed": mixed,
         "speedup_mixed_vs_fp64": fp64["step_s"] / mixed["step_s"],
         "tight_rel_res_1e-12": tight,
+        "krylov_fp64": {k: results[k] for k in ("bicgstab_fp64", "gmres30_fp64")
+                        if k in results},
         "symbolic_s": info["symbolic_s"],
         "upload_s": info["upload_s"],
         "spmv": spmv,

@@ -167,9 +167,14 @@ def run_headline(G, args, nx, ny, nz):
         tight["speedup_mixed_vs_fp64"] = tight["fp64"]["step_s"] / tight["mixed"]["step_s"]
 
     # dominant kernels: the CG SpMV in each precision, timed with HIP events on
-    # the library's own stream (the stream the kernels run on)
-    bytes64 = nnz * 12 + n * 20   # algorithmic: int32 indices, FP64 values/vectors
-    bytes32 = nnz * 8 + n * 12    # algorithmic: int32 indices, FP32 values/vectors
+    # the library's own stream (the stream the kernels run on).  Algorithmic
+    # bytes, CSR-equivalent (int32 column indices and row pointers): FP64
+    # q = A p reads the operator and p once and writes q; the FP32
+    # recursion's kernel (w = A r on the scaled operator, then p = r + beta p,
+    # s = w + beta s in place) reads the operator, r once, p and s, and
+    # writes p and s.
+    bytes64 = nnz * 12 + n * 20
+    bytes32 = nnz * 8 + n * 24
     ms64 = K.time_spmv("fp64", args.spmv_reps)
     ms32 = K.time_spmv("mixed", args.spmv_reps)
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
@@ -577,8 +582,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": spmv["fp32_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": spmv["fp32_GBs"] / HBM_PEAK_GBS,
                      "traffic": spmv["fp32_traffic"],
-                     "kernel": "k_cg_q<float> (SELL-64 CG SpMV + p.Ap partials; "
-                               "dominant kernel of the mixed path)",
+                     "kernel": "k_cg2_dir<float> (SELL-64 SpMV of the FP32 recursion on "
+                               "the Jacobi-scaled operator + direction update p, s and "
+                               "p.s, r.p partials; dominant kernel of the mixed path)",
                      "algorithmic_bytes": spmv["fp32_bytes"]},
         "elastic_48x48x24": elastic,
         "adjoint": adjoint,

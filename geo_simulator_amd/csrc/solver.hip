@@ -60,6 +60,8 @@ struct CgState {
   double rnorm, bnorm, tol_abs, abs_tol, tol;
   double rbase;  // mixed inner: ||r|| (scaled) when the recursion (re)started
   double rtol;   // mixed inner: pause once ||r|| < rtol * rbase
+  double gam[2];       // scaled recursion: r.r at the start of pass k in gam[k & 1]
+  long long fresh_it;  // scaled recursion: the pass that starts a new direction
   long long iteration, max_iter;
   int stop, converged, pause, mixed;
   int brk_rel;   // break down only on p.Ap == 0 or a non-finite step
@@ -109,17 +111,18 @@ __device__ __forceinline__ void put_partials(double (&v)[NV], double *part) {
   }
 }
 
-// Finalize kernels: sum of np x NV partials, in every thread.
-template <int NV>
+// Finalize kernels (workgroups of NT threads): sum of np x NV partials, in
+// every thread.
+template <int NV, int NT = FBS>
 __device__ __forceinline__ void sum_partials(const double *__restrict__ part,
                                              int np, double (&tot)[NV]) {
 #pragma unroll
   for (int i = 0; i < NV; ++i) tot[i] = 0.0;
-  for (int b = threadIdx.x; b < np; b += FBS) {
+  for (int b = threadIdx.x; b < np; b += NT) {
 #pragma unroll
     for (int i = 0; i < NV; ++i) tot[i] += part[b * NV + i];
   }
-  block_sum<NV, FBS>(tot);
+  block_sum<NV, NT>(tot);
 }
 
 // ---- SpMV kernels ------------------------------------------------------------
@@ -317,6 +320,114 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
   }
 }
 
+// ---- two-kernel pass of the scaled FP32 recursion -----------------------------
+// CG on S A S (S = D^-1/2: Jacobi folded into the operator, so z = r) with
+// the direction update moved into the SpMV kernel.  A p_k is not formed from
+// p_k (whose entries other lanes are still writing) but by linearity,
+// s_k = A p_k = A r_k + beta_k s_{k-1}, the recurrence of Chronopoulos and
+// Gear (J. Comput. Appl. Math. 25, 1989); the step length uses the exact
+// products of the pass, alpha = (r, p) / (p, s), an exact line search along
+// p.  Per pass:
+//   KA  gamma = r.r (the previous pass's partials); pause test on ||r||;
+//       beta = gamma / gamma_prev (0 on a fresh start);
+//       w = A r; p = r + beta p; s = w + beta s; partials p.s, r.p
+//   KB  alpha = r.p / p.s; x += alpha p; r -= alpha s; partials r.r
+// Two launches and two reductions per pass instead of three of each.
+template <typename T, class IDX>
+__global__ __launch_bounds__(BS) void k_cg2_dir(int n, int ns, const int *__restrict__ cb,
+                                                const IDX idx, const T *__restrict__ val,
+                                                const T *__restrict__ r,
+                                                T *__restrict__ p, T *__restrict__ sv,
+                                                CgState *__restrict__ st, long long it,
+                                                const double *__restrict__ part_r,
+                                                int np_r, double *__restrict__ part) {
+  if (!active(st, it)) return;
+  double g[1];
+  sum_partials<1, BS>(part_r, np_r, g);
+  const double rn = sqrt(g[0]);
+  const bool pause = rn < st->rtol * st->rbase || rn < st->tol || it >= st->max_iter;
+  const bool fresh = st->fresh_it == it;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    st->rnorm = rn;
+    st->gam[it & 1] = g[0];
+    if (pause) st->pause = 1;
+  }
+  if (pause) return;
+  double beta_d = fresh ? 0.0 : g[0] / st->gam[(it - 1) & 1];
+  if (!isfinite(beta_d)) beta_d = 0.0;
+  const T beta = (T)beta_d;
+  double acc[2] = {0.0, 0.0};
+  auto epi = [&](int row, T w) {
+    const T ri = r[row];
+    T pi = ri, si = w;
+    if (!fresh) {
+      pi = ri + beta * p[row];
+      si = w + beta * sv[row];
+    }
+    p[row] = pi;
+    sv[row] = si;
+    acc[0] += (double)pi * (double)si;
+    acc[1] += (double)ri * (double)pi;
+  };
+  sell_rows<T>(n, ns, cb, idx, val, r, epi);
+  put_partials<2>(acc, part);
+}
+
+template <typename T>
+__global__ __launch_bounds__(FBS) void k_cg2_step(
+    int n, long long it, T *__restrict__ x, T *__restrict__ r, const T *__restrict__ p,
+    const T *__restrict__ sv, CgState *__restrict__ st,
+    const double *__restrict__ part_in, int np_in, double *__restrict__ part) {
+  const long long at = st->iteration;
+  if (!((at == it && !st->stop && !st->pause) || at == it + 1)) return;
+  double t[2];
+  sum_partials<2>(part_in, np_in, t);
+  const double alpha_d = t[1] / t[0];
+  // no positive curvature left along p, or a non-finite step: pause for the
+  // outer step (which then restarts the recursion)
+  const bool brk = !(t[0] > 0.0) || !isfinite(alpha_d);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (brk) {
+      st->pq = 0.0;
+      st->pause = 1;
+    } else {
+      st->pq = t[0];
+      st->alpha = alpha_d;
+      st->iteration = it + 1;
+    }
+  }
+  if (brk) return;
+  const T alpha = (T)alpha_d;
+  double acc[1] = {0.0};
+  const long long stride = (long long)gridDim.x * FBS;
+  for (long long i0 = (long long)blockIdx.x * FBS + threadIdx.x; i0 < n;
+       i0 += UNR * stride) {
+    T pv[UNR], sq[UNR], xv[UNR], rv[UNR];
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        pv[u] = p[i];
+        sq[u] = sv[i];
+        xv[u] = x[i];
+        rv[u] = r[i];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < UNR; ++u) {
+      const long long i = i0 + u * stride;
+      if (i < n) {
+        x[i] = xv[u] + alpha * pv[u];
+        const T ri = rv[u] - alpha * sq[u];
+        r[i] = ri;
+        acc[0] += (double)ri * (double)ri;
+      }
+    }
+  }
+  block_sum<1, FBS>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
 // Set-up: x = 0, r = b, z = D^-1 r, p = z; partials b.b, r.z.
 template <typename T, bool PREC>
 __global__ __launch_bounds__(BS) void k_cg_init(
@@ -419,26 +530,132 @@ __global__ __launch_bounds__(BS) void k_fcg_inner(
   put_partials<1>(acc, part);
 }
 
-// Inner recursion state after a (re)start: rz, rbase; resume passes.
+// The scaled recursion (re)started: rs = S r/||b||; partials rs.rs, one per
+// workgroup of FBS threads (the layout of k_cg2_step's partials).
+__global__ __launch_bounds__(FBS) void k_fcg_inner_scaled(
+    int n, const double *__restrict__ r, double scale, const double *__restrict__ ssc,
+    float *__restrict__ rs, double *__restrict__ part) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
+    const float ri = (float)(ssc[i] * r[i] * scale);
+    rs[i] = ri;
+    acc[0] += (double)ri * (double)ri;
+  }
+  block_sum<1, FBS>(acc);
+  if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
+}
+
+// Inner recursion state after a (re)start: rz, rbase; resume passes.  With
+// rbase < 0 the recursion's residual is its own r.z (the scaled recursion,
+// z = r): rbase = sqrt(r.z), and the pause threshold becomes tol_ratio
+// times it.
 __global__ __launch_bounds__(FBS) void k_fin_inner(CgState *__restrict__ st,
                                                    const double *__restrict__ part,
-                                                   int np, double rbase) {
+                                                   int np, double rbase, double tol_ratio,
+                                                   int restart) {
   double tot[1];
   sum_partials<1>(part, np, tot);
   if (threadIdx.x != 0) return;
   st->rz[st->iteration & 1] = tot[0];
+  if (rbase < 0.0) {
+    rbase = sqrt(tot[0]);
+    st->tol = rbase * tol_ratio;
+  }
+  st->fresh_it = restart ? st->iteration : -1;
   st->rbase = rbase;
   st->rnorm = rbase;
   st->pause = 0;
 }
 
-// d = ||b|| xs (the inner correction in FP64 and in units of b); xs = 0
+// d = ||b|| xs (the inner correction in FP64 and in units of b; S xs for
+// the scaled recursion); xs = 0
 __global__ __launch_bounds__(BS) void k_fcg_expand(int n, float *__restrict__ xs,
                                                    double scale,
+                                                   const double *__restrict__ ssc,
                                                    double *__restrict__ d) {
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
-    d[i] = scale * (double)xs[i];
+    d[i] = ssc ? scale * (ssc[i] * (double)xs[i]) : scale * (double)xs[i];
     xs[i] = 0.0f;
+  }
+}
+
+// FP32 values of S A S, S = diag(ssc) (scalar SELL order), from the FP32
+// values: entry (i, j) becomes s_i a_ij s_j (in FP64, rounded once); padding
+// 0.  One wavefront per slice, blocks of SELL_WPB slices in XCD-contiguous
+// ranges as the SpMV, so the gathered s_j stay in the XCD's L2.
+template <class IDX>
+__global__ __launch_bounds__(BS) void k_scale_sell(int n, int ns, const int *__restrict__ cb,
+                                                   const IDX idx,
+                                                   const float *__restrict__ val,
+                                                   const double *__restrict__ ssc,
+                                                   float *__restrict__ out) {
+  static_assert(IDX::bs == 1, "scalar SELL only");
+  typedef float F4 __attribute__((ext_vector_type(4)));
+  const int nb = (ns + SELL_WPB - 1) / SELL_WPB;
+  const int G = gridDim.x, gi = blockIdx.x;
+  int first = gi, last = nb, stride = G;
+  if (G >= 8) {
+    const int xcd = gi & 7;
+    first = (int)((int64_t)nb * xcd / 8) + (gi >> 3);
+    last = (int)((int64_t)nb * (xcd + 1) / 8);
+    stride = (G >> 3) + (xcd < (G & 7) ? 1 : 0);
+  }
+  const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  for (int b = first; b < last; b += stride) {
+    const int sl = b * SELL_WPB + w;
+    if (sl >= ns) continue;
+    const int row = sl * SELL_C + lane;
+    const bool live = row < n;
+    const double si = live ? ssc[row] : 0.0;
+    for (int c = cb[sl]; c < cb[sl + 1]; ++c) {
+      const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
+      const F4 v = *(const F4 *)(val + e0);
+      int cc[4];
+      bool ok[4];
+      if constexpr (IDX::wide) {
+        typedef int I4 __attribute__((ext_vector_type(4)));
+        const I4 i4 = *(const I4 *)(idx.col + e0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ok[j] = i4[j] >= 0 && live;
+          cc[j] = i4[j];
+        }
+      } else {
+        typedef short S4 __attribute__((ext_vector_type(4)));
+        const S4 i4 = *(const S4 *)(idx.off + e0);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          ok[j] = i4[j] != SELL_PAD16 && live;
+          cc[j] = sl * SELL_C + i4[j];
+        }
+      }
+      F4 o;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const double sj = ok[j] ? ssc[cc[j]] : 0.0;
+        o[j] = ok[j] ? (float)(si * (double)v[j] * sj) : 0.0f;
+      }
+      *(F4 *)(out + e0) = o;
+    }
+  }
+}
+
+// Timing set-up of the two-launch pass: a pass that continues a direction.
+__global__ void k_state_continue(CgState *__restrict__ st) { st->fresh_it = -1; }
+
+// Bad flag (an int) as a double, beside the dots the host reads next.
+__global__ void k_flag_to_dot(const int *__restrict__ flag, double *__restrict__ out) {
+  *out = (double)*flag;
+}
+
+// ssc = sqrt(d) when every d > 0 and finite; flag = 1 otherwise
+__global__ __launch_bounds__(BS) void k_sqrt_dinv(int n, const double *__restrict__ d,
+                                                  double *__restrict__ ssc,
+                                                  int *__restrict__ flag) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    const double v = d[i];
+    if (!(v > 0.0) || !isfinite(v)) *flag = 1;
+    ssc[i] = sqrt(v);
   }
 }
 
@@ -729,6 +946,21 @@ void enqueue_pass(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *z
   hipLaunchKernelGGL((k_cg_beta<T>), gf, FBS, 0, s, n, it, z, p, st, part2, (int)gf);
 }
 
+// One pass of the scaled recursion: KA (SpMV + direction), KB (step).
+// part_r holds the r.r partials of the pass's residual (grid_fat(n) of them).
+template <typename T>
+void enqueue_pass2(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *p,
+                   T *sv, CgState *st, long long it, double *part1, double *part_r) {
+  const int n = o.n();
+  const unsigned gf = grid_fat(n);
+  with_idx(*o.P, [&](auto idx) {
+    hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+                       o.P->cb.get(), idx, val, r, p, sv, st, it, part_r, (int)gf, part1);
+  });
+  hipLaunchKernelGGL((k_cg2_step<T>), gf, FBS, 0, s, n, it, x, r, p, sv, st, part1,
+                     o.grid, part_r);
+}
+
 struct PcgResult {
   long long iterations = 0, outer = 0, spmv = 0;
   double rnorm = 0.0, bnorm = 0.0;
@@ -744,7 +976,7 @@ template <typename T> struct PcgWork {
 template <typename T>
 PcgWork<T> carve_pcg(char *base, Layout &L, int64_t n) {
   PcgWork<T> w;
-  w.part1 = (double *)(base + L.add<double>(GRID_MAX));
+  w.part1 = (double *)(base + L.add<double>(2 * GRID_MAX));
   w.part2 = (double *)(base + L.add<double>(2 * GMAX));
   w.st = (CgState *)(base + L.add<CgState>(1));
   w.r = (T *)(base + L.add<T>(n));
@@ -822,7 +1054,7 @@ void read_dots(gs_ctx *ctx, const double *dev, double *out, int n) {
 // across outer steps; the FP64 line search keeps every step safe when the
 // FP32 operator's rounding (eps32 * cond(A)) makes the recursion drift.  The
 // loop ends on the true residual b - A x.
-PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
+PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
                   bool precond, const gs_cg_opts &opt, char *base, Layout &L) {
   hipStream_t s = ctx->stream;
   const SellOp o = sell_op(A);
@@ -836,15 +1068,29 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
   double *dots = (double *)(base + L.add<double>(8));
   float *xs = (float *)(base + L.add<float>(n));
   const PcgWork<float> iw = carve_pcg<float>(base, L, n);
-  const float *dinv32 = precond ? A->sdinv32.get() : nullptr;
-  const float *val32 = A->sval32.get();
-  const double *val = A->sval.get();
   PcgResult res;
   double h[2];
   hipLaunchKernelGGL(k_fcg_start, gv, BS, 0, s, n, b, x, r, iw.part2);
   hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
   GS_LAUNCH_CHECK("k_fcg_start");
-  read_dots(ctx, dots, h, 1);
+  const bool pending = precond && A->sscaled == 2;
+  if (pending) hipLaunchKernelGGL(k_flag_to_dot, 1, 1, 0, s, A->sscale_bad.get(), dots + 1);
+  read_dots(ctx, dots, h, pending ? 2 : 1);
+  if (pending) {
+    A->sscaled = h[1] == 0.0 ? 1 : -1;
+    A->sscale_bad = DArr<int>();
+    if (A->sscaled < 0) {
+      A->sscale = DArr<double>();
+      A->sval32s = DArr<float>();
+    }
+  }
+  // Jacobi folded into the FP32 operator (S A S, S = D^-1/2) when every
+  // D^-1 entry is positive: the recursion then needs no z vector
+  const bool scaled = precond && A->sscaled == 1;
+  const double *ssc = scaled ? A->sscale.get() : nullptr;
+  const float *dinv32 = precond && !scaled ? A->sdinv32.get() : nullptr;
+  const float *val32 = scaled ? A->sval32s.get() : A->sval32.get();
+  const double *val = A->sval.get();
   const double bnorm = std::sqrt(h[0]);
   res.bnorm = bnorm;
   res.rnorm = bnorm;
@@ -862,8 +1108,12 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
   hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, iw.st, max_iter, target * scale,
                      0.0, 1, opt.inner_tolerance, 0);
   hipLaunchKernelGGL((k_zero<float>), gv, BS, 0, s, n, xs);
+  const unsigned gf = grid_fat(n);
   auto start_inner = [&](bool restart, double rn) {
-    if (dinv32) {
+    if (scaled) {
+      hipLaunchKernelGGL(k_fcg_inner_scaled, gf, FBS, 0, s, n, r, scale, ssc, iw.r,
+                         iw.part2);
+    } else if (dinv32) {
       if (restart)
         hipLaunchKernelGGL((k_fcg_inner<true, true>), gv, BS, 0, s, n, r, scale,
                            dinv32, iw.r, iw.z, iw.p, iw.part2);
@@ -878,7 +1128,12 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
         hipLaunchKernelGGL((k_fcg_inner<false, false>), gv, BS, 0, s, n, r, scale,
                            dinv32, iw.r, iw.z, iw.p, iw.part2);
     }
-    hipLaunchKernelGGL(k_fin_inner, 1, FBS, 0, s, iw.st, iw.part2, (int)gv, rn * scale);
+    // the scaled recursion measures its residual in its own norm ||S r||:
+    // its base is its r.r and its target the same fraction of it as the
+    // outer target is of ||r||
+    hipLaunchKernelGGL(k_fin_inner, 1, FBS, 0, s, iw.st, iw.part2,
+                       (int)(scaled ? gf : gv), scaled ? -1.0 : rn * scale, target / rn,
+                       restart ? 1 : 0);
     GS_LAUNCH_CHECK("k_fcg_inner");
   };
   start_inner(true, bnorm);
@@ -912,9 +1167,14 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
       if (opt.max_inner_iterations > 0)
         left = std::min(left, cycle_start + opt.max_inner_iterations - hs.iteration);
       const long long batch = bt.next(hs.rnorm, hs.iteration, tgt, left);
-      for (long long i = 0; i < batch; ++i)
-        enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
-                            hs.iteration + i, iw.part1, iw.part2);
+      for (long long i = 0; i < batch; ++i) {
+        if (scaled)
+          enqueue_pass2<float>(s, o, val32, xs, iw.r, iw.p, iw.q, iw.st, hs.iteration + i,
+                               iw.part1, iw.part2);
+        else
+          enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
+                              hs.iteration + i, iw.part1, iw.part2);
+      }
       GS_LAUNCH_CHECK("cg batch");
       const long long before = hs.iteration;
       hs = read_state(ctx, iw.st);
@@ -922,9 +1182,11 @@ PcgResult pcg_fcg(gs_ctx *ctx, const gs_csr *A, const double *b, double *x,
     }
     longest = std::max(longest, hs.iteration - cycle_start);
     res.iterations = hs.iteration;
-    const double predicted = hs.rnorm * bnorm;  // the recursion's residual
+    // the residual the recursion predicts: its own reduction applied to the
+    // outer residual it started from
+    const double predicted = hs.rbase > 0.0 ? rn * (hs.rnorm / hs.rbase) : 0.0;
     // outer FP64 step along the FP32 correction
-    hipLaunchKernelGGL(k_fcg_expand, gv, BS, 0, s, n, xs, bnorm, d);
+    hipLaunchKernelGGL(k_fcg_expand, gv, BS, 0, s, n, xs, bnorm, ssc, d);
     with_idx(*o.P, [&](auto idx) {
       hipLaunchKernelGGL((k_fcg_spmv<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                          o.P->cb.get(), idx, val, d, wv, first ? nullptr : ap,
@@ -1583,6 +1845,10 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
     A->sell_force32 = force32;
     A->sval = DArr<double>();
     A->sval32 = DArr<float>();
+    A->sval32s = DArr<float>();
+    A->sscale = DArr<double>();
+    A->sscale_bad = DArr<int>();
+    A->sscaled = 0;
     if (reorder) {  // the SELL order of the rows changed
       A->sdinv = DArr<double>();
       A->sdinv32 = DArr<float>();
@@ -1602,6 +1868,42 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
     A->sdinv32 = dalloc<float>(P.n);
     sell_jacobi(s, P, A, A->sdinv.get(), A->sdinv32.get());
   }
+}
+
+// FP32 values of S A S, S = diag(sqrt(D^-1)), for the mixed solve's
+// two-launch recursion on scalar operators (built once per matrix, after the
+// Jacobi diagonal).  Nothing is
+// waited for here: whether every D^-1 entry was positive (sscaled 2, pending)
+// is read with the solve's first dot product, and when one was not the
+// recursion applies D^-1 to its own z instead (sscaled -1).
+static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
+  if (A->sscaled != 0) return;
+  hipStream_t s = ctx->stream;
+  const SellPattern &P = *A->sell;
+  if (P.bs != 1) {
+    A->sscaled = -1;
+    return;
+  }
+  const int n = (int)P.n;
+  auto ssc = palloc<double>(ctx->pool, P.n);
+  auto bad = palloc<int>(ctx->pool, 1);
+  auto v = palloc<float>(ctx->pool, P.nvals);
+  hipLaunchKernelGGL((k_zero<int>), 1, BS, 0, s, 1, bad.get());
+  hipLaunchKernelGGL(k_sqrt_dinv, grid_vec(n), BS, 0, s, n, A->sdinv.get(), ssc.get(),
+                     bad.get());
+  GS_LAUNCH_CHECK("k_sqrt_dinv");
+  const unsigned g = (unsigned)std::max<int64_t>(
+      1, std::min<int64_t>((P.nslices + SELL_WPB - 1) / SELL_WPB, GRID_MAX));
+  with_idx(P, [&](auto idx) {
+    if constexpr (decltype(idx)::bs == 1)
+      hipLaunchKernelGGL((k_scale_sell<decltype(idx)>), g, BS, 0, s, n, (int)P.nslices,
+                         P.cb.get(), idx, A->sval32.get(), ssc.get(), v.get());
+  });
+  GS_LAUNCH_CHECK("k_scale_sell");
+  A->sscale = std::move(ssc);
+  A->sval32s = std::move(v);
+  A->sscale_bad = std::move(bad);
+  A->sscaled = 2;
 }
 
 // 16-bit anchored column offsets for the CSR engines, built once per matrix;
@@ -1692,6 +1994,7 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   if (A->spmv_flags & GS_SPMV_CSR)
     throw Error(GS_ERR_INVALID_ARG, "CG runs on the SELL engine only");
   ensure_sell(ctx, A, mixed, precond);
+  if (mixed && precond) ensure_scaled(ctx, A);
   Layout L;
   const size_t ob = L.add<double>(n), ox = L.add<double>(n);
   // size the scratch with a dry run of the same layout
@@ -1826,6 +2129,21 @@ void gmres_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b, const gs_cg_opt
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// Resolves a pending scaled operator (sscaled 2) with a synchronous read.
+static void resolve_scaled(gs_ctx *ctx, gs_csr *A) {
+  if (A->sscaled != 2) return;
+  int bad = 0;
+  GS_HIP(hipMemcpyAsync(&bad, A->sscale_bad.get(), sizeof(int), hipMemcpyDeviceToHost,
+                        ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  A->sscaled = bad ? -1 : 1;
+  A->sscale_bad = DArr<int>();
+  if (bad) {
+    A->sscale = DArr<double>();
+    A->sval32s = DArr<float>();
+  }
+}
+
 double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   gs_csr *A = const_cast<gs_csr *>(Ac);
   const int64_t n = A->rows;
@@ -1833,8 +2151,16 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   if (n == 0 || reps <= 0) return 0.0;
   const bool fp32 = precision != GS_PREC_FP64;
   const bool sell = use_sell(A);
-  if (sell) ensure_sell(ctx, A, fp32, false);
+  if (sell) ensure_sell(ctx, A, fp32, fp32);
   else ensure_idx16(ctx, A);
+  // the mixed solve's SpMV on a scalar operator with a positive diagonal is
+  // the two-launch pass's KA on the scaled values
+  bool cg2 = false;
+  if (sell && fp32 && A->sell->bs == 1) {
+    ensure_scaled(ctx, A);
+    resolve_scaled(ctx, A);
+    cg2 = A->sscaled == 1;
+  }
   if (!sell && fp32 && !A->val32) {
     A->val32 = dalloc<float>(A->nnz);
     hipLaunchKernelGGL(k_to_f32, (unsigned)GMAX, BS, 0, s, A->nnz, A->val.get(),
@@ -1842,13 +2168,20 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   }
   const size_t esz = fp32 ? sizeof(float) : sizeof(double);
   Layout L;
-  const size_t opart = L.add<double>(GRID_MAX), ost = L.add<CgState>(1);
+  const size_t opart = L.add<double>(2 * GRID_MAX), ost = L.add<CgState>(1);
+  const size_t opr = L.add<double>(VGMAX);
   const size_t op = L.add<char>((int64_t)(n * esz)), oq = L.add<char>((int64_t)(n * esz));
+  const size_t ow = L.add<char>((int64_t)(n * esz));
   char *base = ctx_scratch(ctx, L.bytes);
   double *part = (double *)(base + opart);
+  double *part_r = (double *)(base + opr);
   CgState *st = (CgState *)(base + ost);
-  void *p = base + op, *q = base + oq;
+  void *p = base + op, *q = base + oq, *wv = base + ow;
   hipLaunchKernelGGL(k_state_setup, 1, 1, 0, s, st, 1ll << 62, 0.0, 0.0, 0, 0.0, 0);
+  if (cg2) {
+    hipLaunchKernelGGL(k_state_continue, 1, 1, 0, s, st);
+    hipLaunchKernelGGL((k_zero<double>), 1, BS, 0, s, VGMAX, part_r);
+  }
   hipEvent_t e0, e1;
   GS_HIP(hipEventCreate(&e0));
   GS_HIP(hipEventCreate(&e1));
@@ -1859,9 +2192,21 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
                         : (const T *)(sell ? (const void *)A->sval.get()
                                            : (const void *)A->val.get());
     hipLaunchKernelGGL((k_fill_pattern<T>), grid_vec(n), BS, 0, s, (int)n, (T *)p);
+    hipLaunchKernelGGL((k_fill_pattern<T>), grid_vec(n), BS, 0, s, (int)n, (T *)q);
+    hipLaunchKernelGGL((k_fill_pattern<T>), grid_vec(n), BS, 0, s, (int)n, (T *)wv);
+    const SellOp o = sell ? sell_op(A) : SellOp{};
     auto once = [&] {
-      if (sell)
-        launch_cg_q<T>(s, sell_op(A), val, (const T *)p, (T *)q, st, 0, part);
+      if (cg2) {
+        // p and s are rewritten in place: beta = 0 (gamma_prev = 0) keeps
+        // them bounded across repetitions
+        with_idx(*o.P, [&](auto idx) {
+          hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx)>), o.grid, BS, 0, s, (int)n,
+                             o.ns(), o.P->cb.get(), idx, (const T *)A->sval32s.get(),
+                             (const T *)p, (T *)q, (T *)wv, st, 0, part_r,
+                             (int)grid_fat(n), part);
+        });
+      } else if (sell)
+        launch_cg_q<T>(s, o, val, (const T *)p, (T *)q, st, 0, part);
       else
         launch_csr_spmv<T>(s, csr_shape(A, fp32), A, val, (const T *)p, (T *)q);
     };

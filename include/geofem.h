@@ -175,7 +175,10 @@ typedef struct {
   int32_t precision;         /* gs_precision, default GS_PREC_FP64 */
   double inner_tolerance;    /* mixed: an FP64 flexible-CG step each time the
                                 FP32 recursion's residual fell by this
-                                factor (default 1e-2) */
+                                factor (default 1e-2; on scalar operators
+                                with a positive Jacobi diagonal the
+                                recursion runs on D^-1/2 A D^-1/2 and the
+                                residual is ||D^-1/2 r||) */
   int64_t max_inner_iterations; /* mixed: cap on FP32 passes between FP64
                                    steps (0 = none) */
   int64_t max_outer_iterations; /* mixed: cap on FP64 steps (default 50;
@@ -271,7 +274,12 @@ gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
 
 /* ---- measurement hooks (no reference counterpart) ----
  * Launch the CG's SpMV kernel `reps` times on the ctx stream between two HIP
- * events and return the average duration per launch in milliseconds. */
+ * events and return the average duration per launch in milliseconds.
+ * GS_PREC_FP64: the FP64 pass's SpMV (q = A p and p.q partials).
+ * GS_PREC_MIXED: the FP32 recursion's SpMV -- on a scalar operator whose
+ * Jacobi diagonal is positive, the two-launch pass's first kernel (w = A r on
+ * the Jacobi-scaled FP32 values, then p and s updated in place and p.s, r.p
+ * partials); otherwise the FP32 q = A p. */
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms);
 /* Time the elastic element kernel (K_e and the gravity load of every

@@ -142,6 +142,49 @@ def test_mixed_heterogeneous_elastic_matches_fp64():
     assert err < 1e-9
 
 
+def test_mixed_heterogeneous_thermal_matches_fp64():
+    """Log-normal conductivity (sigma 1.5) on a scalar operator, where the
+    mixed solve runs its two-launch recursion on the Jacobi-scaled FP32
+    operator: it must reach the FP64 solution at 1e-12."""
+    mesh, dof, _ = W.thermal_box(G, 10, 10, 8)
+    rng = np.random.default_rng(17)
+    k = np.exp(1.5 * rng.standard_normal(mesh.num_elements()))
+    import torch
+    kt = torch.tensor(k, device="cuda")
+    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 1.0, 1e-6, k_per_elem=kt)
+    sol = {}
+    for prec in ("fp64", "mixed"):
+        x, st = (G.ConjugateGradient().with_precision(prec).with_tolerance(1e-12)
+                 .with_abs_tolerance(0.0).with_max_iterations(100000).solve(K, f))
+        assert st.converged, prec
+        assert st.relative_residual < 1e-12
+        sol[prec] = x.cpu().numpy()
+    Ks = K.to_scipy()
+    fh = f.cpu().numpy()
+    assert np.linalg.norm(fh - Ks @ sol["mixed"]) / np.linalg.norm(fh) < 1e-12
+    err = np.linalg.norm(sol["mixed"] - sol["fp64"]) / np.linalg.norm(sol["fp64"])
+    assert err < 1e-9
+
+
+@pytest.mark.parametrize("signs", ["positive", "mixed"])
+def test_mixed_diagonal_operator(signs):
+    """Diagonal operators: Jacobi makes the recursion exact in one pass.  A
+    negative diagonal entry has no square root, so the solve keeps D^-1 in
+    its own z instead of scaling the operator; both reach x = b / d."""
+    n = 3000
+    rng = np.random.default_rng(23)
+    d = rng.uniform(1.0, 10.0, n)
+    if signs == "mixed":
+        d[::7] *= -1.0
+    A = sp.diags(d).tocsr()
+    M = G.CsrMatrix.from_scipy(A)
+    b = rng.standard_normal(n)
+    x, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-12)
+             .with_abs_tolerance(0.0).solve(M, b))
+    assert st.converged
+    assert np.linalg.norm(x - b / d) / np.linalg.norm(b / d) < 1e-12
+
+
 def test_mixed_reports_true_residual_and_counts():
     mesh, dof, exact = W.thermal_box(G, 8, 8, 6)
     K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)

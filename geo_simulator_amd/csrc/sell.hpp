@@ -71,13 +71,17 @@ template <> struct SellVec<double> {
   typedef double t __attribute__((ext_vector_type(4)));
 };
 
-// One lane's dot product over chunks [c0, c1) of its slice.
+// One lane's dot product over chunks [c0, c1) of its slice.  The sum is
+// accumulated in FP64 for either value type: the product of two FP32 values
+// is exact in FP64, so an FP32 row is rounded once, when it is stored (this
+// keeps the FP32 recursion's rounding lower; the memory-bound kernel's time
+// does not change).
 template <typename T, class IDX>
 __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
                                        const T *__restrict__ x, int c0, int c1,
                                        int lane, int base_row) {
   using V = typename SellVec<T>::t;
-  T acc = (T)0;
+  double acc = 0.0;
   int c = c0;
   auto step = [&](const V &v, int j0, int j1, int j2, int j3) {
     const int cc[4] = {j0, j1, j2, j3};
@@ -88,8 +92,8 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
       // its (unwritten) value nor a non-finite x can reach the sum.
       const bool ok = IDX::wide ? cc[j] >= 0 : cc[j] != SELL_PAD16;
       const int col = ok ? (IDX::wide ? cc[j] : base_row + cc[j]) : base_row;
-      const T prod = v[j] * x[col];
-      acc += ok ? prod : (T)0;
+      const double prod = (double)v[j] * (double)x[col];
+      acc += ok ? prod : 0.0;
     }
   };
   for (; c + 2 <= c1; c += 2) {
@@ -124,17 +128,17 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
       step(v0, i0[0], i0[1], i0[2], i0[3]);
     }
   }
-  return acc;
+  return (T)acc;
 }
 
 // One lane's three dot products (block row base_row + lane) over chunks
 // [c0, c1), two chunks in flight.  Each row sums its entries in ascending
-// column order, as the scalar engine does.
+// column order, as the scalar engine does, and in FP64 as it does.
 template <typename T, class IDX>
 __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val,
                                         const T *__restrict__ x, int c0, int c1,
-                                        int lane, int base_row, T (&acc)[3]) {
-  acc[0] = acc[1] = acc[2] = (T)0;
+                                        int lane, int base_row, T (&out)[3]) {
+  double acc[3] = {0.0, 0.0, 0.0};
   auto load = [&](int c, int &col, bool &ok, T (&v)[9]) {
     const size_t e = (size_t)c * SELL_C + lane;
     if constexpr (IDX::wide) {
@@ -150,13 +154,14 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
     for (int m = 0; m < 9; ++m) v[m] = val[((size_t)c * 9 + m) * SELL_C + lane];
   };
   auto fma3 = [&](int col, bool ok, const T (&v)[9]) {
-    const T x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
+    const double x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
-      const T p0 = v[3 * r] * x0, p1 = v[3 * r + 1] * x1, p2 = v[3 * r + 2] * x2;
-      acc[r] += ok ? p0 : (T)0;
-      acc[r] += ok ? p1 : (T)0;
-      acc[r] += ok ? p2 : (T)0;
+      const double p0 = (double)v[3 * r] * x0, p1 = (double)v[3 * r + 1] * x1,
+                   p2 = (double)v[3 * r + 2] * x2;
+      acc[r] += ok ? p0 : 0.0;
+      acc[r] += ok ? p1 : 0.0;
+      acc[r] += ok ? p2 : 0.0;
     }
   };
   int c = c0;
@@ -176,6 +181,8 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
     load(c, ca, oa, va);
     fma3(ca, oa, va);
   }
+#pragma unroll
+  for (int r = 0; r < 3; ++r) out[r] = (T)acc[r];
 }
 
 // Calls epi(row, (A x)[row]) for every row, from the row's lane.  Every

@@ -47,6 +47,10 @@ __device__ __forceinline__ int64_t xcd_block() {
 }
 __device__ __forceinline__ int64_t xcd_item() { return xcd_block() * BS + threadIdx.x; }
 
+// Contributions whose loads a gather thread issues together (the sums still
+// add them one by one in ascending order).
+constexpr int GATHER_GROUP = 8;
+
 // |v| as the ordered bit pattern of a non-negative double (0 for NaN)
 __device__ __forceinline__ unsigned long long abs_bits(double v) {
   const double a = fabs(v);
@@ -225,14 +229,24 @@ __global__ void k_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
   if (s >= nnz) return;
   const int b0 = cptr[s], b1 = cptr[s + 1];
   double v = 0.0;
-  for (int k = b0; k < b1; ++k) {
-    const int t = contrib[k];
-    const int e = t / 100;
-    const int ab = t - 100 * e;
-    const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const double x = FULL ? Ke[(int64_t)e * 100 + ab]
-                          : Ke[(int64_t)e * 55 + (a <= b ? tri_index(a, b) : tri_index(b, a))];
-    v = (k == b0) ? x : v + x;
+  for (int k0 = b0; k0 < b1; k0 += GATHER_GROUP) {
+    int t[GATHER_GROUP];
+    double x[GATHER_GROUP];
+#pragma unroll
+    for (int u = 0; u < GATHER_GROUP; ++u)
+      if (k0 + u < b1) t[u] = contrib[k0 + u];
+#pragma unroll
+    for (int u = 0; u < GATHER_GROUP; ++u)
+      if (k0 + u < b1) {
+        const int e = t[u] / 100;
+        const int ab = t[u] - 100 * e;
+        const int a = ab / 10, b = ab - 10 * (ab / 10);
+        x[u] = FULL ? Ke[(int64_t)e * 100 + ab]
+                    : Ke[(int64_t)e * 55 + (a <= b ? tri_index(a, b) : tri_index(b, a))];
+      }
+#pragma unroll
+    for (int u = 0; u < GATHER_GROUP; ++u)
+      if (k0 + u < b1) v = (k0 + u == b0) ? x[u] : v + x[u];
   }
   val[s] = v;
 }
@@ -519,15 +533,27 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
     const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
     const bool diag = ab0 / 10 == ab0 % 10;  // a diagonal slot: local a == b
     if (d != -1 || diag) {
+      // contributions in groups of GATHER_GROUP: the loads of a group are
+      // issued together, the additions stay in ascending order
       double v = 0.0;
-      for (int k = b0; k < b1; ++k) {
-        const int t = contrib[k];
-        const int e = t / 100;
-        const int ab = t - 100 * e;
-        const int a = ab / 10, b = ab - 10 * (ab / 10);
-        const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
-        const double x = Ke[(int64_t)e * 55 + p];
-        v = (k == b0) ? x : v + x;
+      for (int k0 = b0; k0 < b1; k0 += GATHER_GROUP) {
+        int t[GATHER_GROUP];
+        double x[GATHER_GROUP];
+#pragma unroll
+        for (int u = 0; u < GATHER_GROUP; ++u)
+          if (k0 + u < b1) t[u] = contrib[k0 + u];
+#pragma unroll
+        for (int u = 0; u < GATHER_GROUP; ++u)
+          if (k0 + u < b1) {
+            const int e = t[u] / 100;
+            const int ab = t[u] - 100 * e;
+            const int a = ab / 10, b = ab - 10 * (ab / 10);
+            const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
+            x[u] = Ke[(int64_t)e * 55 + p];
+          }
+#pragma unroll
+        for (int u = 0; u < GATHER_GROUP; ++u)
+          if (k0 + u < b1) v = (k0 + u == b0) ? x[u] : v + x[u];
       }
       if (diag) mine = abs_bits(v);
       if (d >= 0) {
@@ -659,11 +685,23 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
   double f = 0.0;
   const int s = diag_slot[node];
   if (s >= 0) {
-    for (int k = cptr[s]; k < cptr[s + 1]; ++k) {
-      const int t = contrib[k];
-      const int e = t / 100;
-      const int a = (t - 100 * e) / 10;
-      f += Fe[((int64_t)e * 10 + a) * DPN + c];
+    const int k1 = cptr[s + 1];
+    for (int k0 = cptr[s]; k0 < k1; k0 += GATHER_GROUP) {  // loads in groups
+      int t[GATHER_GROUP];
+      double x[GATHER_GROUP];
+#pragma unroll
+      for (int u = 0; u < GATHER_GROUP; ++u)
+        if (k0 + u < k1) t[u] = contrib[k0 + u];
+#pragma unroll
+      for (int u = 0; u < GATHER_GROUP; ++u)
+        if (k0 + u < k1) {
+          const int e = t[u] / 100;
+          const int a = (t[u] - 100 * e) / 10;
+          x[u] = Fe[((int64_t)e * 10 + a) * DPN + c];
+        }
+#pragma unroll
+      for (int u = 0; u < GATHER_GROUP; ++u)
+        if (k0 + u < k1) f += x[u];
     }
   }
   double dv = 0.0;

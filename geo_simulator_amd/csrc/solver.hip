@@ -677,13 +677,42 @@ __global__ __launch_bounds__(BS) void k_fcg_spmv(
 
 // p = d - gamma p; ap = w - gamma ap (first step: p = d, ap = w);
 // partials p.r, p.ap
+// Outer-step scalars kept on the device (one read by the host per step):
+// sc[0] gamma, sc[1] alpha, sc[2] p.Ap of the last direction, sc[3] 1 when
+// the step is taken (p.Ap > 0 and a finite alpha), sc[4] r.r after it.
+// gamma = (d, A p_prev) / (p_prev, A p_prev); 0 on the first step.
+__global__ __launch_bounds__(FBS) void k_fcg_gamma(const double *__restrict__ part, int np,
+                                                   int first, double *__restrict__ sc) {
+  if (first) {
+    if (threadIdx.x == 0) sc[0] = 0.0;
+    return;
+  }
+  double t[1];
+  sum_partials<1>(part, np, t);
+  if (threadIdx.x == 0) sc[0] = t[0] / sc[2];
+}
+
+// alpha = (p, r) / (p, Ap) when p.Ap > 0 and the ratio is finite, else the
+// step is not taken (no descent left)
+__global__ __launch_bounds__(FBS) void k_fcg_alpha(const double *__restrict__ part, int np,
+                                                   double *__restrict__ sc) {
+  double t[2];
+  sum_partials<2>(part, np, t);
+  if (threadIdx.x != 0) return;
+  const bool ok = t[1] > 0.0 && isfinite(t[0] / t[1]);
+  sc[1] = ok ? t[0] / t[1] : 0.0;
+  if (ok) sc[2] = t[1];
+  sc[3] = ok ? 1.0 : 0.0;
+}
+
 __global__ __launch_bounds__(BS) void k_fcg_dir(int n, const double *__restrict__ d,
                                                 const double *__restrict__ w,
                                                 const double *__restrict__ r,
-                                                double gamma, int first,
+                                                const double *__restrict__ sc, int first,
                                                 double *__restrict__ p,
                                                 double *__restrict__ ap,
                                                 double *__restrict__ part) {
+  const double gamma = sc[0];
   double acc[2] = {0.0, 0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     const double pi = first ? d[i] : d[i] - gamma * p[i];
@@ -696,13 +725,16 @@ __global__ __launch_bounds__(BS) void k_fcg_dir(int n, const double *__restrict_
   put_partials<2>(acc, part);
 }
 
-// x += alpha p; r -= alpha ap; partials r.r
-__global__ __launch_bounds__(BS) void k_fcg_update(int n, double alpha,
+// x += alpha p; r -= alpha ap; partials r.r (nothing when the step is
+// not taken)
+__global__ __launch_bounds__(BS) void k_fcg_update(int n, const double *__restrict__ sc,
                                                    const double *__restrict__ p,
                                                    const double *__restrict__ ap,
                                                    double *__restrict__ x,
                                                    double *__restrict__ r,
                                                    double *__restrict__ part) {
+  if (sc[3] == 0.0) return;
+  const double alpha = sc[1];
   double acc[1] = {0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     x[i] += alpha * p[i];
@@ -1137,7 +1169,8 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
     GS_LAUNCH_CHECK("k_fcg_inner");
   };
   start_inner(true, bnorm);
-  double rn = bnorm, pap = 0.0;
+  double rn = bnorm;
+  double *const sc = dots + 2;  // the outer step's device scalars
   bool first = true, checked = false;
   Batcher bt;
   CgState hs = read_state(ctx, iw.st);
@@ -1193,26 +1226,19 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
                          iw.part1);
     });
     ++res.spmv;
-    double gamma = 0.0;
-    if (!first) {
-      hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part1, o.grid, dots);
-      read_dots(ctx, dots, h, 1);
-      gamma = h[0] / pap;
-    }
-    hipLaunchKernelGGL(k_fcg_dir, gv, BS, 0, s, n, d, wv, r, gamma, first ? 1 : 0, p,
-                       ap, iw.part2);
-    hipLaunchKernelGGL((k_fin_sum<2>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
-    GS_LAUNCH_CHECK("k_fcg_dir");
-    read_dots(ctx, dots, h, 2);
-    if (!(h[1] > 0.0) || !std::isfinite(h[0] / h[1])) break;  // no descent left
-    const double alpha = h[0] / h[1];
-    pap = h[1];
-    first = false;
-    hipLaunchKernelGGL(k_fcg_update, gv, BS, 0, s, n, alpha, p, ap, x, r, iw.part2);
-    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, dots);
+    hipLaunchKernelGGL(k_fcg_gamma, 1, FBS, 0, s, iw.part1, o.grid, first ? 1 : 0, sc);
+    hipLaunchKernelGGL(k_fcg_dir, gv, BS, 0, s, n, d, wv, r, sc, first ? 1 : 0, p, ap,
+                       iw.part2);
+    hipLaunchKernelGGL(k_fcg_alpha, 1, FBS, 0, s, iw.part2, (int)gv, sc);
+    hipLaunchKernelGGL(k_fcg_update, gv, BS, 0, s, n, sc, p, ap, x, r, iw.part2);
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, sc + 4);
     GS_LAUNCH_CHECK("k_fcg_update");
-    read_dots(ctx, dots, h, 1);
-    rn = std::sqrt(h[0]);
+    double hs4[4];  // alpha, p.Ap, step taken, r.r
+    read_dots(ctx, sc + 1, hs4, 4);
+    if (hs4[2] == 0.0) break;  // no descent left
+    const double alpha = hs4[0];
+    first = false;
+    rn = std::sqrt(hs4[3]);
     ++res.outer;
     checked = false;
     if (rn < target) {

@@ -99,6 +99,38 @@ def test_time_spmv_reports_positive():
     assert M.time_spmv("fp64", 3) > 0.0 and M.time_spmv("mixed", 3) > 0.0
 
 
+def test_time_spmv_mixed_then_solve_negative_diagonal():
+    """The mixed timing hook on an operator with a negative Jacobi entry
+    times the three-launch recursion's SpMV; a mixed solve afterwards on the
+    same matrix still reaches x = b / d (the operator is diagonal)."""
+    n = 2500
+    rng = np.random.default_rng(29)
+    d = rng.uniform(1.0, 4.0, n)
+    d[5] = -2.0
+    M = G.CsrMatrix.from_scipy(sp.diags(d).tocsr())
+    assert M.time_spmv("mixed", 3) > 0.0
+    b = rng.standard_normal(n)
+    x, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-12)
+             .with_abs_tolerance(0.0).solve(M, b))
+    assert st.converged
+    assert np.linalg.norm(x - b / d) / np.linalg.norm(b / d) < 1e-12
+
+
+def test_mixed_pass_cap():
+    """max_iterations caps the FP32 passes of a mixed solve; an unconverged
+    solve reports its true residual."""
+    mesh, dof, _ = W.thermal_box(G, 8, 8, 6)
+    K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+    x, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-12)
+             .with_abs_tolerance(0.0).with_max_iterations(7).solve(K, f))
+    assert not st.converged and st.iterations <= 7
+    Ks = K.to_scipy()
+    fh = f.cpu().numpy()
+    true = np.linalg.norm(fh - Ks @ x.cpu().numpy()) / np.linalg.norm(fh)
+    assert abs(true - st.relative_residual) <= 1e-6 * true
+    assert 1e-12 < true < 1.0
+
+
 def test_system_plans_per_dofmap():
     """Two DOF maps on one mesh: each fused system equals the separate
     assemble / load / eliminate path bit for bit, also when alternated."""

@@ -41,27 +41,36 @@ constexpr int SELL_WPB = 4;       // slices per workgroup (256 threads)
 constexpr int SELL_SIGMA = 256;   // length-sorting window (rows)
 constexpr int SELL_PAD16 = -32768;
 
-struct SellIdx16 {
+// STREAM: the operator's values bypass the caches (see load_vals)
+template <bool STREAM> struct SellIdx16 {
   const short *__restrict__ off;
   static constexpr bool wide = false;
   static constexpr int bs = 1;
+  static constexpr bool stream = STREAM;
 };
-struct SellIdx32 {
+template <bool STREAM> struct SellIdx32 {
   const int *__restrict__ col;
   static constexpr bool wide = true;
   static constexpr int bs = 1;
+  static constexpr bool stream = STREAM;
 };
 // block variant: the same index streams, one entry per 3x3 block
-struct BSellIdx16 {
+template <bool STREAM> struct BSellIdx16 {
   const short *__restrict__ off;
   static constexpr bool wide = false;
   static constexpr int bs = 3;
+  static constexpr bool stream = STREAM;
 };
-struct BSellIdx32 {
+template <bool STREAM> struct BSellIdx32 {
   const int *__restrict__ col;
   static constexpr bool wide = true;
   static constexpr int bs = 3;
+  static constexpr bool stream = STREAM;
 };
+
+// Operators larger than this (values and indices, in the precision a
+// kernel reads them) stream their values: 7/8 of the 256 MB Infinity Cache.
+constexpr int64_t SELL_RESIDENT_BYTES = 224ll << 20;
 
 template <typename T> struct SellVec;
 template <> struct SellVec<float> {
@@ -70,6 +79,18 @@ template <> struct SellVec<float> {
 template <> struct SellVec<double> {
   typedef double t __attribute__((ext_vector_type(4)));
 };
+
+// Value loads.  An operator larger than the Infinity Cache streams its values
+// past the caches (non-temporal loads), which leaves the cached lines to the
+// pass's vectors and to the index stream: the SpMV itself runs at the same
+// speed, the solves gain (FP64 at 1.08M thermal DOFs 26.3 -> 25.4 ms, at
+// 1.38M elastic DOFs 181 -> 166 ms).  An operator that fits (the thermal FP32
+// one, 194 MB) is kept cached across passes with ordinary loads.
+template <bool STREAM, typename V, typename T>
+__device__ __forceinline__ V load_vals(const T *p) {
+  if constexpr (STREAM) return __builtin_nontemporal_load((const V *)p);
+  else return *(const V *)p;
+}
 
 // One lane's dot product over chunks [c0, c1) of its slice.  The sum is
 // accumulated in FP64 for either value type: the product of two FP32 values
@@ -99,8 +120,8 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
   for (; c + 2 <= c1; c += 2) {
     const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
     const size_t e1 = e0 + SELL_CHUNK;
-    V v0 = *(const V *)(val + e0);
-    V v1 = *(const V *)(val + e1);
+    V v0 = load_vals<IDX::stream, V>(val + e0);
+    V v1 = load_vals<IDX::stream, V>(val + e1);
     if constexpr (IDX::wide) {
       typedef int I4 __attribute__((ext_vector_type(4)));
       const I4 i0 = *(const I4 *)(idx.col + e0);
@@ -117,7 +138,7 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
   }
   if (c < c1) {
     const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
-    V v0 = *(const V *)(val + e0);
+    V v0 = load_vals<IDX::stream, V>(val + e0);
     if constexpr (IDX::wide) {
       typedef int I4 __attribute__((ext_vector_type(4)));
       const I4 i0 = *(const I4 *)(idx.col + e0);
@@ -151,7 +172,7 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
       col = ok ? base_row + j : base_row;
     }
 #pragma unroll
-    for (int m = 0; m < 9; ++m) v[m] = val[((size_t)c * 9 + m) * SELL_C + lane];
+    for (int m = 0; m < 9; ++m) v[m] = load_vals<IDX::stream, T>(val + ((size_t)c * 9 + m) * SELL_C + lane);
   };
   auto fma3 = [&](int col, bool ok, const T (&v)[9]) {
     const double x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];

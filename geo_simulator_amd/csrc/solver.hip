@@ -829,14 +829,23 @@ SellOp sell_op(const gs_csr *A) {
 }
 
 // Calls f(idx) with the pattern's index reader.
-template <class F> void with_idx(const SellPattern &P, F &&f) {
-  if (P.bs == 3) {
-    if (P.idx16) f(BSellIdx16{P.off.get()});
-    else f(BSellIdx32{P.col.get()});
-  } else {
-    if (P.idx16) f(SellIdx16{P.off.get()});
-    else f(SellIdx32{P.col.get()});
-  }
+// Calls f with the pattern's index view; vsize is the byte size of the
+// values the kernel reads, which decides whether they stream (load_vals).
+template <class F> void with_idx(const SellPattern &P, F &&f, int vsize) {
+  const int64_t bytes = P.nvals * vsize + P.entries * (P.idx16 ? 2 : 4);
+  const bool stream = bytes > SELL_RESIDENT_BYTES;
+  auto pick = [&](auto tag) {
+    constexpr bool S = decltype(tag)::value;
+    if (P.bs == 3) {
+      if (P.idx16) f(BSellIdx16<S>{P.off.get()});
+      else f(BSellIdx32<S>{P.col.get()});
+    } else {
+      if (P.idx16) f(SellIdx16<S>{P.off.get()});
+      else f(SellIdx32<S>{P.col.get()});
+    }
+  };
+  if (stream) pick(std::true_type{});
+  else pick(std::false_type{});
 }
 
 template <typename T>
@@ -845,7 +854,7 @@ void launch_cg_q(hipStream_t s, const SellOp &o, const T *val, const T *p, T *q,
   with_idx(*o.P, [&](auto idx) {
     hipLaunchKernelGGL((k_cg_q<T, decltype(idx)>), o.grid, BS, 0, s, o.n(), o.ns(),
                        o.P->cb.get(), idx, val, p, q, st, it, part);
-  });
+  }, sizeof(T));
 }
 
 template <typename T>
@@ -854,7 +863,7 @@ void launch_sell_spmv(hipStream_t s, const SellOp &o, const T *val, const T *x,
   with_idx(*o.P, [&](auto idx) {
     hipLaunchKernelGGL((k_sell_spmv<T, decltype(idx)>), o.grid, BS, 0, s, o.n(),
                        o.ns(), o.P->cb.get(), idx, val, x, y);
-  });
+  }, sizeof(T));
 }
 
 void launch_residual(hipStream_t s, const SellOp &o, const double *val,
@@ -862,7 +871,7 @@ void launch_residual(hipStream_t s, const SellOp &o, const double *val,
   with_idx(*o.P, [&](auto idx) {
     hipLaunchKernelGGL((k_sell_residual<decltype(idx)>), o.grid, BS, 0, s, o.n(),
                        o.ns(), o.P->cb.get(), idx, val, x, b, r, part);
-  });
+  }, 8);
 }
 
 // ---- CSR engine launch shape (rectangular matrices, comparison timing) ------
@@ -988,7 +997,7 @@ void enqueue_pass2(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *
   with_idx(*o.P, [&](auto idx) {
     hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                        o.P->cb.get(), idx, val, r, p, sv, st, it, part_r, (int)gf, part1);
-  });
+  }, sizeof(T));
   hipLaunchKernelGGL((k_cg2_step<T>), gf, FBS, 0, s, n, it, x, r, p, sv, st, part1,
                      o.grid, part_r);
 }
@@ -1224,7 +1233,7 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
       hipLaunchKernelGGL((k_fcg_spmv<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                          o.P->cb.get(), idx, val, d, wv, first ? nullptr : ap,
                          iw.part1);
-    });
+    }, 8);
     ++res.spmv;
     hipLaunchKernelGGL(k_fcg_gamma, 1, FBS, 0, s, iw.part1, o.grid, first ? 1 : 0, sc);
     hipLaunchKernelGGL(k_fcg_dir, gv, BS, 0, s, n, d, wv, r, sc, first ? 1 : 0, p, ap,
@@ -1548,7 +1557,7 @@ void enqueue_bg_pass(hipStream_t s, const SellOp &o, const double *val,
     hipLaunchKernelGGL((k_bg_spmv<1, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                        o.P->cb.get(), idx, val, (const double *)w.ph, w.v,
                        (const double *)w.rh, (const BgState *)w.st, 0, it, w.part_a);
-  });
+  }, 8);
   if (dinv)
     hipLaunchKernelGGL((k_bg_s<true>), gf, FBS, 0, s, n, it, w.r, w.v, dinv, w.s, w.sh,
                        w.st, w.part_a, o.grid, w.part_b);
@@ -1559,7 +1568,7 @@ void enqueue_bg_pass(hipStream_t s, const SellOp &o, const double *val,
     hipLaunchKernelGGL((k_bg_spmv<2, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                        o.P->cb.get(), idx, val, (const double *)w.sh, w.t,
                        (const double *)w.s, (const BgState *)w.st, 2, it, w.part_a);
-  });
+  }, 8);
   hipLaunchKernelGGL(k_bg_x, gf, FBS, 0, s, n, it, x, w.r, w.rh, w.ph, w.s, w.sh, w.t,
                      w.st, w.part_b, (int)gf, w.part_a, o.grid, w.part_c);
 }
@@ -1774,7 +1783,7 @@ PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const doub
       with_idx(*o.P, [&](auto idx) {
         hipLaunchKernelGGL((k_spmv_dot<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
                            o.P->cb.get(), idx, val, zin, w, (const double *)V, part1);
-      });
+      }, 8);
       ++res.spmv;
       // projections i = 0..j; the first sums the SpMV's partials
       for (int i = 0; i <= j; ++i) {
@@ -1924,7 +1933,7 @@ static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
     if constexpr (decltype(idx)::bs == 1)
       hipLaunchKernelGGL((k_scale_sell<decltype(idx)>), g, BS, 0, s, n, (int)P.nslices,
                          P.cb.get(), idx, A->sval32.get(), ssc.get(), v.get());
-  });
+  }, 4);
   GS_LAUNCH_CHECK("k_scale_sell");
   A->sscale = std::move(ssc);
   A->sval32s = std::move(v);
@@ -2230,7 +2239,7 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
                              o.ns(), o.P->cb.get(), idx, (const T *)A->sval32s.get(),
                              (const T *)p, (T *)q, (T *)wv, st, 0, part_r,
                              (int)grid_fat(n), part);
-        });
+        }, sizeof(T));
       } else if (sell)
         launch_cg_q<T>(s, o, val, (const T *)p, (T *)q, st, 0, part);
       else

@@ -149,13 +149,12 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   const SellPattern &S = *P->sell;
   auto val = palloc<double>(pool, P->nnz);
   auto sval = palloc<double>(pool, S.nvals);
-  auto sval32 = palloc<float>(pool, S.nvals);
-  sell_zero_holes(s, S, sval.get(), sval32.get());
+  auto sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
   auto liftv = palloc<double>(pool, P->n_lift);
   auto sdinv = palloc<double>(pool, P->n);
   auto sdinv32 = palloc<float>(pool, P->n);
   const DArr<int> &dperm = sp.elastic ? m->perm3 : m->perm;
-  auto dmax = palloc<unsigned long long>(pool, std::max<int64_t>(1, system_gather_blocks(m)));
+  auto dmax = palloc<unsigned long long>(pool, std::max<int64_t>(1, system_gather_blocks(m, *P)));
   system_gather(s, m, *P, Ke.get(), val.get(), sval.get(), sval32.get(), liftv.get(),
                 dmax.get());
   system_finish(ctx, m, *P, Ke.get(), Fe.get(), val.get(), sval.get(), sval32.get(),

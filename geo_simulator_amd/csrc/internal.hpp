@@ -159,6 +159,11 @@ struct SystemPlan {
   DArr<int> diag;     // n: K_bc nonzero holding row r's diagonal (-1: none)
   DArr<uint8_t> dir;  // n, internal numbering
   DArr<double> g;     // n, internal numbering
+  // 3 DOFs per node: the node slot of each block position of the block view
+  // (-1: padding), and the node slots with lift destinations
+  DArr<int> bslot;
+  DArr<int> lift_slots;
+  int64_t n_lift_slots = 0;
 };
 
 }  // namespace gs
@@ -306,7 +311,7 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
                                         const gs_dofmap *d);
 // values of K_bc (CSR and SELL, FP64 and FP32) and the lift values
 // workgroups of the system gather (entries of its diagonal-maximum array)
-int64_t system_gather_blocks(const gs_mesh *m);
+int64_t system_gather_blocks(const gs_mesh *m, const SystemPlan &P);
 void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
                    const double *Ke, double *val, double *sval, float *sval32,
                    double *liftv, unsigned long long *dmax);

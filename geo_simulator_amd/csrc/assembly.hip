@@ -590,14 +590,26 @@ __device__ __forceinline__ void elastic_slot_sums(
   diag = ab0 / 10 == ab0 % 10;
 #pragma unroll
   for (int m = 0; m < 9; ++m) v[m] = 0.0;
-  for (int k = b0; k < b1; ++k) {
-    const int t = contrib[k];
-    const int e = t / 100;
-    const int ab = t - 100 * e;
-    const int a = ab / 10, b = ab - 10 * (ab / 10);
-    const double *Kr = Ke + (int64_t)e * 900 + (a * 10 + b) * 9;
+  // two contributions' 18 loads issued together; additions in ascending order
+  for (int k0 = b0; k0 < b1; k0 += 2) {
+    double x[2][9];
 #pragma unroll
-    for (int m = 0; m < 9; ++m) v[m] = (k == b0) ? Kr[m] : v[m] + Kr[m];
+    for (int u = 0; u < 2; ++u)
+      if (k0 + u < b1) {
+        const int t = contrib[k0 + u];
+        const int e = t / 100;
+        const int ab = t - 100 * e;
+        const int a = ab / 10, b = ab - 10 * (ab / 10);
+        const double *Kr = Ke + (int64_t)e * 900 + (a * 10 + b) * 9;
+#pragma unroll
+        for (int m = 0; m < 9; ++m) x[u][m] = Kr[m];
+      }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (k0 + u < b1) {
+#pragma unroll
+        for (int m = 0; m < 9; ++m) v[m] = (k0 + u == b0) ? x[u][m] : v[m] + x[u][m];
+      }
   }
 }
 

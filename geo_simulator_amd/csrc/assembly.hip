@@ -557,7 +557,7 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
       }
       if (diag) mine = abs_bits(v);
       if (d >= 0) {
-        val[d] = v;
+        if (val) val[d] = v;  // null: the CSR values are produced on demand
         const int e = pos[d];
         sval[e] = v;
         sval32[e] = (float)v;
@@ -655,7 +655,7 @@ __global__ void k_sys_gather_elastic_b(int64_t nent, const int *__restrict__ bsl
       for (int m = 0; m < 9; ++m)
         if (dd[m] >= 0) {
           out[m] = v[m];
-          val[dd[m]] = v[m];
+          if (val) val[dd[m]] = v[m];  // null: the CSR values are produced on demand
         }
     }
     const int64_t c = eb >> 6, l = eb & 63;
@@ -782,7 +782,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
   const int d = diag[r];
   if (dir[r]) {
     const double cv = __longlong_as_double((long long)*maxbits);
-    val[d] = cv;
+    if (val) val[d] = cv;
     const int e = pos[d];
     sval[e] = cv;
     sval32[e] = (float)cv;
@@ -790,7 +790,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
     dv = cv;
   } else {
     for (int q = lptr[r]; q < lptr[r + 1]; ++q) f -= liftv[q] * g[lcol[q]];
-    dv = d >= 0 ? val[d] : 0.0;
+    dv = d >= 0 ? (val ? val[d] : sval[pos[d]]) : 0.0;
   }
   fbc[dperm ? dperm[r] : r] = f;
   const double di = fabs(dv) > 1e-14 ? 1.0 / dv : 1.0;
@@ -1022,6 +1022,7 @@ void gather_gravity_load(hipStream_t s, const NodePattern &p, int64_t n_nodes,
 gs_csr *dirichlet_eliminate(gs_ctx *ctx, const gs_csr *K, const double *f,
                             const uint8_t *is_dir, const double *g,
                             double *fbc) {
+  ensure_csr_values(ctx, K);
   const int64_t n = K->rows;
   hipStream_t s = ctx->stream;
   auto cnt = dalloc<int>(n > 0 ? n : 1);

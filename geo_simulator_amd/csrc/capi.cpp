@@ -147,7 +147,6 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
                             true, Ke.get(), Fe.get(), flag.get());
   check_flags(read_flag_and_clear(ctx, flag.get()));
   const SellPattern &S = *P->sell;
-  auto val = palloc<double>(pool, P->nnz);
   auto sval = palloc<double>(pool, S.nvals);
   auto sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
   auto liftv = palloc<double>(pool, P->n_lift);
@@ -155,11 +154,14 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   auto sdinv32 = palloc<float>(pool, P->n);
   const DArr<int> &dperm = sp.elastic ? m->perm3 : m->perm;
   auto dmax = palloc<unsigned long long>(pool, std::max<int64_t>(1, system_gather_blocks(m, *P)));
-  system_gather(s, m, *P, Ke.get(), val.get(), sval.get(), sval32.get(), liftv.get(),
+  // K_bc's CSR values are not written here: ensure_csr_values produces them
+  // from the SELL values when a caller asks for the CSR form
+  system_gather(s, m, *P, Ke.get(), nullptr, sval.get(), sval32.get(), liftv.get(),
                 dmax.get());
-  system_finish(ctx, m, *P, Ke.get(), Fe.get(), val.get(), sval.get(), sval32.get(),
+  system_finish(ctx, m, *P, Ke.get(), Fe.get(), nullptr, sval.get(), sval32.get(),
                 liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get(), dmax.get());
-  gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, val);
+  gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, DArr<double>());
+  K->val_lazy = true;
   K->dperm = dperm;
   K->block = dpn;
   K->sell = P->sell;
@@ -735,6 +737,7 @@ gs_status gs_csr_export(gs_ctx *ctx, const gs_csr *A, int64_t *indptr,
   if (!ctx || !A) return GS_ERR_INVALID_ARG;
   return guard(ctx, [&] {
     set_device(ctx);
+    ensure_csr_values(ctx, A);
     const int64_t rows = A->rows, nnz = A->nnz;
     std::vector<int> rp((size_t)rows + 1), ci((size_t)nnz);
     std::vector<double> dv((size_t)nnz);
@@ -843,6 +846,7 @@ gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv) {
   return guard(ctx, [&] {
     require(dinv || A->rows == 0, GS_ERR_INVALID_ARG, "null output");
     set_device(ctx);
+    ensure_csr_values(ctx, A);
     if (A->dperm) {
       auto d = dalloc<double>(A->rows);
       jacobi_diag_inv(ctx->stream, A, d.get());

@@ -214,6 +214,10 @@ struct gs_csr {
   // Row/column numbering: internal index i is caller index dperm[i] (empty:
   // identity).  Vectors cross the C ABI in caller numbering.
   gs::DArr<int> dperm;
+  // The CSR values of a fused system (gs_assemble_*_system) are produced from
+  // its SELL values on first use (ensure_csr_values); the solvers never need
+  // them.
+  bool val_lazy = false;
   // lazily derived, cached (solver)
   int idx16 = 0;              // 16-bit CSR column offsets: 0 unknown, 1 yes, -1 no
   gs::DArr<short> off16;      // nnz
@@ -330,6 +334,8 @@ std::shared_ptr<SellPattern> build_sell_pattern(gs_ctx *ctx, const gs_csr *A,
                                                 bool force32, int block = 1);
 // zero values of a block view that no CSR nonzero writes (bs = 1: nothing)
 void sell_zero_holes(hipStream_t s, const SellPattern &P, double *sval, float *sval32);
+// A->val from A->sval through the SELL pattern's pos, when the values are lazy
+void ensure_csr_values(gs_ctx *ctx, const gs_csr *A);
 // sval[pos[k]] = val[k], padding zero
 template <typename T>
 void sell_values(hipStream_t s, const SellPattern &P, const double *val, T *sval);

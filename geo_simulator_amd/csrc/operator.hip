@@ -454,6 +454,26 @@ template void sell_values<double>(hipStream_t, const SellPattern &, const double
 template void sell_values<float>(hipStream_t, const SellPattern &, const double *,
                                  float *);
 
+__global__ void k_csr_from_sell(int64_t nnz, const int *__restrict__ pos,
+                                const double *__restrict__ sval,
+                                double *__restrict__ val) {
+  const int64_t d = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (d < nnz) val[d] = sval[pos[d]];
+}
+
+void ensure_csr_values(gs_ctx *ctx, const gs_csr *Ac) {
+  gs_csr *A = const_cast<gs_csr *>(Ac);  // a derived cache
+  if (!A->val_lazy || A->val) return;
+  if (!A->sell || !A->sval)
+    throw Error(GS_ERR_INVALID_ARG, "CSR values of a fused system without its SELL values");
+  A->val = dalloc<double>(A->nnz);
+  if (A->nnz > 0) {
+    hipLaunchKernelGGL(k_csr_from_sell, nblk(A->nnz), BS, 0, ctx->stream, A->nnz,
+                       A->sell->pos.get(), A->sval.get(), A->val.get());
+    GS_LAUNCH_CHECK("k_csr_from_sell");
+  }
+}
+
 void sell_zero_holes(hipStream_t s, const SellPattern &P, double *sval, float *sval32) {
   if (P.bs == 1) return;
   hipLaunchKernelGGL((k_fill_zero<double>), GFILL, BS, 0, s, P.nvals, sval);

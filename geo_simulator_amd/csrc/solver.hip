@@ -1875,6 +1875,7 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
   const int block = sell_block(A);
   hipStream_t s = ctx->stream;
   if (!A->sell || A->sell_force32 != force32 || A->sell->bs != block) {
+    ensure_csr_values(ctx, A);  // from the old view's values, before they go
     const bool reorder = !A->sell || A->sell->bs != block;
     A->sell = build_sell_pattern(ctx, A, force32, block);
     A->sell_force32 = force32;
@@ -1891,14 +1892,17 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
   }
   const SellPattern &P = *A->sell;
   if (!A->sval) {
+    ensure_csr_values(ctx, A);
     A->sval = dalloc<double>(P.nvals);
     sell_values<double>(s, P, A->val.get(), A->sval.get());
   }
   if (fp32 && !A->sval32) {
+    ensure_csr_values(ctx, A);
     A->sval32 = dalloc<float>(P.nvals);
     sell_values<float>(s, P, A->val.get(), A->sval32.get());
   }
   if (precond && (!A->sdinv || A->sdinv.n != P.n)) {
+    ensure_csr_values(ctx, A);
     A->sdinv = dalloc<double>(P.n);
     A->sdinv32 = dalloc<float>(P.n);
     sell_jacobi(s, P, A, A->sdinv.get(), A->sdinv32.get());
@@ -1994,6 +1998,7 @@ void csr_spmv(gs_ctx *ctx, const gs_csr *Ac, const double *x, double *y) {
     permute_scatter(s, n, yi, vp, y);
   } else {
     ensure_idx16(ctx, A);
+    ensure_csr_values(ctx, A);
     const Shape sh = csr_shape(A, false);
     const int *dp = A->dperm.get();
     if (!dp) {
@@ -2186,8 +2191,12 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
   if (n == 0 || reps <= 0) return 0.0;
   const bool fp32 = precision != GS_PREC_FP64;
   const bool sell = use_sell(A);
-  if (sell) ensure_sell(ctx, A, fp32, fp32);
-  else ensure_idx16(ctx, A);
+  if (sell) {
+    ensure_sell(ctx, A, fp32, fp32);
+  } else {
+    ensure_idx16(ctx, A);
+    ensure_csr_values(ctx, A);
+  }
   // the mixed solve's SpMV on a scalar operator with a positive diagonal is
   // the two-launch pass's KA on the scaled values
   bool cg2 = false;

@@ -344,3 +344,48 @@ def test_elastic_system_block_view_matches_scalar():
     assert np.linalg.norm(u0[2::3] - uz) / np.linalg.norm(uz) < 1e-8
     um, _ = res[(0, "mixed")]
     assert np.linalg.norm(um - u1) / np.linalg.norm(u1) < 1e-9
+
+
+@pytest.mark.parametrize("elastic", [False, True])
+def test_fused_system_csr_values_on_demand(elastic):
+    """A fused system keeps only its SELL values; its CSR values are produced
+    when a caller asks for the CSR form.  Each consumer, reached first on a
+    fresh matrix, sees the same matrix: the Jacobi diagonal, the CSR SpMV
+    engine, a SELL view rebuilt with 32-bit indices (then a mixed solve), CSR
+    timing, and the export."""
+    from geo_simulator_amd import _lib
+    if elastic:
+        mesh, dof, _ = W.elastic_box(G, 4, 4, 3)
+
+        def build():
+            return G.Assembler.assemble_elasticity_system(
+                mesh, dof, G.IsotropicElasticity(1e11, 0.25), 3300.0, [0.0, 0.0, -9.81])
+    else:
+        mesh, dof, _ = W.thermal_box(G, 6, 6, 4)
+
+        def build():
+            return G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+    Kref, f = build()
+    A = Kref.to_scipy()
+    d = A.diagonal()
+    x = np.random.default_rng(41).standard_normal(A.shape[0])
+    bound = 1e-13 * (abs(A) @ np.abs(x))
+
+    K1, _ = build()
+    dinv = K1.jacobi_diag_inv().cpu().numpy()
+    assert np.array_equal(dinv, np.where(np.abs(d) > 1e-14, 1.0 / d, 1.0))
+
+    K2, _ = build()
+    K2.set_spmv_shape(flags=_lib.GS_SPMV_CSR)
+    assert np.all(np.abs(K2.apply(x) - A @ x) <= bound + 1e-300)
+    assert K2.time_spmv("fp64", 2) > 0.0
+
+    K3, f3 = build()
+    K3.set_spmv_shape(flags=_lib.GS_SPMV_INDEX32)
+    assert np.all(np.abs(K3.apply(x) - A @ x) <= bound + 1e-300)
+    u, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-10)
+             .with_abs_tolerance(0.0).solve(K3, f3))
+    assert st.converged
+    B = K3.to_scipy()
+    assert np.array_equal(B.indptr, A.indptr) and np.array_equal(B.indices, A.indices)
+    assert np.array_equal(B.data, A.data)

@@ -618,7 +618,8 @@ __device__ __forceinline__ void elastic_slot_sums(
 // each of the nine value planes as one contiguous stretch, and every
 // position -- padding and the missing entries of a stored block -- is
 // written once (as zero), with no separate clearing pass.  The kept entries
-// also go to K_bc's CSR values; the lift values are left to
+// also go to K_bc's CSR values when val is given (the fused system passes
+// none and produces them on demand); the lift values are left to
 // k_sys_lift_elastic.  (Scattering node slots into the block view instead
 // evicts partly written lines: measured 4.5x the useful write traffic.)
 __global__ void k_sys_gather_elastic_b(int64_t nent, const int *__restrict__ bslot,

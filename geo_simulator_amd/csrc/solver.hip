@@ -1911,10 +1911,10 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
 
 // FP32 values of S A S, S = diag(sqrt(D^-1)), for the mixed solve's
 // two-launch recursion on scalar operators (built once per matrix, after the
-// Jacobi diagonal).  Nothing is
-// waited for here: whether every D^-1 entry was positive (sscaled 2, pending)
-// is read with the solve's first dot product, and when one was not the
-// recursion applies D^-1 to its own z instead (sscaled -1).
+// Jacobi diagonal).  Nothing is waited for here: whether every D^-1 entry
+// was positive (sscaled 2, pending) is read with the solve's first dot
+// product, and when one was not the recursion applies D^-1 to its own z
+// instead (sscaled -1).
 static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
   if (A->sscaled != 0) return;
   hipStream_t s = ctx->stream;

@@ -15,8 +15,8 @@ library (there is no CPU fallback).
 from ._lib import GeoFemError, LIB_PATH
 from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
 from .dof import DofManager
-from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, JacobiPreconditioner,
-                     SolverStats, norm)
+from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, IdentityPreconditioner,
+                     JacobiPreconditioner, LinearOperator, SolverStats, norm)
 from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        time_elastic_elements,
                        thermal_element_backward)
@@ -28,6 +28,7 @@ from .vtk import VtkWriter
 __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
+    "IdentityPreconditioner", "LinearOperator",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",

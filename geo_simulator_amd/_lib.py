@@ -66,6 +66,9 @@ class SolverStatsC(C.Structure):
 
 
 _vp = C.c_void_p
+
+# gs_operator_fn: int32 (*)(void *user)
+OPERATOR_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
 _pp = C.POINTER(C.c_void_p)
 _i64 = C.c_int64
 _i64p = C.POINTER(C.c_int64)
@@ -115,6 +118,9 @@ SIGNATURES = {
                               C.POINTER(SolverStatsC)]),
     "gs_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                                     C.POINTER(SolverStatsC)]),
+    "gs_cg_solve_operator": (C.c_int, [_vp, _i64, OPERATOR_FN, _vp, OPERATOR_FN, _vp, _vp,
+                                       _vp, _vp, C.POINTER(CgOpts), _vp,
+                                       C.POINTER(SolverStatsC)]),
     "gs_gmres_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), C.c_int32, _vp,
                                  C.POINTER(SolverStatsC)]),
     "gs_thermal_element_backward": (C.c_int, [_vp, _vp, _vp, _vp, _vp, _vp]),

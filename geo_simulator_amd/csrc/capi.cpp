@@ -15,6 +15,9 @@ namespace {
 template <class F> gs_status guard(gs_ctx *ctx, F &&f) {
   try {
     f();
+    // every entry point returns after its work is complete (the header's
+    // contract): callers read the outputs on their own streams
+    if (ctx && ctx->stream) GS_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx) ctx->err.clear();
     return GS_OK;
   } catch (const Error &e) {
@@ -888,6 +891,32 @@ gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
             GS_ERR_INVALID_ARG, "mixed precision needs inner_tolerance > 0");
     set_device(ctx);
     cg_solve(ctx, A, b, o, x, st);
+  });
+}
+
+gs_status gs_cg_solve_operator(gs_ctx *ctx, int64_t n, gs_operator_fn apply,
+                               void *apply_user, gs_operator_fn precond,
+                               void *precond_user, double *in, double *out,
+                               const double *b, const gs_cg_opts *opts, double *x,
+                               gs_solver_stats *st) {
+  if (!ctx || !st) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(n >= 0 && n < INT32_MAX, GS_ERR_INVALID_ARG, "invalid size");
+    require(apply || n == 0, GS_ERR_INVALID_ARG, "null operator");
+    require((b && x && in && out) || n == 0, GS_ERR_INVALID_ARG, "null vector");
+    gs_cg_opts o;
+    gs_cg_opts_default(&o);
+    if (opts) o = *opts;
+    require(o.precision == GS_PREC_FP64, GS_ERR_INVALID_ARG,
+            "a caller's operator is solved in FP64");
+    set_device(ctx);
+    if (n == 0) {  // ||b|| = 0 < 1e-25: converged at once
+      *st = gs_solver_stats{};
+      st->converged = 1;
+      return;
+    }
+    cg_solve_operator(ctx, n, apply, apply_user, precond, precond_user, in, out, b, o, x,
+                      st);
   });
 }
 

@@ -345,6 +345,10 @@ void sell_jacobi(hipStream_t s, const SellPattern &P, const gs_csr *A, double *d
 // ---- solver.hip ----
 void jacobi_diag_inv(hipStream_t s, const gs_csr *A, double *dinv);
 void csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y);
+void cg_solve_operator(gs_ctx *ctx, int64_t n, gs_operator_fn apply, void *apply_user,
+                       gs_operator_fn precond, void *precond_user, double *in,
+                       double *out, const double *b, const gs_cg_opts &o, double *x,
+                       gs_solver_stats *st);
 void cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
               const gs_cg_opts &o, double *x, gs_solver_stats *st);
 void bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b,

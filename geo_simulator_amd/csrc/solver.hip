@@ -2074,6 +2074,141 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
       std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 }
 
+// ---- CG on a caller's operator (src/linalg/iterative.rs:46-119) ------------
+// The vector work of the reference's loop on the device, one kernel per
+// statement; the dot products as fixed-order partial sums read by the host,
+// which also calls the operator and the preconditioner between them.
+__global__ __launch_bounds__(BS) void k_op_dot(int n, const double *__restrict__ a,
+                                               const double *__restrict__ b,
+                                               double *__restrict__ part) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
+    acc[0] += a[i] * b[i];
+  put_partials<1>(acc, part);
+}
+
+// x += alpha p; r -= alpha ap; partials r.r
+__global__ __launch_bounds__(BS) void k_op_xr(int n, double alpha, const double *__restrict__ p,
+                                              const double *__restrict__ ap,
+                                              double *__restrict__ x, double *__restrict__ r,
+                                              double *__restrict__ part) {
+  double acc[1] = {0.0};
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * ap[i];
+    r[i] = ri;
+    acc[0] += ri * ri;
+  }
+  put_partials<1>(acc, part);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(BS) void k_op_dir(int n, double beta, const double *__restrict__ z,
+                                               double *__restrict__ p) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS)
+    p[i] = z[i] + beta * p[i];
+}
+
+__global__ __launch_bounds__(BS) void k_op_copy(int n, const double *__restrict__ a,
+                                                double *__restrict__ b) {
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) b[i] = a[i];
+}
+
+void cg_solve_operator(gs_ctx *ctx, int64_t n64, gs_operator_fn apply, void *apply_user,
+                       gs_operator_fn precond, void *precond_user, double *in,
+                       double *out, const double *b, const gs_cg_opts &o, double *x,
+                       gs_solver_stats *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  *st = gs_solver_stats{};
+  auto finish = [&] {
+    st->solve_time =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  hipStream_t s = ctx->stream;
+  const int n = (int)n64;
+  const unsigned gv = grid_vec(n);
+  // the loop's vectors come from the pool, not the context's scratch buffer:
+  // the callbacks may call other entry points on this context (a CsrMatrix
+  // operator does), which use that buffer
+  auto rv = palloc<double>(ctx->pool, n), pv = palloc<double>(ctx->pool, n);
+  auto zv = palloc<double>(ctx->pool, n), partv = palloc<double>(ctx->pool, GMAX);
+  auto dotv = palloc<double>(ctx->pool, 2);
+  double *r = rv.get(), *p = pv.get(), *zc = zv.get(), *part = partv.get(), *dot = dotv.get();
+  auto dotp = [&](const double *u, const double *v) {
+    hipLaunchKernelGGL(k_op_dot, gv, BS, 0, s, n, u, v, part);
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, part, (int)gv, dot);
+    GS_LAUNCH_CHECK("k_op_dot");
+    double h;
+    read_dots(ctx, dot, &h, 1);
+    return h;
+  };
+  // y = F(v) through the caller's buffers; returns the buffer holding y
+  auto call = [&](gs_operator_fn f, void *user, const double *v) -> double * {
+    hipLaunchKernelGGL(k_op_copy, gv, BS, 0, s, n, v, in);
+    GS_LAUNCH_CHECK("k_op_copy");
+    GS_HIP(hipStreamSynchronize(s));
+    if (f(user) != 0) throw Error(GS_ERR_INVALID_ARG, "operator callback failed");
+    return out;
+  };
+  // the preconditioner: the caller's, or the identity (a copy)
+  auto prec = [&](const double *v) -> const double * {
+    if (!precond) {
+      hipLaunchKernelGGL(k_op_copy, gv, BS, 0, s, n, v, zc);
+      return zc;
+    }
+    return call(precond, precond_user, v);
+  };
+  const double bnorm = std::sqrt(dotp(b, b));
+  if (bnorm < 1e-25) {  // src/linalg/iterative.rs:60-68
+    hipLaunchKernelGGL((k_zero<double>), gv, BS, 0, s, n, x);
+    GS_LAUNCH_CHECK("k_zero");
+    GS_HIP(hipStreamSynchronize(s));
+    st->converged = 1;
+    finish();
+    return;
+  }
+  hipLaunchKernelGGL((k_zero<double>), gv, BS, 0, s, n, x);
+  hipLaunchKernelGGL(k_op_copy, gv, BS, 0, s, n, b, r);
+  const double *z = prec(r);
+  hipLaunchKernelGGL(k_op_copy, gv, BS, 0, s, n, z, p);
+  double rz = dotp(r, z);
+  long long it = 0;
+  bool converged = false;
+  double final_res = bnorm;
+  const bool brk_rel = o.relative_breakdown != 0;
+  while (it < o.max_iterations) {
+    const double *ap = call(apply, apply_user, p);
+    ++st->spmv_count;
+    const double pap = dotp(p, ap);
+    if (brk_rel ? (pap == 0.0 || !std::isfinite(rz / pap)) : std::fabs(pap) < 1e-30) break;
+    const double alpha = rz / pap;
+    hipLaunchKernelGGL(k_op_xr, gv, BS, 0, s, n, alpha, p, ap, x, r, part);
+    hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, part, (int)gv, dot);
+    GS_LAUNCH_CHECK("k_op_xr");
+    double rr;
+    read_dots(ctx, dot, &rr, 1);
+    const double rn = std::sqrt(rr);
+    final_res = rn;
+    if (rn < o.tolerance * bnorm || rn < o.abs_tolerance) {
+      converged = true;
+      break;
+    }
+    z = prec(r);
+    const double rz_new = dotp(r, z);
+    const double beta = rz_new / rz;
+    rz = rz_new;
+    hipLaunchKernelGGL(k_op_dir, gv, BS, 0, s, n, beta, z, p);
+    GS_LAUNCH_CHECK("k_op_dir");
+    ++it;
+  }
+  GS_HIP(hipStreamSynchronize(s));
+  st->iterations = it;
+  st->residual_norm = final_res;
+  st->relative_residual = bnorm > 1e-20 ? final_res / bnorm : 0.0;
+  st->converged = converged;
+  finish();
+}
+
 // BiCGSTAB::solve (src/linalg/iterative.rs:283-291): Jacobi when
 // use_preconditioner, FP64 only.
 void bicgstab_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,

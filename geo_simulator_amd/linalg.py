@@ -147,6 +147,55 @@ class JacobiPreconditioner:
         return np.asarray(r) * self.diag_inv.cpu().numpy()
 
 
+class IdentityPreconditioner:
+    """M = I (src/linalg/preconditioner.rs:65-71)."""
+
+    def apply(self, r):
+        return r.clone() if _is_torch(r) else np.array(r, dtype=np.float64)
+
+
+class LinearOperator:
+    """A caller's operator for `solve_with_operator` (the LinearOperator trait,
+    src/linalg/solver.rs:29-59): `apply(v)` maps a device tensor of `cols`
+    doubles to one of `rows`.  Any object with `apply` and `shape` works;
+    this class wraps a function."""
+
+    def __init__(self, fn, n):
+        self.fn = fn
+        self.shape = (int(n), int(n))
+
+    def apply(self, v):
+        return self.fn(v)
+
+    def rows(self):
+        return self.shape[0]
+
+    def cols(self):
+        return self.shape[1]
+
+
+def _operator_callback(obj, tin, tout, errors):
+    """An OPERATOR_FN that applies obj to the caller buffer tin into tout,
+    synchronously (the library waits for nothing else)."""
+    import torch
+
+    def run(_user):
+        try:
+            y = obj.apply(tin)
+            if not _is_torch(y):
+                y = torch.as_tensor(np.asarray(y, dtype=np.float64), device=tout.device)
+            if y.shape != tout.shape:
+                raise ValueError(f"operator returned shape {tuple(y.shape)}, "
+                                 f"expected {tuple(tout.shape)}")
+            tout.copy_(y)
+            torch.cuda.synchronize(tout.device)
+            return 0
+        except Exception as e:  # reported after the C call returns
+            errors.append(e)
+            return 1
+    return _lib.OPERATOR_FN(run)
+
+
 class ConjugateGradient:
     """Builder and solver with the reference's defaults
     (max_iterations 1000, tolerance 1e-8, abs_tolerance 1e-12, Jacobi on).
@@ -232,6 +281,47 @@ class ConjugateGradient:
     def solve(self, A, b):
         """Solver::solve (src/linalg/iterative.rs:123-131) -> (x, SolverStats)."""
         return _krylov_solve("gs_cg_solve", self.opts(), A, b)
+
+    def solve_with_operator(self, A, b, precond=None):
+        """Solver::solve_with_operator (src/linalg/solver.rs:112-120,
+        src/linalg/iterative.rs:133-140): the reference's CG loop on any
+        operator `A` (a CsrMatrix, a LinearOperator, or anything with `apply`
+        on device tensors and `shape`) with any preconditioner (`apply`; None
+        or IdentityPreconditioner: M = I).  The vector work and the dot
+        products run in the library on the device; A and precond are called
+        back on device tensors.  FP64; `with_preconditioner` does not apply
+        (precond decides).  -> (x, SolverStats); x is a numpy array for a host
+        right-hand side, else a device tensor."""
+        if self.precision != "fp64":
+            raise ValueError("solve_with_operator runs in FP64")
+        ctx = A.ctx if isinstance(A, CsrMatrix) else default_context()
+        n = int(A.shape[0])
+        if int(A.shape[1]) != n:
+            raise ValueError("CG needs a square operator")
+        host = not _is_torch(b)
+        bd = ctx.to_device(b)
+        if bd.numel() != n:
+            raise ValueError("right-hand side length differs from the operator rows")
+        x = ctx.empty(n)
+        tin, tout = ctx.empty(n), ctx.empty(n)
+        errors = []
+        f_apply = _operator_callback(A, tin, tout, errors)
+        f_prec = (_lib.OPERATOR_FN()  # NULL: the identity, applied by the library
+                  if precond is None or isinstance(precond, IdentityPreconditioner)
+                  else _operator_callback(precond, tin, tout, errors))
+        st = _lib.SolverStatsC()
+        o = self.opts()
+        ctx.sync_torch()
+        status = lib().gs_cg_solve_operator(ctx.handle, n, f_apply, None, f_prec, None,
+                                            dptr(tin), dptr(tout), dptr(bd), C.byref(o),
+                                            dptr(x), C.byref(st))
+        if errors:
+            raise errors[0]
+        ctx.check(status, "gs_cg_solve_operator")
+        stats = SolverStats(int(st.iterations), float(st.residual_norm),
+                            float(st.relative_residual), bool(st.converged),
+                            float(st.solve_time), 0, int(st.spmv_count))
+        return (x.cpu().numpy() if host else x), stats
 
     # Solver trait accessors (src/linalg/solver.rs:122-135)
     def set_tolerance(self, t):

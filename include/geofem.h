@@ -229,6 +229,29 @@ gs_status gs_gmres_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
                          const gs_cg_opts *opts, int32_t restart, double *x_dev,
                          gs_solver_stats *stats);
 
+/* Solver::solve_with_operator for ConjugateGradient (src/linalg/solver.rs:
+ * 112-120, src/linalg/iterative.rs:133-140 and the loop at :46-119): CG on a
+ * caller's linear operator with a caller's preconditioner (the traits
+ * LinearOperator, src/linalg/solver.rs:29-59, and Preconditioner,
+ * src/linalg/preconditioner.rs:7-16).  The operator and the preconditioner
+ * are callbacks on two caller-owned device buffers of n doubles: before
+ * each call the library writes the vector to transform into in_dev and
+ * waits for its own work; the callback writes its result into out_dev and
+ * returns 0 once that device work is complete (non-zero aborts the solve
+ * with GS_ERR_INVALID_ARG); a callback may call other entry points on the
+ * same context.  precond NULL is the IdentityPreconditioner.
+ * The loop, thresholds (1e-25, 1e-30) and the stopping tests are the
+ * reference's; uses max_iterations, tolerance, abs_tolerance and
+ * relative_breakdown of opts; precision must be GS_PREC_FP64 and
+ * use_preconditioner is ignored (the callback decides).  b_dev and x_dev in
+ * the operator's own numbering. */
+typedef int32_t (*gs_operator_fn)(void *user);
+gs_status gs_cg_solve_operator(gs_ctx *ctx, int64_t n, gs_operator_fn apply,
+                               void *apply_user, gs_operator_fn precond,
+                               void *precond_user, double *in_dev, double *out_dev,
+                               const double *b_dev, const gs_cg_opts *opts,
+                               double *x_dev, gs_solver_stats *stats);
+
 /* ---- adjoint (new surface; no reference counterpart) ----
  * Element backward of theta_e = log k_e (thermal) or log E_e (elastic):
  * grad_e = -lambda_e^T K_e(theta_e) u_e, with u the full solution (Dirichlet

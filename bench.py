@@ -170,7 +170,8 @@ def run_headline(G, args, nx, ny, nz):
     # the library's own stream (the stream the kernels run on).  Algorithmic
     # bytes, CSR-equivalent (int32 column indices and row pointers): FP64
     # q = A p reads the operator and p once and writes q; the FP32
-    # recursion's kernel (w = A r on the scaled operator, then p = r + beta p,
+    # recursion's kernel (w = A r on the scaled operator, 4-byte fixed-point
+    # values, then p = r + beta p,
     # s = w + beta s in place) reads the operator, r once, p and s, and
     # writes p and s.
     bytes64 = nnz * 12 + n * 20
@@ -583,8 +584,9 @@ def main():
                      "unit": "GB/s", "frac": spmv["fp32_GBs"] / HBM_PEAK_GBS,
                      "traffic": spmv["fp32_traffic"],
                      "kernel": "k_cg2_dir<float> (SELL-64 SpMV of the FP32 recursion on "
-                               "the Jacobi-scaled operator + direction update p, s and "
-                               "p.s, r.p partials; dominant kernel of the mixed path)",
+                               "the Jacobi-scaled operator, int32 fixed-point values, + "
+                               "direction update p, s and p.s, r.p partials; dominant "
+                               "kernel of the mixed path)",
                      "algorithmic_bytes": spmv["fp32_bytes"]},
         "elastic_48x48x24": elastic,
         "adjoint": adjoint,

@@ -231,12 +231,12 @@ struct gs_csr {
   gs::DArr<float> sval32;                 // SELL values, FP32 (mixed precision)
   gs::DArr<double> sdinv;                 // Jacobi inverse diagonal, SELL order
   gs::DArr<float> sdinv32;
-  // mixed solve: S = sqrt(D^-1) (SELL order) and FP32 S A S; sscaled 0
+  // mixed solve: S = sqrt(D^-1) (SELL order) and fixed-point S A S; sscaled 0
   // unknown, 2 built but not yet checked, 1 usable, -1 not applicable (a
   // non-positive D^-1 entry, flagged in sscale_bad)
   int sscaled = 0;
   gs::DArr<double> sscale;
-  gs::DArr<float> sval32s;
+  gs::DArr<int> svalq;  // S A S in int32 fixed point (value * 2^30)
   gs::DArr<int> sscale_bad;
   // SpMV launch shape (0 = automatic); see gs_csr_set_spmv_shape
   int spmv_lanes = 0, spmv_grid = 0, spmv_flags = 0;

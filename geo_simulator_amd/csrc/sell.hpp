@@ -79,6 +79,13 @@ template <> struct SellVec<float> {
 template <> struct SellVec<double> {
   typedef double t __attribute__((ext_vector_type(4)));
 };
+template <> struct SellVec<int> {
+  typedef int t __attribute__((ext_vector_type(4)));
+};
+
+// Fixed-point operator values (int32, value * 2^30): the row sums are taken
+// on the integers in FP64 and scaled once.
+constexpr double SELL_Q30 = 0x1p-30;
 
 // Value loads.  An operator larger than the Infinity Cache streams its values
 // past the caches (non-temporal loads), which leaves the cached lines to the
@@ -97,11 +104,13 @@ __device__ __forceinline__ V load_vals(const T *p) {
 // is exact in FP64, so an FP32 row is rounded once, when it is stored (this
 // keeps the FP32 recursion's rounding lower; the memory-bound kernel's time
 // does not change).
-template <typename T, class IDX>
-__device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
+// TV: the value type -- the vector type T by default, or int32 fixed point
+// (SELL_Q30) for the mixed solve's scaled operator.
+template <typename T, class IDX, typename TV>
+__device__ __forceinline__ T sell_lane(const IDX idx, const TV *__restrict__ val,
                                        const T *__restrict__ x, int c0, int c1,
                                        int lane, int base_row) {
-  using V = typename SellVec<T>::t;
+  using V = typename SellVec<TV>::t;
   double acc = 0.0;
   int c = c0;
   auto step = [&](const V &v, int j0, int j1, int j2, int j3) {
@@ -149,18 +158,20 @@ __device__ __forceinline__ T sell_lane(const IDX idx, const T *__restrict__ val,
       step(v0, i0[0], i0[1], i0[2], i0[3]);
     }
   }
+  if constexpr (std::is_same<TV, int>::value) acc *= SELL_Q30;
   return (T)acc;
 }
 
 // One lane's three dot products (block row base_row + lane) over chunks
 // [c0, c1), two chunks in flight.  Each row sums its entries in ascending
 // column order, as the scalar engine does, and in FP64 as it does.
-template <typename T, class IDX>
-__device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val,
+template <typename T, class IDX, typename TV>
+__device__ __forceinline__ void b3_lane(const IDX idx, const TV *__restrict__ val,
                                         const T *__restrict__ x, int c0, int c1,
                                         int lane, int base_row, T (&out)[3]) {
+  static_assert(!std::is_same<TV, int>::value, "fixed point: scalar view only");
   double acc[3] = {0.0, 0.0, 0.0};
-  auto load = [&](int c, int &col, bool &ok, T (&v)[9]) {
+  auto load = [&](int c, int &col, bool &ok, TV (&v)[9]) {
     const size_t e = (size_t)c * SELL_C + lane;
     if constexpr (IDX::wide) {
       const int j = idx.col[e];
@@ -172,9 +183,9 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
       col = ok ? base_row + j : base_row;
     }
 #pragma unroll
-    for (int m = 0; m < 9; ++m) v[m] = load_vals<IDX::stream, T>(val + ((size_t)c * 9 + m) * SELL_C + lane);
+    for (int m = 0; m < 9; ++m) v[m] = load_vals<IDX::stream, TV>(val + ((size_t)c * 9 + m) * SELL_C + lane);
   };
-  auto fma3 = [&](int col, bool ok, const T (&v)[9]) {
+  auto fma3 = [&](int col, bool ok, const TV (&v)[9]) {
     const double x0 = x[3 * col], x1 = x[3 * col + 1], x2 = x[3 * col + 2];
 #pragma unroll
     for (int r = 0; r < 3; ++r) {
@@ -189,7 +200,7 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
   for (; c + 2 <= c1; c += 2) {
     int ca, cb2;
     bool oa, ob;
-    T va[9], vb[9];
+    TV va[9], vb[9];
     load(c, ca, oa, va);
     load(c + 1, cb2, ob, vb);
     fma3(ca, oa, va);
@@ -198,7 +209,7 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
   if (c < c1) {
     int ca;
     bool oa;
-    T va[9];
+    TV va[9];
     load(c, ca, oa, va);
     fma3(ca, oa, va);
   }
@@ -209,11 +220,11 @@ __device__ __forceinline__ void b3_lane(const IDX idx, const T *__restrict__ val
 // Calls epi(row, (A x)[row]) for every row, from the row's lane.  Every
 // thread of the workgroup returns (no early exit), so callers may follow
 // with workgroup-wide reductions.
-template <typename T, class IDX, class Epi>
+template <typename T, class IDX, typename TV, class Epi>
 __device__ __forceinline__ void sell_rows(int n, int nslices,
                                           const int *__restrict__ cb,
                                           const IDX idx,
-                                          const T *__restrict__ val,
+                                          const TV *__restrict__ val,
                                           const T *__restrict__ x, Epi &epi) {
   const int nb = (nslices + SELL_WPB - 1) / SELL_WPB;
   const int G = gridDim.x, i = blockIdx.x;
@@ -234,15 +245,17 @@ __device__ __forceinline__ void sell_rows(int n, int nslices,
     const int s = b * SELL_WPB + w;
     if (s < nslices) {
       if constexpr (IDX::bs == 3) {
-        const int brow = s * SELL_C + lane;
-        T acc[3];
-        b3_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C, acc);
+        if constexpr (!std::is_same<TV, int>::value) {  // no fixed-point block view
+          const int brow = s * SELL_C + lane;
+          T acc[3];
+          b3_lane<T, IDX, TV>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C, acc);
 #pragma unroll
-        for (int r = 0; r < 3; ++r)
-          if (3 * brow + r < n) epi(3 * brow + r, acc[r]);
+          for (int r = 0; r < 3; ++r)
+            if (3 * brow + r < n) epi(3 * brow + r, acc[r]);
+        }
       } else {
         const int row = s * SELL_C + lane;
-        const T acc = sell_lane<T>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C);
+        const T acc = sell_lane<T, IDX, TV>(idx, val, x, cb[s], cb[s + 1], lane, s * SELL_C);
         if (row < n) epi(row, acc);
       }
     }

@@ -38,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "internal.hpp"
@@ -321,7 +322,8 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
 }
 
 // ---- two-kernel pass of the scaled FP32 recursion -----------------------------
-// CG on S A S (S = D^-1/2: Jacobi folded into the operator, so z = r) with
+// CG on S A S (S = D^-1/2: Jacobi folded into the operator, so z = r; its
+// values in int32 fixed point, 2^-30 resolution, FP32 vectors) with
 // the direction update moved into the SpMV kernel.  A p_k is not formed from
 // p_k (whose entries other lanes are still writing) but by linearity,
 // s_k = A p_k = A r_k + beta_k s_{k-1}, the recurrence of Chronopoulos and
@@ -333,9 +335,9 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
 //       w = A r; p = r + beta p; s = w + beta s; partials p.s, r.p
 //   KB  alpha = r.p / p.s; x += alpha p; r -= alpha s; partials r.r
 // Two launches and two reductions per pass instead of three of each.
-template <typename T, class IDX>
+template <typename T, class IDX, typename TV>
 __global__ __launch_bounds__(BS) void k_cg2_dir(int n, int ns, const int *__restrict__ cb,
-                                                const IDX idx, const T *__restrict__ val,
+                                                const IDX idx, const TV *__restrict__ val,
                                                 const T *__restrict__ r,
                                                 T *__restrict__ p, T *__restrict__ sv,
                                                 CgState *__restrict__ st, long long it,
@@ -579,18 +581,22 @@ __global__ __launch_bounds__(BS) void k_fcg_expand(int n, float *__restrict__ xs
   }
 }
 
-// FP32 values of S A S, S = diag(ssc) (scalar SELL order), from the FP32
-// values: entry (i, j) becomes s_i a_ij s_j (in FP64, rounded once); padding
-// 0.  One wavefront per slice, blocks of SELL_WPB slices in XCD-contiguous
+// Fixed-point values of S A S, S = diag(ssc) (scalar SELL order), from the
+// FP64 values: entry (i, j) becomes round(s_i a_ij s_j 2^30) as int32; padding
+// 0.  With a positive Jacobi diagonal the scaled entries of a symmetric
+// positive definite operator lie in [-1, 1] (the diagonal is 1); an entry
+// outside (-2, 2) sets *bad, and the solve then keeps the unscaled operator.
+// One wavefront per slice, blocks of SELL_WPB slices in XCD-contiguous
 // ranges as the SpMV, so the gathered s_j stay in the XCD's L2.
 template <class IDX>
 __global__ __launch_bounds__(BS) void k_scale_sell(int n, int ns, const int *__restrict__ cb,
                                                    const IDX idx,
-                                                   const float *__restrict__ val,
+                                                   const double *__restrict__ val,
                                                    const double *__restrict__ ssc,
-                                                   float *__restrict__ out) {
+                                                   int *__restrict__ out, int *bad) {
   static_assert(IDX::bs == 1, "scalar SELL only");
-  typedef float F4 __attribute__((ext_vector_type(4)));
+  typedef double D4 __attribute__((ext_vector_type(4)));
+  typedef int Q4 __attribute__((ext_vector_type(4)));
   const int nb = (ns + SELL_WPB - 1) / SELL_WPB;
   const int G = gridDim.x, gi = blockIdx.x;
   int first = gi, last = nb, stride = G;
@@ -601,6 +607,7 @@ __global__ __launch_bounds__(BS) void k_scale_sell(int n, int ns, const int *__r
     stride = (G >> 3) + (xcd < (G & 7) ? 1 : 0);
   }
   const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  bool out_of_range = false;
   for (int b = first; b < last; b += stride) {
     const int sl = b * SELL_WPB + w;
     if (sl >= ns) continue;
@@ -609,7 +616,7 @@ __global__ __launch_bounds__(BS) void k_scale_sell(int n, int ns, const int *__r
     const double si = live ? ssc[row] : 0.0;
     for (int c = cb[sl]; c < cb[sl + 1]; ++c) {
       const size_t e0 = (size_t)c * SELL_CHUNK + lane * SELL_CW;
-      const F4 v = *(const F4 *)(val + e0);
+      const D4 v = *(const D4 *)(val + e0);
       int cc[4];
       bool ok[4];
       if constexpr (IDX::wide) {
@@ -629,15 +636,19 @@ __global__ __launch_bounds__(BS) void k_scale_sell(int n, int ns, const int *__r
           cc[j] = sl * SELL_C + i4[j];
         }
       }
-      F4 o;
+      Q4 o;
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const double sj = ok[j] ? ssc[cc[j]] : 0.0;
-        o[j] = ok[j] ? (float)(si * (double)v[j] * sj) : 0.0f;
+        const double t = ok[j] ? si * v[j] * sj * 0x1p30 : 0.0;
+        const bool fits = t > -0x1.fffp30 && t < 0x1.fffp30;  // false for NaN
+        out_of_range |= !fits;
+        o[j] = fits ? (int)rint(t) : 0;
       }
-      *(F4 *)(out + e0) = o;
+      *(Q4 *)(out + e0) = o;
     }
   }
+  if (out_of_range) *bad = 1;
 }
 
 // Timing set-up of the two-launch pass: a pass that continues a direction.
@@ -989,15 +1000,15 @@ void enqueue_pass(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *z
 
 // One pass of the scaled recursion: KA (SpMV + direction), KB (step).
 // part_r holds the r.r partials of the pass's residual (grid_fat(n) of them).
-template <typename T>
-void enqueue_pass2(hipStream_t s, const SellOp &o, const T *val, T *x, T *r, T *p,
+template <typename T, typename TV>
+void enqueue_pass2(hipStream_t s, const SellOp &o, const TV *val, T *x, T *r, T *p,
                    T *sv, CgState *st, long long it, double *part1, double *part_r) {
   const int n = o.n();
   const unsigned gf = grid_fat(n);
   with_idx(*o.P, [&](auto idx) {
-    hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
+    hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx), TV>), o.grid, BS, 0, s, n, o.ns(),
                        o.P->cb.get(), idx, val, r, p, sv, st, it, part_r, (int)gf, part1);
-  }, sizeof(T));
+  }, sizeof(TV));
   hipLaunchKernelGGL((k_cg2_step<T>), gf, FBS, 0, s, n, it, x, r, p, sv, st, part1,
                      o.grid, part_r);
 }
@@ -1122,15 +1133,17 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
     A->sscale_bad = DArr<int>();
     if (A->sscaled < 0) {
       A->sscale = DArr<double>();
-      A->sval32s = DArr<float>();
+      A->svalq = DArr<int>();
     }
   }
-  // Jacobi folded into the FP32 operator (S A S, S = D^-1/2) when every
-  // D^-1 entry is positive: the recursion then needs no z vector
+  // Jacobi folded into the recursion's operator (S A S, S = D^-1/2, fixed
+  // point) when every D^-1 entry is positive and every scaled entry fits:
+  // the recursion then needs no z vector
   const bool scaled = precond && A->sscaled == 1;
   const double *ssc = scaled ? A->sscale.get() : nullptr;
   const float *dinv32 = precond && !scaled ? A->sdinv32.get() : nullptr;
-  const float *val32 = scaled ? A->sval32s.get() : A->sval32.get();
+  const float *val32 = A->sval32.get();
+  const int *valq = A->svalq.get();
   const double *val = A->sval.get();
   const double bnorm = std::sqrt(h[0]);
   res.bnorm = bnorm;
@@ -1211,8 +1224,8 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
       const long long batch = bt.next(hs.rnorm, hs.iteration, tgt, left);
       for (long long i = 0; i < batch; ++i) {
         if (scaled)
-          enqueue_pass2<float>(s, o, val32, xs, iw.r, iw.p, iw.q, iw.st, hs.iteration + i,
-                               iw.part1, iw.part2);
+          enqueue_pass2<float, int>(s, o, valq, xs, iw.r, iw.p, iw.q, iw.st,
+                                    hs.iteration + i, iw.part1, iw.part2);
         else
           enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
                               hs.iteration + i, iw.part1, iw.part2);
@@ -1881,7 +1894,7 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
     A->sell_force32 = force32;
     A->sval = DArr<double>();
     A->sval32 = DArr<float>();
-    A->sval32s = DArr<float>();
+    A->svalq = DArr<int>();
     A->sscale = DArr<double>();
     A->sscale_bad = DArr<int>();
     A->sscaled = 0;
@@ -1909,12 +1922,15 @@ static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond) {
   }
 }
 
-// FP32 values of S A S, S = diag(sqrt(D^-1)), for the mixed solve's
+// Fixed-point values of S A S, S = diag(sqrt(D^-1)), for the mixed solve's
 // two-launch recursion on scalar operators (built once per matrix, after the
-// Jacobi diagonal).  Nothing is waited for here: whether every D^-1 entry
-// was positive (sscaled 2, pending) is read with the solve's first dot
-// product, and when one was not the recursion applies D^-1 to its own z
-// instead (sscaled -1).
+// Jacobi diagonal).  The scaled entries lie in [-1, 1], where int32 at 2^-30
+// resolution holds them 2^7 times finer than FP32's 24-bit significand at 1
+// in the same 4 bytes: the recursion's operator rounding, which delays its
+// convergence, drops accordingly.  Nothing is waited for here: whether every
+// D^-1 entry was positive and every scaled entry fit (sscaled 2, pending) is
+// read with the solve's first dot product, and when not the recursion
+// applies D^-1 to its own z on the FP32 operator instead (sscaled -1).
 static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
   if (A->sscaled != 0) return;
   hipStream_t s = ctx->stream;
@@ -1926,7 +1942,7 @@ static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
   const int n = (int)P.n;
   auto ssc = palloc<double>(ctx->pool, P.n);
   auto bad = palloc<int>(ctx->pool, 1);
-  auto v = palloc<float>(ctx->pool, P.nvals);
+  auto v = palloc<int>(ctx->pool, P.nvals);
   hipLaunchKernelGGL((k_zero<int>), 1, BS, 0, s, 1, bad.get());
   hipLaunchKernelGGL(k_sqrt_dinv, grid_vec(n), BS, 0, s, n, A->sdinv.get(), ssc.get(),
                      bad.get());
@@ -1936,11 +1952,11 @@ static void ensure_scaled(gs_ctx *ctx, gs_csr *A) {
   with_idx(P, [&](auto idx) {
     if constexpr (decltype(idx)::bs == 1)
       hipLaunchKernelGGL((k_scale_sell<decltype(idx)>), g, BS, 0, s, n, (int)P.nslices,
-                         P.cb.get(), idx, A->sval32.get(), ssc.get(), v.get());
+                         P.cb.get(), idx, A->sval.get(), ssc.get(), v.get(), bad.get());
   }, 4);
   GS_LAUNCH_CHECK("k_scale_sell");
   A->sscale = std::move(ssc);
-  A->sval32s = std::move(v);
+  A->svalq = std::move(v);
   A->sscale_bad = std::move(bad);
   A->sscaled = 2;
 }
@@ -2315,7 +2331,7 @@ static void resolve_scaled(gs_ctx *ctx, gs_csr *A) {
   A->sscale_bad = DArr<int>();
   if (bad) {
     A->sscale = DArr<double>();
-    A->sval32s = DArr<float>();
+    A->svalq = DArr<int>();
   }
 }
 
@@ -2379,8 +2395,8 @@ double time_spmv(gs_ctx *ctx, const gs_csr *Ac, int precision, int reps) {
         // p and s are rewritten in place: beta = 0 (gamma_prev = 0) keeps
         // them bounded across repetitions
         with_idx(*o.P, [&](auto idx) {
-          hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx)>), o.grid, BS, 0, s, (int)n,
-                             o.ns(), o.P->cb.get(), idx, (const T *)A->sval32s.get(),
+          hipLaunchKernelGGL((k_cg2_dir<T, decltype(idx), int>), o.grid, BS, 0, s, (int)n,
+                             o.ns(), o.P->cb.get(), idx, (const int *)A->svalq.get(),
                              (const T *)p, (T *)q, (T *)wv, st, 0, part_r,
                              (int)grid_fat(n), part);
         }, sizeof(T));

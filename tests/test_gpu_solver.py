@@ -217,6 +217,34 @@ def test_mixed_diagonal_operator(signs):
     assert np.linalg.norm(x - b / d) / np.linalg.norm(b / d) < 1e-12
 
 
+def test_mixed_scaled_entry_out_of_fixed_point_range():
+    """The scaled operator S A S is held in int32 fixed point for entries in
+    (-2, 2), which holds for every symmetric positive definite matrix.  A
+    symmetric indefinite matrix with a positive diagonal and |a_ij| = 3
+    sqrt(a_ii a_jj) does not fit, and the solve keeps the unscaled FP32
+    operator.  Spectrum {4, -2, 1} (2x2 blocks [[1, 3], [3, 1]] and unit
+    rows): FP64 CG ends in three steps; the mixed solve's outer line search
+    assumes a positive definite matrix, so it need not converge, but what it
+    reports must be the true residual of what it returns."""
+    nb, n1 = 200, 600
+    blk = sp.csr_matrix(np.array([[1.0, 3.0], [3.0, 1.0]]))
+    A = sp.block_diag([blk] * nb + [sp.identity(n1)]).tocsr()
+    M = G.CsrMatrix.from_scipy(A)
+    b = np.random.default_rng(29).standard_normal(A.shape[0])
+    xr = spla.spsolve(A.tocsc(), b)
+    cg = lambda p: (G.ConjugateGradient().with_precision(p).with_tolerance(1e-12)
+                    .with_abs_tolerance(0.0).with_max_iterations(2000))
+    x, st = cg("fp64").solve(M, b)
+    assert st.converged and st.iterations <= 4
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-12
+    x, st = cg("mixed").solve(M, b)
+    true = np.linalg.norm(b - A @ x) / np.linalg.norm(b)
+    assert np.all(np.isfinite(x))
+    assert abs(true - st.relative_residual) <= 1e-3 * true + 1e-15
+    assert st.converged == (true <= 1e-12 * (1 + 1e-3))
+    assert true < 1e-6
+
+
 def test_mixed_reports_true_residual_and_counts():
     mesh, dof, exact = W.thermal_box(G, 8, 8, 6)
     K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)

@@ -869,7 +869,7 @@ gs_status gs_cg_opts_default(gs_cg_opts *o) {
   o->abs_tolerance = 1e-12;
   o->use_preconditioner = 1;
   o->precision = GS_PREC_FP64;
-  o->inner_tolerance = 1e-2;
+  o->inner_tolerance = 0.0;
   o->max_inner_iterations = 0;
   o->max_outer_iterations = 50;
   o->relative_breakdown = 0;
@@ -887,8 +887,8 @@ gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
     if (opts) o = *opts;
     require(o.precision == GS_PREC_FP64 || o.precision == GS_PREC_MIXED,
             GS_ERR_INVALID_ARG, "unknown precision");
-    require(o.precision == GS_PREC_FP64 || o.inner_tolerance > 0.0,
-            GS_ERR_INVALID_ARG, "mixed precision needs inner_tolerance > 0");
+    require(o.precision == GS_PREC_FP64 || o.inner_tolerance >= 0.0,
+            GS_ERR_INVALID_ARG, "mixed precision needs inner_tolerance >= 0");
     set_device(ctx);
     cg_solve(ctx, A, b, o, x, st);
   });

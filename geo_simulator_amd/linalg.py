@@ -203,7 +203,8 @@ class ConjugateGradient:
     Extensions: `with_precision("mixed")` runs FP64 flexible CG around an
     FP32 Jacobi-PCG recursion (residual, solution and search directions in
     FP64; the FP32 recursion reduces its residual by `inner_tolerance` between
-    FP64 steps), stopping on the same FP64 criterion ||b - A x|| < tol ||b||
+    FP64 steps; 0, the default, picks 1e-3 for the Jacobi-scaled fixed-point
+    operator of a scalar matrix and 1e-2 otherwise), stopping on the same FP64 criterion ||b - A x|| < tol ||b||
     (or < abs_tol)."""
 
     name = "ConjugateGradient"
@@ -214,7 +215,7 @@ class ConjugateGradient:
         self.abs_tolerance = 1e-12
         self.use_preconditioner = True
         self.precision = "fp64"
-        self.inner_tolerance = 1e-2
+        self.inner_tolerance = 0.0
         self.max_inner_iterations = 0
         self.max_outer_iterations = 50
         self.relative_breakdown = False

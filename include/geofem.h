@@ -175,10 +175,11 @@ typedef struct {
   int32_t precision;         /* gs_precision, default GS_PREC_FP64 */
   double inner_tolerance;    /* mixed: an FP64 flexible-CG step each time the
                                 FP32 recursion's residual fell by this
-                                factor (default 1e-2; on scalar operators
-                                with a positive Jacobi diagonal the
-                                recursion runs on D^-1/2 A D^-1/2 and the
-                                residual is ||D^-1/2 r||) */
+                                factor (default 0: automatic, 1e-3 on the
+                                scaled operator below, else 1e-2; on scalar
+                                operators with a positive Jacobi diagonal
+                                the recursion runs on D^-1/2 A D^-1/2 and
+                                the residual is ||D^-1/2 r||) */
   int64_t max_inner_iterations; /* mixed: cap on FP32 passes between FP64
                                    steps (0 = none) */
   int64_t max_outer_iterations; /* mixed: cap on FP64 steps (default 50;

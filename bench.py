@@ -53,6 +53,22 @@ def sync():
     torch.cuda.synchronize()
 
 
+def host_cpu():
+    """The host the CPU baseline ran on: CPU model, logical CPUs the OS
+    reports, and those this process may use."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
+
+
 def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
     """Oracle (C restatement of the reference algorithm) on the same workload:
     parallel element loop, serial triplet merge / CSR / Dirichlet / CG, as in
@@ -549,7 +565,8 @@ def main():
                           f"({per_iter*1e3:.1f} ms/iter) and extrapolated to the GPU "
                           f"FP64 iteration count {results['fp64']['iterations']}"),
                "bicgstab_s_per_pass": bicg_per_iter,
-               "gmres30_s_per_step": gm_per_step}
+               "gmres30_s_per_step": gm_per_step,
+               "host": host_cpu()}
         log(f"[bench] CPU baseline {total:.1f} s (asm {t_asm:.1f} s, {per_iter*1e3:.1f} ms/iter)")
 
     mixed, fp64 = results["mixed"], results["fp64"]

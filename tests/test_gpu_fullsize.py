@@ -70,3 +70,40 @@ def test_fullsize_thermal_cg_matches_oracle(oracle, c2):
     xm = xm.cpu().numpy()
     err_m = np.linalg.norm(xm - exact) / np.linalg.norm(exact)
     assert err_m < 2e-6
+
+
+def test_fullsize_elastic_system_bitexact_and_closed_form(oracle):
+    """The 48x48x24 elastic column in metres (1,383,123 DOFs, 111,247,603
+    nonzeros after the elimination; src/fem/assembly.rs:389-437, 906-937,
+    168-230): the fused system equals the oracle's bit for bit, and both
+    precisions reach the closed-form u_z of
+    benchmarks/elasticity_halfspace_gravity.rs:23-27 (exact at the nodes)
+    with u_x = u_y = 0.  The oracle's serial CG is not run at this size (about
+    two minutes); its parity with the GPU CG is pinned on the thermal box."""
+    lx, ly, lz = 1e6, 1e6, 6e5
+    E, nu, rho, g = 1e11, 0.25, 3300.0, 9.81
+    xyz, conn = oracle.generate_cube(48, 48, 24, lx, ly, lz)
+    mask, vals = oracle.elastic_dirichlet(xyz, lx, ly)
+    Ko, fo = oracle.apply_dirichlet(oracle.assemble_elastic(xyz, conn, E, nu),
+                                    oracle.assemble_gravity_load(xyz, conn, rho, [0.0, 0.0, -g]),
+                                    mask, vals)
+    mesh = G.Mesh(xyz, conn)
+    dof = G.DofManager(mesh.num_nodes(), 3)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    K, f = G.Assembler.assemble_elasticity_system(mesh, dof, G.IsotropicElasticity(E, nu),
+                                                  rho, [0.0, 0.0, -g])
+    A = K.to_scipy()
+    assert A.shape == (1383123, 1383123) and A.nnz == 111247603
+    assert np.array_equal(A.indptr, Ko.indptr)
+    assert np.array_equal(A.indices, Ko.indices)
+    assert np.array_equal(A.data, Ko.data)
+    assert np.array_equal(f.cpu().numpy(), np.asarray(fo))
+    del A, Ko, fo
+    uz = oracle.elastic_analytic_uz(xyz[:, 2], lz, rho, g, E, nu)
+    for prec in ("fp64", "mixed"):
+        u, st = (G.ConjugateGradient().with_precision(prec).with_max_iterations(20000)
+                 .with_tolerance(1e-8).solve(K, f))
+        assert st.converged, prec
+        u = u.cpu().numpy().reshape(-1, 3)
+        assert np.linalg.norm(u[:, 2] - uz) / np.linalg.norm(uz) < 1e-8, prec
+        assert np.abs(u[:, :2]).max() < 1e-8 * np.abs(uz).max(), prec

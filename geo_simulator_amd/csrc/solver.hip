@@ -321,7 +321,7 @@ __global__ __launch_bounds__(FBS) void k_cg_beta(
   }
 }
 
-// ---- two-kernel pass of the scaled FP32 recursion -----------------------------
+// ---- two-kernel pass of the scaled recursion (FP32 vectors) -------------------
 // CG on S A S (S = D^-1/2: Jacobi folded into the operator, so z = r; its
 // values in int32 fixed point, 2^-30 resolution, FP32 vectors) with
 // the direction update moved into the SpMV kernel.  A p_k is not formed from

@@ -302,8 +302,8 @@ gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
  * GS_PREC_FP64: the FP64 pass's SpMV (q = A p and p.q partials).
  * GS_PREC_MIXED: the FP32 recursion's SpMV -- on a scalar operator whose
  * Jacobi diagonal is positive, the two-launch pass's first kernel (w = A r on
- * the Jacobi-scaled FP32 values, then p and s updated in place and p.s, r.p
- * partials); otherwise the FP32 q = A p. */
+ * the Jacobi-scaled values in int32 fixed point, FP32 vectors, then p and s
+ * updated in place and p.s, r.p partials); otherwise the FP32 q = A p. */
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms);
 /* Time the elastic element kernel (K_e and the gravity load of every

@@ -63,6 +63,7 @@ struct CgState {
   double rtol;   // mixed inner: pause once ||r|| < rtol * rbase
   double gam[2];       // scaled recursion: r.r at the start of pass k in gam[k & 1]
   long long fresh_it;  // scaled recursion: the pass that starts a new direction
+  long long alpha_it;  // the last pass whose K2 made its update (written by K2 only)
   long long iteration, max_iter;
   int stop, converged, pause, mixed;
   int brk_rel;   // break down only on p.Ap == 0 or a non-finite step
@@ -192,9 +193,12 @@ __global__ __launch_bounds__(BS) void k_spmv(int n, const int *__restrict__ rp,
 // The two vector kernels of a pass run on at most VGMAX workgroups of FBS
 // threads.  Each workgroup first sums the previous kernel's partials itself
 // (same order everywhere, so every workgroup takes the same decisions) and
-// workgroup 0 records the scalars.  No workgroup reads a state word that the
-// leader of the same kernel writes, except `iteration`, which the leader only
-// advances when the pass completes -- so seeing it+1 also means "live".
+// workgroup 0 records the scalars.  A kernel's entry test reads only words
+// that an earlier kernel wrote, or words whose change by its own leader leads
+// every workgroup to the same decision anyway: K3 runs when K2 of the same
+// pass recorded its update in alpha_it, a word K3 never writes (K3's leader
+// advances `iteration` and may set `pause` while other workgroups of K3 are
+// still starting, so those two cannot gate K3).
 
 // K2 (pass it): pq; break if |pq| < 1e-30 (src/linalg/iterative.rs:85);
 // alpha = rz/pq; x += alpha p; r -= alpha q; z = D^-1 r; partials r.r, r.z.
@@ -221,6 +225,7 @@ __global__ __launch_bounds__(FBS) void k_cg_alpha(
       else st->stop = 1;
     } else {
       st->alpha = alpha_d;
+      st->alpha_it = it;
     }
   }
   if (brk) return;
@@ -274,8 +279,7 @@ template <typename T>
 __global__ __launch_bounds__(FBS) void k_cg_beta(
     int n, long long it, const T *__restrict__ z, T *__restrict__ p,
     CgState *__restrict__ st, const double *__restrict__ part_in, int np_in) {
-  const long long at = st->iteration;
-  if (!((at == it && !st->stop && !st->pause) || at == it + 1)) return;
+  if (st->alpha_it != it) return;
   double tot[2];
   sum_partials<2>(part_in, np_in, tot);
   const double rn = sqrt(tot[0]);
@@ -454,6 +458,7 @@ __global__ void k_state_setup(CgState *__restrict__ st, long long max_iter,
                               double tol, double abs_tol, int mixed, double rtol,
                               int brk_rel) {
   CgState s{};
+  s.alpha_it = -1;
   s.brk_rel = brk_rel;
   s.max_iter = max_iter;
   s.tol = tol;

@@ -102,7 +102,6 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
                                            double poisson,
                                            const double *E_per_elem_dev,
                                            gs_csr **out);
-/* Assembler::assemble_gravity_load (src/fem/assembly.rs:906-937). */
 /* ElementMatrix::thermal_mass (src/fem/element.rs:74-105: M_e = int rho c_p
  * N^T N dV with the 5-point rule) for every element, scattered like the
  * stiffness (DOF = node id; the reference has the element routine only).
@@ -122,6 +121,7 @@ gs_status gs_assemble_thermal_mass(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *
 gs_status gs_apply_neumann_bcs(gs_ctx *ctx, gs_mesh *mesh, int64_t n_faces,
                                const int64_t *element_ids, const int32_t *local_faces,
                                const double *fluxes, double *f_dev);
+/* Assembler::assemble_gravity_load (src/fem/assembly.rs:906-937). */
 gs_status gs_assemble_gravity_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double density,
                                    const double gravity[3], double *f_dev);

@@ -339,57 +339,6 @@ def run_elastic(G, args):
     return out
 
 
-def fd_central(solve_obs, theta, e_list, h, obs_ref):
-    """Central differences of L = 0.5 ||y(theta) - obs||^2 in theta_e,
-    evaluated as 0.5 sum (y+ - y-)(y+ + y- - 2 obs) / (2h): the same
-    difference quotient, without the cancellation of forming L first.
-    solve_obs returns (y - y0, y0) at the observations."""
-    import torch
-    out = []
-    for e in e_list:
-        tp = theta.clone()
-        tm = theta.clone()
-        tp[e] += h
-        tm[e] -= h
-        dp, y0 = solve_obs(tp)
-        dm, _ = solve_obs(tm)
-        dl = 0.5 * torch.sum((dp - dm) * (2.0 * y0 + dp + dm - 2.0 * obs_ref)).item()
-        out.append(dl / (tp[e] - tm[e]).item())  # 2h as represented
-    return np.array(out)
-
-
-def corrected_solver(G, assemble, theta0, y0, cg):
-    """theta -> y(theta) - y0 for theta near theta0, solved from the residual,
-    K(theta) dy = f - K(theta) y0, so the difference of two nearby solutions
-    keeps its relative accuracy.
-
-    The residual is formed without cancellation.  K depends linearly on the
-    element parameters k_e = exp(theta_e) (the loads do not depend on them),
-    so with w_e = expm1(theta_e - theta0_e)
-        f - K(theta) y0 = r0 - (K(theta) - K(theta0)) y0,  r0 = f - K(theta0) y0,
-        (K(theta) - K(theta0)) y0 = (1/S) (r0 - r_S),
-    where r_S is the residual of y0 under theta_S = theta0 + log1p(S w) with S
-    scaling the largest |S w_e| to 1/4: an O(1) change of the perturbed
-    elements, whose residual difference carries full relative accuracy,
-    while f - K(theta) y0 formed directly loses the digits that the 1e-6
-    relative change of k_e shares with the rest of the row.  The solve uses
-    the assembled K(theta); dy is solved to the solver's tolerance of
-    ||f - K(theta) y0||."""
-    import torch
-    K0, f0 = assemble(theta0)
-    r0 = f0 - K0.apply(y0)
-
-    def solve(theta):
-        w = torch.expm1(theta - theta0)
-        S = 0.25 / float(w.abs().max())
-        KS, fS = assemble(theta0 + torch.log1p(S * w))
-        d = r0 - (fS - KS.apply(y0))
-        K, _ = assemble(theta)
-        dy, _ = cg.solve(K, r0 - d / S)
-        return dy
-    return solve
-
-
 def run_adjoint(G, args):
     """Inversion gradients: thermal d L / d log k_e and elastic d L / d log E_e
     from the adjoint (forward solve, reverse solve with the same SPD operator,
@@ -398,6 +347,7 @@ def run_adjoint(G, args):
     from geo_simulator_amd import workloads as W
     from geo_simulator_amd.adjoint import (ElasticProblem, ThermalProblem, elastic_solve,
                                            thermal_solve)
+    from geo_simulator_amd.gradcheck import corrected_solver, fd_central, fd_direction
     dev = "cuda"
     out = {}
     cases = []
@@ -463,7 +413,7 @@ def run_adjoint(G, args):
                     mesh, dof, mat1, 3300.0, [0.0, 0.0, -9.81],
                     E_per_elem=_t.exp(th).contiguous())
             th0 = torch.tensor(theta, device=dev)
-            corr = corrected_solver(G, asm, th0, y0_full, fdcg)
+            corr = corrected_solver(asm, th0, y0_full, fdcg)
             solve_obs = lambda th: (corr(th)[obs_t], y0_full[obs_t])
             rng = np.random.Generator(np.random.PCG64(77))
             elems = np.sort(rng.choice(len(g), min(args.fd_elems, len(g)), replace=False))
@@ -471,10 +421,7 @@ def run_adjoint(G, args):
             with torch.no_grad():
                 fd = fd_central(solve_obs, th0, elems, 1e-6, y_obs)
                 v = rng.standard_normal(len(g))
-                vt = torch.tensor(v, device=dev)
-                dp, y0 = solve_obs(th0 + 1e-6 * vt)
-                dm, _ = solve_obs(th0 - 1e-6 * vt)
-                fdv = 0.5 * torch.sum((dp - dm) * (2.0 * y0 + dp + dm - 2.0 * y_obs)).item() / 2e-6
+                fdv = fd_direction(solve_obs, th0, torch.tensor(v, device=dev), 1e-6, y_obs)
             scale = np.abs(g).max()
             rel = np.abs(fd - g[elems]) / np.maximum(np.abs(fd), 1e-3 * scale)
             rel_pure = np.abs(fd - g[elems]) / np.abs(fd)

@@ -1,0 +1,138 @@
+"""Generate the full-size inversion-gradient fixtures from the FP64 oracle.
+
+Writes tests/golden/adjoint_thermal_64x64x32.npz and
+tests/golden/adjoint_elastic_48x48x24.npz.  Each holds the gradient of
+L = 0.5 ||y_obs - y(theta)||^2 computed end to end by the oracle (the C
+restatement of the reference's assembly, Dirichlet elimination and serial
+Jacobi-PCG): observation solve at theta_true, forward solve at theta, adjoint
+solve K_bc lambda = dL/dy (zero on Dirichlet DOFs), element backward
+-lambda_e^T K_e y_e.  Inputs are the seeded workloads of
+geo_simulator_amd/workloads.py (configs 4 and 5), solved to rel-res 1e-12
+(abs_tol 0), the tolerance the GPU tests use.
+
+A full gradient (786,432 / 331,776 doubles) is not stored: the fixture keeps
+the gradient on 4,096 seeded elements, its inner products with four seeded
+standard-normal vectors over all elements, its largest magnitude, the loss,
+and the oracle's iteration counts.
+
+    python tests/golden/make_adjoint_golden.py [thermal|elastic]
+
+The thermal case takes about a minute on 8 cores, the elastic one about a
+quarter of an hour (serial CG at 1.38M DOFs, as in the reference).
+"""
+from __future__ import annotations
+
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+N_SUBSET = 4096
+N_PROJ = 4
+SUBSET_SEED = 91
+PROJ_SEED = 100
+
+
+def subset_and_projections(ne):
+    """The seeded element subset and projection vectors (shared with the tests)."""
+    rng = np.random.Generator(np.random.PCG64(SUBSET_SEED))
+    sub = np.sort(rng.choice(ne, size=min(N_SUBSET, ne), replace=False))
+    vs = [np.random.Generator(np.random.PCG64(PROJ_SEED + k)).standard_normal(ne)
+          for k in range(N_PROJ)]
+    return sub, np.stack(vs)
+
+
+def _solve(O, Kb, rhs):
+    x, st = O.cg_solve(Kb, rhs, 200000, 1e-12, 0.0)
+    assert st.converged, st
+    return x, st
+
+
+def thermal():
+    import geo_simulator_amd as G
+    from geo_simulator_amd import workloads as W
+    from oracle import oracle as O
+    mesh, dof, theta, theta_true, obs = W.thermal_adjoint_case(G)
+    xyz, conn = mesh.nodes, mesh.elements
+    mask = dof.dirichlet_mask
+    vals = dof.dirichlet_values
+    f = O.assemble_thermal_load(xyz, conn, 1e-6)
+    nthr = os.cpu_count() or 1
+
+    def system(k):
+        return O.apply_dirichlet(O.assemble_thermal(xyz, conn, 1.0, k_elem=k, nthreads=nthr),
+                                 f, mask, vals)
+
+    Kt, ft = system(np.exp(theta_true))
+    y_true, _ = _solve(O, Kt, ft)
+    del Kt
+    k = np.exp(theta)
+    Kb, fb = system(k)
+    y, st_f = _solve(O, Kb, fb)
+    rhs = np.zeros(len(y))
+    rhs[obs] = y[obs] - y_true[obs]
+    rhs[mask == 1] = 0.0
+    s = np.linalg.norm(rhs)
+    lam, st_a = _solve(O, Kb, rhs / s)
+    lam *= s
+    g = O.thermal_element_backward(xyz, conn, k, y, lam)
+    return g, 0.5 * float(np.sum((y[obs] - y_true[obs]) ** 2)), st_f, st_a
+
+
+def elastic():
+    import geo_simulator_amd as G
+    from geo_simulator_amd import workloads as W
+    from oracle import oracle as O
+    mesh, dof, theta, theta_true, obs = W.elastic_adjoint_case(G)
+    xyz, conn = mesh.nodes, mesh.elements
+    mask = dof.dirichlet_mask
+    vals = dof.dirichlet_values
+    f = O.assemble_gravity_load(xyz, conn, 3300.0, [0.0, 0.0, -9.81])
+    nthr = os.cpu_count() or 1
+
+    def system(E):
+        return O.apply_dirichlet(O.assemble_elastic(xyz, conn, 1.0, 0.25, E_elem=E,
+                                                    nthreads=nthr), f, mask, vals)
+
+    Kt, ft = system(np.exp(theta_true))
+    y_true, _ = _solve(O, Kt, ft)
+    del Kt
+    E = np.exp(theta)
+    Kb, fb = system(E)
+    y, st_f = _solve(O, Kb, fb)
+    rhs = np.zeros(len(y))
+    rhs[obs] = y[obs] - y_true[obs]
+    rhs[mask == 1] = 0.0
+    # the reverse solve on the unit-norm right-hand side (the library's
+    # scale-free convention; CG is scale invariant with abs_tol 0)
+    s = np.linalg.norm(rhs)
+    lam, st_a = _solve(O, Kb, rhs / s)
+    lam *= s
+    g = O.elastic_element_backward(xyz, conn, E, 0.25, y, lam)
+    return g, 0.5 * float(np.sum((y[obs] - y_true[obs]) ** 2)), st_f, st_a
+
+
+def write(name, fn):
+    t0 = time.perf_counter()
+    g, loss, st_f, st_a = fn()
+    sub, vs = subset_and_projections(len(g))
+    path = os.path.join(HERE, f"adjoint_{name}.npz")
+    np.savez_compressed(path, subset=sub.astype(np.int64), g_subset=g[sub],
+                        projections=vs @ g, max_abs=np.abs(g).max(), loss=loss,
+                        forward_iterations=st_f.iterations,
+                        adjoint_iterations=st_a.iterations, n_elems=len(g))
+    print(f"{path}: {time.perf_counter() - t0:.0f} s, loss {loss:.6e}, "
+          f"it {st_f.iterations}/{st_a.iterations}, max|g| {np.abs(g).max():.3e}", flush=True)
+
+
+if __name__ == "__main__":
+    which = sys.argv[1:] or ["thermal", "elastic"]
+    if "thermal" in which:
+        write("thermal_64x64x32", thermal)
+    if "elastic" in which:
+        write("elastic_48x48x24", elastic)

@@ -72,6 +72,23 @@ def test_fullsize_thermal_cg_matches_oracle(oracle, c2):
     assert err_m < 2e-6
 
 
+def test_fullsize_thermal_fp64_bar(oracle, c2):
+    """The plain FP64 path's accuracy bar at full size: the GPU FP64 CG and
+    the oracle's serial CG (src/linalg/iterative.rs:46-131), both to rel-res
+    1e-13 with abs_tol 0, give solutions within 1e-10 relative L2."""
+    Ko, fo, K, f, exact = c2
+    xo, so = oracle.cg_solve(Ko, fo, 20000, 1e-13, 0.0)
+    x, st = (G.ConjugateGradient().with_max_iterations(20000).with_tolerance(1e-13)
+             .with_abs_tolerance(0.0).solve(K, f))
+    assert st.converged and so.converged
+    x = x.cpu().numpy()
+    rel = np.linalg.norm(x - xo) / np.linalg.norm(xo)
+    print(f"[fullsize] FP64 CG vs oracle CG at 1e-13: rel L2 {rel:.2e} "
+          f"({st.iterations} / {so.iterations} iterations)", flush=True)
+    assert rel <= 1e-10, rel
+    assert abs(st.iterations - so.iterations) <= 5
+
+
 def test_fullsize_elastic_system_bitexact_and_closed_form(oracle):
     """The 48x48x24 elastic column in metres (1,383,123 DOFs, 111,247,603
     nonzeros after the elimination; src/fem/assembly.rs:389-437, 906-937,

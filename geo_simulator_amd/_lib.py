@@ -116,6 +116,8 @@ SIGNATURES = {
     "gs_cg_opts_default": (C.c_int, [C.POINTER(CgOpts)]),
     "gs_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                               C.POINTER(SolverStatsC)]),
+    "gs_cg_solve_noalloc": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
+                                      C.POINTER(SolverStatsC)]),
     "gs_bicgstab_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                                     C.POINTER(SolverStatsC)]),
     "gs_cg_solve_operator": (C.c_int, [_vp, _i64, OPERATOR_FN, _vp, OPERATOR_FN, _vp, _vp,

@@ -300,6 +300,7 @@ gs_status gs_ctx_create(int device, gs_ctx **out) {
     GS_HIP(hipSetDevice(device));
     GS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     GS_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+    noalloc_setup(device);  // the no-allocation CG's scratch address on this device
   });
   if (st != GS_OK) {
     create_error = "gs_ctx_create: " + ctx->err;
@@ -891,6 +892,19 @@ gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
             GS_ERR_INVALID_ARG, "mixed precision needs inner_tolerance >= 0");
     set_device(ctx);
     cg_solve(ctx, A, b, o, x, st);
+  });
+}
+
+gs_status gs_cg_solve_noalloc(gs_ctx *ctx, const gs_csr *A, const double *b,
+                              const gs_cg_opts *opts, double *x, gs_solver_stats *st) {
+  if (!ctx || !A || !st) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((b && x) || A->rows == 0, GS_ERR_INVALID_ARG, "null vector");
+    gs_cg_opts o;
+    gs_cg_opts_default(&o);
+    if (opts) o = *opts;
+    set_device(ctx);
+    cg_solve_noalloc(ctx, A, b, o, x, st);
   });
 }
 

@@ -38,6 +38,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 #include <vector>
 
@@ -2043,6 +2044,110 @@ void csr_spmv(gs_ctx *ctx, const gs_csr *Ac, const double *x, double *y) {
   GS_HIP(hipStreamSynchronize(s));
 }
 
+// Stats of a finished solve.
+static void fill_stats(const PcgResult &r, std::chrono::steady_clock::time_point t0,
+                       gs_solver_stats *st) {
+  st->iterations = r.iterations;
+  st->residual_norm = r.rnorm;
+  st->relative_residual = r.bnorm > 1e-20 ? r.rnorm / r.bnorm : 0.0;
+  st->converged = r.converged;
+  st->outer_iterations = r.outer;
+  st->spmv_count = r.spmv;
+  st->solve_time =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// The FP64 solve on a prepared SELL view, all scratch carved from base (the
+// layout of fp64_layout): b and x gathered into the SELL order, the PCG, x
+// scattered back.
+static Layout fp64_layout(int64_t n) {
+  Layout L;
+  L.add<double>(n);
+  L.add<double>(n);
+  carve_pcg<double>(nullptr, L, n);
+  return L;
+}
+
+static PcgResult fp64_solve(gs_ctx *ctx, const gs_csr *A, const double *b,
+                            const gs_cg_opts &o, double *x, char *base, bool precond) {
+  hipStream_t s = ctx->stream;
+  const int64_t n = A->rows;
+  Layout L;
+  const size_t ob = L.add<double>(n), ox = L.add<double>(n);
+  double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
+  const PcgWork<double> w = carve_pcg<double>(base, L, n);
+  const int *vp = A->sell->vperm.get();
+  permute_gather(s, n, b, vp, bi);
+  const PcgResult r = pcg_core<double>(ctx, sell_op(A), A->sval.get(),
+                                       precond ? A->sdinv.get() : nullptr, bi, xi,
+                                       o.max_iterations, o.tolerance, o.abs_tolerance,
+                                       o.relative_breakdown != 0, w);
+  permute_scatter(s, n, xi, vp, x);
+  GS_LAUNCH_CHECK("cg_solve");
+  GS_HIP(hipStreamSynchronize(s));
+  return r;
+}
+
+// ---- CG without allocation ------------------------------------------------------
+// gs_cg_solve_noalloc's scratch: the FP64 solve's vectors (b and x in SELL
+// order, r, z, p, q), its partial sums and its state, for operators of up to
+// NOALLOC_MAX_ROWS rows, in one file-scope device array (about 96 MiB, held
+// by every device the library runs on until the process ends).  Host code
+// never uses the array's name as an address: noalloc_setup looks up each
+// device's copy once, with that device current, and the solve passes that
+// address to its kernels.  Every call on one device shares that device's
+// copy, so calls that overlap on one device race.
+constexpr int64_t NOALLOC_MAX_ROWS = int64_t(1) << 21;
+constexpr size_t noalloc_bytes() {
+  return 6 * align256((size_t)NOALLOC_MAX_ROWS * sizeof(double)) +
+         align256(2 * GRID_MAX * sizeof(double)) + align256(2 * GMAX * sizeof(double)) +
+         align256(sizeof(CgState));
+}
+static __device__ __align__(256) char scratch[noalloc_bytes()];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static std::mutex noalloc_mu;
+static char *noalloc_ptr[NOALLOC_MAX_DEVICES] = {};
+
+void noalloc_setup(int device) {
+  if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;
+  std::lock_guard<std::mutex> g(noalloc_mu);
+  if (noalloc_ptr[device]) return;
+  void *ptr = nullptr;
+  GS_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+  noalloc_ptr[device] = (char *)ptr;
+}
+
+void cg_solve_noalloc(gs_ctx *ctx, const gs_csr *A, const double *b, const gs_cg_opts &o,
+                      double *x, gs_solver_stats *st) {
+  const auto t0 = std::chrono::steady_clock::now();
+  memset(st, 0, sizeof(*st));
+  if (o.precision != GS_PREC_FP64)
+    throw Error(GS_ERR_INVALID_ARG, "the no-allocation CG runs in FP64 only");
+  if (A->rows != A->cols) throw Error(GS_ERR_INVALID_ARG, "CG needs a square matrix");
+  const int64_t n = A->rows;
+  if (n > NOALLOC_MAX_ROWS)
+    throw Error(GS_ERR_TOO_LARGE, "the no-allocation CG takes at most 2^21 rows");
+  if (n == 0) {  // ||b|| = 0 < 1e-25: converged at once
+    st->converged = 1;
+    return;
+  }
+  const bool precond = o.use_preconditioner != 0;
+  const bool force32 = (A->spmv_flags & GS_SPMV_INDEX32) != 0;
+  if ((A->spmv_flags & GS_SPMV_CSR) || !A->sell || A->sell_force32 != force32 ||
+      A->sell->bs != sell_block(A) || !A->sval || (precond && !A->sdinv))
+    throw Error(GS_ERR_INVALID_ARG,
+                "the no-allocation CG needs a matrix whose solver arrays exist: one "
+                "returned by a gs_assemble_*_system call or solved once with gs_cg_solve");
+  char *base = ctx->device >= 0 && ctx->device < NOALLOC_MAX_DEVICES
+                   ? noalloc_ptr[ctx->device]
+                   : nullptr;
+  if (!base) throw Error(GS_ERR_HIP, "no-allocation scratch not set up on this device");
+  if (fp64_layout(n).bytes > noalloc_bytes())
+    throw Error(GS_ERR_TOO_LARGE, "the no-allocation CG's scratch is too small");
+  fill_stats(fp64_solve(ctx, A, b, o, x, base, precond), t0, st);
+}
+
 void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
               const gs_cg_opts &o, double *x, gs_solver_stats *st) {
   gs_csr *A = const_cast<gs_csr *>(Ac);  // derived caches only
@@ -2060,44 +2165,29 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   if (A->spmv_flags & GS_SPMV_CSR)
     throw Error(GS_ERR_INVALID_ARG, "CG runs on the SELL engine only");
   ensure_sell(ctx, A, mixed, precond);
-  if (mixed && precond) ensure_scaled(ctx, A);
+  if (!mixed) {
+    char *base = ctx_scratch(ctx, fp64_layout(n).bytes);
+    fill_stats(fp64_solve(ctx, A, b, o, x, base, precond), t0, st);
+    return;
+  }
+  if (precond) ensure_scaled(ctx, A);
   Layout L;
   const size_t ob = L.add<double>(n), ox = L.add<double>(n);
   // size the scratch with a dry run of the same layout
   Layout dry = L;
-  if (mixed) {
-    for (int i = 0; i < 5; ++i) dry.add<double>(n);
-    dry.add<double>(8);
-    dry.add<float>(n);
-    carve_pcg<float>(nullptr, dry, n);
-  } else {
-    carve_pcg<double>(nullptr, dry, n);
-  }
+  for (int i = 0; i < 5; ++i) dry.add<double>(n);
+  dry.add<double>(8);
+  dry.add<float>(n);
+  carve_pcg<float>(nullptr, dry, n);
   char *base = ctx_scratch(ctx, dry.bytes);
   double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
   const int *vp = A->sell->vperm.get();
   permute_gather(s, n, b, vp, bi);
-  PcgResult r;
-  if (!mixed) {
-    const PcgWork<double> w = carve_pcg<double>(base, L, n);
-    r = pcg_core<double>(ctx, sell_op(A), A->sval.get(),
-                         precond ? A->sdinv.get() : nullptr, bi, xi,
-                         o.max_iterations, o.tolerance, o.abs_tolerance,
-                         o.relative_breakdown != 0, w);
-  } else {
-    r = pcg_fcg(ctx, A, bi, xi, precond, o, base, L);
-  }
+  const PcgResult r = pcg_fcg(ctx, A, bi, xi, precond, o, base, L);
   permute_scatter(s, n, xi, vp, x);
   GS_LAUNCH_CHECK("cg_solve");
   GS_HIP(hipStreamSynchronize(s));
-  st->iterations = r.iterations;
-  st->residual_norm = r.rnorm;
-  st->relative_residual = r.bnorm > 1e-20 ? r.rnorm / r.bnorm : 0.0;
-  st->converged = r.converged;
-  st->outer_iterations = r.outer;
-  st->spmv_count = r.spmv;
-  st->solve_time =
-      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  fill_stats(r, t0, st);
 }
 
 // ---- CG on a caller's operator (src/linalg/iterative.rs:46-119) ------------

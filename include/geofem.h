@@ -208,6 +208,20 @@ gs_status gs_cg_opts_default(gs_cg_opts *opts);
 gs_status gs_cg_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
                       const gs_cg_opts *opts, double *x_dev,
                       gs_solver_stats *stats);
+/* gs_cg_solve in FP64 without allocating device memory (no reference
+ * counterpart: for callers that cannot allocate, such as code under HIP graph
+ * capture or tight serving loops).  The solver's vectors, partial sums and
+ * state live in a fixed device array of the library (about 96 MiB on every
+ * device the library runs on, held until the process ends), so A may have at
+ * most 2^21 rows (GS_ERR_TOO_LARGE otherwise).  A must already hold its
+ * solver arrays: a matrix returned by gs_assemble_*_system, or one solved
+ * once with gs_cg_solve under the same gs_csr_set_spmv_shape flags
+ * (GS_ERR_INVALID_ARG otherwise, as for a precision other than FP64).
+ * Results equal gs_cg_solve's bit for bit.  Calls on one device share the
+ * array: calls that overlap on one device race. */
+gs_status gs_cg_solve_noalloc(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
+                              const gs_cg_opts *opts, double *x_dev,
+                              gs_solver_stats *stats);
 /* BiCGSTAB::new().with_max_iterations().with_tolerance().with_abs_tolerance()
  * .with_preconditioner() and Solver::solve (src/linalg/iterative.rs:149-291),
  * for general (nonsymmetric) square matrices: the same loop, stopping tests

@@ -20,6 +20,8 @@ from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, IdentityPrec
 from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        time_elastic_elements,
                        thermal_element_backward)
+from .rheology import (BackwardEuler, MaxwellViscoelasticity, NewtonianViscosity,
+                       StressHistory, TimeStepStats, update_stresses_maxwell)
 from .runtime import Context, default_context
 from .fields import FieldData, ScalarField, VectorField
 from .boundary import BoundaryConditions, BoundaryFace, NeumannBC, Tet10Face
@@ -33,7 +35,8 @@ __all__ = [
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",
     "BoundaryConditions", "BoundaryFace", "NeumannBC", "Tet10Face",
-    "default_context",
+    "default_context", "NewtonianViscosity", "MaxwellViscoelasticity", "StressHistory",
+    "update_stresses_maxwell", "BackwardEuler", "TimeStepStats",
 ]
 
 

@@ -107,6 +107,16 @@ SIGNATURES = {
     "gs_assemble_elasticity_system": (C.c_int, [_vp, _vp, _vp, C.c_double,
                                                 C.c_double, _vp, C.c_double, _dp,
                                                 _pp, _vp]),
+    "gs_assemble_viscosity_stiffness": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double,
+                                                  _vp, _vp, _pp]),
+    "gs_assemble_viscosity_system": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double, _vp,
+                                               _vp, _vp, _pp, _vp]),
+    "gs_assemble_maxwell_viscoelastic": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double,
+                                                   C.c_double, C.c_double, _vp, _pp, _vp]),
+    "gs_assemble_maxwell_system": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, _vp, _vp, _pp, _vp]),
+    "gs_update_stresses_maxwell": (C.c_int, [_vp, _vp, _vp, C.c_double, C.c_double,
+                                             C.c_double, C.c_double, _vp, _vp, _vp, _vp]),
     "gs_csr_info": (C.c_int, [_vp, _i64p, _i64p, _i64p]),
     "gs_csr_export": (C.c_int, [_vp, _vp, _i64p, _i64p, _dp]),
     "gs_csr_from_host": (C.c_int, [_vp, _i64, _i64, _i64, _i64p, _i64p, _dp, _pp]),

@@ -239,6 +239,82 @@ class Assembler:
         return CsrMatrix(ctx, h), fbc
 
 
+    # ---- viscous flow and Maxwell viscoelasticity ------------------------------
+    @staticmethod
+    def assemble_viscosity_stiffness_parallel(mesh, dof_mgr, material, mu_per_elem=None,
+                                              zeta_per_elem=None, ctx=None):
+        """src/fem/assembly.rs:646-694 (material: NewtonianViscosity; per-element
+        viscosity / bulk viscosity: additive overload)."""
+        ctx = ctx or default_context()
+        me = _elem_array(ctx, mu_per_elem, mesh.num_elements())
+        ze = _elem_array(ctx, zeta_per_elem, mesh.num_elements())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_viscosity_stiffness(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), material.viscosity,
+            material.bulk_viscosity, dptr(me), dptr(ze), C.byref(h)),
+            "assemble_viscosity_stiffness_parallel")
+        return CsrMatrix(ctx, h)
+
+    @staticmethod
+    def assemble_viscosity_system(mesh, dof_mgr, material, f=None, mu_per_elem=None,
+                                  zeta_per_elem=None, ctx=None):
+        """assemble_viscosity_stiffness_parallel + apply_dirichlet_bcs on the
+        device (src/linalg/picard.rs:264-268) -> (K_bc, f_bc tensor); f is
+        the right-hand side before the elimination (None: zero)."""
+        ctx = ctx or default_context()
+        me = _elem_array(ctx, mu_per_elem, mesh.num_elements())
+        ze = _elem_array(ctx, zeta_per_elem, mesh.num_elements())
+        fd = None if f is None else ctx.to_device(f)
+        if fd is not None and fd.numel() != dof_mgr.total_dofs():
+            raise ValueError("load vector length differs from the DOF count")
+        fbc = ctx.empty(dof_mgr.total_dofs())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_viscosity_system(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), material.viscosity,
+            material.bulk_viscosity, dptr(me), dptr(ze), dptr(fd), C.byref(h), dptr(fbc)),
+            "assemble_viscosity_system")
+        return CsrMatrix(ctx, h), fbc
+
+    @staticmethod
+    def assemble_maxwell_viscoelastic_parallel(mesh, dof_mgr, material, dt, device=False,
+                                               ctx=None):
+        """src/fem/assembly.rs:959-1020 -> (K, f_history); the stress history
+        is mesh.stress_history (a rheology.StressHistory)."""
+        from .rheology import _stress_of
+        ctx = ctx or default_context()
+        sig = _stress_of(mesh, ctx)
+        f = ctx.empty(dof_mgr.total_dofs())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_maxwell_viscoelastic(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), material.youngs_modulus,
+            material.poisson_ratio, material.viscosity, float(dt), dptr(sig), C.byref(h),
+            dptr(f)), "assemble_maxwell_viscoelastic_parallel")
+        return CsrMatrix(ctx, h), _out(ctx, f, device)
+
+    @staticmethod
+    def assemble_maxwell_system(mesh, dof_mgr, material, dt, f_ext, ctx=None):
+        """BackwardEuler::step's assembly (src/fem/time_integration.rs:99-110):
+        Maxwell K and f_history, f = f_ext + f_history, apply_dirichlet_bcs,
+        on the device -> (K_bc, f_bc tensor)."""
+        from .rheology import _stress_of
+        ctx = ctx or default_context()
+        sig = _stress_of(mesh, ctx)
+        fe = ctx.to_device(f_ext)
+        if fe.numel() != dof_mgr.total_dofs():
+            raise ValueError("load vector length differs from the DOF count")
+        fbc = ctx.empty(dof_mgr.total_dofs())
+        h = C.c_void_p()
+        ctx.sync_torch()
+        ctx.check(lib().gs_assemble_maxwell_system(
+            ctx.handle, mesh.handle(ctx), dof_mgr.handle(ctx), material.youngs_modulus,
+            material.poisson_ratio, material.viscosity, float(dt), dptr(sig), dptr(fe),
+            C.byref(h), dptr(fbc)), "assemble_maxwell_system")
+        return CsrMatrix(ctx, h), fbc
+
+
 def thermal_element_backward(mesh, k_per_elem, u, lam, ctx=None):
     """grad_e = -lambda_e^T K_e(k_e) u_e = dL/dlog k_e (device tensors)."""
     ctx = ctx or default_context()

@@ -730,7 +730,9 @@ __global__ void k_compact_flags(int64_t n, const int *__restrict__ flag,
 
 
 
-// Per DOF row (internal numbering): load (k_gather_load's sum), then either
+// Per DOF row (internal numbering): load (k_gather_load's sum; none when Fe
+// is null) plus the caller's vector fext (caller numbering, nullable: the
+// f_ext + f_history of src/fem/time_integration.rs:101-110), then either
 // the constrained row (c e_i, f = g c) or the lift f -= K_ji g_i in the
 // plan's order; the RHS lands at caller index dperm[r]; Jacobi inverse
 // diagonal (src/linalg/preconditioner.rs:37-51) in SELL order.
@@ -739,6 +741,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
                           const int *__restrict__ cptr,
                           const int *__restrict__ contrib,
                           const double *__restrict__ Fe,
+                          const double *__restrict__ fext,
                           const uint8_t *__restrict__ dir,
                           const double *__restrict__ g,
                           const int *__restrict__ lptr,
@@ -759,7 +762,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
   const int c = (int)(r - DPN * node);
   double f = 0.0;
   const int s = diag_slot[node];
-  if (s >= 0) {
+  if (s >= 0 && Fe) {
     const int k1 = cptr[s + 1];
     for (int k0 = cptr[s]; k0 < k1; k0 += GATHER_GROUP) {  // loads in groups
       int t[GATHER_GROUP];
@@ -779,6 +782,8 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
         if (k0 + u < k1) f += x[u];
     }
   }
+  const int64_t out = dperm ? dperm[r] : r;
+  if (fext) f = fext[out] + f;
   double dv = 0.0;
   const int d = diag[r];
   if (dir[r]) {
@@ -793,7 +798,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
     for (int q = lptr[r]; q < lptr[r + 1]; ++q) f -= liftv[q] * g[lcol[q]];
     dv = d >= 0 ? (val ? val[d] : sval[pos[d]]) : 0.0;
   }
-  fbc[dperm ? dperm[r] : r] = f;
+  fbc[out] = f;
   const double di = fabs(dv) > 1e-14 ? 1.0 / dv : 1.0;
   const int si = siperm[r];
   sdinv[si] = di;
@@ -1209,7 +1214,8 @@ void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
 }
 
 void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, const double *Fe, double *val, double *sval,
+                   const double *Ke, const double *Fe, const double *fext, double *val,
+                   double *sval,
                    float *sval32, const double *liftv, const int *dperm,
                    double *fbc, double *sdinv, float *sdinv32,
                    const unsigned long long *dmax) {
@@ -1229,13 +1235,13 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
   const int *sip = P.sell->siperm.get(), *pos = P.sell->pos.get();
   if (P.dpn == 1) {
     hipLaunchKernelGGL((k_sys_rhs<1>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
-                       p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
+                       p.contrib_ptr.get(), p.contrib.get(), Fe, fext, P.dir.get(),
                        P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
                        P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
                        dperm, fbc, sdinv, sdinv32);
   } else {
     hipLaunchKernelGGL((k_sys_rhs<3>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
-                       p.contrib_ptr.get(), p.contrib.get(), Fe, P.dir.get(),
+                       p.contrib_ptr.get(), p.contrib.get(), Fe, fext, P.dir.get(),
                        P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
                        P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
                        dperm, fbc, sdinv, sdinv32);

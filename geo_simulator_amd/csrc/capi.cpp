@@ -39,9 +39,10 @@ void require(bool ok, gs_status code, const char *msg) {
 void set_device(gs_ctx *ctx) { GS_HIP(hipSetDevice(ctx->device)); }
 
 // Element-kernel failure flags -> reference panic conditions as statuses.
-void check_flags(int flags) {
+void check_flags(int flags, int mat = GS_MAT_ELASTIC) {
   if (flags & 4)
-    throw Error(GS_ERR_INVALID_MATERIAL, "Young's modulus must be positive");
+    throw Error(GS_ERR_INVALID_MATERIAL, mat == GS_MAT_VISCOUS ? "Viscosity must be positive"
+                                                                 : "Young's modulus must be positive");
   if (flags & 1) throw Error(GS_ERR_SINGULAR_JACOBIAN, "Singular Jacobian");
   if (flags & 2)
     throw Error(GS_ERR_NONFINITE, "non-finite Jacobian determinant");
@@ -92,39 +93,39 @@ gs_csr *assemble_thermal_K(gs_ctx *ctx, gs_mesh *m, double k,
   return K;
 }
 
-gs_csr *assemble_elastic_K(gs_ctx *ctx, gs_mesh *m, double E, double nu,
-                           const double *E_elem, bool with_load, double rho,
-                           const double g[3], double *f_dev) {
+// K of an elastic-family law (full structural pattern, DOF 3 node + c) and,
+// when the kind has a load, its assembled load in caller numbering (f_dev).
+gs_csr *assemble_dof3_K(gs_ctx *ctx, gs_mesh *m, const ElasticKind &kind, double *f_dev) {
   build_dof3_pattern(ctx, m);
   const NodePattern &p = m->pat;
   hipStream_t s = ctx->stream;
+  const bool with_load = kind.load != GS_LOAD_NONE;
   auto Ke = dalloc<double>(m->n_elems * 900);
   DArr<double> Fe;
   if (with_load) Fe = dalloc<double>(m->n_elems * 30);
   auto flag = dalloc<int>(1);
   GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
-  const double g0[3] = {0.0, 0.0, 0.0};
-  launch_elastic_elements(s, m->n_elems, m->xyz.get(), m->conn.get(), E, nu,
-                          E_elem, rho, g ? g : g0, with_load, Ke.get(),
-                          with_load ? Fe.get() : nullptr, flag.get());
-  check_flags(read_flag_and_clear(ctx, flag.get()));
+  launch_elastic_family(s, m->n_elems, m->xyz.get(), m->conn.get(), kind, Ke.get(),
+                        with_load ? Fe.get() : nullptr, flag.get());
+  check_flags(read_flag_and_clear(ctx, flag.get()), kind.mat);
   auto val = dalloc<double>(m->dof3.nnz);
   gather_elastic(s, p, m->n_nodes, Ke.get(), val.get());
-  if (with_load) gather_gravity_load(s, p, m->n_nodes, Fe.get(), nullptr, f_dev);
+  if (with_load) gather_gravity_load(s, p, m->n_nodes, Fe.get(), m->perm.get(), f_dev);
   build_perm3(ctx, m);
   GS_HIP(hipStreamSynchronize(s));
   gs_csr *K = new_csr(ctx, 3 * m->n_nodes, 3 * m->n_nodes, m->dof3.nnz,
                       m->dof3.row_ptr, m->dof3.col, val);
   K->dperm = m->perm3;
+  K->block = 3;
   return K;
 }
 
 struct SystemParams {
-  bool elastic = false;
+  bool elastic = false;  // 3 DOFs per node: the elastic family of `ek`
   double k = 0.0, Q = 0.0;
   const double *k_elem = nullptr;
-  double E = 0.0, nu = 0.0, rho = 0.0, g[3] = {0.0, 0.0, 0.0};
-  const double *E_elem = nullptr;
+  ElasticKind ek;
+  const double *fext = nullptr;  // added to the assembled load (caller numbering)
 };
 
 // Element kernel + the cached system plan's numeric phase: K_bc (CSR and its
@@ -138,17 +139,19 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   const auto &pool = ctx->pool;
   const int64_t E = m->n_elems;
   const int dpn = sp.elastic ? 3 : 1;
+  const bool with_load = !sp.elastic || sp.ek.load != GS_LOAD_NONE;
   auto Ke = palloc<double>(pool, E * (sp.elastic ? 900 : 55));
-  auto Fe = palloc<double>(pool, E * 10 * dpn);
+  DArr<double> Fe;
+  if (with_load) Fe = palloc<double>(pool, E * 10 * dpn);
   auto flag = palloc<int>(pool, 1);
   GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
   if (sp.elastic)
-    launch_elastic_elements(s, E, m->xyz.get(), m->conn.get(), sp.E, sp.nu, sp.E_elem,
-                            sp.rho, sp.g, true, Ke.get(), Fe.get(), flag.get());
+    launch_elastic_family(s, E, m->xyz.get(), m->conn.get(), sp.ek, Ke.get(), Fe.get(),
+                          flag.get());
   else
     launch_thermal_elements(s, E, m->xyz.get(), m->conn.get(), sp.k, sp.k_elem, sp.Q,
                             true, Ke.get(), Fe.get(), flag.get());
-  check_flags(read_flag_and_clear(ctx, flag.get()));
+  check_flags(read_flag_and_clear(ctx, flag.get()), sp.elastic ? sp.ek.mat : GS_MAT_ELASTIC);
   const SellPattern &S = *P->sell;
   auto sval = palloc<double>(pool, S.nvals);
   auto sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
@@ -161,7 +164,7 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   // from the SELL values when a caller asks for the CSR form
   system_gather(s, m, *P, Ke.get(), nullptr, sval.get(), sval32.get(), liftv.get(),
                 dmax.get());
-  system_finish(ctx, m, *P, Ke.get(), Fe.get(), nullptr, sval.get(), sval32.get(),
+  system_finish(ctx, m, *P, Ke.get(), Fe.get(), sp.fext, nullptr, sval.get(), sval32.get(),
                 liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get(), dmax.get());
   gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, DArr<double>());
   K->val_lazy = true;
@@ -276,6 +279,35 @@ struct EdgeTable {
     return next;
   }
 };
+
+void check_maxwell(double E, double nu, double visc) {
+  // MaxwellViscoelasticity::new (src/mechanics/constitutive.rs:312-325)
+  check_material(E, nu);
+  require(visc > 0.0, GS_ERR_INVALID_MATERIAL, "Viscosity must be positive");
+}
+
+ElasticKind viscous_kind(double mu, double zeta, const double *mu_elem,
+                         const double *zeta_elem) {
+  // NewtonianViscosity::new (src/mechanics/constitutive.rs:179-186); a
+  // per-element viscosity is checked by the element kernel
+  if (!mu_elem) require(mu > 0.0, GS_ERR_INVALID_MATERIAL, "Viscosity must be positive");
+  ElasticKind k;
+  k.mat = GS_MAT_VISCOUS;
+  k.p0 = mu;
+  k.p1 = zeta;
+  k.e0 = mu_elem;
+  k.e1 = zeta_elem;
+  return k;
+}
+
+ElasticKind maxwell_kind(double E, double nu, const double *stress) {
+  ElasticKind k;
+  k.p0 = E;
+  k.p1 = nu;
+  k.load = GS_LOAD_STRESS;
+  k.sig = stress;
+  return k;
+}
 
 }  // namespace
 
@@ -580,8 +612,11 @@ gs_status gs_assemble_elasticity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
       check_material(E, nu);
     }
     set_device(ctx);
-    *out = assemble_elastic_K(ctx, mesh, E, nu, E_elem, false, 0.0, nullptr, nullptr);
-    (*out)->block = 3;
+    ElasticKind k;
+    k.p0 = E;
+    k.p1 = nu;
+    k.e0 = E_elem;
+    *out = assemble_dof3_K(ctx, mesh, k, nullptr);
   });
 }
 
@@ -718,12 +753,108 @@ gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
     set_device(ctx);
     SystemParams sp;
     sp.elastic = true;
-    sp.E = E;
-    sp.nu = nu;
-    sp.E_elem = E_elem;
-    sp.rho = rho;
-    for (int c = 0; c < 3; ++c) sp.g[c] = g[c];
+    sp.ek.p0 = E;
+    sp.ek.p1 = nu;
+    sp.ek.e0 = E_elem;
+    sp.ek.load = GS_LOAD_BODY;
+    for (int c = 0; c < 3; ++c) sp.ek.bf[c] = rho * g[c];
     *K_bc = assemble_system(ctx, mesh, dof, sp, f_bc);
+  });
+}
+
+gs_status gs_assemble_viscosity_stiffness(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                          double mu, double zeta, const double *mu_elem,
+                                          const double *zeta_elem, gs_csr **out) {
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Viscous flow requires 3 DOF per node");
+    const ElasticKind k = viscous_kind(mu, zeta, mu_elem, zeta_elem);
+    set_device(ctx);
+    *out = assemble_dof3_K(ctx, mesh, k, nullptr);
+  });
+}
+
+gs_status gs_assemble_viscosity_system(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                       double mu, double zeta, const double *mu_elem,
+                                       const double *zeta_elem, const double *f,
+                                       gs_csr **K_bc, double *f_bc) {
+  if (!ctx || !K_bc) return GS_ERR_INVALID_ARG;
+  *K_bc = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Viscous flow requires 3 DOF per node");
+    require(f_bc || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null output");
+    SystemParams sp;
+    sp.elastic = true;
+    sp.ek = viscous_kind(mu, zeta, mu_elem, zeta_elem);
+    sp.fext = f;
+    set_device(ctx);
+    *K_bc = assemble_system(ctx, mesh, dof, sp, f_bc);
+  });
+}
+
+gs_status gs_assemble_maxwell_viscoelastic(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                           double E, double nu, double visc, double dt,
+                                           const double *stress, gs_csr **out,
+                                           double *f_history) {
+  (void)dt;  // unused by the reference's element routine (element.rs:260)
+  if (!ctx || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Maxwell viscoelasticity requires 3 DOF per node");
+    check_maxwell(E, nu, visc);
+    require(stress || mesh->n_elems == 0, GS_ERR_INVALID_ARG,
+            "Mesh must have stress_history for viscoelasticity");
+    require(f_history || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    *out = assemble_dof3_K(ctx, mesh, maxwell_kind(E, nu, stress), f_history);
+  });
+}
+
+gs_status gs_assemble_maxwell_system(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                     double E, double nu, double visc, double dt,
+                                     const double *stress, const double *f_ext,
+                                     gs_csr **K_bc, double *f_bc) {
+  (void)dt;
+  if (!ctx || !K_bc) return GS_ERR_INVALID_ARG;
+  *K_bc = nullptr;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Maxwell viscoelasticity requires 3 DOF per node");
+    check_maxwell(E, nu, visc);
+    require(stress || mesh->n_elems == 0, GS_ERR_INVALID_ARG,
+            "Mesh must have stress_history for viscoelasticity");
+    require((f_ext && f_bc) || mesh->n_nodes == 0, GS_ERR_INVALID_ARG, "null argument");
+    SystemParams sp;
+    sp.elastic = true;
+    sp.ek = maxwell_kind(E, nu, stress);
+    sp.fext = f_ext;
+    set_device(ctx);
+    *K_bc = assemble_system(ctx, mesh, dof, sp, f_bc);
+  });
+}
+
+gs_status gs_update_stresses_maxwell(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                     double E, double nu, double visc, double dt,
+                                     const double *u_n, const double *u_next,
+                                     const double *stress, double *stress_next) {
+  if (!ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    check_mesh_dof(mesh, dof);
+    require(dof->dpn == 3, GS_ERR_BAD_DOFS, "Maxwell viscoelasticity requires 3 DOF per node");
+    check_maxwell(E, nu, visc);
+    require((u_n && u_next && stress && stress_next) || mesh->n_elems == 0,
+            GS_ERR_INVALID_ARG, "null argument");
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    auto flag = palloc<int>(ctx->pool, 1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    launch_maxwell_stress(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(), E, nu, visc,
+                          dt, u_n, u_next, stress, stress_next, flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()));
   });
 }
 

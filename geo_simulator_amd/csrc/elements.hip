@@ -249,11 +249,46 @@ __global__ __launch_bounds__(128) void k_thermal_elements(
 // ---------------------------------------------------------------------------
 constexpr int EL_PER_WG = 4;
 
-template <bool PER_ELEM, bool WITH_LOAD>
+// The isotropic Voigt laws share D's pattern: a normal block with Dd on the
+// diagonal and Do off it, and Ds on the shear diagonal.
+//   MAT_ELASTIC (src/mechanics/constitutive.rs:63-102): factor = E/((1+nu)(1-2nu)),
+//     Dd = (1-nu) factor, Do = nu factor, Ds = ((1-2nu)/2) factor;
+//   MAT_VISCOUS (src/mechanics/constitutive.rs:227-255): lambda_v = zeta - 2mu/3,
+//     Dd = lambda_v + 2mu, Do = lambda_v, Ds = 2mu.
+// The element load is none, the body force w N_i (rho g)_c
+// (src/mechanics/body_force.rs:49-84), or the stress-history force
+// -sum_q w B^T sigma_e (src/mechanics/element.rs:257-295).
+struct DLaw {
+  double Dd, Do, Ds;
+  bool ok;
+};
+
+template <int MAT>
+__device__ __forceinline__ DLaw d_law(const ElasticKind &k, int64_t e) {
+  DLaw d;
+  if constexpr (MAT == GS_MAT_ELASTIC) {
+    const double Ee = k.e0 ? k.e0[e] : k.p0, nu = k.p1;
+    const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
+    d.Dd = (1.0 - nu) * factor;
+    d.Do = nu * factor;
+    d.Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+    d.ok = Ee > 0.0;
+  } else {
+    const double mu = k.e0 ? k.e0[e] : k.p0, zeta = k.e1 ? k.e1[e] : k.p1;
+    const double f2mu = 2.0 * mu;
+    const double lv = zeta - (2.0 * mu / 3.0);
+    d.Dd = lv + f2mu;
+    d.Do = lv;
+    d.Ds = f2mu;
+    d.ok = mu > 0.0;
+  }
+  return d;
+}
+
+template <int MAT, int LOAD>
 __global__ __launch_bounds__(128) void k_elastic_elements(
     int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
-    double Emod, double nu, const double *__restrict__ E_elem, double bfx,
-    double bfy, double bfz, double *__restrict__ Ke, double *__restrict__ Fe,
+    const ElasticKind kind, double *__restrict__ Ke, double *__restrict__ Fe,
     int *__restrict__ flag) {
   __shared__ double sX[EL_PER_WG][10][3];
   __shared__ double sJi[EL_PER_WG][4][3][3];
@@ -299,18 +334,23 @@ __global__ __launch_bounds__(128) void k_elastic_elements(
   __syncthreads();
   if (live && a < 30) {
     const int ia = a / 3, ca = a - 3 * (a / 3);
-    const double Ee = PER_ELEM ? E_elem[e] : Emod;
-    if (!(Ee > 0.0)) bad |= FLAG_MATERIAL;
-    // D (src/mechanics/constitutive.rs:63-102): unscaled entries times factor.
-    const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
-    const double Dd = (1.0 - nu) * factor;
-    const double Do = nu * factor;
-    const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+    const DLaw law = d_law<MAT>(kind, e);
+    if (!law.ok) bad |= FLAG_MATERIAL;
+    const double Dd = law.Dd, Do = law.Do, Ds = law.Ds;
     double K[30];
 #pragma unroll
     for (int b = 0; b < 30; ++b) K[b] = 0.0;
     double f = 0.0;
-    const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
+    const double bfa = LOAD == GS_LOAD_BODY ? kind.bf[ca] : 0.0;
+    // the stress components of the three Voigt rows k1 < k2 < k3 where
+    // column a of B is structurally nonzero
+    double sk1 = 0.0, sk2 = 0.0, sk3 = 0.0;
+    if (LOAD == GS_LOAD_STRESS) {
+      const double *sg = kind.sig + e * 6;
+      sk1 = sg[ca];
+      sk2 = sg[ca == 2 ? 4 : 3];
+      sk3 = sg[ca == 1 ? 4 : 5];
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const double w = sW[le][q];
@@ -337,13 +377,16 @@ __global__ __launch_bounds__(128) void k_elastic_elements(
           K[3 * j + cb] += w * s;
         }
       }
-      if (WITH_LOAD) f += w * kT.N[q][ia] * bfa;
+      if (LOAD == GS_LOAD_BODY) f += w * kT.N[q][ia] * bfa;
+      // (B^T sigma)[a]: the dense product's nonzero terms in ascending Voigt
+      // row; f_history -= w * B^T sigma
+      if (LOAD == GS_LOAD_STRESS) f -= w * (Bk1 * sk1 + Bk2 * sk2 + Bk3 * sk3);
     }
 #pragma unroll
     for (int j = 0; j < 10; ++j)
 #pragma unroll
       for (int cb = 0; cb < 3; ++cb) sK[le][(ia * 10 + j) * 9 + ca * 3 + cb] = K[3 * j + cb];
-    if (WITH_LOAD) Fe[e * 30 + a] = f;
+    if (LOAD != GS_LOAD_NONE) Fe[e * 30 + a] = f;
   }
   if (bad) atomicOr(flag, bad);
   __syncthreads();
@@ -740,6 +783,90 @@ inline unsigned blocks_for(int64_t work, int bs) {
   return (unsigned)((work + bs - 1) / bs);
 }
 
+// update_stresses_maxwell (src/mechanics/stress_update.rs:39-147), one thread
+// per element: du = u_next - u_n on the element's 30 DOFs (DOF 3 node + c),
+// the strain increment B du averaged over the 4-point rule's points (each
+// B du as the dense product's nonzero terms in ascending column), the
+// elastic predictor sigma_n + D_el d_eps in the reference's expression form,
+// and its deviatoric part relaxed by 1 / (1 + dt / tau_M), tau_M = mu / G.
+__global__ __launch_bounds__(128) void k_maxwell_stress(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn, double Emod,
+    double nu, double visc, double dt, const double *__restrict__ u_n,
+    const double *__restrict__ u_next, const double *__restrict__ sig_in,
+    double *__restrict__ sig_out, int *__restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  double X[10][3];
+  load_nodes(xyz, conn, e, X);
+  double du[30];
+#pragma unroll
+  for (int a = 0; a < 10; ++a) {
+    const int64_t nd = conn[e * 10 + a];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) du[3 * a + c] = u_next[nd * 3 + c] - u_n[nd * 3 + c];
+  }
+  double avg[6];
+#pragma unroll
+  for (int v = 0; v < 6; ++v) avg[v] = 0.0;
+  int bad = 0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+    jacobian(q, X, J);
+    bad |= det_flags(inverse3(J, Ji));
+    gradients(q, Ji, G);
+    // Voigt rows [xx, yy, zz, xy, yz, zx]; columns 3i + c (strain.rs:38-71)
+    double d0 = G[0][0] * du[0], d1 = G[0][1] * du[1], d2 = G[0][2] * du[2];
+    double d3 = G[0][1] * du[0], d4 = G[0][2] * du[1], d5 = G[0][2] * du[0];
+    d3 = d3 + G[0][0] * du[1];
+    d4 = d4 + G[0][1] * du[2];
+    d5 = d5 + G[0][0] * du[2];
+#pragma unroll
+    for (int i = 1; i < 10; ++i) {
+      const double x = du[3 * i], y = du[3 * i + 1], z = du[3 * i + 2];
+      d0 = d0 + G[i][0] * x;
+      d3 = d3 + G[i][1] * x;
+      d5 = d5 + G[i][2] * x;
+      d1 = d1 + G[i][1] * y;
+      d3 = d3 + G[i][0] * y;
+      d4 = d4 + G[i][2] * y;
+      d2 = d2 + G[i][2] * z;
+      d4 = d4 + G[i][1] * z;
+      d5 = d5 + G[i][0] * z;
+    }
+    avg[0] += d0;
+    avg[1] += d1;
+    avg[2] += d2;
+    avg[3] += d3;
+    avg[4] += d4;
+    avg[5] += d5;
+  }
+  if (bad) atomicOr(flag, bad);
+#pragma unroll
+  for (int v = 0; v < 6; ++v) avg[v] /= 4.0;
+  const double Gm = Emod / (2.0 * (1.0 + nu));
+  const double tau = visc / (Emod / (2.0 * (1.0 + nu)));
+  const double factor = 1.0 / (1.0 + dt / tau);
+  const double Kb = Emod / (3.0 * (1.0 - 2.0 * nu));
+  const double lambda = Kb - 2.0 * Gm / 3.0;
+  const double *sn = sig_in + e * 6;
+  const double tr = avg[0] + avg[1] + avg[2];
+  const double t0 = sn[0] + lambda * tr + 2.0 * Gm * avg[0];
+  const double t1 = sn[1] + lambda * tr + 2.0 * Gm * avg[1];
+  const double t2 = sn[2] + lambda * tr + 2.0 * Gm * avg[2];
+  const double t3 = sn[3] + Gm * avg[3];
+  const double t4 = sn[4] + Gm * avg[4];
+  const double t5 = sn[5] + Gm * avg[5];
+  const double p = (t0 + t1 + t2) / 3.0;
+  double *so = sig_out + e * 6;
+  so[0] = (t0 - p) * factor + p;
+  so[1] = (t1 - p) * factor + p;
+  so[2] = (t2 - p) * factor + p;
+  so[3] = t3 * factor;
+  so[4] = t4 * factor;
+  so[5] = t5 * factor;
+}
+
 }  // namespace
 
 void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
@@ -762,25 +889,53 @@ void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
   GS_LAUNCH_CHECK("k_thermal_elements");
 }
 
+void launch_elastic_family(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                           const ElasticKind &k, double *Ke, double *Fe, int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E, EL_PER_WG);
+  auto go = [&](auto mat, auto load) {
+    constexpr int M = decltype(mat)::value, L = decltype(load)::value;
+    hipLaunchKernelGGL((k_elastic_elements<M, L>), g, 128, 0, s, E, xyz, conn, k, Ke, Fe,
+                       flag);
+  };
+  using ME = std::integral_constant<int, GS_MAT_ELASTIC>;
+  using MV = std::integral_constant<int, GS_MAT_VISCOUS>;
+  using LN = std::integral_constant<int, GS_LOAD_NONE>;
+  using LB = std::integral_constant<int, GS_LOAD_BODY>;
+  using LS = std::integral_constant<int, GS_LOAD_STRESS>;
+  if (k.mat == GS_MAT_VISCOUS) {
+    if (k.load == GS_LOAD_BODY) go(MV{}, LB{});
+    else go(MV{}, LN{});
+  } else {
+    if (k.load == GS_LOAD_BODY) go(ME{}, LB{});
+    else if (k.load == GS_LOAD_STRESS) go(ME{}, LS{});
+    else go(ME{}, LN{});
+  }
+  GS_LAUNCH_CHECK("k_elastic_elements");
+}
+
 void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double Emod, double nu,
                              const double *E_elem, double rho, const double g3[3],
                              bool with_load, double *Ke, double *Fe, int *flag) {
+  ElasticKind k;
+  k.mat = GS_MAT_ELASTIC;
+  k.p0 = Emod;
+  k.p1 = nu;
+  k.e0 = E_elem;
+  k.load = with_load ? GS_LOAD_BODY : GS_LOAD_NONE;
+  for (int c = 0; c < 3; ++c) k.bf[c] = rho * g3[c];
+  launch_elastic_family(s, E, xyz, conn, k, Ke, Fe, flag);
+}
+
+void launch_maxwell_stress(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                           double Emod, double nu, double visc, double dt, const double *u_n,
+                           const double *u_next, const double *sig_in, double *sig_out,
+                           int *flag) {
   if (E == 0) return;
-  const unsigned g = blocks_for(E, EL_PER_WG);
-  const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
-  if (E_elem) {
-    if (with_load)
-      hipLaunchKernelGGL((k_elastic_elements<true, true>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-    else
-      hipLaunchKernelGGL((k_elastic_elements<true, false>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-  } else {
-    if (with_load)
-      hipLaunchKernelGGL((k_elastic_elements<false, true>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-    else
-      hipLaunchKernelGGL((k_elastic_elements<false, false>), g, 128, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-  }
-  GS_LAUNCH_CHECK("k_elastic_elements");
+  hipLaunchKernelGGL(k_maxwell_stress, blocks_for(E, 128), 128, 0, s, E, xyz, conn, Emod, nu,
+                     visc, dt, u_n, u_next, sig_in, sig_out, flag);
+  GS_LAUNCH_CHECK("k_maxwell_stress");
 }
 
 void launch_thermal_backward(hipStream_t s, int64_t E, const double *xyz,

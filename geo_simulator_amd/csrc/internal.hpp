@@ -249,6 +249,25 @@ void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double k, const double *k_elem,
                              double Q, bool with_load, double *Ke, double *Fe,
                              int *flag);
+// The elastic element family: K_e = sum_q w B^T D B for an isotropic Voigt
+// law (elastic or Newtonian viscous) and an optional element load.
+enum : int { GS_MAT_ELASTIC = 0, GS_MAT_VISCOUS = 1 };
+enum : int { GS_LOAD_NONE = 0, GS_LOAD_BODY = 1, GS_LOAD_STRESS = 2 };
+struct ElasticKind {
+  int mat = GS_MAT_ELASTIC;
+  double p0 = 0.0, p1 = 0.0;      // elastic: E, nu; viscous: mu, zeta
+  const double *e0 = nullptr;     // per-element E or mu (nullable, device)
+  const double *e1 = nullptr;     // per-element zeta (viscous, nullable)
+  int load = GS_LOAD_NONE;
+  double bf[3] = {0.0, 0.0, 0.0}; // body force rho g (GS_LOAD_BODY)
+  const double *sig = nullptr;    // per-element stress [E][6] (GS_LOAD_STRESS)
+};
+void launch_elastic_family(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                           const ElasticKind &k, double *Ke, double *Fe, int *flag);
+void launch_maxwell_stress(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                           double Emod, double nu, double visc, double dt, const double *u_n,
+                           const double *u_next, const double *sig_in, double *sig_out,
+                           int *flag);
 void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double Emod, double nu,
                              const double *E_elem, double rho, const double g[3],
@@ -322,7 +341,8 @@ void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
 // c = max(1, max |K_ii|), constrained rows, RHS (load + lift, written at
 // caller index dperm[r]) and the Jacobi inverse diagonal in SELL order
 void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, const double *Fe, double *val, double *sval,
+                   const double *Ke, const double *Fe, const double *fext, double *val,
+                   double *sval,
                    float *sval32, const double *liftv, const int *dperm,
                    double *fbc, double *sdinv, float *sdinv32,
                    const unsigned long long *dmax);

@@ -40,6 +40,9 @@ class Mesh:
         self.elements = np.ascontiguousarray(elements, dtype=np.int64).reshape(-1, 10)
         self.field_data = FieldData()  # nodal fields (src/mesh/geometry.rs:44)
         self.cell_data = FieldData()   # element fields (src/mesh/geometry.rs:45)
+        # per-element stress for viscoelasticity (src/mesh/geometry.rs:47): a
+        # rheology.StressHistory or None
+        self.stress_history = None
         self._handles = {}
 
     # reference accessors

@@ -145,6 +145,66 @@ gs_status gs_assemble_elasticity_system(gs_ctx *ctx, gs_mesh *mesh,
                                         double density, const double gravity[3],
                                         gs_csr **K_bc, double *f_bc_dev);
 
+/* ---- values-only re-assembly for iterative callers ----
+ * The calls below reuse the mesh's cached CSR pattern and contributor lists
+ * (and, for the *_system calls, the cached elimination plan of the DOF map),
+ * so each new viscosity field or stress history costs only the element
+ * kernel and the numeric gathers. */
+/* Assembler::assemble_viscosity_stiffness_parallel (src/fem/assembly.rs:646-694)
+ * with NewtonianViscosity{viscosity, bulk_viscosity}
+ * (src/mechanics/constitutive.rs:156-255): K = sum_e int B^T D B dV,
+ * D = (zeta - 2 mu / 3) on the normal block, plus 2 mu on its diagonal and on
+ * the shear diagonal.  mu_per_elem_dev / zeta_per_elem_dev (nullable, n_elems
+ * doubles) replace mu / zeta per element (additive overload for Picard-type
+ * callers whose effective viscosity varies, src/linalg/picard.rs:262-275).
+ * mu <= 0 returns GS_ERR_INVALID_MATERIAL ("Viscosity must be positive"). */
+gs_status gs_assemble_viscosity_stiffness(gs_ctx *ctx, gs_mesh *mesh,
+                                          const gs_dofmap *dof, double viscosity,
+                                          double bulk_viscosity,
+                                          const double *mu_per_elem_dev,
+                                          const double *zeta_per_elem_dev,
+                                          gs_csr **out);
+/* assemble_viscosity_stiffness_parallel followed by apply_dirichlet_bcs
+ * (Picard's assemble / eliminate pair, src/linalg/picard.rs:264-268),
+ * device-resident: f_dev (nullable: zero) is the right-hand side before the
+ * elimination, in caller numbering. */
+gs_status gs_assemble_viscosity_system(gs_ctx *ctx, gs_mesh *mesh,
+                                       const gs_dofmap *dof, double viscosity,
+                                       double bulk_viscosity,
+                                       const double *mu_per_elem_dev,
+                                       const double *zeta_per_elem_dev,
+                                       const double *f_dev, gs_csr **K_bc,
+                                       double *f_bc_dev);
+/* Assembler::assemble_maxwell_viscoelastic_parallel (src/fem/assembly.rs:959-1020)
+ * with MaxwellViscoelasticity::new(E, nu, viscosity)
+ * (src/mechanics/constitutive.rs:290-325) and ElasticityElement::
+ * maxwell_viscoelastic (src/mechanics/element.rs:257-295): K with the elastic
+ * D, and f_history = -sum_e sum_q w B^T sigma_e.  stress_dev is the stress
+ * history (Mesh::stress_history, src/mesh/state.rs:12-16): n_elems x 6 Voigt
+ * [xx, yy, zz, xy, yz, xz] in element order.  dt is unused, as in the
+ * reference. */
+gs_status gs_assemble_maxwell_viscoelastic(gs_ctx *ctx, gs_mesh *mesh,
+                                           const gs_dofmap *dof, double youngs,
+                                           double poisson, double viscosity, double dt,
+                                           const double *stress_dev, gs_csr **K,
+                                           double *f_history_dev);
+/* BackwardEuler::step's assembly (src/fem/time_integration.rs:99-110): the
+ * Maxwell K and f_history, f = f_ext + f_history, then apply_dirichlet_bcs,
+ * device-resident; f_ext_dev in caller numbering. */
+gs_status gs_assemble_maxwell_system(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                     double youngs, double poisson, double viscosity,
+                                     double dt, const double *stress_dev,
+                                     const double *f_ext_dev, gs_csr **K_bc,
+                                     double *f_bc_dev);
+/* update_stresses_maxwell (src/mechanics/stress_update.rs:39-147):
+ * stress_next_dev (n_elems x 6) from stress_dev and the displacements
+ * u_n_dev, u_next_dev (DOF 3 node + c, caller numbering). */
+gs_status gs_update_stresses_maxwell(gs_ctx *ctx, gs_mesh *mesh, const gs_dofmap *dof,
+                                     double youngs, double poisson, double viscosity,
+                                     double dt, const double *u_n_dev,
+                                     const double *u_next_dev, const double *stress_dev,
+                                     double *stress_next_dev);
+
 /* ---- sparse matrix handle (sprs CsMat<f64> counterpart) ---- */
 gs_status gs_csr_info(const gs_csr *A, int64_t *rows, int64_t *cols,
                       int64_t *nnz);

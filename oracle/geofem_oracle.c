@@ -305,6 +305,100 @@ int or_elastic_stiffness(const double nodes[10][3], double E, double nu,
   return elastic_stiffness_D(nodes, D, K);
 }
 
+/* NewtonianViscosity::new / constitutive_matrix
+ * (src/mechanics/constitutive.rs:179-186, 227-255): lambda_v = zeta - 2mu/3
+ * on the normal block, 2mu added on its diagonal and on the shear diagonal. */
+int or_viscous_constitutive(double mu, double zeta, double D[6][6]) {
+  if (!(mu > 0.0)) return OR_ERR_INVALID_MATERIAL;
+  const double f2mu = 2.0 * mu;
+  const double lv = zeta - (2.0 * mu / 3.0);
+  memset(D, 0, sizeof(double) * 36);
+  for (int i = 0; i < 3; ++i) {
+    for (int j = 0; j < 3; ++j) D[i][j] = lv;
+    D[i][i] += f2mu;
+  }
+  D[3][3] = f2mu; D[4][4] = f2mu; D[5][5] = f2mu;
+  return OR_OK;
+}
+
+/* ElasticityElement::viscosity_matrix (src/mechanics/element.rs:87-118):
+ * the elastic element's B^T D B with the viscous D. */
+int or_viscosity_stiffness(const double nodes[10][3], double mu, double zeta,
+                           double K[30][30]) {
+  double D[6][6];
+  int st = or_viscous_constitutive(mu, zeta, D);
+  if (st != OR_OK) return st;
+  return elastic_stiffness_D(nodes, D, K);
+}
+
+/* ElasticityElement::maxwell_viscoelastic (src/mechanics/element.rs:257-295):
+ * K_e with the elastic D, and f_history -= w * B^T sigma_n (B^T sigma as a
+ * dense product, ascending Voigt row). */
+int or_maxwell_element(const double nodes[10][3], double E, double nu,
+                       const double sigma[6], double K[30][30], double fe[30]) {
+  double D[6][6];
+  int st = or_constitutive(E, nu, D);
+  if (st != OR_OK) return st;
+  st = elastic_stiffness_D(nodes, D, K);
+  if (st != OR_OK) return st;
+  for (int a = 0; a < 30; ++a) fe[a] = 0.0;
+  for (int q = 0; q < 4; ++q) {
+    const double *L = Q4_PTS[q];
+    double J[3][3], dNdx[10][3], B[6][30];
+    or_jacobian(L, nodes, J);
+    const double w = Q4_W[q] * fabs(or_det3(J));
+    if (or_shape_derivs_cart(L, nodes, dNdx) != OR_OK) return OR_ERR_SINGULAR_JACOBIAN;
+    or_b_matrix(dNdx, B);
+    for (int a = 0; a < 30; ++a) {
+      double bs = B[0][a] * sigma[0];
+      for (int k = 1; k < 6; ++k) bs += B[k][a] * sigma[k];
+      fe[a] -= w * bs;
+    }
+  }
+  return OR_OK;
+}
+
+/* update_stresses_maxwell for one element (src/mechanics/stress_update.rs:
+ * 53-145): the strain increment B du averaged over the 4-point rule's points,
+ * the elastic predictor, the deviatoric part relaxed by 1/(1 + dt/tau_M). */
+int or_maxwell_stress(const double nodes[10][3], double E, double nu, double visc,
+                      double dt, const double du[30], const double sn[6], double out[6]) {
+  const double G = E / (2.0 * (1.0 + nu));
+  const double tau = visc / (E / (2.0 * (1.0 + nu)));
+  const double factor = 1.0 / (1.0 + dt / tau);
+  double d[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int q = 0; q < 4; ++q) {
+    double dNdx[10][3], B[6][30];
+    if (or_shape_derivs_cart(Q4_PTS[q], nodes, dNdx) != OR_OK) return OR_ERR_SINGULAR_JACOBIAN;
+    or_b_matrix(dNdx, B);
+    for (int v = 0; v < 6; ++v) {
+      double s = B[v][0] * du[0];
+      for (int m = 1; m < 30; ++m) s += B[v][m] * du[m];
+      d[v] += s;
+    }
+  }
+  for (int v = 0; v < 6; ++v) d[v] /= 4.0;
+  const double Kb = E / (3.0 * (1.0 - 2.0 * nu));
+  const double lambda = Kb - 2.0 * G / 3.0;
+  double t[6];
+  t[0] = sn[0] + lambda * (d[0] + d[1] + d[2]) + 2.0 * G * d[0];
+  t[1] = sn[1] + lambda * (d[0] + d[1] + d[2]) + 2.0 * G * d[1];
+  t[2] = sn[2] + lambda * (d[0] + d[1] + d[2]) + 2.0 * G * d[2];
+  t[3] = sn[3] + G * d[3];
+  t[4] = sn[4] + G * d[4];
+  t[5] = sn[5] + G * d[5];
+  const double p = (t[0] + t[1] + t[2]) / 3.0;
+  double sdev[6] = {t[0] - p, t[1] - p, t[2] - p, t[3], t[4], t[5]};
+  for (int v = 0; v < 6; ++v) sdev[v] = sdev[v] * factor;
+  out[0] = sdev[0] + p;
+  out[1] = sdev[1] + p;
+  out[2] = sdev[2] + p;
+  out[3] = sdev[3];
+  out[4] = sdev[4];
+  out[5] = sdev[5];
+  return OR_OK;
+}
+
 /* src/mechanics/body_force.rs:49-84: f[3i+c] += (w * N_i) * (rho g)_c. */
 int or_gravity_load(const double nodes[10][3], double rho, const double g[3],
                     double fe[30]) {
@@ -655,35 +749,11 @@ int or_assemble_thermal_mass(int64_t n_nodes, const double *xyz, int64_t n_elems
 
 /* src/fem/assembly.rs:389-437: parallel 30x30 element matrices, then serial
  * scatter with global_dof(node,c) = 3*node + c (src/fem/dof.rs:87-106). */
-int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
-                        const int64_t *conn, double E, double nu,
-                        const double *E_elem, int nthreads, or_csr *out) {
-  memset(out, 0, sizeof(*out));
-  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
-  double D[6][6];
-  if (!E_elem) {
-    const int st = or_constitutive(E, nu, D);
-    if (st != OR_OK) return st;
-  } else {
-    for (int64_t e = 0; e < n_elems; ++e)
-      if (or_constitutive(E_elem[e], nu, D) != OR_OK) return OR_ERR_INVALID_MATERIAL;
-  }
-  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 900 * sizeof(double));
-  if (!Ke) return OR_ERR_OOM;
-  int err = OR_OK;
-  set_threads(nthreads);
-#pragma omp parallel for schedule(static)
-  for (int64_t e = 0; e < n_elems; ++e) {
-    double nodes[10][3], De[6][6];
-    gather_nodes(xyz, conn + 10 * e, nodes);
-    or_constitutive(E_elem ? E_elem[e] : E, nu, De);
-    if (elastic_stiffness_D(nodes, (const double(*)[6])De,
-                            (double(*)[30])(Ke + 900 * e)) != OR_OK) {
-#pragma omp atomic write
-      err = OR_ERR_SINGULAR_JACOBIAN;
-    }
-  }
-  if (err != OR_OK) { free(Ke); return err; }
+/* Serial 900-triplet scatter of 30x30 element matrices with
+ * global_dof(node,c) = 3*node + c (src/fem/dof.rs:87-106,
+ * src/fem/assembly.rs:415-434), then the CSR conversion. Frees Ke. */
+static int scatter_dof3(int64_t n_nodes, int64_t n_elems, const int64_t *conn,
+                        double *Ke, or_csr *out) {
   const int64_t ndof = 3 * n_nodes;
   const int64_t nt = 900 * n_elems;
   const int small = ndof < 2147483647LL;
@@ -719,8 +789,124 @@ int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
       }
   }
   free(Ke);
-  err = triplets_to_csr(ndof, ndof, nt, ti32, tj32, ti64, tj64, tv, out);
+  const int err = triplets_to_csr(ndof, ndof, nt, ti32, tj32, ti64, tj64, tv, out);
   free(tv); free(ti32); free(tj32); free(ti64); free(tj64);
+  return err;
+}
+
+/* The element matrices of one isotropic Voigt law (kind 0: elastic D(E, nu),
+ * E per element when e0; kind 1: viscous D(mu, zeta), mu / zeta per element
+ * when e0 / e1), in parallel as Rayon's par_iter; optional stress-history
+ * loads (kind 0 only) into Fe. */
+static int element_matrices_dof3(const double *xyz, int64_t n_elems, const int64_t *conn,
+                                 int kind, double p0, double p1, const double *e0,
+                                 const double *e1, const double *sigma, double *Ke,
+                                 double *Fe, int nthreads) {
+  double D[6][6];
+  for (int64_t e = 0; e < (e0 || e1 ? n_elems : 1); ++e) {
+    const double a = e0 ? e0[e] : p0, b = e1 ? e1[e] : p1;
+    const int st = kind == 0 ? or_constitutive(a, b, D) : or_viscous_constitutive(a, b, D);
+    if (st != OR_OK) return st;
+  }
+  int err = OR_OK;
+  set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], De[6][6];
+    gather_nodes(xyz, conn + 10 * e, nodes);
+    const double a = e0 ? e0[e] : p0, b = e1 ? e1[e] : p1;
+    int st;
+    if (sigma) {
+      st = or_maxwell_element((const double(*)[3])nodes, a, b, sigma + 6 * e,
+                              (double(*)[30])(Ke + 900 * e), Fe + 30 * e);
+    } else {
+      if (kind == 0) or_constitutive(a, b, De);
+      else or_viscous_constitutive(a, b, De);
+      st = elastic_stiffness_D(nodes, (const double(*)[6])De, (double(*)[30])(Ke + 900 * e));
+    }
+    if (st != OR_OK) {
+#pragma omp atomic write
+      err = OR_ERR_SINGULAR_JACOBIAN;
+    }
+  }
+  return err;
+}
+
+/* src/fem/assembly.rs:389-437: parallel 30x30 element matrices, then serial
+ * scatter with global_dof(node,c) = 3*node + c (src/fem/dof.rs:87-106). */
+int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double E, double nu,
+                        const double *E_elem, int nthreads, or_csr *out) {
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 900 * sizeof(double));
+  if (!Ke) return OR_ERR_OOM;
+  const int err = element_matrices_dof3(xyz, n_elems, conn, 0, E, nu, E_elem, NULL, NULL, Ke,
+                                        NULL, nthreads);
+  if (err != OR_OK) { free(Ke); return err; }
+  return scatter_dof3(n_nodes, n_elems, conn, Ke, out);
+}
+
+/* Assembler::assemble_viscosity_stiffness_parallel (src/fem/assembly.rs:646-694):
+ * the same parallel element loop and serial scatter with the viscous D;
+ * mu_elem / zeta_elem (nullable) replace mu / zeta per element. */
+int or_assemble_viscosity(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                          const int64_t *conn, double mu, double zeta,
+                          const double *mu_elem, const double *zeta_elem, int nthreads,
+                          or_csr *out) {
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 900 * sizeof(double));
+  if (!Ke) return OR_ERR_OOM;
+  const int err = element_matrices_dof3(xyz, n_elems, conn, 1, mu, zeta, mu_elem, zeta_elem,
+                                        NULL, Ke, NULL, nthreads);
+  if (err != OR_OK) { free(Ke); return err; }
+  return scatter_dof3(n_nodes, n_elems, conn, Ke, out);
+}
+
+/* Assembler::assemble_maxwell_viscoelastic_parallel (src/fem/assembly.rs:959-1020):
+ * element (K_e, f_e) in parallel (MaxwellViscoelasticity::new's checks,
+ * src/mechanics/constitutive.rs:312-325), then K scattered and f_history
+ * summed serially in element order.  dt is unused, as in the reference. */
+int or_assemble_maxwell(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double E, double nu, double visc, double dt,
+                        const double *stress, int nthreads, or_csr *out, double *f_hist) {
+  (void)dt;
+  memset(out, 0, sizeof(*out));
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  if (!(visc > 0.0)) return OR_ERR_INVALID_MATERIAL;
+  double *Ke = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 900 * sizeof(double));
+  double *Fe = (double *)malloc((size_t)(n_elems > 0 ? n_elems : 1) * 30 * sizeof(double));
+  if (!Ke || !Fe) { free(Ke); free(Fe); return OR_ERR_OOM; }
+  const int err = element_matrices_dof3(xyz, n_elems, conn, 0, E, nu, NULL, NULL, stress, Ke,
+                                        Fe, nthreads);
+  if (err != OR_OK) { free(Ke); free(Fe); return err; }
+  for (int64_t i = 0; i < 3 * n_nodes; ++i) f_hist[i] = 0.0;
+  for (int64_t e = 0; e < n_elems; ++e)
+    for (int i = 0; i < 10; ++i)
+      for (int c = 0; c < 3; ++c) f_hist[3 * conn[10 * e + i] + c] += Fe[30 * e + 3 * i + c];
+  free(Fe);
+  return scatter_dof3(n_nodes, n_elems, conn, Ke, out);
+}
+
+/* update_stresses_maxwell (src/mechanics/stress_update.rs:39-147) over all
+ * elements; u_n, u_next indexed by DOF 3 node + c. */
+int or_update_stresses_maxwell(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                               const int64_t *conn, double E, double nu, double visc,
+                               double dt, const double *u_n, const double *u_next,
+                               const double *sig_in, double *sig_out) {
+  if (check_conn(n_nodes, n_elems, conn) != OR_OK) return OR_ERR_INVALID_ARG;
+  int err = OR_OK;
+  for (int64_t e = 0; e < n_elems; ++e) {
+    double nodes[10][3], du[30];
+    const int64_t *c = conn + 10 * e;
+    gather_nodes(xyz, c, nodes);
+    for (int i = 0; i < 10; ++i)
+      for (int k = 0; k < 3; ++k) du[3 * i + k] = u_next[3 * c[i] + k] - u_n[3 * c[i] + k];
+    if (or_maxwell_stress((const double(*)[3])nodes, E, nu, visc, dt, du, sig_in + 6 * e,
+                          sig_out + 6 * e) != OR_OK)
+      err = OR_ERR_SINGULAR_JACOBIAN;
+  }
   return err;
 }
 

@@ -74,6 +74,13 @@ int or_elastic_stiffness(const double nodes[10][3], double E, double nu,
                          double Ke[30][30]);
 int or_gravity_load(const double nodes[10][3], double rho, const double g[3],
                     double fe[30]);
+int or_viscous_constitutive(double mu, double zeta, double D[6][6]);
+int or_viscosity_stiffness(const double nodes[10][3], double mu, double zeta,
+                           double Ke[30][30]);
+int or_maxwell_element(const double nodes[10][3], double E, double nu,
+                       const double sigma[6], double Ke[30][30], double fe[30]);
+int or_maxwell_stress(const double nodes[10][3], double E, double nu, double visc,
+                      double dt, const double du[30], const double sn[6], double out[6]);
 
 /* ---- mesh generator (src/mesh_generator_improved.rs:19-169) ---- */
 void or_cube_sizes(int64_t nx, int64_t ny, int64_t nz, int64_t *n_nodes,
@@ -99,6 +106,17 @@ int or_assemble_thermal(int64_t n_nodes, const double *xyz, int64_t n_elems,
 int or_assemble_elastic(int64_t n_nodes, const double *xyz, int64_t n_elems,
                         const int64_t *conn, double E, double nu,
                         const double *E_elem, int nthreads, or_csr *out);
+int or_assemble_viscosity(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                          const int64_t *conn, double mu, double zeta,
+                          const double *mu_elem, const double *zeta_elem, int nthreads,
+                          or_csr *out);
+int or_assemble_maxwell(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                        const int64_t *conn, double E, double nu, double visc, double dt,
+                        const double *stress, int nthreads, or_csr *out, double *f_hist);
+int or_update_stresses_maxwell(int64_t n_nodes, const double *xyz, int64_t n_elems,
+                               const int64_t *conn, double E, double nu, double visc,
+                               double dt, const double *u_n, const double *u_next,
+                               const double *sig_in, double *sig_out);
 void or_thermal_mass(const double nodes[10][3], double density, double specific_heat,
                      double M[10][10]);
 int or_assemble_thermal_mass(int64_t n_nodes, const double *xyz, int64_t n_elems,

@@ -106,6 +106,19 @@ def _declare(L):
     L.or_assemble_elastic.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                       C.c_double, C.c_double, _dp, C.c_int,
                                       C.POINTER(_Csr)]
+    L.or_viscous_constitutive.argtypes = [C.c_double, C.c_double, _dp]
+    L.or_viscosity_stiffness.argtypes = [_dp, C.c_double, C.c_double, _dp]
+    L.or_maxwell_element.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp, _dp]
+    L.or_maxwell_stress.argtypes = [_dp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                    _dp, _dp, _dp]
+    L.or_assemble_viscosity.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_double,
+                                        C.c_double, _dp, _dp, C.c_int, C.POINTER(_Csr)]
+    L.or_assemble_maxwell.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_double,
+                                      C.c_double, C.c_double, C.c_double, _dp, C.c_int,
+                                      C.POINTER(_Csr), _dp]
+    L.or_update_stresses_maxwell.argtypes = [C.c_int64, _dp, C.c_int64, _i64p, C.c_double,
+                                             C.c_double, C.c_double, C.c_double, _dp, _dp,
+                                             _dp, _dp]
     L.or_assemble_thermal_load.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                            C.c_double, _dp]
     L.or_face_nodes.argtypes = [C.c_int, C.POINTER(C.c_int)]
@@ -270,6 +283,39 @@ def elastic_stiffness(nodes, E, nu):
     return out
 
 
+def viscous_constitutive(mu, zeta=0.0):
+    """NewtonianViscosity::constitutive_matrix (src/mechanics/constitutive.rs:227-255)."""
+    out = np.zeros((6, 6))
+    _check(lib().or_viscous_constitutive(mu, zeta, _d(out)), "viscous_constitutive")
+    return out
+
+
+def viscosity_stiffness(nodes, mu, zeta=0.0):
+    """ElasticityElement::viscosity_matrix (src/mechanics/element.rs:87-118)."""
+    out = np.zeros((30, 30))
+    _check(lib().or_viscosity_stiffness(_d(_f64(nodes, (10, 3))), mu, zeta, _d(out)),
+           "viscosity_stiffness")
+    return out
+
+
+def maxwell_element(nodes, E, nu, sigma):
+    """ElasticityElement::maxwell_viscoelastic (src/mechanics/element.rs:257-295)
+    -> (K_e, f_history_e)."""
+    K = np.zeros((30, 30))
+    f = np.zeros(30)
+    _check(lib().or_maxwell_element(_d(_f64(nodes, (10, 3))), E, nu, _d(_f64(sigma)),
+                                    _d(K), _d(f)), "maxwell_element")
+    return K, f
+
+
+def maxwell_stress(nodes, E, nu, visc, dt, du, sigma_n):
+    """One element of update_stresses_maxwell (src/mechanics/stress_update.rs:53-145)."""
+    out = np.zeros(6)
+    _check(lib().or_maxwell_stress(_d(_f64(nodes, (10, 3))), E, nu, visc, dt, _d(_f64(du)),
+                                   _d(_f64(sigma_n)), _d(out)), "maxwell_stress")
+    return out
+
+
 def gravity_load(nodes, rho, g):
     out = np.zeros(30)
     _check(lib().or_gravity_load(_d(_f64(nodes, (10, 3))), rho, _d(_f64(g)),
@@ -358,6 +404,44 @@ def assemble_elastic(xyz, conn, E, nu, E_elem=None, nthreads=0):
                                      nu, None if ee is None else _d(ee), nthreads,
                                      C.byref(m)), "assemble_elastic")
     return _take_csr(m)
+
+
+def assemble_viscosity(xyz, conn, mu, zeta=0.0, mu_elem=None, zeta_elem=None, nthreads=0):
+    """Assembler::assemble_viscosity_stiffness_parallel (src/fem/assembly.rs:646-694)."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    me = None if mu_elem is None else _f64(mu_elem)
+    ze = None if zeta_elem is None else _f64(zeta_elem)
+    m = _Csr()
+    _check(lib().or_assemble_viscosity(len(xyz), _d(xyz), len(conn), _i(conn), mu, zeta,
+                                       None if me is None else _d(me),
+                                       None if ze is None else _d(ze), nthreads,
+                                       C.byref(m)), "assemble_viscosity")
+    return _take_csr(m)
+
+
+def assemble_maxwell(xyz, conn, E, nu, visc, dt, stress, nthreads=0):
+    """Assembler::assemble_maxwell_viscoelastic_parallel (src/fem/assembly.rs:959-1020)
+    -> (K, f_history); stress (E, 6) per element."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    sg = _f64(stress)
+    m = _Csr()
+    f = np.zeros(3 * len(xyz))
+    _check(lib().or_assemble_maxwell(len(xyz), _d(xyz), len(conn), _i(conn), E, nu, visc, dt,
+                                     _d(sg), nthreads, C.byref(m), _d(f)), "assemble_maxwell")
+    return _take_csr(m), f
+
+
+def update_stresses_maxwell(xyz, conn, E, nu, visc, dt, u_n, u_next, stress):
+    """update_stresses_maxwell (src/mechanics/stress_update.rs:39-147) -> (E, 6)."""
+    xyz = _f64(xyz)
+    conn = np.ascontiguousarray(conn, dtype=np.int64)
+    out = np.zeros((len(conn), 6))
+    _check(lib().or_update_stresses_maxwell(len(xyz), _d(xyz), len(conn), _i(conn), E, nu,
+                                            visc, dt, _d(_f64(u_n)), _d(_f64(u_next)),
+                                            _d(_f64(stress)), _d(out)), "update_stresses_maxwell")
+    return out
 
 
 def assemble_thermal_load(xyz, conn, Q):

@@ -328,11 +328,18 @@ def run_elastic(G, args):
     log(f"[bench] elastic SpMV fp64 {ms64*1e3:.1f} us fp32 {ms32*1e3:.1f} us")
     # element kernel alone: K_e (900 doubles), F_e (30) and the connectivity
     # per element, each node's coordinates once
-    eb = mesh.num_elements() * (900 * 8 + 30 * 8 + 40) + mesh.num_nodes() * 24
-    kv = {"algorithmic_bytes": eb}
-    for v in ("valu", "mfma"):
+    ne, nn = mesh.num_elements(), mesh.num_nodes()
+    kv = {"note": "B^T D B + gravity load of every element; valu = the assembly kernel "
+                  "(FP64, one lane per 3x3 node block, bit-identical to the reference "
+                  "order), mfma = v_mfma_f64_16x16x4f64, *_f32 = FP32 B^T D B (VALU, "
+                  "v_mfma_f32_16x16x4f32) writing FP32 K_e; valu_rows = one lane per K_e "
+                  "row (the earlier FP64 layout); rel_diff vs the FP64 valu K_e"}
+    for v in ("valu", "mfma", "valu_f32", "mfma_f32", "valu_rows"):
+        sz = 4 if v.endswith("_f32") else 8
+        eb = ne * (900 * sz + 30 * sz + 40) + nn * 24
         ms, d = G.time_elastic_elements(mesh, mat, None, v, reps=20)
-        kv[v] = {"ms": ms, "GBs": eb / ms / 1e6, "rel_diff_vs_valu": d}
+        kv[v] = {"ms": ms, "algorithmic_bytes": eb, "GBs": eb / ms / 1e6,
+                 "frac": eb / ms / 1e6 / HBM_PEAK_GBS, "rel_diff_vs_valu": d}
         log(f"[bench] elastic element kernel {v}: {ms:.3f} ms ({eb / ms / 1e6:.0f} GB/s) "
             f"max rel diff {d:.1e}")
     out["element_kernel"] = kv

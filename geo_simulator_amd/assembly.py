@@ -343,7 +343,7 @@ def elastic_element_backward(mesh, E_per_elem, poisson, u, lam, ctx=None):
     return g
 
 
-ELEMENT_VARIANTS = {"valu": 0, "mfma": 1}
+ELEMENT_VARIANTS = {"valu": 0, "mfma": 1, "valu_f32": 2, "mfma_f32": 3, "valu_rows": 4}
 
 
 def time_elastic_elements(mesh, material, E_per_elem=None, variant="valu", reps=10,
@@ -351,9 +351,12 @@ def time_elastic_elements(mesh, material, E_per_elem=None, variant="valu", reps=
     """Average time (ms) of one launch of the elastic element kernel (K_e and
     gravity load of every element) and the largest K_e entry difference from
     the VALU kernel relative to the largest entry -> (ms, rel_diff).
-    variant "valu" is the kernel assembly uses; "mfma" evaluates
-    K_e = sum_q w_q B_q^T D B_q as one 30x24 by 24x30 product on the matrix
-    cores (v_mfma_f64_16x16x4f64)."""
+    variant "valu" is the kernel assembly uses (FP64, one lane per 3x3 node
+    block); "mfma" evaluates K_e = sum_q w_q B_q^T D B_q as one 30x24 by
+    24x30 product on the matrix cores (v_mfma_f64_16x16x4f64); "valu_f32" and
+    "mfma_f32" are the same two in FP32 (v_mfma_f32_16x16x4f32), FP32 K_e;
+    "valu_rows" is the FP64 kernel with one lane per K_e row, staged through
+    LDS."""
     ctx = ctx or default_context()
     if variant not in ELEMENT_VARIANTS:
         raise ValueError(f"unknown element kernel variant {variant!r}")

@@ -1156,7 +1156,7 @@ gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
                                    double *max_rel_diff) {
   if (!ctx || !mesh || !avg_ms || !max_rel_diff) return GS_ERR_INVALID_ARG;
   return guard(ctx, [&] {
-    require(variant == 0 || variant == 1, GS_ERR_INVALID_ARG, "unknown kernel variant");
+    require(variant >= 0 && variant <= 4, GS_ERR_INVALID_ARG, "unknown kernel variant");
     require(reps >= 1, GS_ERR_INVALID_ARG, "reps must be positive");
     if (E_elem) {
       require(nu > -1.0 && nu < 0.5, GS_ERR_INVALID_MATERIAL,
@@ -1168,21 +1168,38 @@ gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
     hipStream_t s = ctx->stream;
     const int64_t ne = mesh->n_elems;
     const auto &pool = ctx->pool;
+    const bool f32 = variant == 2 || variant == 3;
     auto ref = palloc<double>(pool, ne * 900);
-    auto Ke = palloc<double>(pool, ne * 900);
+    auto Ke = palloc<double>(pool, f32 ? 1 : ne * 900);
     auto Fe = palloc<double>(pool, ne * 30);
+    auto Ke32 = palloc<float>(pool, f32 ? ne * 900 : 1);
+    auto Fe32 = palloc<float>(pool, f32 ? ne * 30 : 1);
     auto flag = palloc<int>(pool, 1);
     auto bits = palloc<unsigned long long>(pool, 2);
     GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
     GS_HIP(hipMemsetAsync(bits.get(), 0, 2 * sizeof(unsigned long long), s));
     const double g[3] = {0.0, 0.0, -9.81};
+    ElasticKind k;
+    k.p0 = E;
+    k.p1 = nu;
+    k.e0 = E_elem;
+    k.load = GS_LOAD_BODY;
+    for (int c = 0; c < 3; ++c) k.bf[c] = 3300.0 * g[c];
     auto run = [&](int v, double *out) {
-      if (v == 0)
-        launch_elastic_elements(s, ne, mesh->xyz.get(), mesh->conn.get(), E, nu, E_elem,
-                                3300.0, g, true, out, Fe.get(), flag.get());
-      else
-        launch_elastic_elements_mfma(s, ne, mesh->xyz.get(), mesh->conn.get(), E, nu,
-                                     E_elem, 3300.0, g, true, out, Fe.get(), flag.get());
+      switch (v) {
+      case 0: launch_elastic_family(s, ne, mesh->xyz.get(), mesh->conn.get(), k, out,
+                                    Fe.get(), flag.get()); break;
+      case 1: launch_elastic_elements_mfma(s, ne, mesh->xyz.get(), mesh->conn.get(), E, nu,
+                                           E_elem, 3300.0, g, true, out, Fe.get(), flag.get());
+              break;
+      case 2: launch_elastic_family_f32(s, ne, mesh->xyz.get(), mesh->conn.get(), k,
+                                        Ke32.get(), Fe32.get(), flag.get()); break;
+      case 3: launch_elastic_elements_mfma_f32(s, ne, mesh->xyz.get(), mesh->conn.get(), E,
+                                               nu, E_elem, 3300.0, g, true, Ke32.get(),
+                                               Fe32.get(), flag.get()); break;
+      default: launch_elastic_family_rows(s, ne, mesh->xyz.get(), mesh->conn.get(), k, out,
+                                          Fe.get(), flag.get());
+      }
     };
     run(0, ref.get());
     run(variant, Ke.get());
@@ -1198,7 +1215,10 @@ gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     check_flags(read_flag_and_clear(ctx, flag.get()));
-    launch_maxdiff(s, ne * 900, Ke.get(), ref.get(), bits.get());
+    if (f32)
+      launch_maxdiff_f32(s, ne * 900, Ke32.get(), ref.get(), bits.get());
+    else
+      launch_maxdiff(s, ne * 900, Ke.get(), ref.get(), bits.get());
     unsigned long long hb[2] = {0, 0};
     GS_HIP(hipMemcpyAsync(ctx->pinned, bits.get(), sizeof(hb), hipMemcpyDeviceToHost, s));
     GS_HIP(hipStreamSynchronize(s));

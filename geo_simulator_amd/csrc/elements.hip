@@ -285,6 +285,15 @@ __device__ __forceinline__ DLaw d_law(const ElasticKind &k, int64_t e) {
   return d;
 }
 
+// d_law from the law's two values (elastic: E, nu; viscous: mu, zeta).
+template <int MAT>
+__device__ __forceinline__ DLaw d_law_of(double a, double b) {
+  ElasticKind k;
+  k.p0 = a;
+  k.p1 = b;
+  return d_law<MAT>(k, 0);
+}
+
 template <int MAT, int LOAD>
 __global__ __launch_bounds__(128) void k_elastic_elements(
     int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
@@ -396,6 +405,251 @@ __global__ __launch_bounds__(128) void k_elastic_elements(
   }
 }
 
+// Entry (r, c) of inverse3's adjugate/det inverse and its det, with
+// inverse3's expressions (so every lane's det carries the same bits).
+__device__ __forceinline__ double inverse_entry(const double m[3][3], int rc, double &det) {
+  const double m12_23 = m[1][1] * m[2][2] - m[2][1] * m[1][2];
+  const double m11_23 = m[1][0] * m[2][2] - m[2][0] * m[1][2];
+  const double m11_22 = m[1][0] * m[2][1] - m[2][0] * m[1][1];
+  det = m[0][0] * m12_23 - m[0][1] * m11_23 + m[0][2] * m11_22;
+  switch (rc) {
+  case 0: return m12_23 / det;
+  case 1: return (m[0][2] * m[2][1] - m[2][2] * m[0][1]) / det;
+  case 2: return (m[0][1] * m[1][2] - m[1][1] * m[0][2]) / det;
+  case 3: return -m11_23 / det;
+  case 4: return (m[0][0] * m[2][2] - m[2][0] * m[0][2]) / det;
+  case 5: return (m[0][2] * m[1][0] - m[1][2] * m[0][0]) / det;
+  case 6: return m11_22 / det;
+  case 7: return (m[0][1] * m[2][0] - m[2][1] * m[0][0]) / det;
+  default: return (m[0][0] * m[1][1] - m[1][0] * m[0][1]) / det;
+  }
+}
+
+template <int NE, int NT, typename TG>
+__device__ __forceinline__ void stage_jacobians(int nel, double (&sX)[NE][10][3],
+                                                double (&sJ)[NE][4][3][3],
+                                                double (&sW)[NE][4], TG (&sG)[NE][4][10][3],
+                                                int &bad, const Tables *tb = &kT);
+
+// The workgroup's element geometry in LDS, one lane per entry so the
+// latency-bound part (J from ten nodes, nine FP64 divisions of the inverse)
+// runs on many lanes instead of one per element and point: coordinates,
+// J and J^-1 per quadrature point (jacobian's and inverse3's expressions),
+// w = |det| / 24, and the forty gradients (gradients' expression), rounded
+// to TG.  nel live elements from e0; every thread of the workgroup calls it.
+template <int NE, int NT, typename TG>
+__device__ __forceinline__ void stage_geometry(int nel, int64_t e0, const double *__restrict__ xyz,
+                                               const int *__restrict__ conn,
+                                               double (&sX)[NE][10][3],
+                                               double (&sJ)[NE][4][3][3],
+                                               double (&sW)[NE][4], TG (&sG)[NE][4][10][3],
+                                               int &bad) {
+  static_assert(NE * 36 <= NT, "one lane per J entry");
+  const int t = threadIdx.x, nt = NT;
+  for (int k = t; k < nel * 30; k += nt) {
+    const int le = k / 30, r = k - 30 * (k / 30);
+    const int node = r / 3, c = r - 3 * (r / 3);
+    sX[le][node][c] = xyz[(int64_t)conn[(e0 + le) * 10 + node] * 3 + c];
+  }
+  __syncthreads();
+  stage_jacobians<NE, NT, TG>(nel, sX, sJ, sW, sG, bad);
+}
+
+// The part of stage_geometry after the coordinates are in sX.
+template <int NE, int NT, typename TG>
+__device__ __forceinline__ void stage_jacobians(int nel, double (&sX)[NE][10][3],
+                                                double (&sJ)[NE][4][3][3],
+                                                double (&sW)[NE][4], TG (&sG)[NE][4][10][3],
+                                                int &bad, const Tables *tb) {
+  static_assert(NE * 36 <= NT, "one lane per J entry");
+  const int t = threadIdx.x, nt = NT;
+  double jv = 0.0;  // J[r][c] of this lane's (element, point, entry)
+  if (t < nel * 36) {
+    const int k = t;
+    const int le = k / 36, r9 = k - 36 * (k / 36);
+    const int q = r9 / 9, rc = r9 - 9 * (r9 / 9);
+    const int r = rc / 3, c = rc - 3 * (rc / 3);
+    double acc = 0.0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i) acc += sX[le][i][r] * tb->dl[q][i][c];
+    jv = acc;
+  }
+  __syncthreads();
+  if (t < nel * 36) {
+    const int k = t;
+    const int le = k / 36, r9 = k - 36 * (k / 36);
+    const int q = r9 / 9, rc = r9 - 9 * (r9 / 9);
+    sJ[le][q][rc / 3][rc - 3 * (rc / 3)] = jv;
+  }
+  __syncthreads();
+  double iv = 0.0;
+  if (t < nel * 36) {
+    const int k = t;
+    const int le = k / 36, r9 = k - 36 * (k / 36);
+    const int q = r9 / 9, rc = r9 - 9 * (r9 / 9);
+    double m[3][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) m[r][c] = sJ[le][q][r][c];
+    double det;
+    iv = inverse_entry(m, rc, det);
+    if (rc == 0) {
+      bad |= det_flags(det);
+      sW[le][q] = QW * fabs(det);
+    }
+  }
+  __syncthreads();  // every lane has read J
+  if (t < nel * 36) {
+    const int k = t;
+    const int le = k / 36, r9 = k - 36 * (k / 36);
+    const int q = r9 / 9, rc = r9 - 9 * (r9 / 9);
+    sJ[le][q][rc / 3][rc - 3 * (rc / 3)] = iv;  // J^-1 replaces J
+  }
+  __syncthreads();
+  for (int k = t; k < nel * 120; k += nt) {
+    const int le = k / 120, r = k - 120 * (k / 120);
+    const int q = r / 30, ic = r - 30 * (r / 30);
+    const int i = ic / 3, c = ic - 3 * (ic / 3);
+    const double v0 = tb->dl[q][i][0], v1 = tb->dl[q][i][1], v2 = tb->dl[q][i][2];
+    sG[le][q][i][c] = (TG)(sJ[le][q][0][c] * v0 + sJ[le][q][1][c] * v1 + sJ[le][q][2][c] * v2);
+  }
+  __syncthreads();
+}
+
+// ---------------------------------------------------------------------------
+// Elasticity family, one lane per 3x3 node block (i, j) of K_e: 100 lanes per
+// element, EB_ELEMS elements per workgroup of EB_THREADS.
+//
+//   1. the workgroup stages its elements' geometry in LDS (stage_jacobians:
+//      J^-1 and w per quadrature point, the forty gradients per element);
+//      the law parameters (per element or uniform) and stress history come
+//      through LDS too;
+//   2. lane (i, j) sums the nine entries (3i+ci, 3j+cb) of B^T (D B) over the
+//      four points.  Column 3i+ci of B is structurally nonzero in three
+//      Voigt rows (ci=0: 0,3,5; ci=1: 1,3,4; ci=2: 2,4,5) and (D B)[k][3j+cb]
+//      has one structural term (D[k][cb] grad_j[cb] on the normal rows, Ds
+//      times a gradient entry on the shear rows), so each entry is the dense
+//      reference product's nonzero terms in ascending Voigt row, the dropped
+//      terms being exact zeros; with ci and cb unrolled no lane selects;
+//   3. a wavefront's 64 blocks (64 x 9 values, contiguous in the block-major
+//      layout Ke[e*900 + (i*10 + j)*9 + ci*3 + cb]) go through LDS so that
+//      every store writes 64 consecutive values.
+// T = double is the assembly kernel (bit-identical to the reference order);
+// T = float is the FP32 B^T D B variant (geometry in FP64, rounded to FP32
+// gradients and weights; products and sums in FP32).
+// ---------------------------------------------------------------------------
+constexpr int EB_ELEMS = 5;
+constexpr int EB_THREADS = 512;
+
+template <typename T, int MAT, int LOAD>
+__global__ __launch_bounds__(EB_THREADS) void k_elastic_blocks(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
+    const ElasticKind kind, T *__restrict__ Ke, T *__restrict__ Fe, int *__restrict__ flag) {
+  __shared__ double sX[EB_ELEMS][10][3];
+  __shared__ double sJ[EB_ELEMS][4][3][3];
+  __shared__ double sW[EB_ELEMS][4];
+  __shared__ double sP[EB_ELEMS][2];  // per-element law parameters (e0, e1)
+  __shared__ double sS[EB_ELEMS][6];  // per-element stress (GS_LOAD_STRESS)
+  __shared__ T sG[EB_ELEMS][4][10][3];
+  __shared__ T sOut[EB_THREADS / 64][64 * 9];
+  const int t = threadIdx.x;
+  const int64_t g = blockIdx.x;
+  const int64_t e0 = g * EB_ELEMS;
+  const int nel = (int)(E - e0 < EB_ELEMS ? E - e0 : EB_ELEMS);
+  int bad = 0;
+  // lanes t < 30 nel load one coordinate each, lanes 160.. the law
+  // parameters (two per element), lanes 192.. the stress components
+  if (t < 30 * nel) {
+    const int le = t / 30, r = t - 30 * (t / 30);
+    const int node = r / 3, c = r - 3 * (r / 3);
+    sX[le][node][c] = xyz[(int64_t)conn[(e0 + le) * 10 + node] * 3 + c];
+  }
+  if (t >= 160 && t < 160 + 2 * nel) {
+    const int le = (t - 160) >> 1;
+    const double *src = (t & 1) ? kind.e1 : kind.e0;
+    sP[le][t & 1] = src ? src[e0 + le] : ((t & 1) ? kind.p1 : kind.p0);
+  }
+  if (LOAD == GS_LOAD_STRESS && t >= 192 && t < 192 + 6 * nel)
+    sS[(t - 192) / 6][(t - 192) % 6] = kind.sig[e0 * 6 + (t - 192)];
+  __syncthreads();
+  {
+    const Tables *tb = &kT;
+    stage_jacobians<EB_ELEMS, EB_THREADS, T>(nel, sX, sJ, sW, sG, bad, tb);
+    const int le = t / 100, p = t - 100 * (t / 100);
+    T K[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) K[m] = (T)0;
+    if (le < nel) {
+      const int64_t e = e0 + le;
+      const int i = p / 10, j = p - 10 * (p / 10);
+      const DLaw law = d_law_of<MAT>(sP[le][0], sP[le][1]);
+      if (!law.ok && p == 0) bad |= FLAG_MATERIAL;
+      const T Dd = (T)law.Dd, Do = (T)law.Do, Ds = (T)law.Ds;
+      T f0 = (T)0, f1 = (T)0, f2 = (T)0;
+      T s0 = (T)0, s1 = (T)0, s2 = (T)0, s3 = (T)0, s4 = (T)0, s5 = (T)0;
+      if (LOAD == GS_LOAD_STRESS && j == 0) {
+        s0 = (T)sS[le][0], s1 = (T)sS[le][1], s2 = (T)sS[le][2];
+        s3 = (T)sS[le][3], s4 = (T)sS[le][4], s5 = (T)sS[le][5];
+      }
+#pragma unroll 1
+      for (int q = 0; q < 4; ++q) {
+        const T w = (T)sW[le][q];
+        const T a0 = sG[le][q][i][0], a1 = sG[le][q][i][1], a2 = sG[le][q][i][2];
+        const T b0 = sG[le][q][j][0], b1 = sG[le][q][j][1], b2 = sG[le][q][j][2];
+        const T dd0 = Dd * b0, dd1 = Dd * b1, dd2 = Dd * b2;
+        const T do0 = Do * b0, do1 = Do * b1, do2 = Do * b2;
+        const T ds0 = Ds * b0, ds1 = Ds * b1, ds2 = Ds * b2;
+        // ci = 0 (rows 0, 3, 5)
+        K[0] += w * (a0 * dd0 + a1 * ds1 + a2 * ds2);
+        K[1] += w * (a0 * do1 + a1 * ds0);
+        K[2] += w * (a0 * do2 + a2 * ds0);
+        // ci = 1 (rows 1, 3, 4)
+        K[3] += w * (a1 * do0 + a0 * ds1);
+        K[4] += w * (a1 * dd1 + a0 * ds0 + a2 * ds2);
+        K[5] += w * (a1 * do2 + a2 * ds1);
+        // ci = 2 (rows 2, 4, 5)
+        K[6] += w * (a2 * do0 + a0 * ds2);
+        K[7] += w * (a2 * do1 + a1 * ds2);
+        K[8] += w * (a2 * dd2 + a1 * ds1 + a0 * ds0);
+        if (LOAD == GS_LOAD_BODY && j == 0) {
+          const T wn = w * (T)tb->N[q][i];
+          f0 += wn * (T)kind.bf[0];
+          f1 += wn * (T)kind.bf[1];
+          f2 += wn * (T)kind.bf[2];
+        }
+        // f_history -= w * (B^T sigma)[3i+ci], nonzero terms in ascending row
+        if (LOAD == GS_LOAD_STRESS && j == 0) {
+          f0 -= w * (a0 * s0 + a1 * s3 + a2 * s5);
+          f1 -= w * (a1 * s1 + a0 * s3 + a2 * s4);
+          f2 -= w * (a2 * s2 + a1 * s4 + a0 * s5);
+        }
+      }
+      if (LOAD != GS_LOAD_NONE && j == 0) {
+        T *fo = Fe + e * 30 + 3 * i;
+        fo[0] = f0;
+        fo[1] = f1;
+        fo[2] = f2;
+      }
+    }
+    // this wavefront's blocks t = 64 wv + l are the values
+    // [(64 wv + l) * 9, + 9) of the group's contiguous output
+    const int wv = t >> 6, l = t & 63;
+    T *buf = sOut[wv];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) buf[l * 9 + m] = K[m];
+    __syncthreads();
+    const int64_t valid = (int64_t)nel * 900 - (int64_t)wv * 576;  // values of this wave
+    T *out = Ke + e0 * 900 + (int64_t)wv * 576;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+      const int idx = k * 64 + l;
+      if (idx < valid) out[idx] = buf[idx];
+    }
+  }
+  if (bad) atomicOr(flag, bad);
+}
+
 // ---------------------------------------------------------------------------
 // Elasticity on the matrix cores (timing variant; not bit-identical).
 //
@@ -413,143 +667,127 @@ __global__ __launch_bounds__(128) void k_elastic_elements(
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // B_q[v][a] for a = 3i+c (src/mechanics/strain.rs:38-71 Voigt layout)
-__device__ __forceinline__ double b_entry(int v, int c, double g0, double g1, double g2) {
-  const double gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
+template <typename T>
+__device__ __forceinline__ T b_entry(int v, int c, T g0, T g1, T g2) {
+  const T gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
   switch (v) {
-  case 0: case 1: case 2: return v == c ? gc : 0.0;
-  case 3: return c == 0 ? g1 : (c == 1 ? g0 : 0.0);
-  case 4: return c == 0 ? 0.0 : (c == 1 ? g2 : g1);
-  default: return c == 0 ? g2 : (c == 1 ? 0.0 : g0);
+  case 0: case 1: case 2: return v == c ? gc : (T)0;
+  case 3: return c == 0 ? g1 : (c == 1 ? g0 : (T)0);
+  case 4: return c == 0 ? (T)0 : (c == 1 ? g2 : g1);
+  default: return c == 0 ? g2 : (c == 1 ? (T)0 : g0);
   }
 }
 
-template <bool PER_ELEM, bool WITH_LOAD>
+template <typename T, bool PER_ELEM, bool WITH_LOAD>
 __global__ __launch_bounds__(256) void k_elastic_elements_mfma(
     int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
     double Emod, double nu, const double *__restrict__ E_elem, double bfx,
-    double bfy, double bfz, double *__restrict__ Ke, double *__restrict__ Fe,
+    double bfy, double bfz, T *__restrict__ Ke, T *__restrict__ Fe,
     int *__restrict__ flag) {
   __shared__ double sX[EL_PER_WG][10][3];
-  __shared__ double sJi[EL_PER_WG][4][3][3];
+  __shared__ double sJ[EL_PER_WG][4][3][3];
   __shared__ double sW[EL_PER_WG][4];
   __shared__ double sG[EL_PER_WG][4][10][3];
-  __shared__ double sK[EL_PER_WG][900];
+  __shared__ T sK[EL_PER_WG][900];
   const int le = threadIdx.x >> 6;  // element (wavefront) within the workgroup
   const int l = threadIdx.x & 63;   // lane
-  const int64_t e = (int64_t)blockIdx.x * EL_PER_WG + le;
+  const int64_t e0 = (int64_t)blockIdx.x * EL_PER_WG;
+  const int64_t e = e0 + le;
   const bool live = e < E;
+  const int nel = (int)(E - e0 < EL_PER_WG ? E - e0 : EL_PER_WG);
   int bad = 0;
-  if (live && l < 30) {
-    const int node = l / 3, r = l - 3 * (l / 3);
-    sX[le][node][r] = xyz[(int64_t)conn[e * 10 + node] * 3 + r];
-  }
-  __syncthreads();
-  if (live && l < 4) {
-    double X[10][3], J[3][3], Ji[3][3];
-#pragma unroll
-    for (int i = 0; i < 10; ++i)
-#pragma unroll
-      for (int r = 0; r < 3; ++r) X[i][r] = sX[le][i][r];
-    jacobian(l, X, J);
-    const double det = inverse3(J, Ji);
-    bad |= det_flags(det);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-      for (int c = 0; c < 3; ++c) sJi[le][l][r][c] = Ji[r][c];
-    sW[le][l] = QW * fabs(det);
-  }
-  __syncthreads();
-  if (live && l < 40) {
-    const int q = l / 10, i = l - 10 * (l / 10);
-    const double v0 = kT.dl[q][i][0], v1 = kT.dl[q][i][1], v2 = kT.dl[q][i][2];
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-      sG[le][q][i][r] =
-          sJi[le][q][0][r] * v0 + sJi[le][q][1][r] * v1 + sJi[le][q][2][r] * v2;
-  }
-  __syncthreads();
+  stage_geometry<EL_PER_WG, 256, double>(nel, e0, xyz, conn, sX, sJ, sW, sG, bad);
   const double Ee = live ? (PER_ELEM ? E_elem[e] : Emod) : 1.0;
   if (live && l == 0 && !(Ee > 0.0)) bad |= FLAG_MATERIAL;
   const double factor = Ee / ((1.0 + nu) * (1.0 - 2.0 * nu));
-  const double Dd = (1.0 - nu) * factor;
-  const double Do = nu * factor;
-  const double Ds = ((1.0 - 2.0 * nu) / 2.0) * factor;
+  const T Dd = (T)((1.0 - nu) * factor);
+  const T Do = (T)(nu * factor);
+  const T Ds = (T)(((1.0 - 2.0 * nu) / 2.0) * factor);
   if (live) {
     // operands: a_op[I][s] = A[16I + l%16][4s + l/16], c_op[J][s] = C[4s + l/16][16J + l%16]
-    double a_op[2][6], c_op[2][6];
+    T a_op[2][6], c_op[2][6];
 #pragma unroll
     for (int s6 = 0; s6 < 6; ++s6) {
       const int kk = 4 * s6 + (l >> 4);
       const int q = kk / 6, v = kk - 6 * (kk / 6);
-      const double w = sW[le][q];
+      const T w = (T)sW[le][q];
 #pragma unroll
-      for (int T = 0; T < 2; ++T) {
-        const int a = 16 * T + (l & 15);
-        double av = 0.0, cv = 0.0;
+      for (int I = 0; I < 2; ++I) {
+        const int a = 16 * I + (l & 15);
+        T av = (T)0, cv = (T)0;
         if (a < 30) {
           const int i = a / 3, c = a - 3 * (a / 3);
-          const double g0 = sG[le][q][i][0], g1 = sG[le][q][i][1], g2 = sG[le][q][i][2];
-          const double bv = b_entry(v, c, g0, g1, g2);
+          const T g0 = (T)sG[le][q][i][0], g1 = (T)sG[le][q][i][1], g2 = (T)sG[le][q][i][2];
+          const T bv = (T)b_entry(v, c, g0, g1, g2);
           av = w * bv;
           if (v < 3) {
-            const double gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
+            const T gc = c == 0 ? g0 : (c == 1 ? g1 : g2);
             cv = (v == c ? Dd : Do) * gc;
           } else {
             cv = Ds * bv;
           }
         }
-        a_op[T][s6] = av;
-        c_op[T][s6] = cv;
+        a_op[I][s6] = av;
+        c_op[I][s6] = cv;
       }
     }
-    dbl4 acc[2][2];
+    typedef T acc4 __attribute__((ext_vector_type(4)));
+    acc4 acc[2][2];
 #pragma unroll
     for (int I = 0; I < 2; ++I)
 #pragma unroll
       for (int J = 0; J < 2; ++J) {
-        acc[I][J] = (dbl4){0.0, 0.0, 0.0, 0.0};
+        acc[I][J] = (acc4){(T)0, (T)0, (T)0, (T)0};
 #pragma unroll
-        for (int s6 = 0; s6 < 6; ++s6)
-          acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op[I][s6], c_op[J][s6],
-                                                           acc[I][J], 0, 0, 0);
+        for (int s6 = 0; s6 < 6; ++s6) {
+          if constexpr (sizeof(T) == 8)
+            acc[I][J] = __builtin_amdgcn_mfma_f64_16x16x4f64(a_op[I][s6], c_op[J][s6],
+                                                             acc[I][J], 0, 0, 0);
+          else
+            acc[I][J] = __builtin_amdgcn_mfma_f32_16x16x4f32(a_op[I][s6], c_op[J][s6],
+                                                             acc[I][J], 0, 0, 0);
+        }
       }
+    // output lane layout (measured on the part): f64 16x16x4 holds row
+    // 4t + l/16 in element t, f32 16x16x4 row 4 (l/16) + t; column l % 16
 #pragma unroll
     for (int I = 0; I < 2; ++I)
 #pragma unroll
       for (int J = 0; J < 2; ++J)
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
-          const int row = 16 * I + 4 * t + (l >> 4), col = 16 * J + (l & 15);
+          const int rr = sizeof(T) == 8 ? 4 * t + (l >> 4) : 4 * (l >> 4) + t;
+          const int row = 16 * I + rr, col = 16 * J + (l & 15);
           if (row < 30 && col < 30)
             sK[le][((row / 3) * 10 + col / 3) * 9 + (row % 3) * 3 + col % 3] = acc[I][J][t];
         }
     if (WITH_LOAD && l < 30) {
       const int ia = l / 3, ca = l - 3 * (l / 3);
-      const double bfa = ca == 0 ? bfx : (ca == 1 ? bfy : bfz);
-      double f = 0.0;
+      const T bfa = (T)(ca == 0 ? bfx : (ca == 1 ? bfy : bfz));
+      T f = (T)0;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) f += sW[le][q] * kT.N[q][ia] * bfa;
+      for (int q = 0; q < 4; ++q) f += (T)sW[le][q] * (T)kT.N[q][ia] * bfa;
       Fe[e * 30 + l] = f;
     }
   }
   if (bad) atomicOr(flag, bad);
   __syncthreads();
   if (live) {
-    double *out = Ke + e * 900;
+    T *out = Ke + e * 900;
     for (int t = l; t < 900; t += 64) out[t] = sK[le][t];
   }
 }
 
 // max |a - b| and max |b| as non-negative double bit patterns (integer max is
 // order independent)
-__global__ void k_maxdiff(int64_t n, const double *__restrict__ a,
+template <typename TA>
+__global__ void k_maxdiff(int64_t n, const TA *__restrict__ a,
                           const double *__restrict__ b,
                           unsigned long long *__restrict__ out) {
   unsigned long long md = 0, mb = 0;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (int64_t)gridDim.x * blockDim.x) {
-    const double d = fabs(a[i] - b[i]), v = fabs(b[i]);
+    const double d = fabs((double)a[i] - b[i]), v = fabs(b[i]);
     const unsigned long long bd = (unsigned long long)__double_as_longlong(d == d ? d : __builtin_inf());
     const unsigned long long bv = (unsigned long long)__double_as_longlong(v);
     md = bd > md ? bd : md;
@@ -889,14 +1127,18 @@ void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
   GS_LAUNCH_CHECK("k_thermal_elements");
 }
 
-void launch_elastic_family(hipStream_t s, int64_t E, const double *xyz, const int *conn,
-                           const ElasticKind &k, double *Ke, double *Fe, int *flag) {
+template <typename T, bool ROWS>
+static void launch_family(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                          const ElasticKind &k, T *Ke, T *Fe, int *flag) {
   if (E == 0) return;
-  const unsigned g = blocks_for(E, EL_PER_WG);
   auto go = [&](auto mat, auto load) {
     constexpr int M = decltype(mat)::value, L = decltype(load)::value;
-    hipLaunchKernelGGL((k_elastic_elements<M, L>), g, 128, 0, s, E, xyz, conn, k, Ke, Fe,
-                       flag);
+    if constexpr (ROWS)
+      hipLaunchKernelGGL((k_elastic_elements<M, L>), blocks_for(E, EL_PER_WG), 128, 0, s, E,
+                         xyz, conn, k, Ke, Fe, flag);
+    else
+      hipLaunchKernelGGL((k_elastic_blocks<T, M, L>), blocks_for(E, EB_ELEMS), EB_THREADS, 0, s,
+                         E, xyz, conn, k, Ke, Fe, flag);
   };
   using ME = std::integral_constant<int, GS_MAT_ELASTIC>;
   using MV = std::integral_constant<int, GS_MAT_VISCOUS>;
@@ -912,6 +1154,21 @@ void launch_elastic_family(hipStream_t s, int64_t E, const double *xyz, const in
     else go(ME{}, LN{});
   }
   GS_LAUNCH_CHECK("k_elastic_elements");
+}
+
+void launch_elastic_family(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                           const ElasticKind &k, double *Ke, double *Fe, int *flag) {
+  launch_family<double, false>(s, E, xyz, conn, k, Ke, Fe, flag);
+}
+
+void launch_elastic_family_rows(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                                const ElasticKind &k, double *Ke, double *Fe, int *flag) {
+  launch_family<double, true>(s, E, xyz, conn, k, Ke, Fe, flag);
+}
+
+void launch_elastic_family_f32(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                               const ElasticKind &k, float *Ke, float *Fe, int *flag) {
+  launch_family<float, false>(s, E, xyz, conn, k, Ke, Fe, flag);
 }
 
 void launch_elastic_elements(hipStream_t s, int64_t E, const double *xyz,
@@ -958,25 +1215,42 @@ void launch_elastic_backward(hipStream_t s, int64_t E, const double *xyz,
   GS_LAUNCH_CHECK("k_elastic_backward");
 }
 
+template <typename T>
+static void launch_mfma(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                        double Emod, double nu, const double *E_elem, double rho,
+                        const double g3[3], bool with_load, T *Ke, T *Fe, int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E, EL_PER_WG);
+  const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
+  auto go = [&](auto pe, auto wl) {
+    constexpr bool P = decltype(pe)::value, W = decltype(wl)::value;
+    hipLaunchKernelGGL((k_elastic_elements_mfma<T, P, W>), g, 256, 0, s, E, xyz, conn, Emod, nu,
+                       E_elem, bfx, bfy, bfz, Ke, Fe, flag);
+  };
+  using Y = std::true_type;
+  using N = std::false_type;
+  if (E_elem) {
+    if (with_load) go(Y{}, Y{});
+    else go(Y{}, N{});
+  } else {
+    if (with_load) go(N{}, Y{});
+    else go(N{}, N{});
+  }
+  GS_LAUNCH_CHECK("k_elastic_elements_mfma");
+}
+
 void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
                                   const int *conn, double Emod, double nu,
                                   const double *E_elem, double rho, const double g3[3],
                                   bool with_load, double *Ke, double *Fe, int *flag) {
-  if (E == 0) return;
-  const unsigned g = blocks_for(E, EL_PER_WG);
-  const double bfx = rho * g3[0], bfy = rho * g3[1], bfz = rho * g3[2];
-  if (E_elem) {
-    if (with_load)
-      hipLaunchKernelGGL((k_elastic_elements_mfma<true, true>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-    else
-      hipLaunchKernelGGL((k_elastic_elements_mfma<true, false>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-  } else {
-    if (with_load)
-      hipLaunchKernelGGL((k_elastic_elements_mfma<false, true>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-    else
-      hipLaunchKernelGGL((k_elastic_elements_mfma<false, false>), g, 256, 0, s, E, xyz, conn, Emod, nu, E_elem, bfx, bfy, bfz, Ke, Fe, flag);
-  }
-  GS_LAUNCH_CHECK("k_elastic_elements_mfma");
+  launch_mfma<double>(s, E, xyz, conn, Emod, nu, E_elem, rho, g3, with_load, Ke, Fe, flag);
+}
+
+void launch_elastic_elements_mfma_f32(hipStream_t s, int64_t E, const double *xyz,
+                                      const int *conn, double Emod, double nu,
+                                      const double *E_elem, double rho, const double g3[3],
+                                      bool with_load, float *Ke, float *Fe, int *flag) {
+  launch_mfma<float>(s, E, xyz, conn, Emod, nu, E_elem, rho, g3, with_load, Ke, Fe, flag);
 }
 
 void launch_thermal_mass(hipStream_t s, int64_t E, const double *xyz, const int *conn,
@@ -1003,7 +1277,14 @@ void launch_face_flux(hipStream_t s, int64_t F, const double *xyz, const int *co
 void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,
                     unsigned long long *out) {
   if (n == 0) return;
-  hipLaunchKernelGGL(k_maxdiff, 2048, 256, 0, s, n, a, b, out);
+  hipLaunchKernelGGL(k_maxdiff<double>, 2048, 256, 0, s, n, a, b, out);
+  GS_LAUNCH_CHECK("k_maxdiff");
+}
+
+void launch_maxdiff_f32(hipStream_t s, int64_t n, const float *a, const double *b,
+                        unsigned long long *out) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_maxdiff<float>, 2048, 256, 0, s, n, a, b, out);
   GS_LAUNCH_CHECK("k_maxdiff");
 }
 

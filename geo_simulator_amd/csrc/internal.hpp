@@ -286,6 +286,16 @@ void launch_elastic_elements_mfma(hipStream_t s, int64_t E, const double *xyz,
 // out[0] = bits of max |a - b|, out[1] = bits of max |b| (out preset by caller)
 void launch_maxdiff(hipStream_t s, int64_t n, const double *a, const double *b,
                     unsigned long long *out);
+void launch_maxdiff_f32(hipStream_t s, int64_t n, const float *a, const double *b,
+                        unsigned long long *out);
+void launch_elastic_family_rows(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                                const ElasticKind &k, double *Ke, double *Fe, int *flag);
+void launch_elastic_family_f32(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                               const ElasticKind &k, float *Ke, float *Fe, int *flag);
+void launch_elastic_elements_mfma_f32(hipStream_t s, int64_t E, const double *xyz,
+                                      const int *conn, double Emod, double nu,
+                                      const double *E_elem, double rho, const double g3[3],
+                                      bool with_load, float *Ke, float *Fe, int *flag);
 void launch_thermal_backward(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, const double *k_elem,
                              const double *u, const double *lam, double *grad,

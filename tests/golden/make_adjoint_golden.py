@@ -108,9 +108,12 @@ def elastic():
     rhs = np.zeros(len(y))
     rhs[obs] = y[obs] - y_true[obs]
     rhs[mask == 1] = 0.0
-    # the reverse solve on the unit-norm right-hand side (the library's
-    # scale-free convention; CG is scale invariant with abs_tol 0)
-    s = np.linalg.norm(rhs)
+    # the reverse solve on the right-hand side scaled to the forward load's
+    # norm: CG is scale invariant with abs_tol 0, and at that scale the
+    # reference's absolute breakdown test |p.Ap| < 1e-30 stays out of reach
+    # (on a unit-norm right-hand side the elastic operator's 1e15-sized
+    # entries drive p.Ap below it before rel-res 1e-12)
+    s = np.linalg.norm(rhs) / np.linalg.norm(fb)
     lam, st_a = _solve(O, Kb, rhs / s)
     lam *= s
     g = O.elastic_element_backward(xyz, conn, E, 0.25, y, lam)

@@ -69,11 +69,11 @@ def host_cpu():
     return {"cpu_model": model, "nproc": os.cpu_count(), "usable_cpus": usable}
 
 
-def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
+def cpu_baseline(nx, ny, nz, threads):
     """Oracle (C restatement of the reference algorithm) on the same workload:
     parallel element loop, serial triplet merge / CSR / Dirichlet / CG, as in
-    the reference.  Assembly is timed in full; CG is timed over
-    `budget_iters` iterations and extrapolated to `gpu_iters`."""
+    the reference, all timed: assembly, and Jacobi-CG to rel-res 1e-8.  A few
+    BiCGSTAB passes and GMRES steps give their per-pass costs."""
     from oracle import oracle as O
     xyz, conn = O.generate_cube(nx, ny, nz, 1000.0, 1000.0, 600.0)
     mask, vals = O.thermal_dirichlet(xyz, 600.0)
@@ -84,17 +84,16 @@ def cpu_baseline(nx, ny, nz, gpu_iters, threads, budget_iters=25):
     t_asm = time.perf_counter() - t0
     del K
     t0 = time.perf_counter()
-    _, st = O.cg_solve(Kbc, fbc, max_iter=budget_iters, tol=1e-8)
+    _, st = O.cg_solve(Kbc, fbc, max_iter=5000, tol=1e-8)
     t_cg = time.perf_counter() - t0
     per_iter = t_cg / max(1, st.iterations + 1)
-    total = t_asm + per_iter * (gpu_iters + 1)
     t0 = time.perf_counter()
     _, sb = O.bicgstab_solve(Kbc, fbc, max_iter=5, tol=1e-8)
     bicg_per_iter = (time.perf_counter() - t0) / max(1, sb.iterations)
     t0 = time.perf_counter()
     _, sg = O.gmres_solve(Kbc, fbc, max_iter=10, restart=30, tol=1e-8)
     gm_per_step = (time.perf_counter() - t0) / max(1, sg.iterations)
-    return total, t_asm, per_iter, bicg_per_iter, gm_per_step
+    return t_asm + t_cg, t_asm, t_cg, st, per_iter, bicg_per_iter, gm_per_step
 
 
 def make_cg(G, prec, tol, inner_tol=None, abs_tol=None, max_iter=200000):
@@ -202,9 +201,10 @@ def run_headline(G, args, nx, ny, nz):
                 matched[prec] = dict(reached=False)
                 continue
             cg = make_cg(G, prec, pick, args.inner_tol)
-            wall, ta, ts, (_, _, x, st) = timed_steps(assemble, cg, 1, 1)
+            wall, ta, ts, (_, _, x, st) = timed_steps(assemble, cg, 3, 1)
             err = float(np.linalg.norm(x.cpu().numpy() - exact) / np.linalg.norm(exact))
-            matched[prec] = dict(reached=True, rel_tol=pick, step_s=wall, iterations=st.iterations,
+            matched[prec] = dict(reached=True, rel_tol=pick, step_s=wall, steps=3,
+                                 iterations=st.iterations,
                                  outer=st.outer_iterations, rel_l2_vs_closed_form=err)
             log(f"[bench] rel L2 <= 1e-10 {prec}: tol {pick:g} {wall*1e3:.2f} ms "
                 f"it={st.iterations} relL2={err:.2e}")
@@ -226,25 +226,34 @@ def run_headline(G, args, nx, ny, nz):
     gbs64 = bytes64 / (ms64 * 1e-3) / 1e9
     gbs32 = bytes32 / (ms32 * 1e-3) / 1e9
     log(f"[bench] SpMV fp64 {ms64*1e3:.1f} us {gbs64:.0f} GB/s; fp32 {ms32*1e3:.1f} us {gbs32:.0f} GB/s")
+    # the box's achievable HBM rates on buffers 4x the Infinity Cache
+    from geo_simulator_amd.linalg import time_stream
+    gib = 2 ** 30
+    ms_r = time_stream(gib, "read", 10)
+    ms_c = time_stream(gib, "copy", 10)
+    ceiling = {"buffer_bytes": gib,
+               "read_GBs": gib / (ms_r * 1e-3) / 1e9,
+               "copy_GBs": 2 * gib / (ms_c * 1e-3) / 1e9,
+               "kernels": "k_stream_sum (16-byte loads, FP64 sum), k_stream_copy"}
+    log(f"[bench] HBM ceiling (1 GiB): read {ceiling['read_GBs']:.0f} GB/s, "
+        f"copy {ceiling['copy_GBs']:.0f} GB/s")
     if args.copy_probe_mib > 0:
-        m = args.copy_probe_mib * 2 ** 20 // 8
-        src = torch.ones(m, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(3):
-            dst.copy_(src)
-        sync()
-        del src, dst
-    traffic32 = traffic64 = None
+        # counter calibration: a copy of known size (k_stream_copy) in the
+        # profiled command
+        time_stream(args.copy_probe_mib * 2 ** 20, "copy", 3)
+    traffic32 = traffic64 = traffic_tree = None
     pmc_path = os.path.join(ROOT, "profiles", f"spmv_{args.config}_pmc.json")
     if os.path.exists(pmc_path):
         with open(pmc_path) as fh:
             pmc = json.load(fh)
         traffic32 = pmc.get("fp32", {}).get("hbm_bytes")
         traffic64 = pmc.get("fp64", {}).get("hbm_bytes")
+        traffic_tree = pmc.get("tree")
     spmv = {"fp64_us": ms64 * 1e3, "fp64_GBs": gbs64, "fp64_bytes": bytes64,
             "fp64_frac": gbs64 / HBM_PEAK_GBS, "fp64_traffic": traffic64,
             "fp32_us": ms32 * 1e3, "fp32_GBs": gbs32, "fp32_bytes": bytes32,
-            "fp32_frac": gbs32 / HBM_PEAK_GBS, "fp32_traffic": traffic32}
+            "fp32_frac": gbs32 / HBM_PEAK_GBS, "fp32_traffic": traffic32,
+            "traffic_measured_on": traffic_tree, "hbm_ceiling": ceiling}
     if not args.no_bicgstab:
         # BiCGSTAB (FP64, Jacobi) on the same system: the general-matrix solver
         # on the same SELL operator, two SpMVs per pass
@@ -285,8 +294,18 @@ def run_headline(G, args, nx, ny, nz):
             note="solve only, restart 30, on the assembled system of the fp64 step")
         log(f"[bench] GMRES(30) fp64: solve {ms:.2f} ms it={st.iterations} "
             f"({ms / max(1, st.iterations) * 1e3:.1f} us/step) relL2={err:.2e}")
-    info = dict(n=n, nnz=nnz, elements=mesh.num_elements(), symbolic_s=t_symbolic,
-                upload_s=t_upload)
+    # numeric assembly against its algorithmic bytes: coordinates n*24,
+    # connectivity E*40, the K_bc values nnz_bc*8 and the right-hand side n*8
+    ne = mesh.num_elements()
+    asm_bytes = n * 24 + ne * 40 + nnz * 8 + n * 8
+    asm_s = results["fp64"]["asm_s"]
+    assembly = {"bytes": asm_bytes, "s": asm_s, "GBs": asm_bytes / asm_s / 1e9,
+                "frac": asm_bytes / asm_s / 1e9 / HBM_PEAK_GBS,
+                "note": "wall clock of the plan-driven numeric assembly (element kernel, "
+                        "system gather, diagonal maximum, right-hand side and Jacobi "
+                        "diagonal) per step, host synchronisation included"}
+    info = dict(n=n, nnz=nnz, elements=ne, symbolic_s=t_symbolic,
+                upload_s=t_upload, assembly=assembly)
     return results, tight, matched, spmv, info
 
 
@@ -318,6 +337,11 @@ def run_elastic(G, args):
             log(f"[bench] elastic {key}: {wall*1e3:.1f} ms/step asm {ta*1e3:.1f} ms "
                 f"it={st.iterations} outer={st.outer_iterations} relL2={err:.2e}")
     out["speedup_mixed_vs_fp64"] = out["fp64"]["step_s"] / out["mixed"]["step_s"]
+    nn3, ne_, nnz_ = 3 * mesh.num_nodes(), mesh.num_elements(), K.nnz()
+    asm_bytes = mesh.num_nodes() * 24 + ne_ * 40 + nnz_ * 8 + nn3 * 8
+    asm_s = out["fp64"]["asm_s"]
+    out["assembly"] = {"bytes": asm_bytes, "s": asm_s, "GBs": asm_bytes / asm_s / 1e9,
+                       "frac": asm_bytes / asm_s / 1e9 / HBM_PEAK_GBS}
     out["tight_speedup_mixed_vs_fp64"] = (out["tight_fp64"]["step_s"] /
                                           out["tight_mixed"]["step_s"])
     nnz, n = K.nnz(), 3 * mesh.num_nodes()
@@ -538,15 +562,17 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline (oracle, {threads} threads for the element loop)")
-        total, t_asm, per_iter, bicg_per_iter, gm_per_step = cpu_baseline(
-            nx, ny, nz, results["fp64"]["iterations"], threads)
+        total, t_asm, t_cg, cst, per_iter, bicg_per_iter, gm_per_step = cpu_baseline(
+            nx, ny, nz, threads)
         cpu = {"value": total, "unit": "s", "cores": threads, "kind": "port",
-               "sample": (f"oracle C restatement on the same {nx}x{ny}x{nz} workload: "
-                          f"full assembly+load+Dirichlet timed ({t_asm:.2f} s, element "
-                          f"loop on {threads} threads, merge/CSR/BC serial as in the "
-                          f"reference), serial Jacobi-CG timed over 25 iterations "
-                          f"({per_iter*1e3:.1f} ms/iter) and extrapolated to the GPU "
-                          f"FP64 iteration count {results['fp64']['iterations']}"),
+               "sample": (f"oracle C restatement on the same {nx}x{ny}x{nz} workload, "
+                          f"one full step timed: assembly+load+Dirichlet {t_asm:.2f} s "
+                          f"(element loop on {threads} threads, merge/CSR/BC serial as "
+                          f"in the reference) and serial Jacobi-CG to rel-res 1e-8 "
+                          f"{t_cg:.2f} s ({cst.iterations} iterations, "
+                          f"{per_iter*1e3:.1f} ms/iter; GPU FP64: "
+                          f"{results['fp64']['iterations']})"),
+               "cg_iterations": cst.iterations, "assembly_s": t_asm, "cg_s": t_cg,
                "bicgstab_s_per_pass": bicg_per_iter,
                "gmres30_s_per_step": gm_per_step,
                "host": host_cpu()}
@@ -580,10 +606,23 @@ def main():
                         if k in results},
         "symbolic_s": info["symbolic_s"],
         "upload_s": info["upload_s"],
+        "assembly": info["assembly"],
         "spmv": spmv,
         "roofline": {"bound": "hbm", "achieved": spmv["fp32_GBs"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": spmv["fp32_GBs"] / HBM_PEAK_GBS,
                      "traffic": spmv["fp32_traffic"],
+                     "traffic_measured_on": spmv["traffic_measured_on"],
+                     "cache_resident": True,
+                     "cache_note": "the 4-byte operator (194 MB) fits the 256 MB Infinity "
+                                   "Cache, so this kernel runs above the box's HBM "
+                                   "ceiling (spmv.hbm_ceiling); the FP64 SpMV, which "
+                                   "streams from HBM, is given beside it",
+                     "fp64_spmv": {"achieved": spmv["fp64_GBs"],
+                                   "frac": spmv["fp64_frac"],
+                                   "frac_of_read_ceiling": spmv["fp64_GBs"] /
+                                   spmv["hbm_ceiling"]["read_GBs"],
+                                   "traffic": spmv["fp64_traffic"],
+                                   "kernel": "k_cg_q<double> (SELL-64 SpMV, FP64)"},
                      "kernel": "k_cg2_dir<float> (SELL-64 SpMV of the FP32 recursion on "
                                "the Jacobi-scaled operator, int32 fixed-point values, + "
                                "direction update p, s and p.s, r.p partials; dominant "

@@ -139,6 +139,7 @@ SIGNATURES = {
     "gs_elastic_element_backward": (C.c_int, [_vp, _vp, _vp, C.c_double, _vp,
                                               _vp, _vp]),
     "gs_csr_set_spmv_shape": (C.c_int, [_vp, C.c_int32, C.c_int32, C.c_int32]),
+    "gs_time_stream": (C.c_int, [_vp, C.c_size_t, C.c_int32, C.c_int32, _dp]),
     "gs_time_spmv": (C.c_int, [_vp, _vp, C.c_int32, C.c_int32, _dp]),
     "gs_time_elastic_elements": (C.c_int, [_vp, _vp, C.c_double, C.c_double, _vp,
                                            C.c_int32, C.c_int32, _dp, _dp]),

@@ -28,6 +28,7 @@ SOURCES = [
     ("assembly.hip", ["-ffp-contract=off"]),
     ("operator.hip", []),
     ("solver.hip", []),
+    ("measure.hip", []),
     ("capi.cpp", []),
 ]
 

@@ -1231,6 +1231,17 @@ gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
   });
 }
 
+gs_status gs_time_stream(gs_ctx *ctx, size_t bytes, int32_t kind, int32_t reps,
+                         double *avg_ms) {
+  if (!ctx || !avg_ms) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(kind == 0 || kind == 1, GS_ERR_INVALID_ARG, "unknown stream kind");
+    require(reps >= 1, GS_ERR_INVALID_ARG, "reps must be positive");
+    set_device(ctx);
+    *avg_ms = time_stream(ctx, bytes, kind, reps);
+  });
+}
+
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms) {
   if (!ctx || !A || !avg_ms) return GS_ERR_INVALID_ARG;

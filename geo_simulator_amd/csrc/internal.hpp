@@ -379,6 +379,7 @@ void cg_solve_operator(gs_ctx *ctx, int64_t n, gs_operator_fn apply, void *apply
                        gs_operator_fn precond, void *precond_user, double *in,
                        double *out, const double *b, const gs_cg_opts &o, double *x,
                        gs_solver_stats *st);
+double time_stream(gs_ctx *ctx, size_t bytes, int kind, int reps);
 void noalloc_setup(int device);
 void cg_solve_noalloc(gs_ctx *ctx, const gs_csr *A, const double *b, const gs_cg_opts &o,
                       double *x, gs_solver_stats *st);

@@ -332,6 +332,16 @@ class ConjugateGradient:
         self.abs_tolerance = float(t)
 
 
+def time_stream(nbytes: int, kind: str = "read", reps: int = 10, ctx=None) -> float:
+    """Average milliseconds of one streaming pass over nbytes of device memory:
+    "read" sums the values (16-byte loads), "copy" copies them."""
+    ctx = ctx or default_context()
+    out = C.c_double()
+    ctx.check(lib().gs_time_stream(ctx.handle, int(nbytes), 0 if kind == "read" else 1,
+                                   int(reps), C.byref(out)), "gs_time_stream")
+    return out.value
+
+
 def norm(v) -> float:
     """SolverUtils::norm (src/linalg/solver.rs:153-155)."""
     return float(np.sqrt(np.sum(np.asarray(v, dtype=np.float64) ** 2)))

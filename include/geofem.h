@@ -380,10 +380,19 @@ gs_status gs_csr_set_spmv_shape(gs_csr *A, int32_t lanes, int32_t workgroups,
  * updated in place and p.s, r.p partials); otherwise the FP32 q = A p. */
 gs_status gs_time_spmv(gs_ctx *ctx, const gs_csr *A, int32_t precision,
                        int32_t reps, double *avg_ms);
+/* The device's achievable HBM rate: kind 0 streams `bytes` of FP64 values
+ * once and sums them (16-byte loads, read-dominant), kind 1 copies them to a
+ * second buffer; average milliseconds per pass over `reps` passes, timed with
+ * HIP events on the ctx stream (the buffers come from the ctx's pool). */
+gs_status gs_time_stream(gs_ctx *ctx, size_t bytes, int32_t kind, int32_t reps,
+                         double *avg_ms);
 /* Time the elastic element kernel (K_e and the gravity load of every
- * element; variant 0: VALU, the one assembly uses; variant 1: matrix cores,
- * v_mfma_f64_16x16x4f64) over `reps` launches, and report the largest entry
- * difference against the VALU kernel relative to the largest entry. */
+ * element; variant 0: VALU, the one assembly uses (one lane per 3x3 node
+ * block); 1: matrix cores, v_mfma_f64_16x16x4f64; 2: FP32 B^T D B on the
+ * VALU (FP32 K_e); 3: FP32 on the matrix cores, v_mfma_f32_16x16x4f32;
+ * 4: FP64 VALU with one lane per K_e row) over `reps` launches, and report
+ * the largest entry difference against variant 0 relative to the largest
+ * entry. */
 gs_status gs_time_elastic_elements(gs_ctx *ctx, gs_mesh *mesh, double E,
                                    double poisson, const double *E_per_elem_dev,
                                    int32_t variant, int32_t reps, double *avg_ms,

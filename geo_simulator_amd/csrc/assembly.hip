@@ -514,31 +514,37 @@ __global__ void k_plan_fill(int64_t n, const int *__restrict__ rp,
 }
 
 // Numeric phase, thermal: the segmented sum of k_gather_thermal per node
-// slot, written where the plan sends it.
+// slot, written where the plan sends it (spos: the SELL entry of a kept
+// slot, -2 - q for lift entry q, -1 dropped).  A diagonal slot also sums
+// the element loads of the same contributions (k_gather_load's sum, same
+// order) and leaves the node's load and diagonal value for k_sys_rhs.
 __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
                                      const int *__restrict__ contrib,
                                      const double *__restrict__ Ke,
-                                     const int *__restrict__ dst,
-                                     const int *__restrict__ pos,
-                                     double *__restrict__ val,
+                                     const double *__restrict__ Fe,
+                                     const int *__restrict__ spos,
+                                     const int *__restrict__ slot_row,
                                      double *__restrict__ sval,
                                      float *__restrict__ sval32,
                                      double *__restrict__ liftv,
+                                     double *__restrict__ fload,
+                                     double *__restrict__ dval,
                                      unsigned long long *__restrict__ dmax) {
   const int64_t s = xcd_item();
   unsigned long long mine = 0;
   if (s < nnz) {
-    const int d = dst[s];
+    const int d = spos[s];
     const int b0 = cptr[s], b1 = cptr[s + 1];
     const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
     const bool diag = ab0 / 10 == ab0 % 10;  // a diagonal slot: local a == b
     if (d != -1 || diag) {
       // contributions in groups of GATHER_GROUP: the loads of a group are
       // issued together, the additions stay in ascending order
-      double v = 0.0;
+      const bool load = diag && Fe;
+      double v = 0.0, fl = 0.0;
       for (int k0 = b0; k0 < b1; k0 += GATHER_GROUP) {
         int t[GATHER_GROUP];
-        double x[GATHER_GROUP];
+        double x[GATHER_GROUP], y[GATHER_GROUP];
 #pragma unroll
         for (int u = 0; u < GATHER_GROUP; ++u)
           if (k0 + u < b1) t[u] = contrib[k0 + u];
@@ -550,17 +556,24 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
             const int a = ab / 10, b = ab - 10 * (ab / 10);
             const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
             x[u] = Ke[(int64_t)e * 55 + p];
+            if (load) y[u] = Fe[(int64_t)e * 10 + a];
           }
 #pragma unroll
         for (int u = 0; u < GATHER_GROUP; ++u)
-          if (k0 + u < b1) v = (k0 + u == b0) ? x[u] : v + x[u];
+          if (k0 + u < b1) {
+            v = (k0 + u == b0) ? x[u] : v + x[u];
+            if (load) fl += y[u];
+          }
       }
-      if (diag) mine = abs_bits(v);
+      if (diag) {
+        mine = abs_bits(v);
+        const int r = slot_row[s];
+        dval[r] = v;
+        if (load) fload[r] = fl;
+      }
       if (d >= 0) {
-        if (val) val[d] = v;  // null: the CSR values are produced on demand
-        const int e = pos[d];
-        sval[e] = v;
-        sval32[e] = (float)v;
+        sval[d] = v;
+        sval32[d] = (float)v;
       } else if (d <= -2) {
         liftv[-2 - d] = v;
       }
@@ -570,29 +583,24 @@ __global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
 }
 
 // The 3x3 block sums of node slot s (contributions in ascending element
-// order, as k_gather_elastic) and the slot's nine destinations; diag: a
-// diagonal slot.
+// order, as k_gather_elastic); diag: a diagonal slot, whose contributions
+// also give the node's three load components (k_gather_load's sums, same
+// order) when Fe is given.
 __device__ __forceinline__ void elastic_slot_sums(
-    int64_t s, const int *__restrict__ rp, const int *__restrict__ slot_row,
-    const int *__restrict__ cptr, const int *__restrict__ contrib,
-    const double *__restrict__ Ke, const int *__restrict__ dst, double (&v)[9],
-    int (&dd)[9], bool &diag) {
-  const int r = slot_row[s];
-  const int base = rp[r], deg = rp[r + 1] - base;
-  const int j = (int)(s - base);
-#pragma unroll
-  for (int ci = 0; ci < 3; ++ci)
-#pragma unroll
-    for (int cj = 0; cj < 3; ++cj)
-      dd[3 * ci + cj] = dst[9ll * base + 3ll * ci * deg + 3ll * j + cj];
+    int64_t s, const int *__restrict__ cptr, const int *__restrict__ contrib,
+    const double *__restrict__ Ke, const double *__restrict__ Fe, double (&v)[9],
+    double (&fl)[3], bool &diag) {
   const int b0 = cptr[s], b1 = cptr[s + 1];
   const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
   diag = ab0 / 10 == ab0 % 10;
+  const bool load = diag && Fe;
 #pragma unroll
   for (int m = 0; m < 9; ++m) v[m] = 0.0;
-  // two contributions' 18 loads issued together; additions in ascending order
+#pragma unroll
+  for (int c = 0; c < 3; ++c) fl[c] = 0.0;
+  // two contributions' loads issued together; additions in ascending order
   for (int k0 = b0; k0 < b1; k0 += 2) {
-    double x[2][9];
+    double x[2][9], y[2][3];
 #pragma unroll
     for (int u = 0; u < 2; ++u)
       if (k0 + u < b1) {
@@ -603,35 +611,59 @@ __device__ __forceinline__ void elastic_slot_sums(
         const double *Kr = Ke + (int64_t)e * 900 + (a * 10 + b) * 9;
 #pragma unroll
         for (int m = 0; m < 9; ++m) x[u][m] = Kr[m];
+        if (load) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) y[u][c] = Fe[((int64_t)e * 10 + a) * 3 + c];
+        }
       }
 #pragma unroll
     for (int u = 0; u < 2; ++u)
       if (k0 + u < b1) {
 #pragma unroll
         for (int m = 0; m < 9; ++m) v[m] = (k0 + u == b0) ? x[u][m] : v[m] + x[u][m];
+        if (load) {
+#pragma unroll
+          for (int c = 0; c < 3; ++c) fl[c] += y[u][c];
+        }
       }
   }
+}
+
+// Node slot s's nine destinations in the plan (dst of the full K's 3x3 block)
+__device__ __forceinline__ void elastic_slot_dst(int64_t s, const int *__restrict__ rp,
+                                                 const int *__restrict__ slot_row,
+                                                 const int *__restrict__ dst, int (&dd)[9]) {
+  const int r = slot_row[s];
+  const int base = rp[r], deg = rp[r + 1] - base;
+  const int j = (int)(s - base);
+#pragma unroll
+  for (int ci = 0; ci < 3; ++ci)
+#pragma unroll
+    for (int cj = 0; cj < 3; ++cj)
+      dd[3 * ci + cj] = dst[9ll * base + 3ll * ci * deg + 3ll * j + cj];
 }
 
 // Numeric phase, elastic, in the order of the 3x3-block SELL view: one
 // thread per stored block position (chunk c, lane l), so a wavefront writes
 // each of the nine value planes as one contiguous stretch, and every
 // position -- padding and the missing entries of a stored block -- is
-// written once (as zero), with no separate clearing pass.  The kept entries
-// also go to K_bc's CSR values when val is given (the fused system passes
-// none and produces them on demand); the lift values are left to
-// k_sys_lift_elastic.  (Scattering node slots into the block view instead
-// evicts partly written lines: measured 4.5x the useful write traffic.)
+// written once (as zero), with no separate clearing pass.  bmask (from the
+// plan) marks the block's kept entries; the lift values, and the diagonal
+// blocks with no kept entry, are left to k_sys_lift_elastic.  A diagonal
+// block leaves the node's load and diagonal values for k_sys_rhs.
+// (Scattering node slots into the block view instead evicts partly written
+// lines: measured 4.5x the useful write traffic.)
 __global__ void k_sys_gather_elastic_b(int64_t nent, const int *__restrict__ bslot,
-                                       const int *__restrict__ rp,
+                                       const unsigned short *__restrict__ bmask,
                                        const int *__restrict__ slot_row,
                                        const int *__restrict__ cptr,
                                        const int *__restrict__ contrib,
                                        const double *__restrict__ Ke,
-                                       const int *__restrict__ dst,
-                                       double *__restrict__ val,
+                                       const double *__restrict__ Fe,
                                        double *__restrict__ sval,
                                        float *__restrict__ sval32,
+                                       double *__restrict__ fload,
+                                       double *__restrict__ dval,
                                        unsigned long long *__restrict__ dmax) {
   const int64_t eb = xcd_item();
   unsigned long long mine = 0;
@@ -641,23 +673,23 @@ __global__ void k_sys_gather_elastic_b(int64_t nent, const int *__restrict__ bsl
 #pragma unroll
     for (int m = 0; m < 9; ++m) out[m] = 0.0;
     if (sl >= 0) {
-      double v[9];
-      int dd[9];
+      const unsigned mask = bmask[eb];
+      double v[9], fl[3];
       bool diag;
-      elastic_slot_sums(sl, rp, slot_row, cptr, contrib, Ke, dst, v, dd, diag);
+      elastic_slot_sums(sl, cptr, contrib, Ke, Fe, v, fl, diag);
       if (diag) {
+        const int r = slot_row[sl];
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
           const unsigned long long b = abs_bits(v[4 * c]);
           mine = b > mine ? b : mine;
+          dval[3ll * r + c] = v[4 * c];
+          if (Fe) fload[3ll * r + c] = fl[c];
         }
       }
 #pragma unroll
       for (int m = 0; m < 9; ++m)
-        if (dd[m] >= 0) {
-          out[m] = v[m];
-          if (val) val[dd[m]] = v[m];  // null: the CSR values are produced on demand
-        }
+        if ((mask >> m) & 1u) out[m] = v[m];
     }
     const int64_t c = eb >> 6, l = eb & 63;
 #pragma unroll
@@ -670,8 +702,10 @@ __global__ void k_sys_gather_elastic_b(int64_t nent, const int *__restrict__ bsl
   wg_max_store(mine, dmax);
 }
 
-// The lift values f_j -= K_ji g_i of the node slots that have any (the
-// plan's list), one thread per slot.
+// The node slots the block view does not hold in full (the plan's list):
+// lift values f_j -= K_ji g_i, and the diagonal blocks of nodes whose three
+// DOFs are all constrained (their diagonal counts in c = max |K_ii|).  One
+// thread per slot; one maximum per workgroup into dmax.
 __global__ void k_sys_lift_elastic(int64_t nls, const int *__restrict__ lslots,
                                    const int *__restrict__ rp,
                                    const int *__restrict__ slot_row,
@@ -679,41 +713,75 @@ __global__ void k_sys_lift_elastic(int64_t nls, const int *__restrict__ lslots,
                                    const int *__restrict__ contrib,
                                    const double *__restrict__ Ke,
                                    const int *__restrict__ dst,
-                                   double *__restrict__ liftv) {
+                                   double *__restrict__ liftv,
+                                   unsigned long long *__restrict__ dmax) {
   const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i >= nls) return;
-  double v[9];
-  int dd[9];
-  bool diag;
-  elastic_slot_sums(lslots[i], rp, slot_row, cptr, contrib, Ke, dst, v, dd, diag);
+  unsigned long long mine = 0;
+  if (i < nls) {
+    const int s = lslots[i];
+    double v[9], fl[3];
+    bool diag;
+    elastic_slot_sums(s, cptr, contrib, Ke, nullptr, v, fl, diag);
+    int dd[9];
+    elastic_slot_dst(s, rp, slot_row, dst, dd);
 #pragma unroll
-  for (int m = 0; m < 9; ++m)
-    if (dd[m] <= -2) liftv[-2 - dd[m]] = v[m];
+    for (int m = 0; m < 9; ++m)
+      if (dd[m] <= -2) liftv[-2 - dd[m]] = v[m];
+    if (diag) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const unsigned long long b = abs_bits(v[4 * c]);
+        mine = b > mine ? b : mine;
+      }
+    }
+  }
+  wg_max_store(mine, dmax);
 }
 
 // Plan, elastic: the node slot of each block position of the K_bc block view
-// (bslot, -1 for padding), and a flag per slot with any lift destination.
+// (bslot, -1 for padding) with its kept-entry mask (bmask, bit m: entry m of
+// the 3x3 block), and a flag per slot that k_sys_lift_elastic must visit:
+// any lift destination, or a diagonal slot with no kept entry.
 __global__ void k_plan_bslot(int64_t nnz, const int *__restrict__ rp,
                              const int *__restrict__ slot_row,
+                             const int *__restrict__ col,
                              const int *__restrict__ dst, const int *__restrict__ pos,
-                             int *__restrict__ bslot, int *__restrict__ lflag) {
+                             int *__restrict__ bslot, unsigned short *__restrict__ bmask,
+                             int *__restrict__ lflag) {
   const int64_t s = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (s >= nnz) return;
   const int r = slot_row[s];
   const int base = rp[r], deg = rp[r + 1] - base;
   const int j = (int)(s - base);
   int keep = -1, lift = 0;
+  unsigned mask = 0;
   for (int ci = 0; ci < 3; ++ci)
     for (int cj = 0; cj < 3; ++cj) {
       const int d = dst[9ll * base + 3ll * ci * deg + 3ll * j + cj];
-      if (d >= 0 && keep < 0) keep = d;
+      if (d >= 0) {
+        if (keep < 0) keep = d;
+        mask |= 1u << (3 * ci + cj);
+      }
       if (d <= -2) lift = 1;
     }
-  lflag[s] = lift;
+  const bool diag = col[s] == r;
+  lflag[s] = lift || (diag && keep < 0);
   if (keep >= 0) {
     const int64_t vi = pos[keep];
-    bslot[(vi / 64 / 9) * 64 + vi % 64] = (int)s;
+    const int64_t eb = (vi / 64 / 9) * 64 + vi % 64;
+    bslot[eb] = (int)s;
+    bmask[eb] = (unsigned short)mask;
   }
+}
+
+// Plan, thermal: the SELL entry of each kept node slot, or its lift code
+// (-2 - q), or -1 (dropped), so the gather reads one destination per slot.
+__global__ void k_plan_spos(int64_t nnz, const int *__restrict__ dst,
+                            const int *__restrict__ pos, int *__restrict__ spos) {
+  const int64_t s = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (s >= nnz) return;
+  const int d = dst[s];
+  spos[s] = d >= 0 ? pos[d] : d;
 }
 
 __global__ void k_fill_int(int64_t n, int v, int *__restrict__ out) {
@@ -730,17 +798,18 @@ __global__ void k_compact_flags(int64_t n, const int *__restrict__ flag,
 
 
 
-// Per DOF row (internal numbering): load (k_gather_load's sum; none when Fe
-// is null) plus the caller's vector fext (caller numbering, nullable: the
-// f_ext + f_history of src/fem/time_integration.rs:101-110), then either
-// the constrained row (c e_i, f = g c) or the lift f -= K_ji g_i in the
-// plan's order; the RHS lands at caller index dperm[r]; Jacobi inverse
-// diagonal (src/linalg/preconditioner.rs:37-51) in SELL order.
+// Per DOF row (internal numbering): the load the gather summed for the
+// node's diagonal slot (none when fload is null) plus the caller's vector
+// fext (caller numbering, nullable: the f_ext + f_history of
+// src/fem/time_integration.rs:101-110), then either the constrained row
+// (c e_i, f = g c) or the lift f -= K_ji g_i in the plan's order; the RHS
+// lands at caller index dperm[r]; Jacobi inverse diagonal
+// (src/linalg/preconditioner.rs:37-51) in SELL order from the gathered
+// diagonal value.
 template <int DPN>
 __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
-                          const int *__restrict__ cptr,
-                          const int *__restrict__ contrib,
-                          const double *__restrict__ Fe,
+                          const double *__restrict__ fload,
+                          const double *__restrict__ dval,
                           const double *__restrict__ fext,
                           const uint8_t *__restrict__ dir,
                           const double *__restrict__ g,
@@ -751,52 +820,29 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
                           const int *__restrict__ pos,
                           const int *__restrict__ siperm,
                           const unsigned long long *__restrict__ maxbits,
-                          double *__restrict__ val, double *__restrict__ sval,
+                          double *__restrict__ sval,
                           float *__restrict__ sval32,
                           const int *__restrict__ dperm,
                           double *__restrict__ fbc, double *__restrict__ sdinv,
                           float *__restrict__ sdinv32) {
   const int64_t r = xcd_item();
   if (r >= n) return;
-  const int64_t node = r / DPN;
-  const int c = (int)(r - DPN * node);
-  double f = 0.0;
-  const int s = diag_slot[node];
-  if (s >= 0 && Fe) {
-    const int k1 = cptr[s + 1];
-    for (int k0 = cptr[s]; k0 < k1; k0 += GATHER_GROUP) {  // loads in groups
-      int t[GATHER_GROUP];
-      double x[GATHER_GROUP];
-#pragma unroll
-      for (int u = 0; u < GATHER_GROUP; ++u)
-        if (k0 + u < k1) t[u] = contrib[k0 + u];
-#pragma unroll
-      for (int u = 0; u < GATHER_GROUP; ++u)
-        if (k0 + u < k1) {
-          const int e = t[u] / 100;
-          const int a = (t[u] - 100 * e) / 10;
-          x[u] = Fe[((int64_t)e * 10 + a) * DPN + c];
-        }
-#pragma unroll
-      for (int u = 0; u < GATHER_GROUP; ++u)
-        if (k0 + u < k1) f += x[u];
-    }
-  }
   const int64_t out = dperm ? dperm[r] : r;
-  if (fext) f = fext[out] + f;
-  double dv = 0.0;
   const int d = diag[r];
+  double f, dv;
   if (dir[r]) {
     const double cv = __longlong_as_double((long long)*maxbits);
-    if (val) val[d] = cv;
     const int e = pos[d];
     sval[e] = cv;
     sval32[e] = (float)cv;
     f = g[r] * cv;
     dv = cv;
   } else {
+    const bool has = diag_slot[r / DPN] >= 0;
+    f = (has && fload) ? fload[r] : 0.0;
+    if (fext) f = fext[out] + f;
     for (int q = lptr[r]; q < lptr[r + 1]; ++q) f -= liftv[q] * g[lcol[q]];
-    dv = d >= 0 ? (val ? val[d] : sval[pos[d]]) : 0.0;
+    dv = d >= 0 ? dval[r] : 0.0;
   }
   fbc[out] = f;
   const double di = fabs(dv) > 1e-14 ? 1.0 / dv : 1.0;
@@ -1155,19 +1201,27 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
   pat.col = P->col;
   pat.dperm = dperm;
   P->sell = build_sell_pattern(ctx, &pat, false, P->dpn == 3 ? 3 : 1);
-  if (dpn == 3) {
+  const NodePattern &np = m->pat;
+  if (dpn == 1) {
+    // the gather's destinations: SELL entry, lift code or dropped, per slot
+    P->spos = dalloc<int>(nnzK);
+    if (nnzK > 0)
+      hipLaunchKernelGGL(k_plan_spos, nblk(nnzK), BS, 0, s, nnzK, P->dst.get(),
+                         P->sell->pos.get(), P->spos.get());
+    GS_LAUNCH_CHECK("k_plan_spos");
+  } else {
     // the block-view order of the elastic numeric gather
     const SellPattern &S = *P->sell;
-    const NodePattern &np = m->pat;
     P->bslot = dalloc<int>(S.entries);
+    P->bmask = dalloc<unsigned short>(S.entries);
     if (S.entries > 0)
       hipLaunchKernelGGL(k_fill_int, nblk(S.entries), BS, 0, s, S.entries, -1,
                          P->bslot.get());
     if (np.nnz > 0) {
       auto lflag = dalloc<int>(np.nnz), lscan = dalloc<int>(np.nnz);
       hipLaunchKernelGGL(k_plan_bslot, nblk(np.nnz), BS, 0, s, np.nnz, np.row_ptr.get(),
-                         np.slot_row.get(), P->dst.get(), S.pos.get(), P->bslot.get(),
-                         lflag.get());
+                         np.slot_row.get(), np.col.get(), P->dst.get(), S.pos.get(),
+                         P->bslot.get(), P->bmask.get(), lflag.get());
       GS_LAUNCH_CHECK("k_plan_bslot");
       scan_inclusive(ctx, lflag.get(), lscan.get(), np.nnz, rocprim::plus<int>());
       P->n_lift_slots = read_int(ctx, lscan.get() + np.nnz - 1);
@@ -1184,41 +1238,39 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
 }
 
 int64_t system_gather_blocks(const gs_mesh *m, const SystemPlan &P) {
-  return P.dpn == 1 ? nblk(m->pat.nnz) : nblk(P.sell->entries);
+  if (P.dpn == 1) return nblk(m->pat.nnz);
+  return nblk(P.sell->entries) + (int64_t)nblk(P.n_lift_slots);
 }
 
-void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, double *val, double *sval, float *sval32,
-                   double *liftv, unsigned long long *dmax) {
+void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P, const double *Ke,
+                   const double *Fe, double *sval, float *sval32, double *liftv,
+                   double *fload, double *dval, unsigned long long *dmax) {
   const NodePattern &p = m->pat;
   if (p.nnz == 0) return;
-  const int *pos = P.sell->pos.get();
-  if (P.dpn == 1)
+  if (P.dpn == 1) {
     hipLaunchKernelGGL(k_sys_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
-                       p.contrib_ptr.get(), p.contrib.get(), Ke, P.dst.get(), pos,
-                       val, sval, sval32, liftv, dmax);
-  else {
+                       p.contrib_ptr.get(), p.contrib.get(), Ke, Fe, P.spos.get(),
+                       p.slot_row.get(), sval, sval32, liftv, fload, dval, dmax);
+  } else {
     const int64_t nent = P.sell->entries;
     if (nent > 0)
       hipLaunchKernelGGL(k_sys_gather_elastic_b, nblk(nent), BS, 0, s, nent,
-                         P.bslot.get(), p.row_ptr.get(), p.slot_row.get(),
-                         p.contrib_ptr.get(), p.contrib.get(), Ke, P.dst.get(), val,
-                         sval, sval32, dmax);
+                         P.bslot.get(), P.bmask.get(), p.slot_row.get(),
+                         p.contrib_ptr.get(), p.contrib.get(), Ke, Fe, sval, sval32,
+                         fload, dval, dmax);
     if (P.n_lift_slots > 0)
       hipLaunchKernelGGL(k_sys_lift_elastic, nblk(P.n_lift_slots), BS, 0, s,
                          P.n_lift_slots, P.lift_slots.get(), p.row_ptr.get(),
                          p.slot_row.get(), p.contrib_ptr.get(), p.contrib.get(), Ke,
-                         P.dst.get(), liftv);
+                         P.dst.get(), liftv, dmax + nblk(nent));
   }
   GS_LAUNCH_CHECK("k_sys_gather");
 }
 
-void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, const double *Fe, const double *fext, double *val,
-                   double *sval,
-                   float *sval32, const double *liftv, const int *dperm,
-                   double *fbc, double *sdinv, float *sdinv32,
-                   const unsigned long long *dmax) {
+void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P, const double *fload,
+                   const double *dval, const double *fext, double *sval, float *sval32,
+                   const double *liftv, const int *dperm, double *fbc, double *sdinv,
+                   float *sdinv32, const unsigned long long *dmax) {
   if (P.n == 0) return;
   hipStream_t s = ctx->stream;
   const NodePattern &p = m->pat;
@@ -1226,26 +1278,22 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
   const double one = 1.0;
   unsigned long long one_bits;
   memcpy(&one_bits, &one, sizeof(one));
-  // c = max(1, max |K_ii|): the gather left one maximum per workgroup
+  // c = max(1, max |K_ii|): the gathers left one maximum per workgroup
   hipLaunchKernelGGL(k_set_u64, 1, 1, 0, s, maxbits.get(), one_bits);
   const int64_t np = p.nnz > 0 ? system_gather_blocks(m, P) : 0;
   if (np > 0)
     hipLaunchKernelGGL(k_max_partials, (unsigned)std::min<int64_t>(256, nblk(np)), BS, 0, s,
                        np, dmax, maxbits.get());
   const int *sip = P.sell->siperm.get(), *pos = P.sell->pos.get();
-  if (P.dpn == 1) {
-    hipLaunchKernelGGL((k_sys_rhs<1>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
-                       p.contrib_ptr.get(), p.contrib.get(), Fe, fext, P.dir.get(),
-                       P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
-                       P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
-                       dperm, fbc, sdinv, sdinv32);
-  } else {
-    hipLaunchKernelGGL((k_sys_rhs<3>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(),
-                       p.contrib_ptr.get(), p.contrib.get(), Fe, fext, P.dir.get(),
-                       P.g.get(), P.lift_ptr.get(), P.lift_col.get(), liftv,
-                       P.diag.get(), pos, sip, maxbits.get(), val, sval, sval32,
-                       dperm, fbc, sdinv, sdinv32);
-  }
+  auto go = [&](auto dpn) {
+    constexpr int D = decltype(dpn)::value;
+    hipLaunchKernelGGL((k_sys_rhs<D>), nblk(P.n), BS, 0, s, P.n, p.diag_slot.get(), fload,
+                       dval, fext, P.dir.get(), P.g.get(), P.lift_ptr.get(), P.lift_col.get(),
+                       liftv, P.diag.get(), pos, sip, maxbits.get(), sval, sval32, dperm, fbc,
+                       sdinv, sdinv32);
+  };
+  if (P.dpn == 1) go(std::integral_constant<int, 1>{});
+  else go(std::integral_constant<int, 3>{});
   GS_LAUNCH_CHECK("k_sys_rhs");
   GS_HIP(hipStreamSynchronize(s));
 }

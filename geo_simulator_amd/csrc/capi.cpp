@@ -160,12 +160,15 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   auto sdinv32 = palloc<float>(pool, P->n);
   const DArr<int> &dperm = sp.elastic ? m->perm3 : m->perm;
   auto dmax = palloc<unsigned long long>(pool, std::max<int64_t>(1, system_gather_blocks(m, *P)));
+  auto fload = palloc<double>(pool, P->n);  // per DOF: the gathered load
+  auto dval = palloc<double>(pool, P->n);   // per DOF: the gathered diagonal
   // K_bc's CSR values are not written here: ensure_csr_values produces them
   // from the SELL values when a caller asks for the CSR form
-  system_gather(s, m, *P, Ke.get(), nullptr, sval.get(), sval32.get(), liftv.get(),
-                dmax.get());
-  system_finish(ctx, m, *P, Ke.get(), Fe.get(), sp.fext, nullptr, sval.get(), sval32.get(),
-                liftv.get(), dperm.get(), f_bc, sdinv.get(), sdinv32.get(), dmax.get());
+  system_gather(s, m, *P, Ke.get(), with_load ? Fe.get() : nullptr, sval.get(), sval32.get(),
+                liftv.get(), fload.get(), dval.get(), dmax.get());
+  system_finish(ctx, m, *P, with_load ? fload.get() : nullptr, dval.get(), sp.fext,
+                sval.get(), sval32.get(), liftv.get(), dperm.get(), f_bc, sdinv.get(),
+                sdinv32.get(), dmax.get());
   gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, DArr<double>());
   K->val_lazy = true;
   K->dperm = dperm;

@@ -162,6 +162,8 @@ struct SystemPlan {
   // 3 DOFs per node: the node slot of each block position of the block view
   // (-1: padding), and the node slots with lift destinations
   DArr<int> bslot;
+  DArr<unsigned short> bmask;  // kept entries of each block position (bit m)
+  DArr<int> spos;              // thermal: per node slot, SELL entry / lift code / -1
   DArr<int> lift_slots;
   int64_t n_lift_slots = 0;
 };
@@ -345,17 +347,13 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
 // values of K_bc (CSR and SELL, FP64 and FP32) and the lift values
 // workgroups of the system gather (entries of its diagonal-maximum array)
 int64_t system_gather_blocks(const gs_mesh *m, const SystemPlan &P);
-void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, double *val, double *sval, float *sval32,
-                   double *liftv, unsigned long long *dmax);
-// c = max(1, max |K_ii|), constrained rows, RHS (load + lift, written at
-// caller index dperm[r]) and the Jacobi inverse diagonal in SELL order
-void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P,
-                   const double *Ke, const double *Fe, const double *fext, double *val,
-                   double *sval,
-                   float *sval32, const double *liftv, const int *dperm,
-                   double *fbc, double *sdinv, float *sdinv32,
-                   const unsigned long long *dmax);
+void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P, const double *Ke,
+                   const double *Fe, double *sval, float *sval32, double *liftv,
+                   double *fload, double *dval, unsigned long long *dmax);
+void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P, const double *fload,
+                   const double *dval, const double *fext, double *sval, float *sval32,
+                   const double *liftv, const int *dperm, double *fbc, double *sdinv,
+                   float *sdinv32, const unsigned long long *dmax);
 
 // ---- operator.hip ----
 // block = 3 builds the 3x3-block view (rows must be a multiple of 3; a block

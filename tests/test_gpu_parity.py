@@ -431,3 +431,93 @@ def test_bad_dofs_and_material():
             mesh, G.DofManager(mesh.num_nodes(), 3), G.IsotropicElasticity(1.0, 0.25),
             E_per_elem=np.array([1.0, 1.0, -1.0, 1.0, 1.0, 1.0]))
     assert ei.value.code == G._lib.GS_ERR_INVALID_MATERIAL
+
+
+@pytest.mark.parametrize("case", ["thermal", "elastic"])
+def test_fused_system_max_diagonal_at_constrained_node(oracle, case):
+    """c = max(1, max_i |K_ii|) runs over every diagonal of K, also those of
+    constrained rows (src/fem/assembly.rs:184-193): the node holding the
+    largest diagonal is constrained (all three DOFs in the elastic case, so
+    no entry of its diagonal block is kept in K_bc)."""
+    xyz, conn, mesh = mesh_pair(oracle, 2, 3, 2, 1e6, 1e6, 6e5)
+    p = np.exp(np.random.default_rng(8).normal(0.0, 0.5, len(conn)))
+    if case == "elastic":
+        mask, vals = oracle.elastic_dirichlet(xyz, 1e6, 1e6)
+        dof = G.DofManager(mesh.num_nodes(), 3)
+        Ko = oracle.assemble_elastic(xyz, conn, 1.0, 0.25, E_elem=p * 1e11)
+        fo = oracle.assemble_gravity_load(xyz, conn, 3300.0, [0, 0, -9.81])
+        node = int(np.argmax(np.abs(Ko.diagonal()))) // 3
+        mask[3 * node:3 * node + 3] = 1
+        vals[3 * node:3 * node + 3] = [1e-3, -2e-3, 3e-3]
+    else:
+        mask, vals = oracle.thermal_dirichlet(xyz, 6e5)
+        dof = G.DofManager(mesh.num_nodes(), 1)
+        Ko = oracle.assemble_thermal(xyz, conn, 1.0, k_elem=p * 3.0)
+        fo = oracle.assemble_thermal_load(xyz, conn, 1e-6)
+        node = int(np.argmax(np.abs(Ko.diagonal())))
+        mask[node] = 1
+        vals[node] = 640.0
+    assert xyz[node, 2] > 1.0 and xyz[node, 2] < 6e5 - 1.0  # an interior node
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    Kbo, fbo = oracle.apply_dirichlet(Ko, fo, mask, vals)
+    if case == "elastic":
+        Kf, ff = G.Assembler.assemble_elasticity_system(
+            mesh, dof, G.IsotropicElasticity(1.0, 0.25), 3300.0, [0, 0, -9.81],
+            E_per_elem=p * 1e11)
+    else:
+        Kf, ff = G.Assembler.assemble_thermal_system(mesh, dof, 1.0, 1e-6, k_per_elem=p * 3.0)
+    assert same_csr(Kf.to_scipy(), Kbo)
+    assert np.array_equal(ff.cpu().numpy(), fbo)
+
+
+def star_mesh(n_tets):
+    """n_tets Tet10 elements sharing only vertex 0 (rotated copies of one
+    tet), so node 0 couples to 9 n_tets + 1 nodes: rows far longer than a
+    structured mesh's 65."""
+    base = np.array([[0, 0, 0], [1.0, 0.1, 0.0], [0.2, 1.1, 0.1], [0.1, 0.2, 0.9]])
+    xyz, conn = [np.zeros(3)], []
+    for k in range(n_tets):
+        th = 2 * np.pi * k / n_tets
+        ph = 0.3 * np.sin(3 * th)
+        rz = np.array([[np.cos(th), -np.sin(th), 0], [np.sin(th), np.cos(th), 0], [0, 0, 1]])
+        rx = np.array([[1, 0, 0], [0, np.cos(ph), -np.sin(ph)], [0, np.sin(ph), np.cos(ph)]])
+        v = base @ (rz @ rx).T * (1.0 + 0.01 * k)
+        ids = [0]
+        for p in v[1:]:
+            ids.append(len(xyz))
+            xyz.append(p)
+        for a, b in ((0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)):
+            ids.append(len(xyz))
+            xyz.append(0.5 * (v[a] + v[b]))
+        conn.append(ids)
+    return np.array(xyz), np.array(conn, dtype=np.int64)
+
+
+@pytest.mark.parametrize("n_tets", [20, 30])
+def test_thermal_system_long_rows(oracle, n_tets):
+    """Rows longer than one accumulator pass of the row gather (20 tets: 181
+    entries, 1-byte codes) and longer than 255 (30 tets: 271 entries,
+    2-byte codes), with lift entries in the long row: the fused system
+    equals the oracle's assembly + elimination bit for bit."""
+    xyz, conn = star_mesh(n_tets)
+    mesh = G.Mesh(xyz, conn)
+    k = np.exp(np.random.default_rng(n_tets).normal(0.0, 0.5, len(conn)))
+    mask = np.zeros(len(xyz), dtype=np.uint8)
+    mask[conn[::3, 1]] = 1   # a vertex of every third tet
+    mask[conn[1::4, 7]] = 1  # an edge node of every fourth
+    vals = np.where(mask == 1, 1.0 + xyz[:, 2], 0.0)
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    Ko = oracle.assemble_thermal(xyz, conn, 1.0, k_elem=k)
+    assert np.diff(Ko.indptr).max() == 9 * n_tets + 1
+    fo = oracle.assemble_thermal_load(xyz, conn, 2.5)
+    Kbo, fbo = oracle.apply_dirichlet(Ko, fo, mask, vals)
+    Kf, ff = G.Assembler.assemble_thermal_system(mesh, dof, 1.0, 2.5, k_per_elem=k)
+    assert same_csr(Kf.to_scipy(), Kbo)
+    assert np.array_equal(ff.cpu().numpy(), fbo)
+    # the row gather's SELL values also feed the solver: one CG solve
+    x, st = G.ConjugateGradient().with_tolerance(1e-12).with_max_iterations(2000).solve(Kf, ff)
+    assert st.converged
+    x = x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
+    xo = np.linalg.solve(Kbo.toarray(), fbo)
+    assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)

@@ -20,10 +20,12 @@
 
 #include <climits>
 #include <cstring>
+#include <type_traits>
 
 #include <rocprim/rocprim.hpp>
 
 #include "internal.hpp"
+#include "sell.hpp"
 
 namespace gs {
 namespace {
@@ -50,6 +52,38 @@ __device__ __forceinline__ int64_t xcd_item() { return xcd_block() * BS + thread
 // Contributions whose loads a gather thread issues together (the sums still
 // add them one by one in ascending order).
 constexpr int GATHER_GROUP = 8;
+
+// Accumulators per pass of the thermal row gather (a multiple of the SELL
+// chunk width, so a pass writes whole chunks): 64 x LW doubles of LDS per
+// workgroup.
+constexpr int ROW_LW = 68;
+static_assert(ROW_LW % SELL_CW == 0, "a pass covers whole SELL chunks");
+// incidence rows whose loads a lane of the row gather issues together
+constexpr int ROW_UNROLL = 4;
+
+typedef double d2 __attribute__((ext_vector_type(2)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+// The ten accumulator codes of one incidence: 1-byte codes at a 2-byte
+// aligned address, 2-byte codes at a 4-byte aligned one.
+__device__ __forceinline__ void load_codes(const uint8_t *p, int (&code)[10]) {
+  const unsigned short *w = (const unsigned short *)p;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const unsigned v = w[j];
+    code[2 * j] = (int)(v & 0xffu);
+    code[2 * j + 1] = (int)(v >> 8);
+  }
+}
+__device__ __forceinline__ void load_codes(const unsigned short *p, int (&code)[10]) {
+  const unsigned *w = (const unsigned *)p;
+#pragma unroll
+  for (int j = 0; j < 5; ++j) {
+    const unsigned v = w[j];
+    code[2 * j] = (int)(v & 0xffffu);
+    code[2 * j + 1] = (int)(v >> 16);
+  }
+}
 
 // |v| as the ordered bit pattern of a non-negative double (0 for NaN)
 __device__ __forceinline__ unsigned long long abs_bits(double v) {
@@ -513,73 +547,115 @@ __global__ void k_plan_fill(int64_t n, const int *__restrict__ rp,
   }
 }
 
-// Numeric phase, thermal: the segmented sum of k_gather_thermal per node
-// slot, written where the plan sends it (spos: the SELL entry of a kept
-// slot, -2 - q for lift entry q, -1 dropped).  A diagonal slot also sums
-// the element loads of the same contributions (k_gather_load's sum, same
-// order) and leaves the node's load and diagonal value for k_sys_rhs.
-__global__ void k_sys_gather_thermal(int64_t nnz, const int *__restrict__ cptr,
-                                     const int *__restrict__ contrib,
-                                     const double *__restrict__ Ke,
-                                     const double *__restrict__ Fe,
-                                     const int *__restrict__ spos,
-                                     const int *__restrict__ slot_row,
-                                     double *__restrict__ sval,
-                                     float *__restrict__ sval32,
-                                     double *__restrict__ liftv,
-                                     double *__restrict__ fload,
-                                     double *__restrict__ dval,
-                                     unsigned long long *__restrict__ dmax) {
-  const int64_t s = xcd_item();
+// Numeric phase, thermal, in the order of the SELL view: a workgroup per
+// slice, a lane per SELL row (node r).  The element kernel left row a of
+// every K_e (and f_e[a]) in the incidence slot of (e, a): step j of the
+// slice holds the j-th incidence (ascending element) of each of its 64
+// nodes side by side, so the lanes' loads of a step are contiguous.  The
+// lane walks its node's incidence rows
+// and adds each into accumulators in LDS, one per entry of r's K_bc row
+// followed by its lift entries, as icode (from the plan) directs.  Each slot
+// so receives its contributions in ascending element order, as in
+// k_gather_thermal: the sums start from -0.0, which leaves the first term
+// unchanged.  The additions are LDS atomics without return (ds_add_f64): a
+// lane only ever adds to its own accumulators, and one wavefront's LDS
+// operations complete in program order, so the order of every sum stays
+// fixed while the lane issues its additions without waiting on each.  A pass covers LW accumulators; longer rows take further passes
+// over their incidence rows.  The lane then writes whole SELL chunks (entries
+// and padding zeros, 32 contiguous bytes per lane), its lift values, the
+// node's load (k_gather_load's sum, same order) and diagonal, and the
+// workgroup leaves the largest |K_ii| in dmax[blockIdx.x].
+template <typename C, int LW>
+__global__ __launch_bounds__(64) void k_sys_rows_thermal(
+    int64_t n, const int *__restrict__ sperm, const int *__restrict__ cb,
+    const int *__restrict__ brp, const int *__restrict__ lptr,
+    const int *__restrict__ diag, const int *__restrict__ ib, const int *__restrict__ icnt,
+    const double *__restrict__ rows, const C *__restrict__ icode, int with_load,
+    double *__restrict__ sval, float *__restrict__ sval32, double *__restrict__ liftv,
+    double *__restrict__ fload, double *__restrict__ dval,
+    unsigned long long *__restrict__ dmax) {
+  __shared__ double acc[LW * 64];
+  const int lane = threadIdx.x;
+  const int64_t sl = xcd_block();
+  const int64_t srow = sl * 64 + lane;
+  int r = -1, nk = 0, nl = 0, lb = 0, ni = 0, kd = -1;
+  if (srow < n) {
+    r = sperm[srow];
+    const int b0 = brp[r];
+    nk = brp[r + 1] - b0;
+    lb = lptr[r];
+    nl = lptr[r + 1] - lb;
+    ni = icnt[srow];
+    const int d = diag[r];
+    if (d >= 0 && ni > 0) kd = d - b0;
+  }
+  const int c0 = cb[sl], nch = cb[sl + 1] - c0;
+  // incidence slot of step j: (ib[sl] + j) * 64 + lane
+  const int64_t q0 = (int64_t)ib[sl] * 64 + lane;
+  int span = max(nk + nl, SELL_CW * nch);
+  for (int off = 32; off > 0; off >>= 1) span = max(span, __shfl_xor(span, off, 64));
+  double *o64 = sval + (int64_t)c0 * SELL_CHUNK + lane * SELL_CW;
+  float *o32 = sval32 + (int64_t)c0 * SELL_CHUNK + lane * SELL_CW;
+  double fl = 0.0;
   unsigned long long mine = 0;
-  if (s < nnz) {
-    const int d = spos[s];
-    const int b0 = cptr[s], b1 = cptr[s + 1];
-    const int ab0 = b1 > b0 ? contrib[b0] % 100 : 1;
-    const bool diag = ab0 / 10 == ab0 % 10;  // a diagonal slot: local a == b
-    if (d != -1 || diag) {
-      // contributions in groups of GATHER_GROUP: the loads of a group are
-      // issued together, the additions stay in ascending order
-      const bool load = diag && Fe;
-      double v = 0.0, fl = 0.0;
-      for (int k0 = b0; k0 < b1; k0 += GATHER_GROUP) {
-        int t[GATHER_GROUP];
-        double x[GATHER_GROUP], y[GATHER_GROUP];
+  for (int p0 = 0; p0 < span; p0 += LW) {
 #pragma unroll
-        for (int u = 0; u < GATHER_GROUP; ++u)
-          if (k0 + u < b1) t[u] = contrib[k0 + u];
+    for (int k = 0; k < LW; ++k) acc[k * 64 + lane] = -0.0;
+    // ROW_UNROLL incidence rows per trip: their loads are issued together,
+    // the additions stay in ascending order
+    for (int i = 0; i < ni; i += ROW_UNROLL) {
+      d2 x[ROW_UNROLL][6];
+      int code[ROW_UNROLL][10];
 #pragma unroll
-        for (int u = 0; u < GATHER_GROUP; ++u)
-          if (k0 + u < b1) {
-            const int e = t[u] / 100;
-            const int ab = t[u] - 100 * e;
-            const int a = ab / 10, b = ab - 10 * (ab / 10);
-            const int p = a <= b ? tri_index(a, b) : tri_index(b, a);
-            x[u] = Ke[(int64_t)e * 55 + p];
-            if (load) y[u] = Fe[(int64_t)e * 10 + a];
+      for (int u = 0; u < ROW_UNROLL; ++u)
+        if (i + u < ni) {
+          const int64_t q = q0 + 64ll * (i + u);
+          const d2 *row = (const d2 *)(rows + q * 12);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) x[u][j] = row[j];
+          load_codes(icode + q * 10, code[u]);
+        }
+#pragma unroll
+      for (int u = 0; u < ROW_UNROLL; ++u)
+        if (i + u < ni) {
+          if (p0 == 0 && with_load) fl += x[u][5].x;
+#pragma unroll
+          for (int b = 0; b < 10; ++b) {
+            const unsigned kk = (unsigned)(code[u][b] - p0);
+            if (kk < (unsigned)LW)
+              unsafeAtomicAdd(&acc[kk * 64 + lane], (b & 1) ? x[u][b / 2].y : x[u][b / 2].x);
           }
+        }
+    }
+    // this pass's SELL chunks, lift values and diagonal
+    const int cend = min(nch, (p0 + LW) / SELL_CW);
+    for (int c = p0 / SELL_CW; c < cend; ++c) {
+      double v[SELL_CW];
 #pragma unroll
-        for (int u = 0; u < GATHER_GROUP; ++u)
-          if (k0 + u < b1) {
-            v = (k0 + u == b0) ? x[u] : v + x[u];
-            if (load) fl += y[u];
-          }
+      for (int q = 0; q < SELL_CW; ++q) {
+        const int k = SELL_CW * c + q;
+        v[q] = k < nk ? acc[(k - p0) * 64 + lane] : 0.0;
       }
-      if (diag) {
-        mine = abs_bits(v);
-        const int r = slot_row[s];
-        dval[r] = v;
-        if (load) fload[r] = fl;
-      }
-      if (d >= 0) {
-        sval[d] = v;
-        sval32[d] = (float)v;
-      } else if (d <= -2) {
-        liftv[-2 - d] = v;
-      }
+      d2 *w64 = (d2 *)(o64 + (int64_t)c * SELL_CHUNK);
+      w64[0] = (d2){v[0], v[1]};
+      w64[1] = (d2){v[2], v[3]};
+      *(f4 *)(o32 + (int64_t)c * SELL_CHUNK) =
+          (f4){(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+    }
+    const int kl1 = min(p0 + LW, nk + nl);
+    for (int k = max(p0, nk); k < kl1; ++k) liftv[lb + k - nk] = acc[(k - p0) * 64 + lane];
+    if (kd >= p0 && kd < p0 + LW) {
+      const double dv = acc[(kd - p0) * 64 + lane];
+      dval[r] = dv;
+      mine = abs_bits(dv);
     }
   }
-  wg_max_store(mine, dmax);
+  if (with_load && ni > 0) fload[r] = fl;
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(mine, off, 64);
+    mine = o > mine ? o : mine;
+  }
+  if (lane == 0) dmax[blockIdx.x] = mine;
 }
 
 // The 3x3 block sums of node slot s (contributions in ascending element
@@ -774,14 +850,91 @@ __global__ void k_plan_bslot(int64_t nnz, const int *__restrict__ rp,
   }
 }
 
-// Plan, thermal: the SELL entry of each kept node slot, or its lift code
-// (-2 - q), or -1 (dropped), so the gather reads one destination per slot.
-__global__ void k_plan_spos(int64_t nnz, const int *__restrict__ dst,
-                            const int *__restrict__ pos, int *__restrict__ spos) {
+// Incidences of node r: the contributions (e, a, a) of its diagonal slot,
+// counted (cnt) and numbered from base (incpos[e*10 + a]), in slot order.
+__global__ void k_inc_count(int64_t n, const int *__restrict__ diag_slot,
+                            const int *__restrict__ cptr, const int *__restrict__ contrib,
+                            int *__restrict__ cnt) {
+  const int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (r >= n) return;
+  const int ds = diag_slot[r];
+  int c = 0;
+  if (ds >= 0)
+    for (int i = cptr[ds]; i < cptr[ds + 1]; ++i) {
+      const int ab = contrib[i] % 100;
+      c += ab / 10 == ab % 10;
+    }
+  cnt[r] = c;
+}
+// Per slice: the incidence count of each SELL row (icnt) and the slice's
+// largest (nib[s], its steps of 64 incidence slots).
+__global__ void k_slice_inc(int64_t n, int64_t nslices, const int *__restrict__ sperm,
+                            const int *__restrict__ cnt, int *__restrict__ icnt,
+                            int *__restrict__ nib) {
+  const int64_t sl = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (sl >= nslices) return;
+  int m = 0;
+  for (int l = 0; l < 64; ++l) {
+    const int64_t srow = sl * 64 + l;
+    const int c = srow < n ? cnt[sperm[srow]] : 0;
+    icnt[srow] = c;
+    m = max(m, c);
+  }
+  nib[sl] = m;
+}
+// incpos[e*10 + a] = the slot of incidence (e, a): one thread per SELL row
+__global__ void k_inc_pos(int64_t n, const int *__restrict__ sperm,
+                          const int *__restrict__ diag_slot, const int *__restrict__ cptr,
+                          const int *__restrict__ contrib, const int *__restrict__ ib,
+                          int *__restrict__ incpos) {
+  const int64_t srow = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (srow >= n) return;
+  const int ds = diag_slot[sperm[srow]];
+  if (ds < 0) return;
+  int64_t q = (int64_t)ib[srow / 64] * 64 + (srow & 63);
+  for (int i = cptr[ds]; i < cptr[ds + 1]; ++i) {
+    const int t = contrib[i];
+    const int ab = t % 100;
+    if (ab / 10 == ab % 10) {
+      incpos[(t / 100) * 10 + ab / 10] = (int)q;
+      q += 64;
+    }
+  }
+}
+
+// Plan, thermal: the accumulator of every triplet (e, a, b) in the row
+// gather of its row r = conn[e][a], stored at icode[incpos[e*10 + a]*10 + b]:
+// k for the row's K_bc entry k, kept + q for its lift entry q; in a
+// constrained row the diagonal is entry 0 and the dropped entries go to
+// accumulator kept + lift, which is never written out.
+template <typename C>
+__global__ void k_plan_icode(int64_t nnz, const int *__restrict__ slot_row,
+                             const int *__restrict__ col, const int *__restrict__ dst,
+                             const int *__restrict__ brp, const int *__restrict__ lptr,
+                             const int *__restrict__ cptr, const int *__restrict__ contrib,
+                             const int *__restrict__ incpos, C *__restrict__ icode) {
   const int64_t s = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (s >= nnz) return;
+  const int r = slot_row[s];
   const int d = dst[s];
-  spos[s] = d >= 0 ? pos[d] : d;
+  const int nk = brp[r + 1] - brp[r];
+  int code;
+  if (d >= 0) code = d - brp[r];
+  else if (d <= -2) code = nk + (-2 - d) - lptr[r];
+  else code = col[s] == r ? 0 : nk + lptr[r + 1] - lptr[r];
+  for (int i = cptr[s]; i < cptr[s + 1]; ++i) {
+    const int t = contrib[i];
+    const int e = t / 100, ab = t - 100 * e;
+    icode[(int64_t)incpos[e * 10 + ab / 10] * 10 + ab % 10] = (C)code;
+  }
+}
+
+// *out = max(*out, longest row of rp)
+__global__ void k_max_rowlen(int64_t n, const int *__restrict__ rp, int *__restrict__ out) {
+  const int64_t r = (int64_t)blockIdx.x * BS + threadIdx.x;
+  int m = r < n ? rp[r + 1] - rp[r] : 0;
+  for (int off = 32; off > 0; off >>= 1) m = max(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0 && m > 0) atomicMax(out, m);
 }
 
 __global__ void k_fill_int(int64_t n, int v, int *__restrict__ out) {
@@ -1203,12 +1356,57 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
   P->sell = build_sell_pattern(ctx, &pat, false, P->dpn == 3 ? 3 : 1);
   const NodePattern &np = m->pat;
   if (dpn == 1) {
-    // the gather's destinations: SELL entry, lift code or dropped, per slot
-    P->spos = dalloc<int>(nnzK);
-    if (nnzK > 0)
-      hipLaunchKernelGGL(k_plan_spos, nblk(nnzK), BS, 0, s, nnzK, P->dst.get(),
-                         P->sell->pos.get(), P->spos.get());
-    GS_LAUNCH_CHECK("k_plan_spos");
+    // incidence slots of the row gather, and their accumulator codes
+    const int64_t E = m->n_elems;
+    const SellPattern &S = *P->sell;
+    P->ib = dalloc<int>(S.nslices + 1);
+    P->icnt = dalloc<int>(S.nslices * 64);
+    P->incpos = dalloc<int>(10 * E);
+    GS_HIP(hipMemsetAsync(P->ib.get(), 0, sizeof(int), s));
+    if (S.nslices > 0) {
+      auto cnt = dalloc<int>(n), nib = dalloc<int>(S.nslices);
+      hipLaunchKernelGGL(k_inc_count, nblk(n), BS, 0, s, n, np.diag_slot.get(),
+                         np.contrib_ptr.get(), np.contrib.get(), cnt.get());
+      GS_LAUNCH_CHECK("k_inc_count");
+      hipLaunchKernelGGL(k_slice_inc, nblk(S.nslices), BS, 0, s, n, S.nslices,
+                         S.sperm.get(), cnt.get(), P->icnt.get(), nib.get());
+      GS_LAUNCH_CHECK("k_slice_inc");
+      scan_inclusive(ctx, nib.get(), P->ib.get() + 1, S.nslices, rocprim::plus<int>());
+      const int steps = read_int(ctx, P->ib.get() + S.nslices);
+      if ((int64_t)steps * 64 >= (int64_t)INT32_MAX)
+        throw Error(GS_ERR_TOO_LARGE, "too many incidence slots for int32 indices");
+      P->n_islots = (int64_t)steps * 64;
+      hipLaunchKernelGGL(k_inc_pos, nblk(n), BS, 0, s, n, S.sperm.get(),
+                         np.diag_slot.get(), np.contrib_ptr.get(), np.contrib.get(),
+                         P->ib.get(), P->incpos.get());
+      GS_LAUNCH_CHECK("k_inc_pos");
+    }
+    int maxlen = 0;
+    if (n > 0) {
+      auto ml = dalloc<int>(1);
+      hipLaunchKernelGGL(k_set_int, 1, 1, 0, s, ml.get(), 0);
+      hipLaunchKernelGGL(k_max_rowlen, nblk(n), BS, 0, s, n, rp, ml.get());
+      GS_LAUNCH_CHECK("k_max_rowlen");
+      maxlen = read_int(ctx, ml.get());
+    }
+    // codes run to the row length (the dropped entries of a constrained row
+    // go past its single kept entry)
+    if (maxlen > 65535)
+      throw Error(GS_ERR_TOO_LARGE, "a node couples to more than 65535 nodes");
+    auto go = [&](auto *out) {
+      if (nnzK > 0)
+        hipLaunchKernelGGL(k_plan_icode, nblk(nnzK), BS, 0, s, nnzK, np.slot_row.get(),
+                           np.col.get(), P->dst.get(), P->row_ptr.get(), P->lift_ptr.get(),
+                           np.contrib_ptr.get(), np.contrib.get(), P->incpos.get(), out);
+      GS_LAUNCH_CHECK("k_plan_icode");
+    };
+    if (maxlen <= 255) {
+      P->icode8 = dalloc<uint8_t>(10 * P->n_islots);
+      go(P->icode8.get());
+    } else {
+      P->icode16 = dalloc<unsigned short>(10 * P->n_islots);
+      go(P->icode16.get());
+    }
   } else {
     // the block-view order of the elastic numeric gather
     const SellPattern &S = *P->sell;
@@ -1238,26 +1436,36 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
 }
 
 int64_t system_gather_blocks(const gs_mesh *m, const SystemPlan &P) {
-  if (P.dpn == 1) return nblk(m->pat.nnz);
+  if (P.dpn == 1) return P.sell->nslices;
   return nblk(P.sell->entries) + (int64_t)nblk(P.n_lift_slots);
 }
 
 void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P, const double *Ke,
-                   const double *Fe, double *sval, float *sval32, double *liftv,
-                   double *fload, double *dval, unsigned long long *dmax) {
+                   const double *Fe, bool with_load, double *sval, float *sval32,
+                   double *liftv, double *fload, double *dval, unsigned long long *dmax) {
   const NodePattern &p = m->pat;
   if (p.nnz == 0) return;
   if (P.dpn == 1) {
-    hipLaunchKernelGGL(k_sys_gather_thermal, nblk(p.nnz), BS, 0, s, p.nnz,
-                       p.contrib_ptr.get(), p.contrib.get(), Ke, Fe, P.spos.get(),
-                       p.slot_row.get(), sval, sval32, liftv, fload, dval, dmax);
+    const SellPattern &S = *P.sell;
+    auto go = [&](const auto *code) {
+      hipLaunchKernelGGL((k_sys_rows_thermal<std::remove_cv_t<std::remove_pointer_t<
+                              decltype(code)>>, ROW_LW>),
+                         (unsigned)S.nslices, 64, 0, s, P.n, S.sperm.get(), S.cb.get(),
+                         P.row_ptr.get(), P.lift_ptr.get(), P.diag.get(), P.ib.get(),
+                         P.icnt.get(),
+                         Ke, code, (int)with_load, sval, sval32, liftv, fload, dval, dmax);
+    };
+    if (S.nslices > 0) {
+      if (P.icode8) go((const uint8_t *)P.icode8.get());
+      else go((const unsigned short *)P.icode16.get());
+    }
   } else {
     const int64_t nent = P.sell->entries;
     if (nent > 0)
       hipLaunchKernelGGL(k_sys_gather_elastic_b, nblk(nent), BS, 0, s, nent,
                          P.bslot.get(), P.bmask.get(), p.slot_row.get(),
-                         p.contrib_ptr.get(), p.contrib.get(), Ke, Fe, sval, sval32,
-                         fload, dval, dmax);
+                         p.contrib_ptr.get(), p.contrib.get(), Ke, with_load ? Fe : nullptr,
+                         sval, sval32, fload, dval, dmax);
     if (P.n_lift_slots > 0)
       hipLaunchKernelGGL(k_sys_lift_elastic, nblk(P.n_lift_slots), BS, 0, s,
                          P.n_lift_slots, P.lift_slots.get(), p.row_ptr.get(),

@@ -140,9 +140,10 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   const int64_t E = m->n_elems;
   const int dpn = sp.elastic ? 3 : 1;
   const bool with_load = !sp.elastic || sp.ek.load != GS_LOAD_NONE;
-  auto Ke = palloc<double>(pool, E * (sp.elastic ? 900 : 55));
+  // thermal: one 12-double row per incidence slot of the plan, the load inside
+  auto Ke = palloc<double>(pool, sp.elastic ? E * 900 : P->n_islots * 12);
   DArr<double> Fe;
-  if (with_load) Fe = palloc<double>(pool, E * 10 * dpn);
+  if (with_load && sp.elastic) Fe = palloc<double>(pool, E * 10 * dpn);
   auto flag = palloc<int>(pool, 1);
   GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
   if (sp.elastic)
@@ -150,7 +151,7 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
                           flag.get());
   else
     launch_thermal_elements(s, E, m->xyz.get(), m->conn.get(), sp.k, sp.k_elem, sp.Q,
-                            true, Ke.get(), Fe.get(), flag.get());
+                            true, Ke.get(), nullptr, flag.get(), P->incpos.get());
   check_flags(read_flag_and_clear(ctx, flag.get()), sp.elastic ? sp.ek.mat : GS_MAT_ELASTIC);
   const SellPattern &S = *P->sell;
   auto sval = palloc<double>(pool, S.nvals);
@@ -164,7 +165,7 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   auto dval = palloc<double>(pool, P->n);   // per DOF: the gathered diagonal
   // K_bc's CSR values are not written here: ensure_csr_values produces them
   // from the SELL values when a caller asks for the CSR form
-  system_gather(s, m, *P, Ke.get(), with_load ? Fe.get() : nullptr, sval.get(), sval32.get(),
+  system_gather(s, m, *P, Ke.get(), Fe.get(), with_load, sval.get(), sval32.get(),
                 liftv.get(), fload.get(), dval.get(), dmax.get());
   system_finish(ctx, m, *P, with_load ? fload.get() : nullptr, dval.get(), sp.fext,
                 sval.get(), sval32.get(), liftv.get(), dperm.get(), f_bc, sdinv.get(),

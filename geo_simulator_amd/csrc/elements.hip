@@ -12,6 +12,8 @@
 // from all ten nodes, so curved (non-affine) Tet10s are handled.
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "internal.hpp"
 
 namespace gs {
@@ -173,17 +175,28 @@ __device__ __forceinline__ void load_nodes(const double *__restrict__ xyz,
   }
 }
 
+// index of entry (a, b) of a symmetric 10x10 matrix in its packed upper
+// triangle (row-major, a <= b)
+__host__ __device__ constexpr int tri_p(int a, int b) {
+  return a <= b ? 10 * a - (a * (a - 1)) / 2 + (b - a) : 10 * b - (b * (b - 1)) / 2 + (a - b);
+}
+
 // ---------------------------------------------------------------------------
 // Thermal: one thread per element (a wavefront handles a 64-element batch).
 // Writes the 55 upper-triangle entries of K_e (bitwise symmetric, because
 // grad_i . grad_j is evaluated with the same products in either order) in
 // element-major layout Ke[e*55 + p(a,b)], and optionally f_e (Fe[e*10+a]).
+// ROWS: row a of K_e goes to the incidence slot i = incpos[e*10 + a] of the
+// system assembly's row gather instead, Ke[i*12 + b] (b < 10), with f_e[a]
+// at i*12 + 10 and a zero at i*12 + 11: 96 bytes, three whole 32-byte
+// sectors per row, written as six 16-byte stores.
 // ---------------------------------------------------------------------------
-template <bool PER_ELEM, bool WITH_LOAD>
+template <bool PER_ELEM, bool WITH_LOAD, bool ROWS>
 __global__ __launch_bounds__(128) void k_thermal_elements(
     int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn,
     double k, const double *__restrict__ k_elem, double Q,
-    double *__restrict__ Ke, double *__restrict__ Fe, int *__restrict__ flag) {
+    double *__restrict__ Ke, double *__restrict__ Fe, const int *__restrict__ incpos,
+    int *__restrict__ flag) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= E) return;
   double X[10][3];
@@ -219,6 +232,18 @@ __global__ __launch_bounds__(128) void k_thermal_elements(
     }
   }
   if (bad) atomicOr(flag, bad);
+  if (ROWS) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+#pragma unroll
+    for (int a = 0; a < 10; ++a) {
+      d2 *o = (d2 *)(Ke + (int64_t)incpos[e * 10 + a] * 12);
+#pragma unroll
+      for (int b = 0; b < 10; b += 2)
+        o[b / 2] = (d2){K[tri_p(a, b)], K[tri_p(a, b + 1)]};
+      o[5] = (d2){F[a], 0.0};
+    }
+    return;
+  }
   double *out = Ke + e * 55;
 #pragma unroll
   for (int p = 0; p < 55; ++p) out[p] = K[p];
@@ -1110,20 +1135,25 @@ __global__ __launch_bounds__(128) void k_maxwell_stress(
 void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double k, const double *k_elem,
                              double Q, bool with_load, double *Ke, double *Fe,
-                             int *flag) {
+                             int *flag, const int *incpos) {
   if (E == 0) return;
-  const unsigned g = blocks_for(E, 128);
-  if (k_elem) {
-    if (with_load)
-      hipLaunchKernelGGL((k_thermal_elements<true, true>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
+  using T = std::true_type;
+  using F = std::false_type;
+  auto go = [&](auto per, auto load) {
+    constexpr bool P = decltype(per)::value, L = decltype(load)::value;
+    if (incpos)
+      hipLaunchKernelGGL((k_thermal_elements<P, L, true>), blocks_for(E, 128), 128, 0, s, E,
+                         xyz, conn, k, k_elem, Q, Ke, Fe, incpos, flag);
     else
-      hipLaunchKernelGGL((k_thermal_elements<true, false>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
-  } else {
-    if (with_load)
-      hipLaunchKernelGGL((k_thermal_elements<false, true>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
-    else
-      hipLaunchKernelGGL((k_thermal_elements<false, false>), g, 128, 0, s, E, xyz, conn, k, k_elem, Q, Ke, Fe, flag);
-  }
+      hipLaunchKernelGGL((k_thermal_elements<P, L, false>), blocks_for(E, 128), 128, 0, s, E,
+                         xyz, conn, k, k_elem, Q, Ke, Fe, incpos, flag);
+  };
+  auto with_load_flag = [&](auto per) {
+    if (with_load) go(per, T{});
+    else go(per, F{});
+  };
+  if (k_elem) with_load_flag(T{});
+  else with_load_flag(F{});
   GS_LAUNCH_CHECK("k_thermal_elements");
 }
 

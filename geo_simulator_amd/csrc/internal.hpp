@@ -163,7 +163,18 @@ struct SystemPlan {
   // (-1: padding), and the node slots with lift destinations
   DArr<int> bslot;
   DArr<unsigned short> bmask;  // kept entries of each block position (bit m)
-  DArr<int> spos;              // thermal: per node slot, SELL entry / lift code / -1
+  // thermal: incidence slots of the row gather.  The incidences (e, a) of
+  // node r = conn[e][a], in ascending element order, take the slots
+  // (ib[s] + j) * 64 + l, j = 0 .. icnt[64 s + l] - 1, of the node's SELL
+  // row 64 s + l: slice s owns ib[s+1] - ib[s] steps of 64 slots, so a
+  // wavefront's loads of step j are contiguous.  incpos[e*10 + a] is the
+  // slot of (e, a); icode[slot*10 + b] names the accumulator of triplet
+  // (e, a, b) (1-byte codes when every row has at most 255 entries, else
+  // 2-byte).
+  DArr<int> incpos, ib, icnt;
+  int64_t n_islots = 0;
+  DArr<uint8_t> icode8;
+  DArr<unsigned short> icode16;
   DArr<int> lift_slots;
   int64_t n_lift_slots = 0;
 };
@@ -250,7 +261,7 @@ namespace gs {
 void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double k, const double *k_elem,
                              double Q, bool with_load, double *Ke, double *Fe,
-                             int *flag);
+                             int *flag, const int *incpos = nullptr);
 // The elastic element family: K_e = sum_q w B^T D B for an isotropic Voigt
 // law (elastic or Newtonian viscous) and an optional element load.
 enum : int { GS_MAT_ELASTIC = 0, GS_MAT_VISCOUS = 1 };
@@ -344,12 +355,16 @@ void permute_scatter(hipStream_t s, int64_t n, const double *src, const int *map
 // System plans and the fused numeric phase (assembly.hip).
 std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
                                         const gs_dofmap *d);
-// values of K_bc (CSR and SELL, FP64 and FP32) and the lift values
 // workgroups of the system gather (entries of its diagonal-maximum array)
 int64_t system_gather_blocks(const gs_mesh *m, const SystemPlan &P);
+// values of K_bc (SELL, FP64 and FP32), the lift values, and per DOF the
+// gathered load (when with_load) and diagonal.  Thermal: Ke holds the
+// incidence rows of k_thermal_elements<.., ROWS = true> (the load inside, Fe
+// unused, one row per incidence slot of the plan); elastic: the block-major
+// K_e and Fe.
 void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P, const double *Ke,
-                   const double *Fe, double *sval, float *sval32, double *liftv,
-                   double *fload, double *dval, unsigned long long *dmax);
+                   const double *Fe, bool with_load, double *sval, float *sval32,
+                   double *liftv, double *fload, double *dval, unsigned long long *dmax);
 void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P, const double *fload,
                    const double *dval, const double *fext, double *sval, float *sval32,
                    const double *liftv, const int *dperm, double *fbc, double *sdinv,

@@ -521,3 +521,20 @@ def test_thermal_system_long_rows(oracle, n_tets):
     x = x.cpu().numpy() if hasattr(x, "cpu") else np.asarray(x)
     xo = np.linalg.solve(Kbo.toarray(), fbo)
     assert np.linalg.norm(x - xo) <= 1e-9 * np.linalg.norm(xo)
+
+
+def test_thermal_system_subnormal_values(oracle):
+    """A conductivity of 1e-312 leaves K's entries subnormal (about 1e-310):
+    the element kernel, the row gather's LDS sums and the elimination keep
+    them bit for bit."""
+    xyz, conn, mesh = mesh_pair(oracle, 3, 3, 2, 1000.0, 1000.0, 600.0, 0.05, 3)
+    mask, vals = oracle.thermal_dirichlet(xyz, 600.0)
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    Ko = oracle.assemble_thermal(xyz, conn, 1e-312)
+    assert 0 < np.abs(Ko.data[Ko.data != 0]).max() < 2.2e-308  # all subnormal
+    fo = oracle.assemble_thermal_load(xyz, conn, 1e-315)
+    Kbo, fbo = oracle.apply_dirichlet(Ko, fo, mask, vals)
+    Kf, ff = G.Assembler.assemble_thermal_system(mesh, dof, 1e-312, 1e-315)
+    assert same_csr(Kf.to_scipy(), Kbo)
+    assert np.array_equal(ff.cpu().numpy(), fbo)

@@ -152,7 +152,6 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   else
     launch_thermal_elements(s, E, m->xyz.get(), m->conn.get(), sp.k, sp.k_elem, sp.Q,
                             true, Ke.get(), nullptr, flag.get(), P->incpos.get());
-  check_flags(read_flag_and_clear(ctx, flag.get()), sp.elastic ? sp.ek.mat : GS_MAT_ELASTIC);
   const SellPattern &S = *P->sell;
   auto sval = palloc<double>(pool, S.nvals);
   auto sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
@@ -170,6 +169,10 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
   system_finish(ctx, m, *P, with_load ? fload.get() : nullptr, dval.get(), sp.fext,
                 sval.get(), sval32.get(), liftv.get(), dperm.get(), f_bc, sdinv.get(),
                 sdinv32.get(), dmax.get());
+  // the element kernels' status, read once the stream has drained: the
+  // gathers index by the plan alone, so values from a rejected element
+  // reach only buffers that are discarded here
+  check_flags(read_flag_and_clear(ctx, flag.get()), sp.elastic ? sp.ek.mat : GS_MAT_ELASTIC);
   gs_csr *K = new_csr(ctx, P->n, P->n, P->nnz, P->row_ptr, P->col, DArr<double>());
   K->val_lazy = true;
   K->dperm = dperm;

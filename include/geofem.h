@@ -8,7 +8,9 @@
  *  - Every call returns gs_status; GS_OK == 0.  A condition on which the
  *    reference panics returns a non-zero status instead, and
  *    gs_last_error(ctx) gives the message.  CG non-convergence is not an
- *    error (stats->converged == 0), as in the reference.
+ *    error (stats->converged == 0), as in the reference.  After an error
+ *    the contents of caller-owned output buffers are unspecified and no
+ *    output handle is returned.
  *  - Pointers whose name ends in _dev are device pointers (HBM of the ctx's
  *    device, e.g. from gs_device_alloc or any allocator on that device);
  *    every other pointer is host memory.  Device handles (mesh, dofmap, csr)

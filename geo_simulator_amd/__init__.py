@@ -8,7 +8,7 @@ The Python names mirror the reference's Rust API:
     Assembler.assemble_thermal_stiffness_parallel / assemble_thermal_load /
     assemble_elasticity_stiffness_parallel / assemble_gravity_load /
     apply_dirichlet_bcs, IsotropicElasticity, ConjugateGradient.solve,
-    SolverStats, JacobiPreconditioner.
+    SolverStats, JacobiPreconditioner, ILUPreconditioner.
 Importing the package needs no GPU; compute calls need one and the built
 library (there is no CPU fallback).
 """
@@ -16,7 +16,8 @@ from ._lib import GeoFemError, LIB_PATH
 from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
 from .dof import DofManager
 from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, IdentityPreconditioner,
-                     JacobiPreconditioner, LinearOperator, SolverStats, norm)
+                     ILUPreconditioner, JacobiPreconditioner, LinearOperator, SolverStats,
+                     norm)
 from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
                        time_elastic_elements,
                        thermal_element_backward)
@@ -30,7 +31,7 @@ from .vtk import VtkWriter
 __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
-    "IdentityPreconditioner", "LinearOperator",
+    "IdentityPreconditioner", "ILUPreconditioner", "LinearOperator",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",

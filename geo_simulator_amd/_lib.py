@@ -21,6 +21,7 @@ GS_ERR_INVALID_MATERIAL = 5
 GS_ERR_NONFINITE = 6
 GS_ERR_TOO_LARGE = 7
 GS_ERR_OOM = 8
+GS_ERR_ZERO_PIVOT = 9
 
 GS_PREC_FP64 = 0
 GS_PREC_MIXED = 1
@@ -37,7 +38,7 @@ _STATUS_NAMES = {
     GS_ERR_BAD_DOFS: "GS_ERR_BAD_DOFS",
     GS_ERR_INVALID_MATERIAL: "GS_ERR_INVALID_MATERIAL",
     GS_ERR_NONFINITE: "GS_ERR_NONFINITE", GS_ERR_TOO_LARGE: "GS_ERR_TOO_LARGE",
-    GS_ERR_OOM: "GS_ERR_OOM",
+    GS_ERR_OOM: "GS_ERR_OOM", GS_ERR_ZERO_PIVOT: "GS_ERR_ZERO_PIVOT",
 }
 
 
@@ -123,6 +124,12 @@ SIGNATURES = {
     "gs_csr_destroy": (None, [_vp]),
     "gs_csr_spmv": (C.c_int, [_vp, _vp, _vp, _vp]),
     "gs_jacobi_diag_inv": (C.c_int, [_vp, _vp, _vp]),
+    "gs_ilu_create": (C.c_int, [_vp, _vp, _pp]),
+    "gs_ilu_update": (C.c_int, [_vp, _vp, _vp]),
+    "gs_ilu_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_ilu_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
+    "gs_ilu_export": (C.c_int, [_vp, _vp, _i64p, _i64p, _dp]),
+    "gs_ilu_destroy": (None, [_vp]),
     "gs_cg_opts_default": (C.c_int, [C.POINTER(CgOpts)]),
     "gs_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                               C.POINTER(SolverStatsC)]),

@@ -4,8 +4,8 @@
 
 Objects go to geo_simulator_amd/_build/, the library to
 geo_simulator_amd/libgeofem.so (in-tree, so it ships with the source tree).
-Element and assembly kernels are compiled with -ffp-contract=off so their
-results are bit-identical to the FP64 restatement of the reference order;
+Element, assembly and ILU(0) kernels are compiled with -ffp-contract=off so
+their results are bit-identical to the FP64 restatement of the reference order;
 the solver keeps contraction (FMA) on.  Packed-FP32 instructions are disabled
 for every GPU object.
 """
@@ -29,6 +29,7 @@ SOURCES = [
     ("operator.hip", []),
     ("solver.hip", []),
     ("measure.hip", []),
+    ("ilu.hip", ["-ffp-contract=off"]),
     ("capi.cpp", []),
 ]
 

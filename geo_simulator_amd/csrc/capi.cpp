@@ -983,6 +983,70 @@ gs_status gs_csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y) 
   });
 }
 
+gs_status gs_ilu_create(gs_ctx *ctx, const gs_csr *A, gs_ilu **out) {
+  if (!ctx || !A || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    set_device(ctx);
+    *out = ilu_create(ctx, A);
+  });
+}
+
+gs_status gs_ilu_update(gs_ctx *ctx, gs_ilu *ilu, const gs_csr *A) {
+  if (!ctx || !ilu || !A || ilu->ctx != ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    set_device(ctx);
+    ilu_update(ilu, A);
+  });
+}
+
+gs_status gs_ilu_apply(gs_ctx *ctx, const gs_ilu *ilu, const double *r, double *z) {
+  if (!ctx || !ilu || ilu->ctx != ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((r && z) || ilu->n == 0, GS_ERR_INVALID_ARG, "null vector");
+    require(r != z || ilu->n == 0, GS_ERR_INVALID_ARG, "r and z must be distinct buffers");
+    set_device(ctx);
+    ilu_apply(ilu, r, z);
+  });
+}
+
+gs_status gs_ilu_info(const gs_ilu *ilu, int64_t *n, int64_t *nnz, int64_t *levels_lower,
+                      int64_t *levels_upper) {
+  if (!ilu) return GS_ERR_INVALID_ARG;
+  if (n) *n = ilu->n;
+  if (nnz) *nnz = ilu->nnz;
+  if (levels_lower) *levels_lower = (int64_t)ilu->lvl_lower.size() - 1;
+  if (levels_upper) *levels_upper = (int64_t)ilu->lvl_upper.size() - 1;
+  return GS_OK;
+}
+
+gs_status gs_ilu_export(gs_ctx *ctx, const gs_ilu *ilu, int64_t *indptr, int64_t *indices,
+                        double *factors) {
+  if (!ctx || !ilu || !indptr || ilu->ctx != ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((indices && factors) || ilu->nnz == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    std::vector<int> rp((size_t)ilu->n + 1), col((size_t)ilu->nnz);
+    GS_HIP(hipMemcpyAsync(rp.data(), ilu->rp.get(), rp.size() * sizeof(int),
+                          hipMemcpyDeviceToHost, ctx->stream));
+    if (ilu->nnz > 0) {
+      GS_HIP(hipMemcpyAsync(col.data(), ilu->col.get(), col.size() * sizeof(int),
+                            hipMemcpyDeviceToHost, ctx->stream));
+      GS_HIP(hipMemcpyAsync(factors, ilu->f.get(), ilu->nnz * sizeof(double),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    }
+    GS_HIP(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < rp.size(); ++i) indptr[i] = rp[i];
+    for (size_t k = 0; k < col.size(); ++k) indices[k] = col[k];
+  });
+}
+
+void gs_ilu_destroy(gs_ilu *ilu) {
+  if (!ilu) return;
+  if (ilu->ctx) (void)hipStreamSynchronize(ilu->ctx->stream);
+  delete ilu;
+}
+
 gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv) {
   if (!ctx || !A) return GS_ERR_INVALID_ARG;
   return guard(ctx, [&] {

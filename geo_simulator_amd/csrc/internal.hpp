@@ -218,6 +218,18 @@ struct gs_dofmap {
   gs::DArr<double> values;  // n_dofs
 };
 
+// ILU(0) factors of a square matrix in the caller's numbering (ilu.hip):
+// pattern, diagonal positions, factors, and the rows of each dependency
+// level of the forward (lower) and backward (upper) substitutions.
+struct gs_ilu {
+  gs_ctx *ctx = nullptr;
+  int64_t n = 0, nnz = 0;
+  gs::DArr<int> rp, col, diag;
+  gs::DArr<double> f;
+  gs::DArr<int> ord_lower, ord_upper;
+  std::vector<int> lvl_lower, lvl_upper;  // level offsets into ord_*
+};
+
 struct gs_csr {
   gs_ctx *ctx = nullptr;
   int64_t rows = 0, cols = 0, nnz = 0;
@@ -369,6 +381,11 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P, const dou
                    const double *dval, const double *fext, double *sval, float *sval32,
                    const double *liftv, const int *dperm, double *fbc, double *sdinv,
                    float *sdinv32, const unsigned long long *dmax);
+
+// ---- ilu.hip ----
+gs_ilu *ilu_create(gs_ctx *ctx, const gs_csr *A);
+void ilu_update(gs_ilu *M, const gs_csr *A);
+void ilu_apply(const gs_ilu *M, const double *r, double *z);
 
 // ---- operator.hip ----
 // block = 3 builds the 3x3-block view (rows must be a multiple of 3; a block

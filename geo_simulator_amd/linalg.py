@@ -2,7 +2,8 @@
 
 Mirrors `CsMat<f64>` + `impl LinearOperator for CsMat<f64>`
 (src/linalg/solver.rs:61-92), `SolverStats` (src/linalg/solver.rs:5-20),
-`JacobiPreconditioner` (src/linalg/preconditioner.rs:29-62) and
+`JacobiPreconditioner` (src/linalg/preconditioner.rs:29-62),
+`ILUPreconditioner` (src/linalg/preconditioner.rs:78-204) and
 `ConjugateGradient` (src/linalg/iterative.rs:7-147).  Vectors may be numpy
 arrays (host, like Vec<f64>: results come back as numpy) or torch tensors on
 the device (results stay on the device).
@@ -145,6 +146,74 @@ class JacobiPreconditioner:
         if _is_torch(r):
             return r * self.diag_inv
         return np.asarray(r) * self.diag_inv.cpu().numpy()
+
+
+class ILUPreconditioner:
+    """ILU(0) with zero fill (src/linalg/preconditioner.rs:78-204), factored
+    and applied on the device, bit-identical to the reference's loops (rows
+    in the caller's numbering).  `ILUPreconditioner(A)` is
+    `ILUPreconditioner::new(&A)`; where the reference returns Err (a row
+    without a diagonal, a pivot below 1e-15) this raises GeoFemError
+    (GS_ERR_ZERO_PIVOT) with the reference's message."""
+
+    def __init__(self, A: CsrMatrix):
+        self.ctx = A.ctx
+        self.handle = None
+        h = C.c_void_p()
+        self.ctx.sync_torch()
+        self.ctx.check(lib().gs_ilu_create(self.ctx.handle, A.handle, C.byref(h)),
+                       "ILUPreconditioner::new")
+        self.handle = h
+        n, z, lo, up = C.c_int64(), C.c_int64(), C.c_int64(), C.c_int64()
+        lib().gs_ilu_info(h, C.byref(n), C.byref(z), C.byref(lo), C.byref(up))
+        self.shape = (n.value, n.value)
+        self._nnz = z.value
+        self.levels = (lo.value, up.value)
+
+    @classmethod
+    def new(cls, A: CsrMatrix) -> "ILUPreconditioner":
+        return cls(A)
+
+    def update(self, A: CsrMatrix) -> None:
+        """ILUPreconditioner::update (:110-124): refactor with A's values."""
+        self.ctx.sync_torch()
+        self.ctx.check(lib().gs_ilu_update(self.ctx.handle, self.handle, A.handle),
+                       "ILUPreconditioner::update")
+
+    def apply(self, r):
+        """z = U^-1 L^-1 r (:168-203); numpy in, numpy out; a device tensor
+        in, a device tensor out."""
+        ctx = self.ctx
+        host = not _is_torch(r)
+        x = ctx.to_device(r)
+        if x.numel() != self.shape[0]:
+            raise ValueError("vector length differs from the factored matrix")
+        z = ctx.empty(self.shape[0])
+        ctx.sync_torch()
+        ctx.check(lib().gs_ilu_apply(ctx.handle, self.handle, dptr(x), dptr(z)),
+                  "ILUPreconditioner::apply")
+        return z.cpu().numpy() if host else z
+
+    def factors(self) -> sp.csr_matrix:
+        """The `factors` field: L (unit diagonal implied) below, U on and above
+        the diagonal, on A's pattern."""
+        n = self.shape[0]
+        indptr = np.empty(n + 1, dtype=np.int64)
+        indices = np.empty(self._nnz, dtype=np.int64)
+        data = np.empty(self._nnz, dtype=np.float64)
+        self.ctx.check(lib().gs_ilu_export(
+            self.ctx.handle, self.handle, indptr.ctypes.data_as(C.POINTER(C.c_int64)),
+            indices.ctypes.data_as(C.POINTER(C.c_int64)),
+            data.ctypes.data_as(C.POINTER(C.c_double))), "gs_ilu_export")
+        return sp.csr_matrix((data, indices, indptr), shape=self.shape)
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                lib().gs_ilu_destroy(self.handle)
+            self.handle = None
+        except Exception:
+            pass
 
 
 class IdentityPreconditioner:

@@ -40,7 +40,8 @@ typedef enum {
   GS_ERR_INVALID_MATERIAL = 5,  /* src/mechanics/constitutive.rs:31-35 */
   GS_ERR_NONFINITE = 6,         /* non-finite Jacobian determinant */
   GS_ERR_TOO_LARGE = 7,         /* exceeds int32 indexing or a fixed limit */
-  GS_ERR_OOM = 8
+  GS_ERR_OOM = 8,
+  GS_ERR_ZERO_PIVOT = 9         /* ILU(0): src/linalg/preconditioner.rs:103,140,159 */
 } gs_status;
 
 typedef enum { GS_PREC_FP64 = 0, GS_PREC_MIXED = 1 } gs_precision;
@@ -49,6 +50,7 @@ typedef struct gs_ctx gs_ctx;
 typedef struct gs_mesh gs_mesh;
 typedef struct gs_dofmap gs_dofmap;
 typedef struct gs_csr gs_csr;
+typedef struct gs_ilu gs_ilu;
 
 /* ---- context / memory (no reference counterpart: device plumbing) ---- */
 gs_status gs_ctx_create(int device, gs_ctx **out);
@@ -305,6 +307,32 @@ gs_status gs_bicgstab_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
 gs_status gs_gmres_solve(gs_ctx *ctx, const gs_csr *A, const double *b_dev,
                          const gs_cg_opts *opts, int32_t restart, double *x_dev,
                          gs_solver_stats *stats);
+
+/* ---- ILU(0) preconditioner (src/linalg/preconditioner.rs:78-204) ----
+ * ILUPreconditioner::new (:86-108, factorize_in_place :126-166): incomplete
+ * LU with zero fill of a square A, in A's caller numbering with columns
+ * ascending (the matrix the reference would hold), rows eliminated in
+ * order; bit-identical factors.  GS_ERR_ZERO_PIVOT with the reference's
+ * message ("Row i has no diag", "Zero diag at k", "Zero diag after update at
+ * i") where the reference returns Err. */
+gs_status gs_ilu_create(gs_ctx *ctx, const gs_csr *A, gs_ilu **out);
+/* ILUPreconditioner::update (:110-124): refactor with A's values; A must
+ * have the factored pattern ("Size mismatch" for another size,
+ * GS_ERR_INVALID_ARG for another pattern). */
+gs_status gs_ilu_update(gs_ctx *ctx, gs_ilu *ilu, const gs_csr *A);
+/* Preconditioner::apply for ILUPreconditioner (:168-203): z = U^-1 L^-1 r,
+ * forward then backward substitution in the reference's order; device
+ * vectors of n doubles in A's caller numbering (r_dev and z_dev distinct). */
+gs_status gs_ilu_apply(gs_ctx *ctx, const gs_ilu *ilu, const double *r_dev, double *z_dev);
+/* Size, and the dependency levels of the forward and backward substitutions
+ * (one launch each); no reference counterpart. */
+gs_status gs_ilu_info(const gs_ilu *ilu, int64_t *n, int64_t *nnz, int64_t *levels_lower,
+                      int64_t *levels_upper);
+/* The factors on A's pattern in caller numbering, sprs shape (host arrays:
+ * indptr n+1, indices nnz, factors nnz): the reference's `factors` field. */
+gs_status gs_ilu_export(gs_ctx *ctx, const gs_ilu *ilu, int64_t *indptr, int64_t *indices,
+                        double *factors);
+void gs_ilu_destroy(gs_ilu *ilu);
 
 /* Solver::solve_with_operator for ConjugateGradient (src/linalg/solver.rs:
  * 112-120, src/linalg/iterative.rs:133-140 and the loop at :46-119): CG on a

@@ -1424,6 +1424,82 @@ int or_gmres_solve(const or_csr *A, const double *b, int64_t max_iter,
   return OR_OK;
 }
 
+/* ILUPreconditioner::new (src/linalg/preconditioner.rs:86-108) and
+ * factorize_in_place (:126-166). */
+int or_ilu0_factor(const or_csr *A, double *factors, int64_t *diag, int64_t *bad,
+                   int32_t *kind) {
+  const int64_t n = A->rows;
+  const int64_t *indptr = A->indptr, *indices = A->indices;
+  double *data = factors;
+  for (int64_t p = 0; p < A->nnz; ++p) data[p] = A->data[p];
+  /* 1. the diagonal position of every row (:93-105) */
+  for (int64_t i = 0; i < n; ++i) {
+    int found = 0;
+    for (int64_t idx = indptr[i]; idx < indptr[i + 1]; ++idx)
+      if (indices[idx] == i) {
+        diag[i] = idx;
+        found = 1;
+        break;
+      }
+    if (!found) {
+      *bad = i;
+      *kind = 0;
+      return OR_ERR_ZERO_PIVOT;
+    }
+  }
+  /* 2. elimination by the earlier rows k < i, merge-style update of row i
+   * over row k's upper part (:131-160) */
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t row_start = indptr[i], diag_i = diag[i];
+    for (int64_t k_idx = row_start; k_idx < diag_i; ++k_idx) {
+      const int64_t k = indices[k_idx];
+      const int64_t k_diag = diag[k], k_end = indptr[k + 1];
+      const double diag_val_k = data[k_diag];
+      if (fabs(diag_val_k) < 1e-15) {
+        *bad = k;
+        *kind = 1;
+        return OR_ERR_ZERO_PIVOT;
+      }
+      const double val_ik = data[k_idx] / diag_val_k;
+      data[k_idx] = val_ik;
+      int64_t cur = k_idx + 1;
+      const int64_t row_i_end = indptr[i + 1];
+      for (int64_t kj = k_diag + 1; kj < k_end; ++kj) {
+        const int64_t col_j = indices[kj];
+        while (cur < row_i_end && indices[cur] < col_j) ++cur;
+        if (cur == row_i_end) break;
+        if (indices[cur] == col_j) data[cur] -= val_ik * data[kj];
+      }
+    }
+    if (fabs(data[diag_i]) < 1e-15) {
+      *bad = i;
+      *kind = 2;
+      return OR_ERR_ZERO_PIVOT;
+    }
+  }
+  return OR_OK;
+}
+
+/* ILUPreconditioner::apply (src/linalg/preconditioner.rs:168-203): forward
+ * substitution with the unit lower factor, then backward with the upper. */
+void or_ilu0_apply(const or_csr *A, const double *factors, const int64_t *diag,
+                   const double *r, double *z) {
+  const int64_t n = A->rows;
+  const int64_t *indptr = A->indptr, *indices = A->indices;
+  for (int64_t i = 0; i < n; ++i) z[i] = r[i];
+  for (int64_t i = 0; i < n; ++i) {
+    double sum = 0.0;
+    for (int64_t idx = indptr[i]; idx < diag[i]; ++idx) sum += factors[idx] * z[indices[idx]];
+    z[i] -= sum;
+  }
+  for (int64_t i = n - 1; i >= 0; --i) {
+    double sum = 0.0;
+    for (int64_t idx = diag[i] + 1; idx < indptr[i + 1]; ++idx)
+      sum += factors[idx] * z[indices[idx]];
+    z[i] = (z[i] - sum) / factors[diag[i]];
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Adjoint element backward (no reference counterpart; pinned by FD).        */
 /* With theta_e = log k_e (or log E_e), dK_e/dtheta_e = K_e, so              */

@@ -47,7 +47,8 @@ enum {
   OR_ERR_SINGULAR_JACOBIAN = 3,
   OR_ERR_BAD_DOFS = 4,
   OR_ERR_INVALID_MATERIAL = 5,
-  OR_ERR_OOM = 8
+  OR_ERR_OOM = 8,
+  OR_ERR_ZERO_PIVOT = 9 /* ILU(0): a row without a diagonal, or a pivot below 1e-15 */
 };
 
 /* ---- quadrature (src/fem/quadrature.rs:13-104) ---- */
@@ -159,6 +160,21 @@ int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,
 int or_gmres_solve(const or_csr *A, const double *b, int64_t max_iter,
                    int64_t restart, double tol, double abs_tol, int use_precond,
                    double *x, or_stats *st);
+
+/* ILUPreconditioner::new / update (src/linalg/preconditioner.rs:86-166): the
+ * values of A (square, sorted columns) factored in place into factors (nnz),
+ * L strictly below the diagonal (unit diagonal implied) and U on and above
+ * it, rows in order; diag (rows) receives each row's diagonal position.
+ * OR_ERR_ZERO_PIVOT with *bad = the row: a row without a diagonal entry
+ * (*kind = 0, checked for all rows first), an eliminating row's pivot below
+ * 1e-15 (*kind = 1, *bad = that row k) or a row's own pivot below 1e-15 after
+ * its elimination (*kind = 2), the first in the reference's order. */
+int or_ilu0_factor(const or_csr *A, double *factors, int64_t *diag, int64_t *bad,
+                   int32_t *kind);
+/* ILUPreconditioner::apply (src/linalg/preconditioner.rs:168-203): z = U^-1
+ * L^-1 r with the factors of or_ilu0_factor on A's pattern. */
+void or_ilu0_apply(const or_csr *A, const double *factors, const int64_t *diag,
+                   const double *r, double *z);
 
 /* ---- adjoint element backward (new surface; pinned by central FD) ---- */
 int or_thermal_element_backward(int64_t n_nodes, const double *xyz,

@@ -25,6 +25,7 @@ OR_ERR_INVALID_ARG = 1
 OR_ERR_SINGULAR_JACOBIAN = 3
 OR_ERR_BAD_DOFS = 4
 OR_ERR_INVALID_MATERIAL = 5
+OR_ERR_ZERO_PIVOT = 9
 
 
 class OracleError(RuntimeError):
@@ -144,6 +145,8 @@ def _declare(L):
                                     C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
     L.or_gmres_solve.argtypes = [C.POINTER(_Csr), _dp, C.c_int64, C.c_int64, C.c_double,
                                  C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
+    L.or_ilu0_factor.argtypes = [C.POINTER(_Csr), _dp, _i64p, _i64p, C.POINTER(C.c_int32)]
+    L.or_ilu0_apply.argtypes = [C.POINTER(_Csr), _dp, _i64p, _dp, _dp]
     L.or_thermal_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                               _dp, _dp, _dp, _dp]
     L.or_elastic_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
@@ -532,6 +535,44 @@ def gmres_solve(A, b, max_iter=1000, restart=10, tol=1e-8, abs_tol=1e-12, use_pr
            "gmres_solve")
     return x, SolverStats(st.iterations, st.residual_norm, st.relative_residual,
                           bool(st.converged), st.solve_time)
+
+
+class IluFactorError(ValueError):
+    """ILUPreconditioner::new / update failed: kind 0 'Row {row} has no diag',
+    1 'Zero diag at {row}', 2 'Zero diag after update at {row}'."""
+
+    MESSAGES = ("Row {} has no diag", "Zero diag at {}", "Zero diag after update at {}")
+
+    def __init__(self, row, kind):
+        super().__init__(self.MESSAGES[kind].format(row))
+        self.row, self.kind = row, kind
+
+
+def ilu0_factor(A):
+    """ILUPreconditioner::new (src/linalg/preconditioner.rs:86-166): returns
+    (factors on A's pattern, diagonal positions); raises IluFactorError."""
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    m, keep = _give_csr(A)
+    F = np.empty(A.nnz)
+    diag = np.zeros(A.shape[0], dtype=np.int64)
+    bad = C.c_int64(-1)
+    kind = C.c_int32(-1)
+    code = lib().or_ilu0_factor(C.byref(m), _d(F), _i(diag), C.byref(bad), C.byref(kind))
+    if code == OR_ERR_ZERO_PIVOT:
+        raise IluFactorError(bad.value, kind.value)
+    _check(code, "ilu0_factor")
+    return sp.csr_matrix((F, A.indices.copy(), A.indptr.copy()), shape=A.shape), diag
+
+
+def ilu0_apply(F, diag, r):
+    """ILUPreconditioner::apply (src/linalg/preconditioner.rs:168-203)."""
+    m, keep = _give_csr(F)
+    r = _f64(r)
+    z = np.empty_like(r)
+    dg = np.ascontiguousarray(diag, dtype=np.int64)
+    lib().or_ilu0_apply(C.byref(m), _d(keep[2]), _i(dg), _d(r), _d(z))
+    return z
 
 
 # ---- adjoint --------------------------------------------------------------

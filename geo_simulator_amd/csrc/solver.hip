@@ -1636,12 +1636,26 @@ PcgResult bicgstab_core(gs_ctx *ctx, const SellOp &o, const double *val,
 
 // ---- GMRES (src/linalg/iterative.rs:313-486) ----------------------------------
 // Restarted GMRES(m), right-preconditioned, modified Gram-Schmidt, Givens
-// rotations.  The vectors stay on the device: per inner step z = M v_j,
-// w = A z (SELL, partials v_0.w), one kernel per Gram-Schmidt projection
-// (sums the previous partials itself: h_ij, w -= h_ij v_i, partials of the
-// next projection), and a normalisation kernel (h_{j+1,j}, v_{j+1}).  The
-// host reads the new Hessenberg column and does the rotations, the residual
-// estimate and the back substitution in the reference's order.
+// rotations, entirely on the device: per inner step z = M v_j, w = A z (SELL,
+// partials v_0.w), one kernel per Gram-Schmidt projection (sums the previous
+// partials itself: h_ij, w -= h_ij v_i, partials of the next projection), a
+// normalisation kernel (h_{j+1,j}, v_{j+1}) and a one-thread kernel that
+// applies the earlier rotations to the new Hessenberg column, forms the next
+// rotation, updates g and the residual estimate |g_{j+1}| and decides
+// whether the cycle ends -- the reference's arithmetic in its order.  Every
+// kernel returns at once once the cycle or the solve has ended (GmState),
+// so the host enqueues whole cycles and reads the state once per cycle.
+
+struct GmState {
+  double rnorm;      // residual norm at the start of the cycle
+  double final_res;  // the reference's final_res
+  long long total;   // inner steps over all cycles
+  long long spmv;
+  int k;             // steps of the current cycle
+  int stop;          // 0 running, 1 cycle ended, 2 solve ended
+  int converged;
+  int pad_;
+};
 
 // y = A x (SELL); partials of u.y
 template <class IDX>
@@ -1650,7 +1664,9 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int n, int ns, const int *__res
                                                  const double *__restrict__ x,
                                                  double *__restrict__ y,
                                                  const double *__restrict__ u,
-                                                 double *__restrict__ part) {
+                                                 double *__restrict__ part,
+                                                 const GmState *__restrict__ st) {
+  if (st->stop) return;
   double acc[1] = {0.0};
   auto epi = [&](int row, double sv) {
     y[row] = sv;
@@ -1660,16 +1676,21 @@ __global__ __launch_bounds__(BS) void k_spmv_dot(int n, int ns, const int *__res
   put_partials<1>(acc, part);
 }
 
-// dst = src / s (the reference divides)
-__global__ __launch_bounds__(BS) void k_div(int n, const double *__restrict__ src, double s,
-                                            double *__restrict__ dst) {
-  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) dst[i] = src[i] / s;
+// v_0 = r / ||r|| (the reference divides)
+__global__ __launch_bounds__(BS) void k_gm_first(int n, const double *__restrict__ r,
+                                                 double *__restrict__ v,
+                                                 const GmState *__restrict__ st) {
+  if (st->stop) return;
+  const double rn = st->rnorm;
+  for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) v[i] = r[i] / rn;
 }
 
 // z = D^-1 v
-__global__ __launch_bounds__(BS) void k_precond(int n, const double *__restrict__ v,
-                                                const double *__restrict__ dinv,
-                                                double *__restrict__ z) {
+__global__ __launch_bounds__(BS) void k_gm_precond(int n, const double *__restrict__ v,
+                                                   const double *__restrict__ dinv,
+                                                   double *__restrict__ z,
+                                                   const GmState *__restrict__ st) {
+  if (st->stop) return;
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) z[i] = v[i] * dinv[i];
 }
 
@@ -1681,7 +1702,9 @@ __global__ __launch_bounds__(FBS) void k_gm_project(int n, const double *__restr
                                                     double *__restrict__ w,
                                                     double *__restrict__ hcol, int i,
                                                     const double *__restrict__ part_in,
-                                                    int np_in, double *__restrict__ part) {
+                                                    int np_in, double *__restrict__ part,
+                                                    const GmState *__restrict__ st) {
+  if (st->stop) return;
   double tot[1];
   sum_partials<1>(part_in, np_in, tot);
   const double h = tot[0];
@@ -1696,27 +1719,143 @@ __global__ __launch_bounds__(FBS) void k_gm_project(int n, const double *__restr
   if (threadIdx.x == 0) part[blockIdx.x] = acc[0];
 }
 
-// H[j+1][j] = ||w||; v_{j+1} = w / H[j+1][j] when above 1e-40
+// H[j+1][j] = ||w||; v_{j+1} = w / H[j+1][j] when above 1e-40, else zero
+// (the reference's fresh v_{j+1} of the cycle)
 __global__ __launch_bounds__(FBS) void k_gm_normalize(int n, const double *__restrict__ w,
                                                       double *__restrict__ vnext,
                                                       double *__restrict__ hcol, int j,
                                                       const double *__restrict__ part_in,
-                                                      int np_in) {
+                                                      int np_in, const GmState *__restrict__ st) {
+  if (st->stop) return;
   double tot[1];
   sum_partials<1>(part_in, np_in, tot);
   const double hn = sqrt(tot[0]);
   if (blockIdx.x == 0 && threadIdx.x == 0) hcol[j + 1] = hn;
-  if (!(fabs(hn) > 1e-40)) return;
+  const bool ok = fabs(hn) > 1e-40;
   for (int l = blockIdx.x * FBS + threadIdx.x; l < n; l += gridDim.x * FBS)
-    vnext[l] = w[l] / hn;
+    vnext[l] = ok ? w[l] / hn : 0.0;
+}
+
+// The Hessenberg, rotations and right-hand side of a cycle, on the device
+// (H row-major (m+1) x m, g m+1, cs and sn m, y m).
+struct GmArrays {
+  double *H, *g, *cs, *sn, *y;
+};
+
+// Start of a cycle (src/linalg/iterative.rs:356-381): the solve ends when
+// max_iterations is reached or the true residual meets the tolerance;
+// otherwise H, g, cs, sn are cleared and g[0] = ||r||.
+__global__ __launch_bounds__(FBS) void k_gm_begin(GmState *st, const double *__restrict__ dots,
+                                                  GmArrays a, int m, double target,
+                                                  double abs_tol, long long max_iter) {
+  __shared__ int go;
+  if (threadIdx.x == 0) {
+    go = 0;
+    if (st->stop != 2) {
+      if (st->total >= max_iter) {
+        st->stop = 2;
+      } else {
+        const double r_norm = sqrt(dots[0]);
+        st->final_res = r_norm;
+        st->spmv += 1;
+        if (r_norm < target || r_norm < abs_tol) {
+          st->converged = 1;
+          st->stop = 2;
+        } else {
+          st->rnorm = r_norm;
+          st->k = 0;
+          st->stop = 0;
+          go = 1;
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (!go) return;
+  for (int t = threadIdx.x; t < (m + 1) * m; t += FBS) a.H[t] = 0.0;
+  for (int t = threadIdx.x; t <= m; t += FBS) a.g[t] = t == 0 ? st->rnorm : 0.0;
+  for (int t = threadIdx.x; t < m; t += FBS) {
+    a.cs[t] = 0.0;
+    a.sn[t] = 0.0;
+  }
+}
+
+// src/linalg/iterative.rs:463-473
+__device__ void gm_givens(double a, double b, double &c, double &s, double &rho) {
+#pragma clang fp contract(off)
+  if (fabs(b) < 1e-40) {
+    c = 1.0; s = 0.0; rho = a;
+  } else if (fabs(b) > fabs(a)) {
+    const double tau = a / b;
+    s = 1.0 / sqrt(1.0 + tau * tau);
+    c = s * tau;
+    rho = b * sqrt(1.0 + tau * tau);
+  } else {
+    const double tau = b / a;
+    c = 1.0 / sqrt(1.0 + tau * tau);
+    s = c * tau;
+    rho = a * sqrt(1.0 + tau * tau);
+  }
+}
+
+// End of inner step j (src/linalg/iterative.rs:393-414), one thread.
+__global__ void k_gm_rotate(GmState *st, const double *__restrict__ hcol, GmArrays a, int m,
+                            int j, double target, double abs_tol, long long max_iter) {
+#pragma clang fp contract(off)
+  if (st->stop) return;
+  double *H = a.H;
+  auto HH = [&](int i, int c) -> double & { return H[(size_t)i * m + c]; };
+  for (int i = 0; i <= j + 1; ++i) HH(i, j) = hcol[i];
+  for (int i = 0; i < j; ++i) {
+    const double temp = a.cs[i] * HH(i, j) + a.sn[i] * HH(i + 1, j);
+    HH(i + 1, j) = -a.sn[i] * HH(i, j) + a.cs[i] * HH(i + 1, j);
+    HH(i, j) = temp;
+  }
+  double c, s, rho;
+  gm_givens(HH(j, j), HH(j + 1, j), c, s, rho);
+  a.cs[j] = c;
+  a.sn[j] = s;
+  HH(j, j) = rho;
+  HH(j + 1, j) = 0.0;
+  const double temp = a.cs[j] * a.g[j];
+  a.g[j + 1] = -a.sn[j] * a.g[j];
+  a.g[j] = temp;
+  st->k = j + 1;
+  st->total += 1;
+  st->spmv += 1;
+  const double fr = fabs(a.g[j + 1]);
+  st->final_res = fr;
+  // the cycle ends on the estimate, at max_iterations (the next step's
+  // check), or after its m-th step
+  if (fr < target || fr < abs_tol || st->total >= max_iter || j + 1 == m) st->stop = 1;
+}
+
+// End of a cycle (src/linalg/iterative.rs:416-433): back substitution with
+// its 1e-40 break (one thread), then the solve ends if the estimate met
+// the tolerance.
+__global__ void k_gm_backsub(GmState *st, GmArrays a, int m, double target, double abs_tol) {
+#pragma clang fp contract(off)
+  if (st->stop == 2) return;
+  const int k = st->k;
+  const double *H = a.H;
+  for (int i = 0; i < k; ++i) a.y[i] = 0.0;
+  for (int i = k - 1; i >= 0; --i) {
+    double sum = 0.0;
+    for (int c = i + 1; c < k; ++c) sum += H[(size_t)i * m + c] * a.y[c];
+    if (fabs(H[(size_t)i * m + i]) < 1e-40) break;
+    a.y[i] = (a.g[i] - sum) / H[(size_t)i * m + i];
+  }
 }
 
 // x += M dy, dy = sum_j V_j y_j (accumulated from 0 in j order)
 template <bool PREC>
-__global__ __launch_bounds__(BS) void k_gm_update(int n, int k, const double *__restrict__ V,
+__global__ __launch_bounds__(BS) void k_gm_update(int n, const double *__restrict__ V,
                                                   const double *__restrict__ y,
                                                   const double *__restrict__ dinv,
-                                                  double *__restrict__ x) {
+                                                  double *__restrict__ x,
+                                                  const GmState *__restrict__ st) {
+  if (st->stop == 2) return;
+  const int k = st->k;
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     double dy = 0.0;
     for (int j = 0; j < k; ++j) dy += V[(size_t)j * n + i] * y[j];
@@ -1724,21 +1863,24 @@ __global__ __launch_bounds__(BS) void k_gm_update(int n, int k, const double *__
   }
 }
 
-// src/linalg/iterative.rs:463-473
-void givens(double a, double b, double &c, double &s, double &rho) {
-  if (std::fabs(b) < 1e-40) {
-    c = 1.0; s = 0.0; rho = a;
-  } else if (std::fabs(b) > std::fabs(a)) {
-    const double tau = a / b;
-    s = 1.0 / std::sqrt(1.0 + tau * tau);
-    c = s * tau;
-    rho = b * std::sqrt(1.0 + tau * tau);
-  } else {
-    const double tau = b / a;
-    c = 1.0 / std::sqrt(1.0 + tau * tau);
-    s = c * tau;
-    rho = a * std::sqrt(1.0 + tau * tau);
+__global__ void k_gm_end(GmState *st, double target, double abs_tol) {
+  if (st->stop == 2) return;
+  const double fr = st->final_res;
+  if (fr < target || fr < abs_tol) {
+    st->converged = 1;
+    st->stop = 2;
   }
+}
+
+__global__ void k_gm_init(GmState *st, double b_norm) {
+  st->rnorm = 0.0;
+  st->final_res = b_norm;
+  st->total = 0;
+  st->spmv = 0;
+  st->k = 0;
+  st->stop = 0;
+  st->converged = 0;
+  st->pad_ = 0;
 }
 
 PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const double *dinv,
@@ -1752,10 +1894,16 @@ PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const doub
   double *z = (double *)(base + L.add<double>(n));
   double *V = (double *)(base + L.add<double>((int64_t)n * (m + 1)));
   double *hcol = (double *)(base + L.add<double>(m + 2));
-  double *ydev = (double *)(base + L.add<double>(m + 1));
+  GmArrays a;
+  a.H = (double *)(base + L.add<double>((int64_t)(m + 1) * m));
+  a.g = (double *)(base + L.add<double>(m + 1));
+  a.cs = (double *)(base + L.add<double>(m));
+  a.sn = (double *)(base + L.add<double>(m));
+  a.y = (double *)(base + L.add<double>(m));
   double *part1 = (double *)(base + L.add<double>(GRID_MAX));
   double *part2 = (double *)(base + L.add<double>(GMAX));
   double *dots = (double *)(base + L.add<double>(8));
+  GmState *st = (GmState *)(base + L.add<GmState>(1));
   PcgResult res;
   double h1[1];
   hipLaunchKernelGGL(k_fcg_start, gv, BS, 0, s, n, b, x, r, part2);  // x = 0, ||b||
@@ -1763,6 +1911,7 @@ PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const doub
   GS_LAUNCH_CHECK("k_fcg_start");
   read_dots(ctx, dots, h1, 1);
   const double b_norm = std::sqrt(h1[0]);
+  hipLaunchKernelGGL(k_gm_init, 1, 1, 0, s, st, b_norm);
   res.bnorm = b_norm;
   res.rnorm = b_norm;
   if (b_norm < 1e-25) {  // src/linalg/iterative.rs:346-348
@@ -1771,44 +1920,25 @@ PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const doub
     return res;
   }
   const double target = tol * b_norm;
-  auto done = [&](double v) { return v < target || v < abs_tol; };
-  std::vector<double> H((size_t)(m + 1) * m), g(m + 1), cs(m), sn(m), y(m), col(m + 2);
-  auto HH = [&](int i, int j) -> double & { return H[(size_t)i * m + j]; };
-  long long total = 0;
-  double final_res = b_norm;
-  bool converged = false;
-  while (total < max_iter) {
+  GmState h{};
+  for (;;) {
     launch_residual(s, o, val, x, b, r, part1);  // r = b - A x
     hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, part1, o.grid, dots);
-    GS_LAUNCH_CHECK("gmres residual");
-    read_dots(ctx, dots, h1, 1);
-    ++res.spmv;
-    const double r_norm = std::sqrt(h1[0]);
-    final_res = r_norm;
-    if (done(r_norm)) {
-      converged = true;
-      break;
-    }
-    std::fill(H.begin(), H.end(), 0.0);
-    std::fill(g.begin(), g.end(), 0.0);
-    std::fill(cs.begin(), cs.end(), 0.0);
-    std::fill(sn.begin(), sn.end(), 0.0);
-    hipLaunchKernelGGL(k_div, gv, BS, 0, s, n, (const double *)r, r_norm, V);
-    g[0] = r_norm;
-    int k = 0;
+    hipLaunchKernelGGL(k_gm_begin, 1, FBS, 0, s, st, (const double *)dots, a, m, target,
+                       abs_tol, max_iter);
+    hipLaunchKernelGGL(k_gm_first, gv, BS, 0, s, n, (const double *)r, V, (const GmState *)st);
     for (int j = 0; j < m; ++j) {
-      if (total >= max_iter) break;
       const double *vj = V + (size_t)j * n;
       const double *zin = vj;
       if (dinv) {
-        hipLaunchKernelGGL(k_precond, gv, BS, 0, s, n, vj, dinv, z);
+        hipLaunchKernelGGL(k_gm_precond, gv, BS, 0, s, n, vj, dinv, z, (const GmState *)st);
         zin = z;
       }
       with_idx(*o.P, [&](auto idx) {
         hipLaunchKernelGGL((k_spmv_dot<decltype(idx)>), o.grid, BS, 0, s, n, o.ns(),
-                           o.P->cb.get(), idx, val, zin, w, (const double *)V, part1);
+                           o.P->cb.get(), idx, val, zin, w, (const double *)V, part1,
+                           (const GmState *)st);
       }, 8);
-      ++res.spmv;
       // projections i = 0..j; the first sums the SpMV's partials
       for (int i = 0; i <= j; ++i) {
         const double *nxt = i < j ? V + (size_t)(i + 1) * n : nullptr;
@@ -1816,61 +1946,34 @@ PcgResult gmres_core(gs_ctx *ctx, const SellOp &o, const double *val, const doub
         const int npin = i == 0 ? o.grid : (int)gf;
         double *pout = part2 + (i & 1 ? GMAX / 2 : 0);
         hipLaunchKernelGGL(k_gm_project, gf, FBS, 0, s, n, V + (size_t)i * n, nxt, w, hcol,
-                           i, pin, npin, pout);
+                           i, pin, npin, pout, (const GmState *)st);
       }
       hipLaunchKernelGGL(k_gm_normalize, gf, FBS, 0, s, n, (const double *)w,
                          V + (size_t)(j + 1) * n, hcol, j,
-                         (const double *)(part2 + (j & 1 ? GMAX / 2 : 0)), (int)gf);
+                         (const double *)(part2 + (j & 1 ? GMAX / 2 : 0)), (int)gf,
+                         (const GmState *)st);
+      hipLaunchKernelGGL(k_gm_rotate, 1, 1, 0, s, st, (const double *)hcol, a, m, j, target,
+                         abs_tol, max_iter);
       GS_LAUNCH_CHECK("gmres step");
-      read_dots(ctx, hcol, col.data(), j + 2);
-      for (int i = 0; i <= j + 1; ++i) HH(i, j) = col[i];
-      for (int i = 0; i < j; ++i) {
-        const double temp = cs[i] * HH(i, j) + sn[i] * HH(i + 1, j);
-        HH(i + 1, j) = -sn[i] * HH(i, j) + cs[i] * HH(i + 1, j);
-        HH(i, j) = temp;
-      }
-      double c, sv, rho;
-      givens(HH(j, j), HH(j + 1, j), c, sv, rho);
-      cs[j] = c;
-      sn[j] = sv;
-      HH(j, j) = rho;
-      HH(j + 1, j) = 0.0;
-      const double temp = cs[j] * g[j];
-      g[j + 1] = -sn[j] * g[j];
-      g[j] = temp;
-      k = j + 1;
-      ++total;
-      final_res = std::fabs(g[k]);
-      if (done(final_res)) break;
     }
-    std::fill(y.begin(), y.end(), 0.0);
-    for (int i = k - 1; i >= 0; --i) {
-      double sum = 0.0;
-      for (int j = i + 1; j < k; ++j) sum += HH(i, j) * y[j];
-      if (std::fabs(HH(i, i)) < 1e-40) break;
-      y[i] = (g[i] - sum) / HH(i, i);
-    }
-    if (k > 0) {
-      double *stage = (double *)((char *)ctx->pinned + 2048);
-      memcpy(stage, y.data(), sizeof(double) * k);
-      GS_HIP(hipMemcpyAsync(ydev, stage, sizeof(double) * k, hipMemcpyHostToDevice, s));
-      if (dinv)
-        hipLaunchKernelGGL((k_gm_update<true>), gv, BS, 0, s, n, k, (const double *)V,
-                           (const double *)ydev, dinv, x);
-      else
-        hipLaunchKernelGGL((k_gm_update<false>), gv, BS, 0, s, n, k, (const double *)V,
-                           (const double *)ydev, dinv, x);
-      GS_LAUNCH_CHECK("k_gm_update");
-    }
-    if (done(final_res)) {
-      converged = true;
-      break;
-    }
+    hipLaunchKernelGGL(k_gm_backsub, 1, 1, 0, s, st, a, m, target, abs_tol);
+    if (dinv)
+      hipLaunchKernelGGL((k_gm_update<true>), gv, BS, 0, s, n, (const double *)V,
+                         (const double *)a.y, dinv, x, (const GmState *)st);
+    else
+      hipLaunchKernelGGL((k_gm_update<false>), gv, BS, 0, s, n, (const double *)V,
+                         (const double *)a.y, dinv, x, (const GmState *)st);
+    hipLaunchKernelGGL(k_gm_end, 1, 1, 0, s, st, target, abs_tol);
+    GS_LAUNCH_CHECK("gmres cycle");
+    GS_HIP(hipMemcpyAsync(ctx->pinned, st, sizeof(GmState), hipMemcpyDeviceToHost, s));
+    GS_HIP(hipStreamSynchronize(s));
+    memcpy(&h, ctx->pinned, sizeof(GmState));
+    if (h.stop == 2) break;
   }
-  GS_HIP(hipStreamSynchronize(s));
-  res.iterations = total;
-  res.rnorm = final_res;
-  res.converged = converged;
+  res.iterations = h.total;
+  res.rnorm = h.final_res;
+  res.converged = h.converged != 0;
+  res.spmv = h.spmv;
   return res;
 }
 
@@ -2396,10 +2499,13 @@ void gmres_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b, const gs_cg_opt
   for (int i = 0; i < 3; ++i) dry.add<double>(n);
   dry.add<double>(n * (int64_t)(restart + 1));
   dry.add<double>(restart + 2);
+  dry.add<double>((int64_t)(restart + 1) * restart);
   dry.add<double>(restart + 1);
+  for (int i = 0; i < 3; ++i) dry.add<double>(restart);
   dry.add<double>(GRID_MAX);
   dry.add<double>(GMAX);
   dry.add<double>(8);
+  dry.add<GmState>(1);
   char *base = ctx_scratch(ctx, dry.bytes);
   double *bi = (double *)(base + ob), *xi = (double *)(base + ox);
   const int *vp = A->sell->vperm.get();

@@ -151,3 +151,36 @@ def test_dirichlet_scaling_floor_is_one(oracle):
                                       np.array([5.0, 0.0]))
     assert Kbc.toarray()[0, 0] == 1.0 and fbc[0] == 5.0
     assert fbc[1] == 2.0 - 1e-4 * 5.0
+
+
+@pytest.mark.parametrize("dims", [(1, 1, 1), (3, 2, 2), (4, 3, 5)])
+def test_generator_numbering_first_touch(oracle, dims):
+    """The numbering callers index results by (src/mesh_generator_improved.rs:
+    29-56,139-146): vertex (ix, iy, iz) is node ix + (nx+1)(iy + (ny+1) iz) at
+    (ix dx, iy dy, iz dz); edge midpoints are numbered in the order they are
+    first requested -- elements in order, edges e01, e12, e20, e03, e13, e23 --
+    so the first occurrences of midpoint ids, scanned that way, count up from
+    the number of vertices without gaps; a repeated edge reuses its id."""
+    nx, ny, nz = dims
+    lx, ly, lz = 3.0, 2.0, 1.5
+    xyz, conn = oracle.generate_cube(nx, ny, nz, lx, ly, lz)
+    nv = (nx + 1) * (ny + 1) * (nz + 1)
+    ix, iy, iz = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1),
+                             indexing="ij")
+    ids = (ix + (nx + 1) * (iy + (ny + 1) * iz)).ravel()
+    expect = np.stack([ix.ravel() * lx / nx, iy.ravel() * ly / ny, iz.ravel() * lz / nz], 1)
+    assert np.allclose(xyz[ids], expect, rtol=0, atol=1e-12)
+    assert conn[:, :4].max() < nv and conn[:, 4:].min() >= nv
+    seen, nxt = {}, nv
+    for e in conn:
+        for k in range(6):
+            a, b = [(0, 1), (1, 2), (2, 0), (0, 3), (1, 3), (2, 3)][k]
+            key = tuple(sorted((e[a], e[b])))
+            mid = e[4 + k]
+            if key in seen:
+                assert seen[key] == mid
+            else:
+                assert mid == nxt
+                seen[key] = mid
+                nxt += 1
+    assert nxt == len(xyz)

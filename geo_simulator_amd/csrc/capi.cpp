@@ -339,6 +339,7 @@ gs_status gs_ctx_create(int device, gs_ctx **out) {
     GS_HIP(hipSetDevice(device));
     GS_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
     GS_HIP(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault));
+    GS_HIP(hipEventCreateWithFlags(&ctx->snap_ev, hipEventDisableTiming));
     noalloc_setup(device);  // the no-allocation CG's scratch address on this device
   });
   if (st != GS_OK) {
@@ -358,6 +359,7 @@ void gs_ctx_destroy(gs_ctx *ctx) {
   ctx->ws = DArr<char>();
   ctx->pool.reset();  // frees the cached blocks; live arrays free their own
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->snap_ev) (void)hipEventDestroy(ctx->snap_ev);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }

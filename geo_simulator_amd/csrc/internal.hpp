@@ -186,6 +186,7 @@ struct gs_ctx {
   hipStream_t stream = nullptr;
   std::string err;
   void *pinned = nullptr;  // small pinned staging buffer (4 KiB)
+  hipEvent_t snap_ev = nullptr;  // marks a solver state snapshot (created with the ctx)
   gs::DArr<char> ws;       // solver scratch, grows on demand
   size_t ws_bytes = 0;
   std::shared_ptr<gs::Pool> pool = std::make_shared<gs::Pool>();

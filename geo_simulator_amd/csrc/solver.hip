@@ -962,6 +962,27 @@ CgState read_state(gs_ctx *ctx, const CgState *st) {
   return h;
 }
 
+// State snapshots: a copy of the solver state into the context's pinned
+// buffer (past the part read_state and read_dots use), enqueued between two
+// passes and followed by an event, so the host reads the state of that
+// point while the passes after it keep the GPU busy.
+constexpr size_t SNAP_OFFSET = 2048;
+constexpr long long SNAP_TAIL = 6;  // passes enqueued after each snapshot
+static_assert(SNAP_OFFSET + sizeof(CgState) <= 4096, "snapshot exceeds the pinned buffer");
+
+void snap_enqueue(gs_ctx *ctx, const CgState *st) {
+  GS_HIP(hipMemcpyAsync((char *)ctx->pinned + SNAP_OFFSET, st, sizeof(CgState),
+                        hipMemcpyDeviceToHost, ctx->stream));
+  GS_HIP(hipEventRecord(ctx->snap_ev, ctx->stream));
+}
+
+CgState snap_wait(gs_ctx *ctx) {
+  GS_HIP(hipEventSynchronize(ctx->snap_ev));
+  CgState h;
+  memcpy(&h, (const char *)ctx->pinned + SNAP_OFFSET, sizeof(CgState));
+  return h;
+}
+
 // Passes for the next batch: enough to reach `target` at the decay rate seen
 // since the previous read, within [4, 64].
 struct Batcher {
@@ -1065,21 +1086,37 @@ PcgResult pcg_core(gs_ctx *ctx, const SellOp &o, const T *val, const T *dinv,
   GS_LAUNCH_CHECK("k_cg_init");
   PcgResult res;
   Batcher bt;
-  for (;;) {
-    const CgState h = read_state(ctx, w.st);
-    res.iterations = h.iteration;
-    res.rnorm = h.rnorm;
-    res.bnorm = h.bnorm;
-    res.converged = h.converged != 0;
-    if (h.stop) break;
-    const long long batch = bt.next(h.rnorm, h.iteration,
-                                    std::max(h.tol_abs, h.abs_tol),
-                                    max_iter - h.iteration);
-    for (long long i = 0; i < batch; ++i)
-      enqueue_pass<T>(s, o, val, x, w.r, w.z, w.p, w.q, dinv, w.st, h.iteration + i,
+  // Batches are pipelined: the state is snapshotted SNAP_TAIL passes before
+  // a batch ends and the next batch is enqueued from that snapshot, so the
+  // GPU does not drain between batches; passes enqueued past the stop
+  // return at once.  When the passes in flight should already reach the
+  // target, the host drains the stream and reads the state instead.
+  CgState h = read_state(ctx, w.st);
+  long long next_it = h.iteration;  // the next pass to enqueue
+  while (!h.stop) {
+    const long long need = bt.next(h.rnorm, h.iteration, std::max(h.tol_abs, h.abs_tol),
+                                   max_iter - h.iteration);
+    const long long ahead = next_it - h.iteration;
+    if (ahead >= need || next_it >= max_iter) {
+      h = read_state(ctx, w.st);
+      continue;
+    }
+    const long long batch = std::min(need - ahead, max_iter - next_it);
+    const long long snap_after = batch > SNAP_TAIL ? batch - SNAP_TAIL : batch;
+    for (long long i = 0; i < batch; ++i) {
+      enqueue_pass<T>(s, o, val, x, w.r, w.z, w.p, w.q, dinv, w.st, next_it + i,
                       w.part1, w.part2);
+      if (i + 1 == snap_after) snap_enqueue(ctx, w.st);
+    }
     GS_LAUNCH_CHECK("cg batch");
+    next_it += batch;
+    h = snap_wait(ctx);
   }
+  // passes after the stop returned at once: the state is final
+  res.iterations = h.iteration;
+  res.rnorm = h.rnorm;
+  res.bnorm = h.bnorm;
+  res.converged = h.converged != 0;
   // one SpMV per pass, the converging (uncounted) pass included
   res.spmv = res.iterations + (res.converged && res.bnorm >= 1e-25 ? 1 : 0);
   return res;
@@ -1212,40 +1249,58 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
   // direction: it is cut short and the recursion restarts.
   long long longest = 0, cycle_start = 0;
   while (res.outer < max_outer) {
-    // FP32 passes until the recursion pauses
+    // FP32 passes until the recursion pauses, in pipelined batches (as in
+    // pcg_core); the passes enqueued never go past the cycle's limits (the
+    // caller's cap, the stall window, max_iterations), and the loop leaves
+    // on a drained, current state
     bool stalled = false;
     double best = hs.rnorm;
     long long best_at = hs.iteration;
+    const long long it_begin = hs.iteration;
+    long long next_it = hs.iteration;
     while (!hs.pause && !hs.stop) {
       const long long window = std::max(200ll, 2 * longest);
-      if (opt.max_inner_iterations > 0 &&
+      const bool fresh = next_it == hs.iteration;
+      if (fresh && opt.max_inner_iterations > 0 &&
           hs.iteration - cycle_start >= opt.max_inner_iterations)
         break;  // the caller's cap on passes per cycle
       if (hs.rnorm < 0.5 * best) {
         best = hs.rnorm;
         best_at = hs.iteration;
-      } else if (hs.iteration - best_at >= window) {
+      } else if (fresh && hs.iteration - best_at >= window) {
         stalled = true;
         break;
       }
       const double tgt = std::max(hs.rtol * hs.rbase, hs.tol);
-      long long left = std::min(max_iter - hs.iteration, best_at + window - hs.iteration);
+      // (pass max_iter itself may be enqueued: the scaled recursion pauses
+      // at the start of that pass, the three-launch one already returns)
+      long long limit = std::min(max_iter + 1, best_at + window);
       if (opt.max_inner_iterations > 0)
-        left = std::min(left, cycle_start + opt.max_inner_iterations - hs.iteration);
-      const long long batch = bt.next(hs.rnorm, hs.iteration, tgt, left);
+        limit = std::min(limit, cycle_start + opt.max_inner_iterations);
+      const long long need = bt.next(hs.rnorm, hs.iteration, tgt, limit - hs.iteration);
+      const long long ahead = next_it - hs.iteration;
+      if (ahead >= need || next_it >= limit) {
+        hs = read_state(ctx, iw.st);
+        next_it = std::max(next_it, hs.iteration);
+        continue;
+      }
+      const long long batch = std::min(need - ahead, limit - next_it);
+      const long long snap_after = batch > SNAP_TAIL ? batch - SNAP_TAIL : batch;
       for (long long i = 0; i < batch; ++i) {
         if (scaled)
-          enqueue_pass2<float, int>(s, o, valq, xs, iw.r, iw.p, iw.q, iw.st,
-                                    hs.iteration + i, iw.part1, iw.part2);
+          enqueue_pass2<float, int>(s, o, valq, xs, iw.r, iw.p, iw.q, iw.st, next_it + i,
+                                    iw.part1, iw.part2);
         else
           enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
-                              hs.iteration + i, iw.part1, iw.part2);
+                              next_it + i, iw.part1, iw.part2);
+        if (i + 1 == snap_after) snap_enqueue(ctx, iw.st);
       }
       GS_LAUNCH_CHECK("cg batch");
-      const long long before = hs.iteration;
-      hs = read_state(ctx, iw.st);
-      res.spmv += hs.iteration - before;
+      next_it += batch;
+      hs = snap_wait(ctx);
     }
+    if (next_it > hs.iteration) hs = read_state(ctx, iw.st);  // passes still in flight
+    res.spmv += hs.iteration - it_begin;
     longest = std::max(longest, hs.iteration - cycle_start);
     res.iterations = hs.iteration;
     // the residual the recursion predicts: its own reduction applied to the

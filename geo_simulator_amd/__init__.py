@@ -18,7 +18,8 @@ from .dof import DofManager
 from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, IdentityPreconditioner,
                      ILUPreconditioner, JacobiPreconditioner, LinearOperator, SolverStats,
                      norm)
-from .assembly import (Assembler, IsotropicElasticity, elastic_element_backward,
+from .assembly import (Assembler, ElementMatrix, GaussQuadrature, IsotropicElasticity,
+                       elastic_element_backward,
                        time_elastic_elements,
                        thermal_element_backward)
 from .rheology import (BackwardEuler, MaxwellViscoelasticity, NewtonianViscosity,
@@ -32,7 +33,8 @@ __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
     "IdentityPreconditioner", "ILUPreconditioner", "LinearOperator",
-    "SolverStats", "norm", "Assembler", "IsotropicElasticity",
+    "SolverStats", "norm", "Assembler", "IsotropicElasticity", "ElementMatrix",
+    "GaussQuadrature",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",
     "Context", "FieldData", "ScalarField", "VectorField", "VtkWriter",
     "BoundaryConditions", "BoundaryFace", "NeumannBC", "Tet10Face",

@@ -124,6 +124,8 @@ SIGNATURES = {
     "gs_csr_destroy": (None, [_vp]),
     "gs_csr_spmv": (C.c_int, [_vp, _vp, _vp, _vp]),
     "gs_jacobi_diag_inv": (C.c_int, [_vp, _vp, _vp]),
+    "gs_quadrature_rule": (C.c_int, [C.c_int32, _dp, _dp]),
+    "gs_thermal_element_matrices": (C.c_int, [_vp, _vp, C.c_double, _vp, C.c_int32, _vp]),
     "gs_ilu_create": (C.c_int, [_vp, _vp, _pp]),
     "gs_ilu_update": (C.c_int, [_vp, _vp, _vp]),
     "gs_ilu_apply": (C.c_int, [_vp, _vp, _vp, _vp]),

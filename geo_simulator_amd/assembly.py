@@ -315,6 +315,66 @@ class Assembler:
         return CsrMatrix(ctx, h), fbc
 
 
+class GaussQuadrature:
+    """GaussQuadrature (src/fem/quadrature.rs:1-110): `points` (barycentric
+    [L0, L1, L2, L3]) and `weights` of the library's literal tables, the ones
+    its kernels integrate with."""
+
+    def __init__(self, n_points: int):
+        pts = np.zeros((max(1, int(n_points)), 4))
+        w = np.zeros(max(1, int(n_points)))
+        st = lib().gs_quadrature_rule(int(n_points), pts.ctypes.data_as(C.POINTER(C.c_double)),
+                                      w.ctypes.data_as(C.POINTER(C.c_double)))
+        if st != 0:
+            raise ValueError("tetrahedral rules have 1, 4, 5 or 11 points")
+        self.points = [list(p) for p in pts]
+        self.weights = list(w)
+
+    @classmethod
+    def tet_1point(cls):
+        return cls(1)
+
+    @classmethod
+    def tet_4point(cls):
+        return cls(4)
+
+    @classmethod
+    def tet_5point(cls):
+        return cls(5)
+
+    @classmethod
+    def tet_11point(cls):
+        return cls(11)
+
+    def num_points(self) -> int:
+        return len(self.weights)
+
+
+class ElementMatrix:
+    """Element matrices (src/fem/element.rs) for every element of a mesh at
+    once, on the device."""
+
+    @staticmethod
+    def thermal_stiffness(mesh, conductivity: float, quadrature=None, k_per_elem=None,
+                          ctx=None):
+        """ElementMatrix::thermal_stiffness (src/fem/element.rs:23-61) of each
+        element, integrated with `quadrature` (a GaussQuadrature or its point
+        count; default the reference's 4-point rule): a device tensor
+        (n_elems, 10, 10)."""
+        ctx = ctx or default_context()
+        n = quadrature.num_points() if isinstance(quadrature, GaussQuadrature) else (
+            4 if quadrature is None else int(quadrature))
+        E = mesh.num_elements()
+        kd = _elem_array(ctx, k_per_elem, E)
+        out = ctx.empty(E * 100)
+        ctx.sync_torch()
+        ctx.check(lib().gs_thermal_element_matrices(
+            ctx.handle, mesh.handle(ctx), float(conductivity),
+            dptr(kd) if kd is not None else None, n, dptr(out)),
+            "ElementMatrix::thermal_stiffness")
+        return out.view(E, 10, 10)
+
+
 def thermal_element_backward(mesh, k_per_elem, u, lam, ctx=None):
     """grad_e = -lambda_e^T K_e(k_e) u_e = dL/dlog k_e (device tensors)."""
     ctx = ctx or default_context()

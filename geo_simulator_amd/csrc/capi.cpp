@@ -985,6 +985,29 @@ gs_status gs_csr_spmv(gs_ctx *ctx, const gs_csr *A, const double *x, double *y) 
   });
 }
 
+gs_status gs_quadrature_rule(int32_t n_points, double *points, double *weights) {
+  if (!points || !weights) return GS_ERR_INVALID_ARG;
+  return quad_rule(n_points, points, weights) ? GS_OK : GS_ERR_INVALID_ARG;
+}
+
+gs_status gs_thermal_element_matrices(gs_ctx *ctx, gs_mesh *mesh, double conductivity,
+                                      const double *k_per_elem, int32_t n_points,
+                                      double *Ke) {
+  if (!ctx || !mesh) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require(n_points == 1 || n_points == 4 || n_points == 5 || n_points == 11,
+            GS_ERR_INVALID_ARG, "quadrature rules have 1, 4, 5 or 11 points");
+    require(Ke || mesh->n_elems == 0, GS_ERR_INVALID_ARG, "null output");
+    set_device(ctx);
+    hipStream_t s = ctx->stream;
+    auto flag = palloc<int>(ctx->pool, 1);
+    GS_HIP(hipMemsetAsync(flag.get(), 0, sizeof(int), s));
+    launch_thermal_rule(s, mesh->n_elems, mesh->xyz.get(), mesh->conn.get(), conductivity,
+                        k_per_elem, n_points, Ke, flag.get());
+    check_flags(read_flag_and_clear(ctx, flag.get()));
+  });
+}
+
 gs_status gs_ilu_create(gs_ctx *ctx, const gs_csr *A, gs_ilu **out) {
   if (!ctx || !A || !out) return GS_ERR_INVALID_ARG;
   *out = nullptr;

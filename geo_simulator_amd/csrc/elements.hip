@@ -110,6 +110,91 @@ constexpr Tables5 make_tables5() {
 }
 __device__ constexpr Tables5 kT5 = make_tables5();
 
+// The four rules of GaussQuadrature (src/fem/quadrature.rs:13-104), their
+// literal constants, in barycentric points and weights.
+struct QuadRule {
+  int n;
+  double L[11][4];
+  double w[11];
+};
+constexpr double Q11A2 = 0.0714285714285714, Q11B2 = 0.7857142857142857;
+constexpr double Q11A3 = 0.3994035761667992, Q11B3 = 0.1005964238332008;
+constexpr double Q11W1 = -0.01315555555555556, Q11W2 = 0.007622222222222222,
+                 Q11W3 = 0.024888888888888888;
+constexpr QuadRule make_rule(int n) {
+  QuadRule r{};
+  r.n = n;
+  if (n == 1) {
+    for (int m = 0; m < 4; ++m) r.L[0][m] = 0.25;
+    r.w[0] = 1.0 / 6.0;
+  } else if (n == 4) {
+    for (int q = 0; q < 4; ++q) {
+      for (int m = 0; m < 4; ++m) r.L[q][m] = qpl(q, m);
+      r.w[q] = QW;
+    }
+  } else if (n == 5) {
+    const double P[5][4] = {{0.25, 0.25, 0.25, 0.25},
+                            {Q5B, Q5B, Q5B, 0.5},
+                            {Q5B, Q5B, 0.5, Q5B},
+                            {Q5B, 0.5, Q5B, Q5B},
+                            {0.5, Q5B, Q5B, Q5B}};
+    const double W[5] = {-2.0 / 15.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0, 3.0 / 40.0};
+    for (int q = 0; q < 5; ++q) {
+      for (int m = 0; m < 4; ++m) r.L[q][m] = P[q][m];
+      r.w[q] = W[q];
+    }
+  } else if (n == 11) {
+    const double P[11][4] = {{0.25, 0.25, 0.25, 0.25},
+                             {Q11A2, Q11A2, Q11A2, Q11B2},
+                             {Q11A2, Q11A2, Q11B2, Q11A2},
+                             {Q11A2, Q11B2, Q11A2, Q11A2},
+                             {Q11B2, Q11A2, Q11A2, Q11A2},
+                             {Q11A3, Q11A3, Q11B3, Q11B3},
+                             {Q11A3, Q11B3, Q11A3, Q11B3},
+                             {Q11A3, Q11B3, Q11B3, Q11A3},
+                             {Q11B3, Q11A3, Q11A3, Q11B3},
+                             {Q11B3, Q11A3, Q11B3, Q11A3},
+                             {Q11B3, Q11B3, Q11A3, Q11A3}};
+    for (int q = 0; q < 11; ++q) {
+      for (int m = 0; m < 4; ++m) r.L[q][m] = P[q][m];
+      r.w[q] = q == 0 ? Q11W1 : q < 5 ? Q11W2 : Q11W3;
+    }
+  }
+  return r;
+}
+constexpr QuadRule kRule1 = make_rule(1), kRule4 = make_rule(4), kRule5 = make_rule(5),
+                   kRule11 = make_rule(11);
+
+// dN/dL differences of a rule's points (as Tables for the 4-point rule)
+template <int NQ> struct RuleTables {
+  double dl[NQ][10][3];
+  double w[NQ];
+};
+template <int NQ> constexpr RuleTables<NQ> make_rule_tables(const QuadRule &r) {
+  RuleTables<NQ> t{};
+  for (int q = 0; q < NQ; ++q) {
+    for (int i = 0; i < 10; ++i) {
+      const double d0 = dndl(i, 0, r.L[q]);
+      t.dl[q][i][0] = dndl(i, 1, r.L[q]) - d0;
+      t.dl[q][i][1] = dndl(i, 2, r.L[q]) - d0;
+      t.dl[q][i][2] = dndl(i, 3, r.L[q]) - d0;
+    }
+    t.w[q] = r.w[q];
+  }
+  return t;
+}
+__device__ constexpr RuleTables<1> kRT1 = make_rule_tables<1>(kRule1);
+__device__ constexpr RuleTables<4> kRT4 = make_rule_tables<4>(kRule4);
+__device__ constexpr RuleTables<5> kRT5 = make_rule_tables<5>(kRule5);
+__device__ constexpr RuleTables<11> kRT11 = make_rule_tables<11>(kRule11);
+// the tables of the NQ-point rule, named in device code (a __device__
+// variable's name in host code is a host-side stand-in, not its address)
+template <int NQ> __device__ __forceinline__ const RuleTables<NQ> &rule_tables();
+template <> __device__ __forceinline__ const RuleTables<1> &rule_tables<1>() { return kRT1; }
+template <> __device__ __forceinline__ const RuleTables<4> &rule_tables<4>() { return kRT4; }
+template <> __device__ __forceinline__ const RuleTables<5> &rule_tables<5>() { return kRT5; }
+template <> __device__ __forceinline__ const RuleTables<11> &rule_tables<11>() { return kRT11; }
+
 enum : int { FLAG_SINGULAR = 1, FLAG_NONFINITE = 2, FLAG_MATERIAL = 4 };
 
 // J (src/fem/basis.rs:196-223), accumulated over the ten nodes in order.
@@ -251,6 +336,64 @@ __global__ __launch_bounds__(128) void k_thermal_elements(
 #pragma unroll
     for (int i = 0; i < 10; ++i) Fe[e * 10 + i] = F[i];
   }
+}
+
+// ElementMatrix::thermal_stiffness (src/fem/element.rs:23-61) with any of
+// the four rules: one thread per element, J, det, J^-1 and the gradients at
+// each point (the expressions of k_thermal_elements), K_ij += k (grad N_i .
+// grad N_j) w over the points in order; all 100 entries, row-major per
+// element (Ke[e*100 + 10 i + j]; entry (j, i) carries the same bits).
+template <int NQ, bool PER_ELEM>
+__global__ __launch_bounds__(128) void k_thermal_rule(
+    int64_t E, const double *__restrict__ xyz, const int *__restrict__ conn, double k,
+    const double *__restrict__ k_elem, double *__restrict__ Ke, int *__restrict__ flag) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= E) return;
+  const RuleTables<NQ> *tb = &rule_tables<NQ>();
+  double X[10][3];
+  load_nodes(xyz, conn, e, X);
+  const double kk = PER_ELEM ? k_elem[e] : k;
+  double K[55];
+#pragma unroll
+  for (int p = 0; p < 55; ++p) K[p] = 0.0;
+  int bad = 0;
+  for (int q = 0; q < NQ; ++q) {
+    double J[3][3], Ji[3][3], G[10][3];
+#pragma unroll
+    for (int r = 0; r < 3; ++r) J[r][0] = J[r][1] = J[r][2] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int r = 0; r < 3; ++r) {
+        J[r][0] += X[i][r] * tb->dl[q][i][0];
+        J[r][1] += X[i][r] * tb->dl[q][i][1];
+        J[r][2] += X[i][r] * tb->dl[q][i][2];
+      }
+    const double det = inverse3(J, Ji);
+    bad |= det_flags(det);
+#pragma unroll
+    for (int i = 0; i < 10; ++i) {
+      const double v0 = tb->dl[q][i][0], v1 = tb->dl[q][i][1], v2 = tb->dl[q][i][2];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) G[i][r] = Ji[0][r] * v0 + Ji[1][r] * v1 + Ji[2][r] * v2;
+    }
+    const double w = tb->w[q] * fabs(det);
+    int p = 0;
+#pragma unroll
+    for (int i = 0; i < 10; ++i)
+#pragma unroll
+      for (int j = i; j < 10; ++j) {
+        const double gd = G[i][0] * G[j][0] + G[i][1] * G[j][1] + G[i][2] * G[j][2];
+        K[p] += kk * gd * w;
+        ++p;
+      }
+  }
+  if (bad) atomicOr(flag, bad);
+  double *out = Ke + e * 100;
+#pragma unroll
+  for (int i = 0; i < 10; ++i)
+#pragma unroll
+    for (int j = 0; j < 10; ++j) out[i * 10 + j] = K[tri_p(i, j)];
 }
 
 // ---------------------------------------------------------------------------
@@ -1155,6 +1298,47 @@ void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
   if (k_elem) with_load_flag(T{});
   else with_load_flag(F{});
   GS_LAUNCH_CHECK("k_thermal_elements");
+}
+
+void launch_thermal_rule(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                         double k, const double *k_elem, int npts, double *Ke, int *flag) {
+  if (E == 0) return;
+  const unsigned g = blocks_for(E, 128);
+  auto go = [&](auto per) {
+    constexpr bool P = decltype(per)::value;
+    switch (npts) {
+    case 1:
+      hipLaunchKernelGGL((k_thermal_rule<1, P>), g, 128, 0, s, E, xyz, conn, k, k_elem,
+                         Ke, flag);
+      break;
+    case 4:
+      hipLaunchKernelGGL((k_thermal_rule<4, P>), g, 128, 0, s, E, xyz, conn, k, k_elem,
+                         Ke, flag);
+      break;
+    case 5:
+      hipLaunchKernelGGL((k_thermal_rule<5, P>), g, 128, 0, s, E, xyz, conn, k, k_elem,
+                         Ke, flag);
+      break;
+    default:
+      hipLaunchKernelGGL((k_thermal_rule<11, P>), g, 128, 0, s, E, xyz, conn, k, k_elem,
+                         Ke, flag);
+      break;
+    }
+  };
+  if (k_elem) go(std::true_type{});
+  else go(std::false_type{});
+  GS_LAUNCH_CHECK("k_thermal_rule");
+}
+
+bool quad_rule(int npts, double *points, double *weights) {
+  const QuadRule *r = npts == 1 ? &kRule1 : npts == 4 ? &kRule4 : npts == 5 ? &kRule5
+                    : npts == 11 ? &kRule11 : nullptr;
+  if (!r) return false;
+  for (int q = 0; q < npts; ++q) {
+    for (int m = 0; m < 4; ++m) points[4 * q + m] = r->L[q][m];
+    weights[q] = r->w[q];
+  }
+  return true;
 }
 
 template <typename T, bool ROWS>

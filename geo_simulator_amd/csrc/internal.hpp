@@ -271,6 +271,12 @@ struct gs_csr {
 namespace gs {
 
 // ---- elements.hip ----
+// ElementMatrix::thermal_stiffness with the rule of npts points (1, 4, 5, 11),
+// all 100 entries per element
+void launch_thermal_rule(hipStream_t s, int64_t E, const double *xyz, const int *conn,
+                         double k, const double *k_elem, int npts, double *Ke, int *flag);
+// GaussQuadrature::tet_{npts}point tables (false for another npts)
+bool quad_rule(int npts, double *points, double *weights);
 void launch_thermal_elements(hipStream_t s, int64_t E, const double *xyz,
                              const int *conn, double k, const double *k_elem,
                              double Q, bool with_load, double *Ke, double *Fe,

@@ -95,6 +95,19 @@ gs_status gs_assemble_thermal_stiffness(gs_ctx *ctx, gs_mesh *mesh,
                                         const gs_dofmap *dof, double conductivity,
                                         const double *k_per_elem_dev,
                                         gs_csr **out);
+/* GaussQuadrature::tet_1point / tet_4point / tet_5point / tet_11point
+ * (src/fem/quadrature.rs:13-104): the literal tables the kernels integrate
+ * with; n_points in {1, 4, 5, 11}, points n_points x 4 barycentric, weights
+ * n_points (host arrays). */
+gs_status gs_quadrature_rule(int32_t n_points, double *points, double *weights);
+/* ElementMatrix::thermal_stiffness (src/fem/element.rs:23-61) of every element
+ * of the mesh (caller element order), integrated with the rule of n_points
+ * points (the reference's element routine and assembly use 4; its tests also
+ * 1, 5 and 11): Ke_dev receives n_elems x 100 doubles, entry (i, j) of
+ * element e at e*100 + 10 i + j.  k_per_elem_dev (nullable) overrides k. */
+gs_status gs_thermal_element_matrices(gs_ctx *ctx, gs_mesh *mesh, double conductivity,
+                                      const double *k_per_elem_dev, int32_t n_points,
+                                      double *Ke_dev);
 /* Assembler::assemble_thermal_load (src/fem/assembly.rs:130-154). */
 gs_status gs_assemble_thermal_load(gs_ctx *ctx, gs_mesh *mesh,
                                    const gs_dofmap *dof, double source,

@@ -538,3 +538,45 @@ def test_thermal_system_subnormal_values(oracle):
     Kf, ff = G.Assembler.assemble_thermal_system(mesh, dof, 1e-312, 1e-315)
     assert same_csr(Kf.to_scipy(), Kbo)
     assert np.array_equal(ff.cpu().numpy(), fbo)
+
+
+@pytest.mark.parametrize("n_points", [1, 4, 5, 11])
+def test_thermal_element_matrices_every_rule(oracle, n_points):
+    """ElementMatrix::thermal_stiffness (src/fem/element.rs:23-61) with each
+    GaussQuadrature rule on curved elements and per-element k: all 100 entries
+    of every element bit-identical to the oracle's element routine."""
+    xyz, conn, mesh = mesh_pair(oracle, 3, 2, 2, 1000.0, 800.0, 600.0, 0.08, 11)
+    k = np.exp(np.random.default_rng(n_points).normal(1.0, 0.3, len(conn)))
+    Ke = G.ElementMatrix.thermal_stiffness(mesh, 0.0, n_points, k_per_elem=k).cpu().numpy()
+    for e in range(len(conn)):
+        assert np.array_equal(Ke[e], oracle.thermal_stiffness(xyz[conn[e]], k[e], n_points)), e
+
+
+def test_thermal_element_rule_invariance_and_4pt_assembly(oracle):
+    """On affine elements the integrand is quadratic: the 4-, 5- and 11-point
+    rules agree to rounding (the 1-point rule does not); the 4-point matrices
+    are the assembly's (their scatter equals the assembled K)."""
+    xyz, conn, mesh = mesh_pair(oracle, 2, 2, 2, 1000.0, 1000.0, 600.0)
+    K = {n: G.ElementMatrix.thermal_stiffness(mesh, 3.0, G.GaussQuadrature(n)).cpu().numpy()
+         for n in (1, 4, 5, 11)}
+    scale = np.abs(K[4]).max()
+    assert np.abs(K[5] - K[4]).max() < 1e-13 * scale
+    assert np.abs(K[11] - K[4]).max() < 1e-13 * scale
+    assert np.abs(K[1] - K[4]).max() > 1e-3 * scale
+    rows = np.repeat(conn, 10, axis=1).ravel()
+    cols = np.tile(conn, (1, 10)).ravel()
+    S = sp.csr_matrix((K[4].ravel(), (rows, cols)), shape=(len(xyz), len(xyz)))
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    A = G.Assembler.assemble_thermal_stiffness_parallel(mesh, dof, 3.0).to_scipy()
+    assert np.abs((S - A).toarray()).max() <= 1e-12 * scale
+
+
+def test_thermal_element_matrices_singular_and_bad_rule(oracle):
+    xyz, conn = star_mesh(2)
+    xyz = xyz.copy()
+    xyz[conn[1, 1:]] = xyz[conn[1, 0]]  # element 1 collapsed to a point: J = 0
+    mesh = G.Mesh(xyz, conn)
+    with pytest.raises(G.GeoFemError):
+        G.ElementMatrix.thermal_stiffness(mesh, 1.0, 4)
+    with pytest.raises(G.GeoFemError):
+        G.ElementMatrix.thermal_stiffness(mesh, 1.0, 3)

@@ -8,14 +8,15 @@ The Python names mirror the reference's Rust API:
     Assembler.assemble_thermal_stiffness_parallel / assemble_thermal_load /
     assemble_elasticity_stiffness_parallel / assemble_gravity_load /
     apply_dirichlet_bcs, IsotropicElasticity, ConjugateGradient.solve,
-    SolverStats, JacobiPreconditioner, ILUPreconditioner.
+    SolverStats, JacobiPreconditioner, ILUPreconditioner, AMGPreconditioner.
 Importing the package needs no GPU; compute calls need one and the built
 library (there is no CPU fallback).
 """
 from ._lib import GeoFemError, LIB_PATH
 from .mesh import ImprovedMeshGenerator, Mesh, Tet10Element
 from .dof import DofManager
-from .linalg import (GMRES, BiCGSTAB, ConjugateGradient, CsrMatrix, IdentityPreconditioner,
+from .linalg import (GMRES, AMGPreconditioner, BiCGSTAB, ConjugateGradient, CsrMatrix,
+                     IdentityPreconditioner,
                      ILUPreconditioner, JacobiPreconditioner, LinearOperator, SolverStats,
                      norm)
 from .assembly import (Assembler, ElementMatrix, GaussQuadrature, IsotropicElasticity,
@@ -32,7 +33,7 @@ from .vtk import VtkWriter
 __all__ = [
     "GeoFemError", "LIB_PATH", "ImprovedMeshGenerator", "Mesh", "Tet10Element",
     "DofManager", "ConjugateGradient", "BiCGSTAB", "GMRES", "CsrMatrix", "JacobiPreconditioner",
-    "IdentityPreconditioner", "ILUPreconditioner", "LinearOperator",
+    "IdentityPreconditioner", "ILUPreconditioner", "AMGPreconditioner", "LinearOperator",
     "SolverStats", "norm", "Assembler", "IsotropicElasticity", "ElementMatrix",
     "GaussQuadrature",
     "thermal_element_backward", "elastic_element_backward", "time_elastic_elements",

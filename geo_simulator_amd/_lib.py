@@ -132,6 +132,12 @@ SIGNATURES = {
     "gs_ilu_info": (C.c_int, [_vp, _i64p, _i64p, _i64p, _i64p]),
     "gs_ilu_export": (C.c_int, [_vp, _vp, _i64p, _i64p, _dp]),
     "gs_ilu_destroy": (None, [_vp]),
+    "gs_amg_create": (C.c_int, [_vp, _vp, _i64, _i64, C.c_double, _pp]),
+    "gs_amg_apply": (C.c_int, [_vp, _vp, _vp, _vp]),
+    "gs_amg_levels": (C.c_int, [_vp, _i64p]),
+    "gs_amg_level_info": (C.c_int, [_vp, _i64, _i64p, _i64p, _i64p, _i64p, _i64p]),
+    "gs_amg_export": (C.c_int, [_vp, _vp, _i64, C.c_int32, _i64p, _i64p, _dp]),
+    "gs_amg_destroy": (None, [_vp]),
     "gs_cg_opts_default": (C.c_int, [C.POINTER(CgOpts)]),
     "gs_cg_solve": (C.c_int, [_vp, _vp, _vp, C.POINTER(CgOpts), _vp,
                               C.POINTER(SolverStatsC)]),

@@ -30,6 +30,7 @@ SOURCES = [
     ("solver.hip", []),
     ("measure.hip", []),
     ("ilu.hip", ["-ffp-contract=off"]),
+    ("amg.hip", ["-ffp-contract=off"]),
     ("capi.cpp", []),
 ]
 

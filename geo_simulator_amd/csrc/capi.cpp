@@ -1072,6 +1072,78 @@ void gs_ilu_destroy(gs_ilu *ilu) {
   delete ilu;
 }
 
+gs_status gs_amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels,
+                        int64_t coarse_size, double strength_threshold, gs_amg **out) {
+  if (!ctx || !A || !out) return GS_ERR_INVALID_ARG;
+  *out = nullptr;
+  return guard(ctx, [&] {
+    set_device(ctx);
+    *out = amg_create(ctx, A, max_levels, coarse_size, strength_threshold);
+  });
+}
+
+gs_status gs_amg_apply(gs_ctx *ctx, const gs_amg *amg, const double *r, double *z) {
+  if (!ctx || !amg || amg->ctx != ctx) return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    require((r && z) || amg->n == 0, GS_ERR_INVALID_ARG, "null vector");
+    require(r != z || amg->n == 0, GS_ERR_INVALID_ARG, "r and z must be distinct buffers");
+    set_device(ctx);
+    amg_apply(amg, r, z);
+  });
+}
+
+gs_status gs_amg_levels(const gs_amg *amg, int64_t *levels) {
+  if (!amg || !levels) return GS_ERR_INVALID_ARG;
+  *levels = (int64_t)amg->lv.size();
+  return GS_OK;
+}
+
+gs_status gs_amg_level_info(const gs_amg *amg, int64_t level, int64_t *rows, int64_t *nnz,
+                            int64_t *coarse, int64_t *nnz_p, int64_t *sweep_levels) {
+  if (!amg || level < 0 || level >= (int64_t)amg->lv.size()) return GS_ERR_INVALID_ARG;
+  const AmgLevel &L = amg->lv[(size_t)level];
+  if (rows) *rows = L.A.rows;
+  if (nnz) *nnz = L.A.nnz;
+  if (coarse) *coarse = L.has_p ? L.coarse : 0;
+  if (nnz_p) *nnz_p = L.has_p ? L.P.nnz : 0;
+  if (sweep_levels) *sweep_levels = L.nlev;
+  return GS_OK;
+}
+
+gs_status gs_amg_export(gs_ctx *ctx, const gs_amg *amg, int64_t level, int32_t which,
+                        int64_t *indptr, int64_t *indices, double *data) {
+  if (!ctx || !amg || amg->ctx != ctx || level < 0 || level >= (int64_t)amg->lv.size() ||
+      which < 0 || which > 2)
+    return GS_ERR_INVALID_ARG;
+  return guard(ctx, [&] {
+    const AmgLevel &L = amg->lv[(size_t)level];
+    set_device(ctx);
+    if (which == 2) {
+      require(data || L.A.rows == 0, GS_ERR_INVALID_ARG, "null output");
+      const std::vector<double> d = to_host(ctx, L.dinv.get(), L.A.rows);
+      std::copy(d.begin(), d.end(), data);
+      return;
+    }
+    const DevCsr &M = which == 0 ? L.A : L.P;
+    require(which == 0 || L.has_p, GS_ERR_INVALID_ARG, "this level has no prolongation");
+    require(indptr && ((indices && data) || M.nnz == 0), GS_ERR_INVALID_ARG, "null output");
+    const std::vector<int> rp = to_host(ctx, M.rp.get(), M.rows + 1);
+    const std::vector<int> col = to_host(ctx, M.col.get(), M.nnz);
+    const std::vector<double> val = to_host(ctx, M.val.get(), M.nnz);
+    for (size_t i = 0; i < rp.size(); ++i) indptr[i] = rp[i];
+    for (size_t k = 0; k < col.size(); ++k) {
+      indices[k] = col[k];
+      data[k] = val[k];
+    }
+  });
+}
+
+void gs_amg_destroy(gs_amg *amg) {
+  if (!amg) return;
+  if (amg->ctx) (void)hipStreamSynchronize(amg->ctx->stream);
+  delete amg;
+}
+
 gs_status gs_jacobi_diag_inv(gs_ctx *ctx, const gs_csr *A, double *dinv) {
   if (!ctx || !A) return GS_ERR_INVALID_ARG;
   return guard(ctx, [&] {

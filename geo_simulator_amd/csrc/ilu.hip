@@ -168,30 +168,20 @@ __global__ void k_ilu_same(int64_t m, const int *__restrict__ a, const int *__re
 __global__ void k_ilu_set_u64(unsigned long long *p, unsigned long long v) { *p = v; }
 __global__ void k_ilu_set_int(int *p, int v) { *p = v; }
 
-template <class T> DArr<T> to_device(gs_ctx *ctx, const std::vector<T> &h) {
-  auto d = dalloc<T>((int64_t)h.size());
-  if (!h.empty())
-    GS_HIP(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice,
-                          ctx->stream));
-  GS_HIP(hipStreamSynchronize(ctx->stream));
-  return d;
+// rows grouped by level: ord (rows, level by level, ascending within one)
+// and the level offsets
+void levels(const std::vector<int> &lv, int nlev, std::vector<int> &ptr, std::vector<int> &ord) {
+  ptr.assign((size_t)nlev + 1, 0);
+  for (int v : lv) ++ptr[(size_t)v + 1];
+  for (int l = 0; l < nlev; ++l) ptr[(size_t)l + 1] += ptr[(size_t)l];
+  ord.resize(lv.size());
+  std::vector<int> at(ptr.begin(), ptr.end() - 1);
+  for (size_t i = 0; i < lv.size(); ++i) ord[(size_t)at[(size_t)lv[i]]++] = (int)i;
 }
 
-template <class T> std::vector<T> to_host(gs_ctx *ctx, const T *d, int64_t n) {
-  std::vector<T> h((size_t)n);
-  if (n > 0)
-    GS_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
-  GS_HIP(hipStreamSynchronize(ctx->stream));
-  return h;
-}
+}  // namespace
 
-// A in caller numbering with sorted columns: row pointers, columns, values
-// and, when A is renumbered, the source nonzero of each entry.
-struct CallerCsr {
-  DArr<int> rp, col, src;
-  DArr<double> val;
-};
-
+// A in caller numbering with sorted columns (the matrix the reference holds).
 CallerCsr caller_csr(gs_ctx *ctx, const gs_csr *A) {
   hipStream_t s = ctx->stream;
   const int64_t n = A->rows, nnz = A->nnz;
@@ -248,18 +238,6 @@ CallerCsr caller_csr(gs_ctx *ctx, const gs_csr *A) {
   return C;
 }
 
-// rows grouped by level: ord (rows, level by level, ascending within one)
-// and the level offsets
-void levels(const std::vector<int> &lv, int nlev, std::vector<int> &ptr, std::vector<int> &ord) {
-  ptr.assign((size_t)nlev + 1, 0);
-  for (int v : lv) ++ptr[(size_t)v + 1];
-  for (int l = 0; l < nlev; ++l) ptr[(size_t)l + 1] += ptr[(size_t)l];
-  ord.resize(lv.size());
-  std::vector<int> at(ptr.begin(), ptr.end() - 1);
-  for (size_t i = 0; i < lv.size(); ++i) ord[(size_t)at[(size_t)lv[i]]++] = (int)i;
-}
-
-}  // namespace
 
 void ilu_factor(gs_ilu *M) {
   gs_ctx *ctx = M->ctx;

@@ -231,6 +231,31 @@ struct gs_ilu {
   std::vector<int> lvl_lower, lvl_upper;  // level offsets into ord_*
 };
 
+// AMG hierarchy (amg.hip): per level the operator, prolongation and its
+// transpose in CSR (caller numbering on level 0), the Jacobi inverse
+// diagonal and the V-cycle's vectors; the last level's sweep order.
+namespace gs {
+struct DevCsr {
+  int64_t rows = 0, nnz = 0;
+  DArr<int> rp, col;
+  DArr<double> val;
+};
+struct AmgLevel {
+  DevCsr A, P, PT;      // P and PT empty where the level has none
+  bool has_p = false;
+  int64_t coarse = 0;   // columns of P
+  DArr<double> dinv, x, t, r;
+  int nlev = 0;         // last level: dependency levels of its sweep
+  DArr<int> lptr, ord;
+};
+}  // namespace gs
+
+struct gs_amg {
+  gs_ctx *ctx = nullptr;
+  int64_t n = 0;
+  std::vector<gs::AmgLevel> lv;
+};
+
 struct gs_csr {
   gs_ctx *ctx = nullptr;
   int64_t rows = 0, cols = 0, nnz = 0;
@@ -389,10 +414,39 @@ void system_finish(gs_ctx *ctx, const gs_mesh *m, const SystemPlan &P, const dou
                    const double *liftv, const int *dperm, double *fbc, double *sdinv,
                    float *sdinv32, const unsigned long long *dmax);
 
+// Host <-> device copies on the context's stream, completed before return.
+template <class T> DArr<T> to_device(gs_ctx *ctx, const std::vector<T> &h) {
+  auto d = dalloc<T>((int64_t)h.size());
+  if (!h.empty())
+    GS_HIP(hipMemcpyAsync(d.get(), h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice,
+                          ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  return d;
+}
+template <class T> std::vector<T> to_host(gs_ctx *ctx, const T *d, int64_t n) {
+  std::vector<T> h((size_t)n);
+  if (n > 0)
+    GS_HIP(hipMemcpyAsync(h.data(), d, n * sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+  GS_HIP(hipStreamSynchronize(ctx->stream));
+  return h;
+}
+
 // ---- ilu.hip ----
+// A in caller numbering with sorted columns: row pointers, columns, values
+// and, when A is renumbered, the source nonzero of each entry.
+struct CallerCsr {
+  DArr<int> rp, col, src;
+  DArr<double> val;
+};
+CallerCsr caller_csr(gs_ctx *ctx, const gs_csr *A);
 gs_ilu *ilu_create(gs_ctx *ctx, const gs_csr *A);
 void ilu_update(gs_ilu *M, const gs_csr *A);
 void ilu_apply(const gs_ilu *M, const double *r, double *z);
+
+// ---- amg.hip ----
+gs_amg *amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels, int64_t coarse_size,
+                   double theta);
+void amg_apply(const gs_amg *M, const double *r, double *z);
 
 // ---- operator.hip ----
 // block = 3 builds the 3x3-block view (rows must be a multiple of 3; a block

@@ -3,7 +3,8 @@
 Mirrors `CsMat<f64>` + `impl LinearOperator for CsMat<f64>`
 (src/linalg/solver.rs:61-92), `SolverStats` (src/linalg/solver.rs:5-20),
 `JacobiPreconditioner` (src/linalg/preconditioner.rs:29-62),
-`ILUPreconditioner` (src/linalg/preconditioner.rs:78-204) and
+`ILUPreconditioner` (src/linalg/preconditioner.rs:78-204),
+`AMGPreconditioner` (src/linalg/amg.rs) and
 `ConjugateGradient` (src/linalg/iterative.rs:7-147).  Vectors may be numpy
 arrays (host, like Vec<f64>: results come back as numpy) or torch tensors on
 the device (results stay on the device).
@@ -211,6 +212,95 @@ class ILUPreconditioner:
         try:
             if self.handle and self.ctx.handle:
                 lib().gs_ilu_destroy(self.handle)
+            self.handle = None
+        except Exception:
+            pass
+
+
+class AMGPreconditioner:
+    """Algebraic multigrid (src/linalg/amg.rs): `AMGPreconditioner(A,
+    max_levels, coarse_size, strength_threshold)` is
+    `AMGPreconditioner::new(&A, ...)` (:44-129; defaults 10, 500, 0.25), the
+    hierarchy built as the reference builds it (in A's caller numbering,
+    bit-identical levels); `apply(r)` is one V-cycle (:132-192) on the
+    device, bit-identical to the reference's."""
+
+    def __init__(self, A: CsrMatrix, max_levels: int = 10, coarse_size: int = 500,
+                 strength_threshold: float = 0.25):
+        self.ctx = A.ctx
+        self.handle = None
+        h = C.c_void_p()
+        self.ctx.sync_torch()
+        self.ctx.check(lib().gs_amg_create(self.ctx.handle, A.handle, int(max_levels),
+                                           int(coarse_size), float(strength_threshold),
+                                           C.byref(h)), "AMGPreconditioner::new")
+        self.handle = h
+        nl = C.c_int64()
+        lib().gs_amg_levels(h, C.byref(nl))
+        self.levels = []
+        for l in range(nl.value):
+            v = [C.c_int64() for _ in range(5)]
+            lib().gs_amg_level_info(h, l, *[C.byref(x) for x in v])
+            self.levels.append(dict(rows=v[0].value, nnz=v[1].value, coarse=v[2].value,
+                                    nnz_p=v[3].value, sweep_levels=v[4].value))
+        self.shape = (A.shape[0], A.shape[0])
+
+    @classmethod
+    def new(cls, A: CsrMatrix, max_levels: int = 10, coarse_size: int = 500,
+            strength_threshold: float = 0.25) -> "AMGPreconditioner":
+        return cls(A, max_levels, coarse_size, strength_threshold)
+
+    def apply(self, r):
+        """z = v_cycle(r, 0); numpy in, numpy out; a device tensor in, a
+        device tensor out."""
+        ctx = self.ctx
+        host = not _is_torch(r)
+        x = ctx.to_device(r)
+        if x.numel() != self.shape[0]:
+            raise ValueError("vector length differs from the matrix")
+        z = ctx.empty(self.shape[0])
+        ctx.sync_torch()
+        ctx.check(lib().gs_amg_apply(ctx.handle, self.handle, dptr(x), dptr(z)),
+                  "AMGPreconditioner::apply")
+        return z.cpu().numpy() if host else z
+
+    def _export(self, level, which):
+        info = self.levels[level]
+        rows = info["rows"]
+        if which == 2:
+            d = np.empty(rows)
+            self.ctx.check(lib().gs_amg_export(self.ctx.handle, self.handle, level, 2, None,
+                                               None, d.ctypes.data_as(C.POINTER(C.c_double))),
+                           "gs_amg_export")
+            return d
+        nnz = info["nnz"] if which == 0 else info["nnz_p"]
+        cols = rows if which == 0 else info["coarse"]
+        indptr = np.empty(rows + 1, dtype=np.int64)
+        indices = np.empty(nnz, dtype=np.int64)
+        data = np.empty(nnz, dtype=np.float64)
+        self.ctx.check(lib().gs_amg_export(
+            self.ctx.handle, self.handle, level, which,
+            indptr.ctypes.data_as(C.POINTER(C.c_int64)),
+            indices.ctypes.data_as(C.POINTER(C.c_int64)),
+            data.ctypes.data_as(C.POINTER(C.c_double))), "gs_amg_export")
+        return sp.csr_matrix((data, indices, indptr), shape=(rows, cols))
+
+    def operator(self, level: int) -> sp.csr_matrix:
+        """AMGLevel::a of `level` (caller numbering on level 0)."""
+        return self._export(level, 0)
+
+    def prolongation(self, level: int):
+        """AMGLevel::p of `level` (None where the level has none)."""
+        return self._export(level, 1) if self.levels[level]["coarse"] > 0 else None
+
+    def diag_inv(self, level: int) -> np.ndarray:
+        """AMGLevel::diag_inv of `level`."""
+        return self._export(level, 2)
+
+    def __del__(self):
+        try:
+            if self.handle and self.ctx.handle:
+                lib().gs_amg_destroy(self.handle)
             self.handle = None
         except Exception:
             pass

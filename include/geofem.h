@@ -51,6 +51,7 @@ typedef struct gs_mesh gs_mesh;
 typedef struct gs_dofmap gs_dofmap;
 typedef struct gs_csr gs_csr;
 typedef struct gs_ilu gs_ilu;
+typedef struct gs_amg gs_amg;
 
 /* ---- context / memory (no reference counterpart: device plumbing) ---- */
 gs_status gs_ctx_create(int device, gs_ctx **out);
@@ -346,6 +347,36 @@ gs_status gs_ilu_info(const gs_ilu *ilu, int64_t *n, int64_t *nnz, int64_t *leve
 gs_status gs_ilu_export(gs_ctx *ctx, const gs_ilu *ilu, int64_t *indptr, int64_t *indices,
                         double *factors);
 void gs_ilu_destroy(gs_ilu *ilu);
+
+/* ---- AMG preconditioner (src/linalg/amg.rs) ----
+ * AMGPreconditioner::new(&A, max_levels, coarse_size, strength_threshold)
+ * (:44-129; the reference's defaults 10, 500, 0.25): the hierarchy of a
+ * square A in A's caller numbering with columns ascending, built as the
+ * reference builds it (strength of connection, greedy coarsening, direct
+ * interpolation, Galerkin operators P^T (A P) with |entries| <= 1e-14
+ * dropped); bit-identical levels.  max_levels >= 1 and coarse_size >= 0
+ * (GS_ERR_INVALID_ARG otherwise: the reference's V-cycle cannot run
+ * without a level). */
+gs_status gs_amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels,
+                        int64_t coarse_size, double strength_threshold, gs_amg **out);
+/* Preconditioner::apply for AMGPreconditioner (:188-192): z = one V-cycle
+ * (:132-185) on r, bit-identical to the reference's; device vectors of n
+ * doubles in A's caller numbering (r_dev and z_dev distinct). */
+gs_status gs_amg_apply(gs_ctx *ctx, const gs_amg *amg, const double *r_dev, double *z_dev);
+/* Number of levels (no reference counterpart). */
+gs_status gs_amg_levels(const gs_amg *amg, int64_t *levels);
+/* Level `level`'s sizes: operator rows and stored entries, the columns and
+ * stored entries of its prolongation (0 and 0 where it has none), and the
+ * dependency levels of its in-place sweeps (the last level; 0 elsewhere). */
+gs_status gs_amg_level_info(const gs_amg *amg, int64_t level, int64_t *rows, int64_t *nnz,
+                            int64_t *coarse, int64_t *nnz_p, int64_t *sweep_levels);
+/* Level `level`'s operator (which 0) or prolongation (which 1) in sprs shape
+ * (host arrays: indptr rows+1, indices and data of its stored entries), the
+ * reference's AMGLevel fields a and p; which 2: its Jacobi inverse diagonal
+ * into data (diag_inv). */
+gs_status gs_amg_export(gs_ctx *ctx, const gs_amg *amg, int64_t level, int32_t which,
+                        int64_t *indptr, int64_t *indices, double *data);
+void gs_amg_destroy(gs_amg *amg);
 
 /* Solver::solve_with_operator for ConjugateGradient (src/linalg/solver.rs:
  * 112-120, src/linalg/iterative.rs:133-140 and the loop at :46-119): CG on a

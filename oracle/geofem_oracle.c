@@ -1559,3 +1559,460 @@ int or_elastic_element_backward(int64_t n_nodes, const double *xyz,
   }
   return err;
 }
+
+/* ------------------------------------------------------------------------ */
+/* AMGPreconditioner (src/linalg/amg.rs).                                    */
+/* ------------------------------------------------------------------------ */
+
+static int csr_alloc(or_csr *m, int64_t rows, int64_t cols, int64_t cap) {
+  memset(m, 0, sizeof(*m));
+  m->rows = rows;
+  m->cols = cols;
+  m->indptr = (int64_t *)calloc((size_t)rows + 1, sizeof(int64_t));
+  m->indices = (int64_t *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(int64_t));
+  m->data = (double *)malloc((size_t)(cap > 0 ? cap : 1) * sizeof(double));
+  if (!m->indptr || !m->indices || !m->data) {
+    or_csr_free(m);
+    return OR_ERR_OOM;
+  }
+  return OR_OK;
+}
+
+/* strength_matrix (src/linalg/amg.rs:359-390).  The reference scans every
+ * column j of row i through a.get(i, j), which yields exactly the stored
+ * entries of the row; the loops below walk those entries in ascending column
+ * order.  f64::max ignores a NaN operand, as fmax does. */
+int or_amg_strength(const or_csr *A, double theta, or_csr *S) {
+  const int64_t n = A->rows;
+  if (csr_alloc(S, n, A->cols, A->nnz) != OR_OK) return OR_ERR_OOM;
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    double max_off = 0.0;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+      if (A->indices[p] != i) max_off = fmax(max_off, fabs(A->data[p]));
+    const double threshold = theta * max_off;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+      if (A->indices[p] != i && fabs(A->data[p]) >= threshold) {
+        S->indices[nnz] = A->indices[p];
+        S->data[nnz] = A->data[p];
+        ++nnz;
+      }
+    S->indptr[i + 1] = nnz;
+  }
+  S->nnz = nnz;
+  return OR_OK;
+}
+
+static int cmp_i64(const void *a, const void *b) {
+  const int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+  return (x > y) - (x < y);
+}
+
+/* max-heap of (lambda, node): larger lambda first, then the smaller node */
+typedef struct { int64_t lam, node; } lamnode;
+static int lam_before(lamnode a, lamnode b) {
+  return a.lam > b.lam || (a.lam == b.lam && a.node < b.node);
+}
+static void heap_push(lamnode *h, int64_t *size, lamnode v) {
+  int64_t i = (*size)++;
+  h[i] = v;
+  while (i > 0) {
+    const int64_t up = (i - 1) / 2;
+    if (!lam_before(h[i], h[up])) break;
+    const lamnode t = h[i]; h[i] = h[up]; h[up] = t;
+    i = up;
+  }
+}
+static lamnode heap_pop(lamnode *h, int64_t *size) {
+  const lamnode top = h[0];
+  h[0] = h[--(*size)];
+  int64_t i = 0;
+  for (;;) {
+    const int64_t l = 2 * i + 1, r = l + 1;
+    int64_t m = i;
+    if (l < *size && lam_before(h[l], h[m])) m = l;
+    if (r < *size && lam_before(h[r], h[m])) m = r;
+    if (m == i) break;
+    const lamnode t = h[i]; h[i] = h[m]; h[m] = t;
+    i = m;
+  }
+  return top;
+}
+
+/* coarsen (src/linalg/amg.rs:286-355).  The reference rescans all nodes for
+ * the undecided node of largest lambda (the first such node in index order:
+ * its comparison is strict) at every step; here a max-heap ordered by
+ * (lambda descending, node ascending) holds every undecided node whose
+ * lambda is positive, with stale entries (a node decided since, or whose
+ * lambda has grown since) dropped when they surface.  lambda only grows, so
+ * the valid top is exactly the node the reference's scan picks.  When no
+ * undecided node has a positive lambda the rest become coarse (:312-321). */
+int or_amg_coarsen(const or_csr *A, double theta, uint8_t *cf, int64_t *num_coarse) {
+  or_csr S;
+  int err = or_amg_strength(A, theta, &S);
+  if (err != OR_OK) return err;
+  const int64_t n = A->rows;
+  int64_t *lambda = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+  uint8_t *undecided = (uint8_t *)malloc((size_t)(n > 0 ? n : 1));
+  int64_t cap = n + S.nnz * 8 + 16, size = 0;
+  lamnode *heap = (lamnode *)malloc((size_t)cap * sizeof(lamnode));
+  if (!lambda || !undecided || !heap) {
+    free(lambda); free(undecided); free(heap); or_csr_free(&S);
+    return OR_ERR_OOM;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t c = 0;
+    for (int64_t p = S.indptr[i]; p < S.indptr[i + 1]; ++p) c += S.indices[p] != i;
+    lambda[i] = c;
+    undecided[i] = 1;
+    cf[i] = 0;
+    if (c > 0) heap_push(heap, &size, (lamnode){c, i});
+  }
+  int64_t nc = 0, left = n;
+  while (left > 0) {
+    int64_t max_node = -1;
+    while (size > 0) {
+      const lamnode t = heap_pop(heap, &size);
+      if (undecided[t.node] && lambda[t.node] == t.lam) {
+        max_node = t.node;
+        break;
+      }
+    }
+    if (max_node < 0) { /* max_lambda == 0: the rest are coarse */
+      for (int64_t i = 0; i < n; ++i)
+        if (undecided[i]) {
+          cf[i] = 1;
+          undecided[i] = 0;
+          ++nc;
+        }
+      break;
+    }
+    cf[max_node] = 1;
+    undecided[max_node] = 0;
+    ++nc;
+    --left;
+    for (int64_t p = S.indptr[max_node]; p < S.indptr[max_node + 1]; ++p) {
+      const int64_t j = S.indices[p];
+      if (j != max_node && undecided[j]) {
+        undecided[j] = 0; /* fine */
+        --left;
+        for (int64_t q = S.indptr[j]; q < S.indptr[j + 1]; ++q) {
+          const int64_t k = S.indices[q];
+          if (k != j && undecided[k]) {
+            lambda[k] += 1;
+            if (size == cap) {
+              cap *= 2;
+              lamnode *h2 = (lamnode *)realloc(heap, (size_t)cap * sizeof(lamnode));
+              if (!h2) {
+                free(lambda); free(undecided); free(heap); or_csr_free(&S);
+                return OR_ERR_OOM;
+              }
+              heap = h2;
+            }
+            heap_push(heap, &size, (lamnode){lambda[k], k});
+          }
+        }
+      }
+    }
+  }
+  *num_coarse = nc;
+  free(lambda);
+  free(undecided);
+  free(heap);
+  or_csr_free(&S);
+  return OR_OK;
+}
+
+/* build_prolongation (src/linalg/amg.rs:396-487): a coarse point keeps its
+ * value (weight 1); a fine point takes its strong coarse neighbours (S row
+ * order) with weights |s_ij| / sum, or when that sum is at most 1e-14 all
+ * coarse entries of its A row the same way, or when that too is at most
+ * 1e-14 the first coarse point with weight 1 (if there is one).  Triplets
+ * arrive row by row in ascending column, so to_csr keeps them as they are
+ * (explicit zeros included). */
+int or_amg_prolongation(const or_csr *A, const uint8_t *cf, int64_t num_coarse,
+                        double theta, or_csr *P) {
+  or_csr S;
+  int err = or_amg_strength(A, theta, &S);
+  if (err != OR_OK) return err;
+  const int64_t n = A->rows;
+  int64_t *cidx = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+  if (!cidx || csr_alloc(P, n, num_coarse, A->nnz + n) != OR_OK) {
+    free(cidx); or_csr_free(&S);
+    return OR_ERR_OOM;
+  }
+  int64_t c = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    cidx[i] = 0;
+    if (cf[i]) cidx[i] = c++;
+  }
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    if (cf[i]) {
+      P->indices[nnz] = cidx[i];
+      P->data[nnz++] = 1.0;
+    } else {
+      double sum = 0.0;
+      for (int64_t p = S.indptr[i]; p < S.indptr[i + 1]; ++p)
+        if (cf[S.indices[p]]) sum += fabs(S.data[p]);
+      if (sum > 1e-14) {
+        for (int64_t p = S.indptr[i]; p < S.indptr[i + 1]; ++p)
+          if (cf[S.indices[p]]) {
+            P->indices[nnz] = cidx[S.indices[p]];
+            P->data[nnz++] = fabs(S.data[p]) / sum;
+          }
+      } else {
+        double sum_all = 0.0;
+        for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+          if (cf[A->indices[p]]) sum_all += fabs(A->data[p]);
+        if (sum_all > 1e-14) {
+          for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+            if (cf[A->indices[p]]) {
+              P->indices[nnz] = cidx[A->indices[p]];
+              P->data[nnz++] = fabs(A->data[p]) / sum_all;
+            }
+        } else if (num_coarse > 0) {
+          P->indices[nnz] = 0;
+          P->data[nnz++] = 1.0;
+        }
+      }
+    }
+    P->indptr[i + 1] = nnz;
+  }
+  P->nnz = nnz;
+  free(cidx);
+  or_csr_free(&S);
+  return OR_OK;
+}
+
+/* matrix_multiply (src/linalg/amg.rs:195-226).  The reference forms every
+ * C[i][j] as 0.0 plus a_ik b_kj over the stored entries k of A's row i in
+ * ascending order (b.get(k, j) present), keeping it when |sum| > 1e-14.  A
+ * dense accumulator per row receives the same products in the same order
+ * (for a fixed j they arrive in ascending k); columns never reached hold
+ * the reference's 0.0 and are not kept. */
+int or_csr_matmul(const or_csr *A, const or_csr *B, or_csr *C) {
+  if (A->cols != B->rows) return OR_ERR_INVALID_ARG;
+  const int64_t m = A->rows, nb = B->cols;
+  double *acc = (double *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(double));
+  int64_t *mark = (int64_t *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int64_t));
+  int64_t *touched = (int64_t *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int64_t));
+  int64_t cap = A->nnz + B->nnz + 16;
+  if (!acc || !mark || !touched || csr_alloc(C, m, nb, cap) != OR_OK) {
+    free(acc); free(mark); free(touched);
+    return OR_ERR_OOM;
+  }
+  for (int64_t j = 0; j < nb; ++j) mark[j] = -1;
+  int64_t nnz = 0;
+  for (int64_t i = 0; i < m; ++i) {
+    int64_t nt = 0;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p) {
+      const int64_t k = A->indices[p];
+      const double a = A->data[p];
+      for (int64_t q = B->indptr[k]; q < B->indptr[k + 1]; ++q) {
+        const int64_t j = B->indices[q];
+        if (mark[j] != i) {
+          mark[j] = i;
+          acc[j] = 0.0;
+          touched[nt++] = j;
+        }
+        acc[j] += a * B->data[q];
+      }
+    }
+    qsort(touched, (size_t)nt, sizeof(int64_t), cmp_i64); /* ascending columns */
+    for (int64_t x = 0; x < nt; ++x) {
+      const int64_t j = touched[x];
+      if (fabs(acc[j]) > 1e-14) {
+        if (nnz == cap) {
+          cap *= 2;
+          int64_t *i2 = (int64_t *)realloc(C->indices, (size_t)cap * sizeof(int64_t));
+          if (i2) C->indices = i2;
+          double *d2 = (double *)realloc(C->data, (size_t)cap * sizeof(double));
+          if (d2) C->data = d2;
+          if (!i2 || !d2) {
+            free(acc); free(mark); free(touched); or_csr_free(C);
+            return OR_ERR_OOM;
+          }
+        }
+        C->indices[nnz] = j;
+        C->data[nnz++] = acc[j];
+      }
+    }
+    C->indptr[i + 1] = nnz;
+  }
+  C->nnz = nnz;
+  free(acc);
+  free(mark);
+  free(touched);
+  return OR_OK;
+}
+
+int or_csr_transpose(const or_csr *A, or_csr *T) {
+  if (csr_alloc(T, A->cols, A->rows, A->nnz) != OR_OK) return OR_ERR_OOM;
+  for (int64_t p = 0; p < A->nnz; ++p) T->indptr[A->indices[p] + 1]++;
+  for (int64_t j = 0; j < A->cols; ++j) T->indptr[j + 1] += T->indptr[j];
+  int64_t *cur = (int64_t *)malloc((size_t)(A->cols > 0 ? A->cols : 1) * sizeof(int64_t));
+  if (!cur) { or_csr_free(T); return OR_ERR_OOM; }
+  memcpy(cur, T->indptr, (size_t)A->cols * sizeof(int64_t));
+  for (int64_t i = 0; i < A->rows; ++i)
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p) {
+      const int64_t q = cur[A->indices[p]]++;
+      T->indices[q] = i;
+      T->data[q] = A->data[p];
+    }
+  T->nnz = A->nnz;
+  free(cur);
+  return OR_OK;
+}
+
+/* extract_diag_inv (src/linalg/amg.rs:266-280) */
+void or_amg_diag_inv(const or_csr *A, double *dinv) {
+  for (int64_t i = 0; i < A->rows; ++i) {
+    dinv[i] = 1.0;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+      if (A->indices[p] == i) {
+        if (fabs(A->data[p]) > 1e-14) dinv[i] = 1.0 / A->data[p];
+        break;
+      }
+  }
+}
+
+static int csr_copy(const or_csr *A, or_csr *B) {
+  if (csr_alloc(B, A->rows, A->cols, A->nnz) != OR_OK) return OR_ERR_OOM;
+  memcpy(B->indptr, A->indptr, (size_t)(A->rows + 1) * sizeof(int64_t));
+  memcpy(B->indices, A->indices, (size_t)A->nnz * sizeof(int64_t));
+  memcpy(B->data, A->data, (size_t)A->nnz * sizeof(double));
+  B->nnz = A->nnz;
+  return OR_OK;
+}
+
+void or_amg_free(or_amg *M) {
+  for (int64_t l = 0; l < M->nlevels; ++l) {
+    or_csr_free(&M->A[l]);
+    or_csr_free(&M->P[l]);
+    or_csr_free(&M->PT[l]);
+    free(M->dinv[l]);
+  }
+  free(M->A);
+  free(M->P);
+  free(M->PT);
+  free(M->dinv);
+  memset(M, 0, sizeof(*M));
+}
+
+/* AMGPreconditioner::new (src/linalg/amg.rs:44-129).  A level is kept with
+ * its prolongation while the operator has more than coarse_size rows and
+ * coarsening leaves between 1 and n - 1 coarse points; the Galerkin operator
+ * P^T (A P) becomes the next level's.  When max_levels levels were all
+ * coarsened, the last Galerkin operator is not kept (the loop ends), and
+ * the last level keeps its P unused. */
+int or_amg_create(const or_csr *A, int64_t max_levels, int64_t coarse_size, double theta,
+                  or_amg *out) {
+  memset(out, 0, sizeof(*out));
+  if (max_levels < 1 || A->rows != A->cols) return OR_ERR_INVALID_ARG;
+  out->A = (or_csr *)calloc((size_t)max_levels, sizeof(or_csr));
+  out->P = (or_csr *)calloc((size_t)max_levels, sizeof(or_csr));
+  out->PT = (or_csr *)calloc((size_t)max_levels, sizeof(or_csr));
+  out->dinv = (double **)calloc((size_t)max_levels, sizeof(double *));
+  if (!out->A || !out->P || !out->PT || !out->dinv) { or_amg_free(out); return OR_ERR_OOM; }
+  or_csr cur;
+  int err = csr_copy(A, &cur);
+  if (err != OR_OK) { or_amg_free(out); return err; }
+  for (int64_t level = 0; level < max_levels; ++level) {
+    const int64_t n = cur.rows;
+    const int64_t l = out->nlevels;
+    out->dinv[l] = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    if (!out->dinv[l]) { or_csr_free(&cur); or_amg_free(out); return OR_ERR_OOM; }
+    or_amg_diag_inv(&cur, out->dinv[l]);
+    out->A[l] = cur;
+    out->nlevels = l + 1;
+    if (n <= coarse_size) break;
+    uint8_t *cf = (uint8_t *)malloc((size_t)(n > 0 ? n : 1));
+    int64_t nc = 0;
+    if (!cf) { or_amg_free(out); return OR_ERR_OOM; }
+    err = or_amg_coarsen(&cur, theta, cf, &nc);
+    if (err != OR_OK) { free(cf); or_amg_free(out); return err; }
+    if (nc == 0 || nc >= n) { free(cf); break; }
+    err = or_amg_prolongation(&cur, cf, nc, theta, &out->P[l]);
+    free(cf);
+    if (err == OR_OK) err = or_csr_transpose(&out->P[l], &out->PT[l]);
+    if (err != OR_OK) { or_amg_free(out); return err; }
+    if (level + 1 == max_levels) break; /* the loop's last Galerkin product is dropped */
+    or_csr ap, ac;
+    err = or_csr_matmul(&out->A[l], &out->P[l], &ap);
+    if (err != OR_OK) { or_amg_free(out); return err; }
+    err = or_csr_matmul(&out->PT[l], &ap, &ac);
+    or_csr_free(&ap);
+    if (err != OR_OK) { or_amg_free(out); return err; }
+    cur = ac;
+  }
+  return OR_OK;
+}
+
+/* y = A x, each row summed from 0.0 in stored order (matvec, :230-247) */
+static void amg_matvec(const or_csr *A, const double *x, double *y) {
+  for (int64_t i = 0; i < A->rows; ++i) {
+    double sum = 0.0;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p) sum += A->data[p] * x[A->indices[p]];
+    y[i] = sum;
+  }
+}
+
+/* weighted_jacobi_step (:250-263) with omega 0.67 */
+static void amg_jacobi(const or_csr *A, const double *r, double *x, double *xn,
+                       const double *dinv) {
+  const double omega = 0.67;
+  for (int64_t i = 0; i < A->rows; ++i) {
+    double ax = 0.0;
+    for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p) ax += A->data[p] * x[A->indices[p]];
+    xn[i] = x[i] + omega * (r[i] - ax) * dinv[i];
+  }
+  memcpy(x, xn, (size_t)A->rows * sizeof(double));
+}
+
+/* v_cycle (:132-185): x = 0; on the last level ten in-place sweeps
+ * x_i += (r_i - (A x)_i) / a_ii (a Gauss-Seidel sweep: rows in order, each
+ * reading the entries already updated); otherwise two weighted-Jacobi steps,
+ * the residual restricted by P^T, the coarse correction prolonged by P and
+ * added, two more steps. */
+static int amg_vcycle(const or_amg *M, int64_t level, const double *r, double *x) {
+  const or_csr *A = &M->A[level];
+  const int64_t n = A->rows;
+  const double *dinv = M->dinv[level];
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  if (level == M->nlevels - 1) {
+    for (int s = 0; s < 10; ++s)
+      for (int64_t i = 0; i < n; ++i) {
+        double ax = 0.0;
+        for (int64_t p = A->indptr[i]; p < A->indptr[i + 1]; ++p)
+          ax += A->data[p] * x[A->indices[p]];
+        x[i] += (r[i] - ax) * dinv[i];
+      }
+    return OR_OK;
+  }
+  const or_csr *P = &M->P[level], *PT = &M->PT[level];
+  const int64_t nc = P->cols;
+  double *t = (double *)malloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  double *rc = (double *)malloc((size_t)(nc > 0 ? nc : 1) * sizeof(double));
+  double *ec = (double *)malloc((size_t)(nc > 0 ? nc : 1) * sizeof(double));
+  if (!t || !rc || !ec) { free(t); free(rc); free(ec); return OR_ERR_OOM; }
+  for (int k = 0; k < 2; ++k) amg_jacobi(A, r, x, t, dinv);
+  amg_matvec(A, x, t);
+  for (int64_t i = 0; i < n; ++i) t[i] = r[i] - t[i];
+  amg_matvec(PT, t, rc);
+  int err = amg_vcycle(M, level + 1, rc, ec);
+  if (err == OR_OK) {
+    amg_matvec(P, ec, t);
+    for (int64_t i = 0; i < n; ++i) x[i] += t[i];
+    for (int k = 0; k < 2; ++k) amg_jacobi(A, r, x, t, dinv);
+  }
+  free(t);
+  free(rc);
+  free(ec);
+  return err;
+}
+
+int or_amg_apply(const or_amg *M, const double *r, double *z) {
+  if (M->nlevels < 1) return OR_ERR_INVALID_ARG;
+  return amg_vcycle(M, 0, r, z);
+}

@@ -176,6 +176,41 @@ int or_ilu0_factor(const or_csr *A, double *factors, int64_t *diag, int64_t *bad
 void or_ilu0_apply(const or_csr *A, const double *factors, const int64_t *diag,
                    const double *r, double *z);
 
+/* ---- AMGPreconditioner (src/linalg/amg.rs) ----
+ * Classical Ruge-Stueben-style AMG as the reference builds it.  Matrices are
+ * square with sorted column indices (sprs' CSR).  Efficient loops stand in
+ * for the reference's O(n^2) ones where the result is the same by
+ * construction (noted at each function). */
+/* strength_matrix (:359-390): S[i,j] = A[i,j] for stored j != i with
+ * |A[i,j]| >= theta * max_{stored k != i} |A[i,k]|. */
+int or_amg_strength(const or_csr *A, double theta, or_csr *S);
+/* coarsen (:286-355): cf[i] = 1 for coarse points; returns their count. */
+int or_amg_coarsen(const or_csr *A, double theta, uint8_t *cf, int64_t *num_coarse);
+/* build_prolongation (:396-487): P (rows x num_coarse). */
+int or_amg_prolongation(const or_csr *A, const uint8_t *cf, int64_t num_coarse,
+                        double theta, or_csr *P);
+/* matrix_multiply (:195-226): C = A B, entries with |sum| > 1e-14 kept. */
+int or_csr_matmul(const or_csr *A, const or_csr *B, or_csr *C);
+/* B = A^T (sprs transpose_view().to_owned().to_csr()). */
+int or_csr_transpose(const or_csr *A, or_csr *T);
+/* extract_diag_inv (:266-280). */
+void or_amg_diag_inv(const or_csr *A, double *dinv);
+
+typedef struct {
+  int64_t nlevels;
+  or_csr *A;      /* [nlevels] operators */
+  or_csr *P;      /* [nlevels] prolongations; rows == 0 where a level has none */
+  or_csr *PT;     /* [nlevels] their transposes */
+  double **dinv;  /* [nlevels] extract_diag_inv of each operator */
+} or_amg;
+/* AMGPreconditioner::new (:44-129): max_levels >= 1. */
+int or_amg_create(const or_csr *A, int64_t max_levels, int64_t coarse_size, double theta,
+                  or_amg *out);
+/* Preconditioner::apply for AMGPreconditioner (:188-192): z = v_cycle(r, 0)
+ * (:132-185). */
+int or_amg_apply(const or_amg *M, const double *r, double *z);
+void or_amg_free(or_amg *M);
+
 /* ---- adjoint element backward (new surface; pinned by central FD) ---- */
 int or_thermal_element_backward(int64_t n_nodes, const double *xyz,
                                 int64_t n_elems, const int64_t *conn,

@@ -41,6 +41,11 @@ class _Csr(C.Structure):
                 ("data", C.POINTER(C.c_double))]
 
 
+class _Amg(C.Structure):
+    _fields_ = [("nlevels", C.c_int64), ("A", C.POINTER(_Csr)), ("P", C.POINTER(_Csr)),
+                ("PT", C.POINTER(_Csr)), ("dinv", C.POINTER(C.POINTER(C.c_double)))]
+
+
 class _Stats(C.Structure):
     _fields_ = [("iterations", C.c_int64), ("residual_norm", C.c_double),
                 ("relative_residual", C.c_double), ("converged", C.c_int32),
@@ -147,6 +152,17 @@ def _declare(L):
                                  C.c_double, C.c_int, _dp, C.POINTER(_Stats)]
     L.or_ilu0_factor.argtypes = [C.POINTER(_Csr), _dp, _i64p, _i64p, C.POINTER(C.c_int32)]
     L.or_ilu0_apply.argtypes = [C.POINTER(_Csr), _dp, _i64p, _dp, _dp]
+    L.or_amg_strength.argtypes = [C.POINTER(_Csr), C.c_double, C.POINTER(_Csr)]
+    L.or_amg_coarsen.argtypes = [C.POINTER(_Csr), C.c_double, _u8p, C.POINTER(C.c_int64)]
+    L.or_amg_prolongation.argtypes = [C.POINTER(_Csr), _u8p, C.c_int64, C.c_double,
+                                      C.POINTER(_Csr)]
+    L.or_csr_matmul.argtypes = [C.POINTER(_Csr), C.POINTER(_Csr), C.POINTER(_Csr)]
+    L.or_csr_transpose.argtypes = [C.POINTER(_Csr), C.POINTER(_Csr)]
+    L.or_amg_diag_inv.argtypes = [C.POINTER(_Csr), _dp]
+    L.or_amg_create.argtypes = [C.POINTER(_Csr), C.c_int64, C.c_int64, C.c_double,
+                                C.POINTER(_Amg)]
+    L.or_amg_apply.argtypes = [C.POINTER(_Amg), _dp, _dp]
+    L.or_amg_free.argtypes = [C.POINTER(_Amg)]
     L.or_thermal_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
                                               _dp, _dp, _dp, _dp]
     L.or_elastic_element_backward.argtypes = [C.c_int64, _dp, C.c_int64, _i64p,
@@ -573,6 +589,102 @@ def ilu0_apply(F, diag, r):
     dg = np.ascontiguousarray(diag, dtype=np.int64)
     lib().or_ilu0_apply(C.byref(m), _d(keep[2]), _i(dg), _d(r), _d(z))
     return z
+
+
+# ---- AMG (src/linalg/amg.rs) -------------------------------------------------
+
+def _copy_csr(m: _Csr) -> sp.csr_matrix:
+    """A copy of an oracle-owned matrix (not freed here)."""
+    rows, cols, nnz = m.rows, m.cols, m.nnz
+    indptr = np.ctypeslib.as_array(m.indptr, shape=(rows + 1,)).copy()
+    if nnz > 0:
+        indices = np.ctypeslib.as_array(m.indices, shape=(nnz,)).copy()
+        data = np.ctypeslib.as_array(m.data, shape=(nnz,)).copy()
+    else:
+        indices = np.zeros(0, dtype=np.int64)
+        data = np.zeros(0)
+    return sp.csr_matrix((data, indices, indptr), shape=(rows, cols))
+
+
+def _sorted(A):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return A
+
+
+def amg_strength(A, theta):
+    """strength_matrix (src/linalg/amg.rs:359-390)."""
+    m, keep = _give_csr(_sorted(A))
+    out = _Csr()
+    _check(lib().or_amg_strength(C.byref(m), theta, C.byref(out)), "amg_strength")
+    return _take_csr(out)
+
+
+def amg_coarsen(A, theta):
+    """coarsen (src/linalg/amg.rs:286-355): (coarse mask, number of coarse points)."""
+    A = _sorted(A)
+    m, keep = _give_csr(A)
+    cf = np.zeros(A.shape[0], dtype=np.uint8)
+    nc = C.c_int64(0)
+    _check(lib().or_amg_coarsen(C.byref(m), theta, cf.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                C.byref(nc)), "amg_coarsen")
+    return cf.astype(bool), nc.value
+
+
+def amg_prolongation(A, cf, num_coarse, theta):
+    """build_prolongation (src/linalg/amg.rs:396-487)."""
+    m, keep = _give_csr(_sorted(A))
+    cf8 = np.ascontiguousarray(cf, dtype=np.uint8)
+    out = _Csr()
+    _check(lib().or_amg_prolongation(C.byref(m), cf8.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     num_coarse, theta, C.byref(out)), "amg_prolongation")
+    return _take_csr(out)
+
+
+def csr_matmul(A, B):
+    """matrix_multiply (src/linalg/amg.rs:195-226)."""
+    ma, ka = _give_csr(_sorted(A))
+    mb, kb = _give_csr(_sorted(B))
+    out = _Csr()
+    _check(lib().or_csr_matmul(C.byref(ma), C.byref(mb), C.byref(out)), "csr_matmul")
+    return _take_csr(out)
+
+
+def amg_diag_inv(A):
+    """extract_diag_inv (src/linalg/amg.rs:266-280)."""
+    A = _sorted(A)
+    m, keep = _give_csr(A)
+    d = np.empty(A.shape[0])
+    lib().or_amg_diag_inv(C.byref(m), _d(d))
+    return d
+
+
+class Amg:
+    """AMGPreconditioner::new (src/linalg/amg.rs:44-129) and apply (:188-192)."""
+
+    def __init__(self, A, max_levels=10, coarse_size=500, theta=0.25):
+        self._h = None
+        m, keep = _give_csr(_sorted(A))
+        h = _Amg()
+        _check(lib().or_amg_create(C.byref(m), max_levels, coarse_size, theta, C.byref(h)),
+               "amg_create")
+        self._h = h
+        self.nlevels = h.nlevels
+        self.A = [_copy_csr(h.A[l]) for l in range(h.nlevels)]
+        self.P = [_copy_csr(h.P[l]) if h.P[l].rows > 0 else None for l in range(h.nlevels)]
+        self.dinv = [np.ctypeslib.as_array(h.dinv[l], shape=(h.A[l].rows,)).copy()
+                     if h.A[l].rows > 0 else np.zeros(0) for l in range(h.nlevels)]
+
+    def apply(self, r):
+        r = _f64(r)
+        z = np.empty_like(r)
+        _check(lib().or_amg_apply(C.byref(self._h), _d(r), _d(z)), "amg_apply")
+        return z
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None:
+            lib().or_amg_free(C.byref(self._h))
+            self._h = None
 
 
 # ---- adjoint --------------------------------------------------------------

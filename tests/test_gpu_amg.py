@@ -1,0 +1,153 @@
+"""GPU AMG (AMGPreconditioner, src/linalg/amg.rs) against the oracle's
+restatement: every level's operator, prolongation and Jacobi inverse
+diagonal bit-identical, and the V-cycle bit-identical, on the reference's
+1D Laplacian test, random SPD matrices, a 2D Poisson matrix and an assembled
+thermal system held in the library's locality numbering (the hierarchy is
+built in the caller's numbering, as the reference would hold it); the last
+level's in-place sweeps, run level-scheduled on the device, on general
+nonsymmetric patterns (one-level hierarchies); the max_levels cut; argument
+errors; and CG with AMG through solve_with_operator."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401 (sp.linalg)
+import torch
+
+import geo_simulator_amd as G
+
+pytestmark = pytest.mark.gpu
+
+
+def lap1d(n):
+    return sp.diags([-np.ones(n - 1), 2 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1]).tocsr()
+
+
+def lap2d(m):
+    T = lap1d(m)
+    I = sp.identity(m)
+    return (sp.kron(T, I) + sp.kron(I, T)).tocsr()
+
+
+def random_spd(n, density, seed):
+    rng = np.random.default_rng(seed)
+    B = sp.random(n, n, density=density, random_state=rng, format="csr")
+    B.data = -np.abs(rng.standard_normal(B.nnz))
+    B = B + B.T
+    B.setdiag(0.0)
+    B.eliminate_zeros()
+    d = np.abs(B).sum(axis=1).A1 + rng.uniform(0.1, 1.0, n)
+    A = (B + sp.diags(d)).tocsr()
+    A.sort_indices()
+    return A
+
+
+def random_general(n, density, seed):
+    rng = np.random.default_rng(seed)
+    A = sp.random(n, n, density=density, random_state=rng, format="csr")
+    A.data = rng.standard_normal(A.nnz)
+    d = np.abs(A).sum(axis=1).A1 + 1.0
+    A = (A + sp.diags(d)).tocsr()
+    A.sort_indices()
+    return A
+
+
+def thermal_system(oracle, n=(6, 6, 4)):
+    xyz, conn = oracle.generate_cube(*n, 1000.0, 1000.0, 600.0)
+    mesh = G.Mesh(xyz, conn)
+    mask, vals = oracle.thermal_dirichlet(xyz, 600.0)
+    dof = G.DofManager(mesh.num_nodes(), 1)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], vals[mask == 1])
+    return G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+
+
+def check_same(M, Mo, r):
+    assert len(M.levels) == Mo.nlevels
+    for l in range(Mo.nlevels):
+        A, Ao = M.operator(l), Mo.A[l]
+        assert np.array_equal(A.indptr, Ao.indptr) and np.array_equal(A.indices, Ao.indices)
+        assert np.array_equal(A.data, Ao.data), f"level {l} operator"
+        assert np.array_equal(M.diag_inv(l), Mo.dinv[l])
+        P, Po = M.prolongation(l), Mo.P[l]
+        assert (P is None) == (Po is None)
+        if P is not None:
+            assert np.array_equal(P.indptr, Po.indptr) and np.array_equal(P.indices, Po.indices)
+            assert np.array_equal(P.data, Po.data), f"level {l} prolongation"
+    z = M.apply(r)
+    zo = Mo.apply(r)
+    assert np.array_equal(z, zo)
+    zt = M.apply(torch.tensor(r, device="cuda"))
+    assert zt.is_cuda and np.array_equal(zt.cpu().numpy(), zo)
+
+
+def test_amg_reference_unit_test(oracle):
+    print("[amg] 1D Laplacian n=10 (10, 2, 0.25)", flush=True)
+    A = lap1d(10)
+    M = G.AMGPreconditioner(G.CsrMatrix.from_scipy(A), 10, 2, 0.25)
+    x = M.apply(np.ones(10))
+    assert x.shape == (10,) and np.all(np.isfinite(x))
+    check_same(M, oracle.Amg(A, 10, 2, 0.25), np.ones(10))
+    assert [lv["rows"] for lv in M.levels] == [10, 5, 2]
+
+
+@pytest.mark.parametrize("case", ["spd", "lap2d", "thermal"])
+def test_amg_hierarchy_and_vcycle_bitexact(oracle, case):
+    print(f"[amg] hierarchy {case}", flush=True)
+    if case == "thermal":
+        K, f = thermal_system(oracle)
+        Ks = K.to_scipy()  # the caller's numbering
+        Kd = K
+    else:
+        Ks = random_spd(3000, 0.003, 4) if case == "spd" else lap2d(40)
+        Kd = G.CsrMatrix.from_scipy(Ks)
+    M = G.AMGPreconditioner(Kd, 10, 50, 0.25)
+    Mo = oracle.Amg(Ks, 10, 50, 0.25)
+    assert len(M.levels) >= 3
+    r = np.random.default_rng(5).standard_normal(Ks.shape[0])
+    check_same(M, Mo, r)
+
+
+@pytest.mark.parametrize("n,density,seed", [(2000, 0.004, 1), (300, 0.05, 2), (1, 1.0, 3)])
+def test_amg_sweeps_general_pattern(oracle, n, density, seed):
+    # coarse_size >= n: one level, the V-cycle is the ten in-place sweeps
+    print(f"[amg] sweeps n={n}", flush=True)
+    A = random_general(n, density, seed)
+    M = G.AMGPreconditioner(G.CsrMatrix.from_scipy(A), 10, n, 0.25)
+    assert len(M.levels) == 1
+    if n > 100:
+        assert 1 < M.levels[0]["sweep_levels"] < n
+    r = np.random.default_rng(seed).standard_normal(n)
+    check_same(M, oracle.Amg(A, 10, n, 0.25), r)
+
+
+def test_amg_max_levels_cut(oracle):
+    print("[amg] max_levels cut", flush=True)
+    A = lap2d(20)
+    M = G.AMGPreconditioner(G.CsrMatrix.from_scipy(A), 2, 2, 0.25)
+    assert len(M.levels) == 2 and M.prolongation(1) is not None
+    check_same(M, oracle.Amg(A, 2, 2, 0.25), np.linspace(-1.0, 1.0, 400))
+
+
+def test_amg_errors():
+    A = G.CsrMatrix.from_scipy(lap1d(8))
+    with pytest.raises(G.GeoFemError, match="max_levels"):
+        G.AMGPreconditioner(A, 0, 2, 0.25)
+    with pytest.raises(G.GeoFemError, match="coarse_size"):
+        G.AMGPreconditioner(A, 4, -1, 0.25)
+    with pytest.raises(G.GeoFemError, match="square"):
+        G.AMGPreconditioner(G.CsrMatrix.from_scipy(sp.random(5, 7, density=0.5,
+                                                             format="csr")), 4, 2, 0.25)
+    M = G.AMGPreconditioner(A, 4, 2, 0.25)
+    with pytest.raises(ValueError):
+        M.apply(np.ones(7))
+
+
+def test_amg_cg_thermal(oracle):
+    print("[amg] CG with AMG", flush=True)
+    K, f = thermal_system(oracle, (8, 8, 6))
+    cg = G.ConjugateGradient().with_tolerance(1e-10).with_max_iterations(2000)
+    x_amg, st_amg = cg.solve_with_operator(K, f, G.AMGPreconditioner(K, 10, 100, 0.25))
+    x_jac, st_jac = cg.solve_with_operator(K, f, G.JacobiPreconditioner(K))
+    assert st_amg.converged and st_jac.converged
+    assert st_amg.iterations * 3 < st_jac.iterations, (st_amg.iterations, st_jac.iterations)
+    xo = sp.linalg.spsolve(K.to_scipy().tocsc(), f.cpu().numpy())
+    assert np.linalg.norm(x_amg.cpu().numpy() - xo) <= 1e-8 * np.linalg.norm(xo)

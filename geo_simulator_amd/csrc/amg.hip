@@ -15,14 +15,18 @@
 // for the largest count, and a sparse accumulator for the dense column loop
 // of matrix_multiply (every product reaches its sum in the same order).
 //
-// V-cycle (:132-185) on the device, one thread per row: weighted Jacobi
+// V-cycle (:132-185) on the device, one thread per row (operators and
+// prolongations sliced by 64 rows, a slice's lanes loading their rows'
+// entries side by side): weighted Jacobi
 // (omega 0.67) twice, the residual, restriction by P^T, the coarser cycle,
 // prolongation by P, weighted Jacobi twice.  The last level runs the
 // reference's ten in-place sweeps (Gauss-Seidel: rows in order, each reading
-// the entries already updated) in one workgroup, rows grouped in dependency
-// levels: row i comes after every row j < i that it reads (its new value)
-// and after every row k < i that reads x_i (the old value), so rows of one
-// level neither read nor overwrite each other's entries and run at once.
+// the entries already updated): rows that read no other row and that no
+// other row reads take their ten updates at once, one thread each; the
+// others run in one workgroup, grouped in dependency levels: row i comes
+// after every row j < i that it reads (its new value) and after every row
+// k < i that reads x_i (the old value), so rows of one level neither read
+// nor overwrite each other's entries and run at once.
 // Every row sum starts at 0.0 and adds in ascending column order, compiled
 // without contraction, so the V-cycle equals the reference's bit for bit.
 #include <algorithm>
@@ -236,14 +240,27 @@ std::vector<double> diag_inv(const HCsr &A) {
   return d;
 }
 
-// Dependency levels of the in-place sweep: row i after the rows j < i it
-// reads and the rows k < i that read it.
-void sweep_levels(const HCsr &A, std::vector<int> &ptr, std::vector<int> &ord) {
+// The in-place sweeps' schedule: the isolated rows (every stored column of
+// the row is i, and column i is stored in no other row), whose ten updates
+// touch no other row's entry; and the dependency levels of the others: row i
+// after the rows j < i it reads and the rows k < i that read it.
+void sweep_levels(const HCsr &A, std::vector<int> &iso, std::vector<int> &ptr,
+                  std::vector<int> &ord) {
   const int64_t n = A.rows;
-  std::vector<int> lv((size_t)n, 0);
+  std::vector<int> lv((size_t)n, -1);
   const HCsr T = transpose(A);
-  int nlev = n > 0 ? 1 : 0;
+  auto alone = [](const HCsr &M, int64_t i) {
+    for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p)
+      if (M.col[p] != i) return false;
+    return true;
+  };
+  iso.clear();
+  int nlev = 0;
   for (int64_t i = 0; i < n; ++i) {
+    if (alone(A, i) && alone(T, i)) {
+      iso.push_back((int)i);
+      continue;
+    }
     int v = 0;
     for (int64_t p = A.rp[i]; p < A.rp[i + 1] && A.col[p] < i; ++p) v = std::max(v, lv[A.col[p]] + 1);
     for (int64_t p = T.rp[i]; p < T.rp[i + 1] && T.col[p] < i; ++p) v = std::max(v, lv[T.col[p]] + 1);
@@ -251,11 +268,13 @@ void sweep_levels(const HCsr &A, std::vector<int> &ptr, std::vector<int> &ord) {
     nlev = std::max(nlev, v + 1);
   }
   ptr.assign((size_t)nlev + 1, 0);
-  for (int v : lv) ++ptr[(size_t)v + 1];
+  for (int v : lv)
+    if (v >= 0) ++ptr[(size_t)v + 1];
   for (int l = 0; l < nlev; ++l) ptr[(size_t)l + 1] += ptr[(size_t)l];
-  ord.resize((size_t)n);
+  ord.resize((size_t)ptr[(size_t)nlev]);
   std::vector<int> at(ptr.begin(), ptr.end() - 1);
-  for (int64_t i = 0; i < n; ++i) ord[(size_t)at[(size_t)lv[i]]++] = (int)i;
+  for (int64_t i = 0; i < n; ++i)
+    if (lv[i] >= 0) ord[(size_t)at[(size_t)lv[i]]++] = (int)i;
 }
 
 DevCsr upload(gs_ctx *ctx, const HCsr &A) {
@@ -268,6 +287,56 @@ DevCsr upload(gs_ctx *ctx, const HCsr &A) {
   D.rp = to_device(ctx, rp);
   D.col = to_device(ctx, A.col);
   D.val = to_device(ctx, A.val);
+  // sliced copy: each slice as long as its longest row; rows sorted by
+  // length (descending, then by index) when the natural order pads more
+  // than a quarter
+  const int64_t n = A.rows, ns = (n + 63) / 64;
+  auto len = [&](int64_t i) { return A.rp[i + 1] - A.rp[i]; };
+  auto padded = [&](const std::vector<int64_t> &at) {
+    int64_t tot = 0;
+    for (int64_t sl = 0; sl < ns; ++sl) {
+      int64_t m = 0;
+      for (int64_t q = 64 * sl; q < std::min<int64_t>(n, 64 * sl + 64); ++q)
+        m = std::max(m, len(at.empty() ? q : at[q]));
+      tot += 64 * m;
+    }
+    return tot;
+  };
+  std::vector<int64_t> pos;  // row at each position (empty: the identity)
+  if (padded(pos) * 4 > A.nnz() * 5) {
+    pos.resize((size_t)n);
+    for (int64_t i = 0; i < n; ++i) pos[i] = i;
+    std::stable_sort(pos.begin(), pos.end(),
+                     [&](int64_t a, int64_t b) { return len(a) > len(b); });
+  }
+  auto row_at = [&](int64_t q) { return pos.empty() ? q : pos[q]; };
+  std::vector<int> cb((size_t)ns + 1, 0);
+  for (int64_t sl = 0; sl < ns; ++sl) {
+    int64_t m = 0;
+    for (int64_t q = 64 * sl; q < std::min<int64_t>(n, 64 * sl + 64); ++q)
+      m = std::max(m, len(row_at(q)));
+    if ((int64_t)cb[sl] + m >= INT32_MAX / 64)
+      throw Error(GS_ERR_TOO_LARGE, "AMG level too large for its sliced copy");
+    cb[sl + 1] = cb[sl] + (int)m;
+  }
+  const size_t ne = (size_t)cb[ns] * 64;
+  std::vector<int> ec(ne, -1);
+  std::vector<double> ev(ne, 0.0);
+  for (int64_t q = 0; q < n; ++q) {
+    const int64_t i = row_at(q), sl = q / 64, lane = q % 64;
+    for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+      const size_t at = (size_t)(cb[sl] + (p - A.rp[i])) * 64 + lane;
+      ec[at] = A.col[p];
+      ev[at] = A.val[p];
+    }
+  }
+  int64_t longest = 0;
+  for (int64_t i = 0; i < n; ++i) longest = std::max(longest, len(i));
+  D.ew = longest > 72 ? 8 : 4;
+  D.ecb = to_device(ctx, cb);
+  D.ecol = to_device(ctx, ec);
+  D.eval = to_device(ctx, ev);
+  if (!pos.empty()) D.erow = to_device(ctx, std::vector<int>(pos.begin(), pos.end()));
   return D;
 }
 
@@ -293,48 +362,138 @@ __device__ __forceinline__ double row_sum(int b, int e, const int *__restrict__ 
   return sum;
 }
 
+// Row i of a sliced copy (one lane per row of a 64-row slice, entries
+// loaded side by side by the slice's lanes): the same sum as row_sum, the
+// padding skipped by a select.
+struct Ell {
+  const int *__restrict__ cb;
+  const int *__restrict__ col;
+  const double *__restrict__ val;
+  const int *__restrict__ row;  // row at each position (nullptr: the same)
+};
+inline Ell ell(const DevCsr &M) {
+  return Ell{M.ecb.get(), M.ecol.get(), M.eval.get(), M.erow.get()};
+}
+
+// the row at position q of the slices
+__device__ __forceinline__ int64_t ell_row(const Ell &M, int64_t q) {
+  return M.row ? (int64_t)M.row[q] : q;
+}
+
+// (M x) of the row at position q: W entries per trip, the next trip's
+// columns and values loaded while this trip's gathers are in flight (W = 4
+// keeps more rows in flight on long operators; W = 8 shortens the chain of
+// trips of levels with long rows, whose longest row sets the kernel's time)
+template <int W>
+__device__ __forceinline__ double ell_sum(const Ell &M, int64_t q, const double *x) {
+  const int64_t s = q >> 6;
+  const int lane = (int)(q & 63);
+  const int b = M.cb[s], e = M.cb[s + 1];
+  double sum = 0.0;
+  int c[W];
+  double a[W];
+  auto load = [&](int k) {
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const bool in = k + u < e;
+      const size_t at = (size_t)(in ? k + u : b) * 64 + lane;
+      c[u] = in ? M.col[at] : -1;
+      a[u] = in ? M.val[at] : 0.0;
+    }
+  };
+  if (b < e) load(b);
+  for (int k = b; k < e; k += W) {
+    double v[W];
+    int cc[W];
+    double aa[W];
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      cc[u] = c[u];
+      aa[u] = a[u];
+      v[u] = x[cc[u] >= 0 ? cc[u] : 0];
+    }
+    if (k + W < e) load(k + W);
+#pragma unroll
+    for (int u = 0; u < W; ++u) {
+      const double t = sum + aa[u] * v[u];
+      sum = cc[u] >= 0 ? t : sum;
+    }
+  }
+  return sum;
+}
+
 __global__ void k_amg_zero(int64_t n, double *__restrict__ x) {
   const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i < n) x[i] = 0.0;
 }
 
 // weighted_jacobi_step (:250-263): xo_i = xi_i + omega (r_i - (A xi)_i) dinv_i
-__global__ void k_amg_jacobi(int64_t n, const int *__restrict__ rp, const int *__restrict__ col,
-                             const double *__restrict__ val, const double *__restrict__ dinv,
+template <int W>
+__global__ void k_amg_jacobi(int64_t n, const Ell A, const double *__restrict__ dinv,
                              const double *__restrict__ r, const double *__restrict__ xi,
                              double *__restrict__ xo) {
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const double ax = row_sum(rp[i], rp[i + 1], col, val, xi);
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q >= n) return;
+  const int64_t i = ell_row(A, q);
+  const double ax = ell_sum<W>(A, q, xi);
   const double t = AMG_OMEGA * (r[i] - ax);
   xo[i] = xi[i] + t * dinv[i];
 }
 
-// res_i = r_i - (A x)_i
-__global__ void k_amg_residual(int64_t n, const int *__restrict__ rp, const int *__restrict__ col,
-                               const double *__restrict__ val, const double *__restrict__ r,
-                               const double *__restrict__ x, double *__restrict__ res) {
+// The first step from x = 0 with every operator entry finite: (A x)_i is
+// +0.0, so xo_i = 0.0 + omega (r_i - 0.0) dinv_i, the same operations.
+__global__ void k_amg_jacobi_zero(int64_t n, const double *__restrict__ dinv,
+                                  const double *__restrict__ r, double *__restrict__ xo) {
   const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
   if (i >= n) return;
-  res[i] = r[i] - row_sum(rp[i], rp[i + 1], col, val, x);
+  const double ax = 0.0;
+  const double t = AMG_OMEGA * (r[i] - ax);
+  xo[i] = 0.0 + t * dinv[i];
+}
+
+// res_i = r_i - (A x)_i
+template <int W>
+__global__ void k_amg_residual(int64_t n, const Ell A, const double *__restrict__ r,
+                               const double *__restrict__ x, double *__restrict__ res) {
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q >= n) return;
+  const int64_t i = ell_row(A, q);
+  res[i] = r[i] - ell_sum<W>(A, q, x);
 }
 
 // y = M x (matvec, :230-247)
-__global__ void k_amg_spmv(int64_t n, const int *__restrict__ rp, const int *__restrict__ col,
-                           const double *__restrict__ val, const double *__restrict__ x,
+template <int W>
+__global__ void k_amg_spmv(int64_t n, const Ell M, const double *__restrict__ x,
                            double *__restrict__ y) {
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i < n) y[i] = row_sum(rp[i], rp[i + 1], col, val, x);
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q < n) y[ell_row(M, q)] = ell_sum<W>(M, q, x);
 }
 
 // x_i += (P e)_i  (the correction formed first, then added, :168-175)
-__global__ void k_amg_prolong(int64_t n, const int *__restrict__ rp, const int *__restrict__ col,
-                              const double *__restrict__ val, const double *__restrict__ e,
+template <int W>
+__global__ void k_amg_prolong(int64_t n, const Ell P, const double *__restrict__ e,
                               double *__restrict__ x) {
-  const int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (i >= n) return;
-  const double c = row_sum(rp[i], rp[i + 1], col, val, e);
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q >= n) return;
+  const int64_t i = ell_row(P, q);
+  const double c = ell_sum<W>(P, q, e);
   x[i] = x[i] + c;
+}
+
+// The last level's sweeps of its isolated rows: each row's ten updates at
+// once (they read and write only its own entry).
+__global__ void k_amg_sweeps_iso(int64_t n, const int *__restrict__ iso,
+                                 const int *__restrict__ rp, const int *__restrict__ col,
+                                 const double *__restrict__ val, const double *__restrict__ dinv,
+                                 const double *__restrict__ r, double *__restrict__ x) {
+  const int64_t t = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (t >= n) return;
+  const int i = iso[t];
+  for (int s = 0; s < AMG_SWEEPS; ++s) {
+    const double ax = row_sum(rp[i], rp[i + 1], col, val, x);
+    const double d = (r[i] - ax) * dinv[i];
+    x[i] = x[i] + d;
+  }
 }
 
 // The last level's sweeps (:136-146), one workgroup: levels in order, the
@@ -396,9 +555,13 @@ gs_amg *amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels, int64_t coa
     // the V-cycle ends here when this level has no P or the loop ends
     const bool end = last || level + 1 == max_levels;
     L.A = upload(ctx, cur);
+    L.finite = std::all_of(cur.val.begin(), cur.val.end(),
+                           [](double v) { return std::isfinite(v); });
     if (end) {
-      std::vector<int> ptr, ord;
-      sweep_levels(cur, ptr, ord);
+      std::vector<int> iso, ptr, ord;
+      sweep_levels(cur, iso, ptr, ord);
+      L.n_iso = (int64_t)iso.size();
+      L.iso = to_device(ctx, iso);
       L.nlev = (int)ptr.size() - 1;
       L.lptr = to_device(ctx, ptr);
       L.ord = to_device(ctx, ord);
@@ -423,34 +586,56 @@ static void vcycle(const gs_amg *M, size_t l, const double *r, double *x) {
   const int64_t n = L.A.rows;
   if (n == 0) return;
   const unsigned g = nblk(n);
-  hipLaunchKernelGGL(k_amg_zero, g, BS, 0, s, n, x);
   const int *rp = L.A.rp.get(), *col = L.A.col.get();
   const double *val = L.A.val.get(), *dinv = L.dinv.get();
   if (l + 1 == M->lv.size()) {
-    hipLaunchKernelGGL(k_amg_sweeps, 1, GS_NT, 0, s, L.nlev, L.lptr.get(), L.ord.get(), rp,
-                       col, val, dinv, r, x);
+    hipLaunchKernelGGL(k_amg_zero, g, BS, 0, s, n, x);
+    if (L.n_iso > 0)
+      hipLaunchKernelGGL(k_amg_sweeps_iso, nblk(L.n_iso), BS, 0, s, L.n_iso, L.iso.get(), rp,
+                         col, val, dinv, r, x);
+    if (L.nlev > 0)
+      hipLaunchKernelGGL(k_amg_sweeps, 1, GS_NT, 0, s, L.nlev, L.lptr.get(), L.ord.get(), rp,
+                         col, val, dinv, r, x);
     GS_LAUNCH_CHECK("k_amg_sweeps");
     return;
   }
   double *t = L.t.get();
-  auto smooth = [&] {
+  // x starts at 0: the first step's product A x is +0.0 in every row (each
+  // term a_ij 0.0 is a signed zero and the sum starts at +0.0) when every
+  // entry is finite, so that step reads no operator
+  auto smooth = [&](bool from_zero) {
     for (int k = 0; k < AMG_SMOOTH; ++k) {
       const double *xi = (k & 1) ? t : x;
       double *xo = (k & 1) ? x : t;
-      hipLaunchKernelGGL(k_amg_jacobi, g, BS, 0, s, n, rp, col, val, dinv, r, xi, xo);
+      if (k == 0 && from_zero && L.finite)
+        hipLaunchKernelGGL(k_amg_jacobi_zero, g, BS, 0, s, n, dinv, r, xo);
+      else if (L.A.ew == 8)
+        hipLaunchKernelGGL(k_amg_jacobi<8>, g, BS, 0, s, n, ell(L.A), dinv, r, xi, xo);
+      else
+        hipLaunchKernelGGL(k_amg_jacobi<4>, g, BS, 0, s, n, ell(L.A), dinv, r, xi, xo);
     }
   };
   static_assert(AMG_SMOOTH % 2 == 0, "the smoothing steps end in x");
-  smooth();
-  hipLaunchKernelGGL(k_amg_residual, g, BS, 0, s, n, rp, col, val, r, x, t);
+  if (!L.finite) hipLaunchKernelGGL(k_amg_zero, g, BS, 0, s, n, x);
+  smooth(true);
+  if (L.A.ew == 8)
+    hipLaunchKernelGGL(k_amg_residual<8>, g, BS, 0, s, n, ell(L.A), r, x, t);
+  else
+    hipLaunchKernelGGL(k_amg_residual<4>, g, BS, 0, s, n, ell(L.A), r, x, t);
   const AmgLevel &C = M->lv[l + 1];
-  hipLaunchKernelGGL(k_amg_spmv, nblk(L.coarse), BS, 0, s, L.coarse, L.PT.rp.get(),
-                     L.PT.col.get(), L.PT.val.get(), t, C.r.get());
+  if (L.PT.ew == 8)
+    hipLaunchKernelGGL(k_amg_spmv<8>, nblk(L.coarse), BS, 0, s, L.coarse, ell(L.PT), t,
+                       C.r.get());
+  else
+    hipLaunchKernelGGL(k_amg_spmv<4>, nblk(L.coarse), BS, 0, s, L.coarse, ell(L.PT), t,
+                       C.r.get());
   GS_LAUNCH_CHECK("k_amg_spmv");
   vcycle(M, l + 1, C.r.get(), C.x.get());
-  hipLaunchKernelGGL(k_amg_prolong, g, BS, 0, s, n, L.P.rp.get(), L.P.col.get(),
-                     L.P.val.get(), C.x.get(), x);
-  smooth();
+  if (L.P.ew == 8)
+    hipLaunchKernelGGL(k_amg_prolong<8>, g, BS, 0, s, n, ell(L.P), C.x.get(), x);
+  else
+    hipLaunchKernelGGL(k_amg_prolong<4>, g, BS, 0, s, n, ell(L.P), C.x.get(), x);
+  smooth(false);
   GS_LAUNCH_CHECK("k_amg_jacobi");
 }
 

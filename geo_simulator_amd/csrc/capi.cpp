@@ -1106,7 +1106,7 @@ gs_status gs_amg_level_info(const gs_amg *amg, int64_t level, int64_t *rows, int
   if (nnz) *nnz = L.A.nnz;
   if (coarse) *coarse = L.has_p ? L.coarse : 0;
   if (nnz_p) *nnz_p = L.has_p ? L.P.nnz : 0;
-  if (sweep_levels) *sweep_levels = L.nlev;
+  if (sweep_levels) *sweep_levels = L.nlev + (L.n_iso > 0 ? 1 : 0);
   return GS_OK;
 }
 

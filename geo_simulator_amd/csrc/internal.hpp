@@ -239,13 +239,25 @@ struct DevCsr {
   int64_t rows = 0, nnz = 0;
   DArr<int> rp, col;
   DArr<double> val;
+  // the same rows sliced for the V-cycle's kernels: slices of 64 rows, entry
+  // k of the row at position 64 s + l at (ecb[s] + k) * 64 + l, padded with
+  // column -1; erow: the row at each position (rows sorted by length, when
+  // that pads less), empty for position = row
+  DArr<int> ecb, ecol, erow;
+  DArr<double> eval;
+  int ew = 4;  // entries per trip of the row sums (8 where rows are long)
 };
 struct AmgLevel {
   DevCsr A, P, PT;      // P and PT empty where the level has none
   bool has_p = false;
+  bool finite = true;   // every operator entry finite
   int64_t coarse = 0;   // columns of P
   DArr<double> dinv, x, t, r;
-  int nlev = 0;         // last level: dependency levels of its sweep
+  // last level: the rows that read no other row and that no other row reads
+  // (swept on their own), and the dependency levels of the others' sweeps
+  int64_t n_iso = 0;
+  DArr<int> iso;
+  int nlev = 0;
   DArr<int> lptr, ord;
 };
 }  // namespace gs

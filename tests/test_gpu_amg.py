@@ -2,8 +2,9 @@
 restatement: every level's operator, prolongation and Jacobi inverse
 diagonal bit-identical, and the V-cycle bit-identical, on the reference's
 1D Laplacian test, random SPD matrices, a 2D Poisson matrix and an assembled
-thermal system held in the library's locality numbering (the hierarchy is
-built in the caller's numbering, as the reference would hold it); the last
+thermal system and an elastic (3 DOFs per node) system held in the
+library's locality numbering (the hierarchy is built in the caller's
+numbering, as the reference would hold it); the last
 level's in-place sweeps, run level-scheduled on the device, on general
 nonsymmetric patterns (one-level hierarchies); the max_levels cut; argument
 errors; and CG with AMG through solve_with_operator."""
@@ -60,6 +61,17 @@ def thermal_system(oracle, n=(6, 6, 4)):
     return G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
 
 
+def elastic_system(oracle):
+    lx = ly = 1e6
+    xyz, conn = oracle.generate_cube(3, 3, 4, lx, ly, 6e5)
+    mesh = G.Mesh(xyz, conn)
+    mask, vals = oracle.elastic_dirichlet(xyz, lx, ly)
+    dof = G.DofManager(mesh.num_nodes(), 3)
+    dof.set_dirichlet_many(np.nonzero(mask)[0], 0.0)
+    mat = G.IsotropicElasticity(1e11, 0.25)
+    return G.Assembler.assemble_elasticity_system(mesh, dof, mat, 3300.0, [0, 0, -9.81])
+
+
 def check_same(M, Mo, r):
     assert len(M.levels) == Mo.nlevels
     for l in range(Mo.nlevels):
@@ -89,11 +101,13 @@ def test_amg_reference_unit_test(oracle):
     assert [lv["rows"] for lv in M.levels] == [10, 5, 2]
 
 
-@pytest.mark.parametrize("case", ["spd", "lap2d", "thermal"])
+@pytest.mark.parametrize("case", ["spd", "lap2d", "thermal", "elastic"])
 def test_amg_hierarchy_and_vcycle_bitexact(oracle, case):
     print(f"[amg] hierarchy {case}", flush=True)
-    if case == "thermal":
-        K, f = thermal_system(oracle)
+    if case in ("thermal", "elastic"):
+        # assembled systems held in the library's locality numbering (the
+        # elastic one also in the 3x3-block view): built in caller numbering
+        K, f = thermal_system(oracle) if case == "thermal" else elastic_system(oracle)
         Ks = K.to_scipy()  # the caller's numbering
         Kd = K
     else:

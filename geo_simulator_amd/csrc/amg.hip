@@ -30,6 +30,9 @@
 // Every row sum starts at 0.0 and adds in ascending column order, compiled
 // without contraction, so the V-cycle equals the reference's bit for bit.
 #include <algorithm>
+#include <functional>
+#include <memory>
+#include <thread>
 #include <cmath>
 #include <queue>
 #include <string>
@@ -57,57 +60,108 @@ struct HCsr {
   int64_t nnz() const { return (int64_t)col.size(); }
 };
 
+// Host threads for the row-parallel set-up loops (each row is computed by
+// one thread in the reference's order, so the result does not depend on the
+// thread count).
+inline int host_threads(int64_t rows) {
+  const unsigned h = std::thread::hardware_concurrency();
+  const int64_t t = std::min<int64_t>(h ? h : 1, 16);
+  return (int)std::max<int64_t>(1, std::min<int64_t>(t, rows / 4096));
+}
+
+// A CSR matrix built row by row: make() gives each thread its row function
+// row(i, col, val), which appends row i's entries in order.
+template <class Make>
+HCsr build_rows(int64_t rows, int64_t cols, Make make) {
+  const int T = host_threads(rows);
+  std::vector<std::vector<int>> tc((size_t)T);
+  std::vector<std::vector<double>> tv((size_t)T);
+  HCsr C;
+  C.rows = rows;
+  C.cols = cols;
+  C.rp.assign((size_t)rows + 1, 0);
+  auto work = [&](int t) {
+    auto row = make();
+    const int64_t b = rows * t / T, e = rows * (t + 1) / T;
+    for (int64_t i = b; i < e; ++i) {
+      const size_t before = tc[t].size();
+      row(i, tc[t], tv[t]);
+      C.rp[i + 1] = (int64_t)(tc[t].size() - before);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < T; ++t) pool.emplace_back(work, t);
+  work(0);
+  for (auto &th : pool) th.join();
+  for (int64_t i = 0; i < rows; ++i) C.rp[i + 1] += C.rp[i];
+  C.col.resize((size_t)C.rp[rows]);
+  C.val.resize((size_t)C.rp[rows]);
+  for (int t = 0; t < T; ++t) {
+    const int64_t at = C.rp[rows * t / T];
+    std::copy(tc[t].begin(), tc[t].end(), C.col.begin() + at);
+    std::copy(tv[t].begin(), tv[t].end(), C.val.begin() + at);
+  }
+  return C;
+}
+
 // strength_matrix (:359-390): the stored off-diagonal entries of row i with
 // |a_ij| >= theta * max_{stored k != i} |a_ik| (f64::max skips NaN, as fmax)
 HCsr strength(const HCsr &A, double theta) {
-  HCsr S;
-  S.rows = A.rows;
-  S.cols = A.cols;
-  S.rp.assign((size_t)A.rows + 1, 0);
-  for (int64_t i = 0; i < A.rows; ++i) {
-    double mx = 0.0;
-    for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-      if (A.col[p] != i) mx = std::fmax(mx, std::fabs(A.val[p]));
-    const double thr = theta * mx;
-    for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
-      if (A.col[p] != i && std::fabs(A.val[p]) >= thr) {
-        S.col.push_back(A.col[p]);
-        S.val.push_back(A.val[p]);
-      }
-    S.rp[i + 1] = S.nnz();
-  }
-  return S;
+  return build_rows(A.rows, A.cols, [&] {
+    return [&](int64_t i, std::vector<int> &col, std::vector<double> &val) {
+      double mx = 0.0;
+      for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+        if (A.col[p] != i) mx = std::fmax(mx, std::fabs(A.val[p]));
+      const double thr = theta * mx;
+      for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p)
+        if (A.col[p] != i && std::fabs(A.val[p]) >= thr) {
+          col.push_back(A.col[p]);
+          val.push_back(A.val[p]);
+        }
+    };
+  });
 }
 
-// coarsen (:286-355): 1 marks a coarse point
-std::vector<unsigned char> coarsen(const HCsr &A, double theta, int64_t &num_coarse) {
-  const HCsr S = strength(A, theta);
-  const int64_t n = A.rows;
+// coarsen (:286-355) on the strength matrix S: 1 marks a coarse point.  The
+// undecided points with a positive count sit in buckets by count, each a
+// min-heap of point indices; an entry is stale once its point was decided or
+// its count grew (the point then also sits in the next bucket).  The top
+// bucket's smallest valid index is the point the reference's scan picks.
+std::vector<unsigned char> coarsen(const HCsr &S, int64_t &num_coarse) {
+  const int64_t n = S.rows;
   std::vector<int64_t> lam((size_t)n);
   std::vector<unsigned char> cf((size_t)n, 0), open((size_t)n, 1);
-  // top: the largest count, then the smallest index
-  auto later = [](const std::pair<int64_t, int64_t> &a, const std::pair<int64_t, int64_t> &b) {
-    return a.first < b.first || (a.first == b.first && a.second > b.second);
+  std::vector<std::vector<int>> bucket(1);
+  int64_t top = 0;
+  auto push = [&](int64_t i) {
+    const int64_t l = lam[i];
+    if (l >= (int64_t)bucket.size()) bucket.resize((size_t)l + 1);
+    auto &h = bucket[(size_t)l];
+    h.push_back((int)i);
+    std::push_heap(h.begin(), h.end(), std::greater<int>());
+    top = std::max(top, l);
   };
-  std::priority_queue<std::pair<int64_t, int64_t>, std::vector<std::pair<int64_t, int64_t>>,
-                      decltype(later)>
-      heap(later);
   for (int64_t i = 0; i < n; ++i) {
     int64_t c = 0;
     for (int64_t p = S.rp[i]; p < S.rp[i + 1]; ++p) c += S.col[p] != i;
     lam[i] = c;
-    if (c > 0) heap.emplace(c, i);
+    if (c > 0) push(i);
   }
   int64_t nc = 0, left = n;
   while (left > 0) {
     int64_t pick = -1;
-    while (!heap.empty()) {
-      const auto t = heap.top();
-      heap.pop();
-      if (open[t.second] && lam[t.second] == t.first) {
-        pick = t.second;
-        break;
+    while (top > 0 && pick < 0) {
+      auto &h = bucket[(size_t)top];
+      while (!h.empty()) {
+        std::pop_heap(h.begin(), h.end(), std::greater<int>());
+        const int i = h.back();
+        h.pop_back();
+        if (open[i] && lam[i] == top) {
+          pick = i;
+          break;
+        }
       }
+      if (pick < 0) --top;
     }
     if (pick < 0) {  // no undecided point has a strong connection left
       for (int64_t i = 0; i < n; ++i)
@@ -129,7 +183,10 @@ std::vector<unsigned char> coarsen(const HCsr &A, double theta, int64_t &num_coa
       --left;
       for (int64_t q = S.rp[j]; q < S.rp[j + 1]; ++q) {
         const int64_t k = S.col[q];
-        if (k != j && open[k]) heap.emplace(++lam[k], k);
+        if (k != j && open[k]) {
+          ++lam[k];
+          push(k);
+        }
       }
     }
   }
@@ -137,41 +194,38 @@ std::vector<unsigned char> coarsen(const HCsr &A, double theta, int64_t &num_coa
   return cf;
 }
 
-// build_prolongation (:396-487)
-HCsr prolongation(const HCsr &A, const std::vector<unsigned char> &cf, int64_t nc,
-                  double theta) {
-  const HCsr S = strength(A, theta);
+// build_prolongation (:396-487), with the level's strength matrix S
+HCsr prolongation(const HCsr &A, const HCsr &S, const std::vector<unsigned char> &cf,
+                  int64_t nc) {
   const int64_t n = A.rows;
   std::vector<int> cidx((size_t)n, 0);
   for (int64_t i = 0, c = 0; i < n; ++i)
     if (cf[i]) cidx[i] = (int)c++;
-  HCsr P;
-  P.rows = n;
-  P.cols = nc;
-  P.rp.assign((size_t)n + 1, 0);
-  auto take = [&](const HCsr &M, int64_t i) {  // the coarse entries of row i, |a| / sum
-    double sum = 0.0;
-    for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p)
-      if (cf[M.col[p]]) sum += std::fabs(M.val[p]);
-    if (!(sum > 1e-14)) return false;
-    for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p)
-      if (cf[M.col[p]]) {
-        P.col.push_back(cidx[M.col[p]]);
-        P.val.push_back(std::fabs(M.val[p]) / sum);
+  return build_rows(n, nc, [&] {
+    return [&](int64_t i, std::vector<int> &col, std::vector<double> &val) {
+      // the coarse entries of row i of M, |m_ij| / their sum, unless the sum
+      // is at most 1e-14
+      auto take = [&](const HCsr &M) {
+        double sum = 0.0;
+        for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p)
+          if (cf[M.col[p]]) sum += std::fabs(M.val[p]);
+        if (!(sum > 1e-14)) return false;
+        for (int64_t p = M.rp[i]; p < M.rp[i + 1]; ++p)
+          if (cf[M.col[p]]) {
+            col.push_back(cidx[M.col[p]]);
+            val.push_back(std::fabs(M.val[p]) / sum);
+          }
+        return true;
+      };
+      if (cf[i]) {
+        col.push_back(cidx[i]);
+        val.push_back(1.0);
+      } else if (!take(S) && !take(A) && nc > 0) {
+        col.push_back(0);  // no coarse neighbour at all: the first coarse point
+        val.push_back(1.0);
       }
-    return true;
-  };
-  for (int64_t i = 0; i < n; ++i) {
-    if (cf[i]) {
-      P.col.push_back(cidx[i]);
-      P.val.push_back(1.0);
-    } else if (!take(S, i) && !take(A, i) && nc > 0) {
-      P.col.push_back(0);  // no coarse neighbour at all: the first coarse point
-      P.val.push_back(1.0);
-    }
-    P.rp[i + 1] = P.nnz();
-  }
-  return P;
+    };
+  });
 }
 
 HCsr transpose(const HCsr &A) {
@@ -195,37 +249,35 @@ HCsr transpose(const HCsr &A) {
 
 // matrix_multiply (:195-226): C = A B, |c_ij| > 1e-14 kept
 HCsr matmul(const HCsr &A, const HCsr &B) {
-  HCsr C;
-  C.rows = A.rows;
-  C.cols = B.cols;
-  C.rp.assign((size_t)A.rows + 1, 0);
-  std::vector<double> acc((size_t)std::max<int64_t>(1, B.cols));
-  std::vector<int64_t> seen((size_t)std::max<int64_t>(1, B.cols), -1);
-  std::vector<int> cols;
-  for (int64_t i = 0; i < A.rows; ++i) {
-    cols.clear();
-    for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-      const double a = A.val[p];
-      const int64_t k = A.col[p];
-      for (int64_t q = B.rp[k]; q < B.rp[k + 1]; ++q) {
-        const int j = B.col[q];
-        if (seen[j] != i) {
-          seen[j] = i;
-          acc[j] = 0.0;
-          cols.push_back(j);
+  const int64_t nb = std::max<int64_t>(1, B.cols);
+  return build_rows(A.rows, B.cols, [&] {
+    // per thread: a dense accumulator over B's columns and the row's columns
+    auto acc = std::make_shared<std::vector<double>>((size_t)nb);
+    auto seen = std::make_shared<std::vector<int64_t>>((size_t)nb, -1);
+    auto cols = std::make_shared<std::vector<int>>();
+    return [&, acc, seen, cols](int64_t i, std::vector<int> &col, std::vector<double> &val) {
+      cols->clear();
+      for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
+        const double a = A.val[p];
+        const int64_t k = A.col[p];
+        for (int64_t q = B.rp[k]; q < B.rp[k + 1]; ++q) {
+          const int j = B.col[q];
+          if ((*seen)[j] != i) {
+            (*seen)[j] = i;
+            (*acc)[j] = 0.0;
+            cols->push_back(j);
+          }
+          (*acc)[j] += a * B.val[q];
         }
-        acc[j] += a * B.val[q];
       }
-    }
-    std::sort(cols.begin(), cols.end());
-    for (int j : cols)
-      if (std::fabs(acc[j]) > 1e-14) {
-        C.col.push_back(j);
-        C.val.push_back(acc[j]);
-      }
-    C.rp[i + 1] = C.nnz();
-  }
-  return C;
+      std::sort(cols->begin(), cols->end());
+      for (int j : *cols)
+        if (std::fabs((*acc)[j]) > 1e-14) {
+          col.push_back(j);
+          val.push_back((*acc)[j]);
+        }
+    };
+  });
 }
 
 // extract_diag_inv (:266-280)
@@ -548,9 +600,10 @@ gs_amg *amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels, int64_t coa
     HCsr P;
     int64_t nc = 0;
     if (!last) {
-      const std::vector<unsigned char> cf = coarsen(cur, theta, nc);
+      const HCsr S = strength(cur, theta);
+      const std::vector<unsigned char> cf = coarsen(S, nc);
       last = nc == 0 || nc >= n;
-      if (!last) P = prolongation(cur, cf, nc, theta);
+      if (!last) P = prolongation(cur, S, cf, nc);
     }
     // the V-cycle ends here when this level has no P or the loop ends
     const bool end = last || level + 1 == max_levels;

@@ -595,7 +595,7 @@ __global__ __launch_bounds__(64) void k_sys_rows_thermal(
   int span = max(nk + nl, SELL_CW * nch);
   for (int off = 32; off > 0; off >>= 1) span = max(span, __shfl_xor(span, off, 64));
   double *o64 = sval + (int64_t)c0 * SELL_CHUNK + lane * SELL_CW;
-  float *o32 = sval32 + (int64_t)c0 * SELL_CHUNK + lane * SELL_CW;
+  float *o32 = sval32 ? sval32 + (int64_t)c0 * SELL_CHUNK + lane * SELL_CW : nullptr;
   double fl = 0.0;
   unsigned long long mine = 0;
   for (int p0 = 0; p0 < span; p0 += LW) {
@@ -639,8 +639,9 @@ __global__ __launch_bounds__(64) void k_sys_rows_thermal(
       d2 *w64 = (d2 *)(o64 + (int64_t)c * SELL_CHUNK);
       w64[0] = (d2){v[0], v[1]};
       w64[1] = (d2){v[2], v[3]};
-      *(f4 *)(o32 + (int64_t)c * SELL_CHUNK) =
-          (f4){(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
+      if (sval32)
+        *(f4 *)(o32 + (int64_t)c * SELL_CHUNK) =
+            (f4){(float)v[0], (float)v[1], (float)v[2], (float)v[3]};
     }
     const int kl1 = min(p0 + LW, nk + nl);
     for (int k = max(p0, nk); k < kl1; ++k) liftv[lb + k - nk] = acc[(k - p0) * 64 + lane];
@@ -987,7 +988,7 @@ __global__ void k_sys_rhs(int64_t n, const int *__restrict__ diag_slot,
     const double cv = __longlong_as_double((long long)*maxbits);
     const int e = pos[d];
     sval[e] = cv;
-    sval32[e] = (float)cv;
+    if (sval32) sval32[e] = (float)cv;
     f = g[r] * cv;
     dv = cv;
   } else {

@@ -154,7 +154,11 @@ gs_csr *assemble_system(gs_ctx *ctx, gs_mesh *m, const gs_dofmap *dof,
                             true, Ke.get(), nullptr, flag.get(), P->incpos.get());
   const SellPattern &S = *P->sell;
   auto sval = palloc<double>(pool, S.nvals);
-  auto sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
+  // FP32 values: the elastic solves' mixed recursion reads them, so the
+  // elastic gather writes them; a thermal solve reads its scaled fixed-point
+  // values instead, and the FP32 ones are made when a path needs them
+  DArr<float> sval32;
+  if (sp.elastic) sval32 = palloc<float>(pool, S.nvals);  // every entry written by the gather
   auto liftv = palloc<double>(pool, P->n_lift);
   auto sdinv = palloc<double>(pool, P->n);
   auto sdinv32 = palloc<float>(pool, P->n);

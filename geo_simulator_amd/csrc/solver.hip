@@ -47,6 +47,10 @@
 #include "spmv.hpp"
 
 namespace gs {
+// the SELL view's arrays of A (values in FP64, and FP32 with fp32; the Jacobi
+// diagonal with precond), made where missing
+static void ensure_sell(gs_ctx *ctx, gs_csr *A, bool fp32, bool precond);
+
 namespace {
 
 constexpr int BS = 256;
@@ -1183,6 +1187,9 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
   // point) when every D^-1 entry is positive and every scaled entry fits:
   // the recursion then needs no z vector
   const bool scaled = precond && A->sscaled == 1;
+  // the FP32 values, for the recursion on A itself (made here when the
+  // matrix's assembly did not write them)
+  if (!scaled) ensure_sell(ctx, A, true, precond);
   const double *ssc = scaled ? A->sscale.get() : nullptr;
   const float *dinv32 = precond && !scaled ? A->sdinv32.get() : nullptr;
   const float *val32 = A->sval32.get();
@@ -2322,7 +2329,9 @@ void cg_solve(gs_ctx *ctx, const gs_csr *Ac, const double *b,
   const bool precond = o.use_preconditioner != 0;
   if (A->spmv_flags & GS_SPMV_CSR)
     throw Error(GS_ERR_INVALID_ARG, "CG runs on the SELL engine only");
-  ensure_sell(ctx, A, mixed, precond);
+  // FP64 values (and Jacobi diagonal); the mixed solve makes its FP32 values
+  // only if its recursion runs on A itself (not on the scaled operator)
+  ensure_sell(ctx, A, mixed && sell_block(A) == 3, precond);
   if (!mixed) {
     char *base = ctx_scratch(ctx, fp64_layout(n).bytes);
     fill_stats(fp64_solve(ctx, A, b, o, x, base, precond), t0, st);

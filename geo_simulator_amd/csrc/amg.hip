@@ -10,16 +10,18 @@
 // dropped.  The coarsening is sequential by definition (each choice changes
 // the counts the next one reads), so the hierarchy is built on the host
 // from A in the caller's numbering, with loops that give the reference's
-// result at O(nnz log n) instead of its O(n^2) scans: a max-heap keyed by
-// (count, smallest index) with stale entries skipped stands in for the scan
-// for the largest count, and a sparse accumulator for the dense column loop
-// of matrix_multiply (every product reaches its sum in the same order).
+// result at O(nnz log n) instead of its O(n^2) scans: buckets by count of
+// index min-heaps, stale entries skipped, stand in for the scan for the
+// largest count (smallest index first), and a sparse accumulator for the
+// dense column loop of matrix_multiply (every product reaches its sum in the
+// same order).  Rows of the strength, prolongation and Galerkin matrices are
+// built on host threads, each row by one thread.
 //
 // V-cycle (:132-185) on the device, one thread per row (operators and
 // prolongations sliced by 64 rows, a slice's lanes loading their rows'
-// entries side by side): weighted Jacobi
-// (omega 0.67) twice, the residual, restriction by P^T, the coarser cycle,
-// prolongation by P, weighted Jacobi twice.  The last level runs the
+// entries side by side): weighted Jacobi (omega 0.67) twice, the residual,
+// restriction by P^T, the coarser cycle, prolongation by P, weighted Jacobi
+// twice.  The last level runs the
 // reference's ten in-place sweeps (Gauss-Seidel: rows in order, each reading
 // the entries already updated): rows that read no other row and that no
 // other row reads take their ten updates at once, one thread each; the
@@ -30,12 +32,11 @@
 // Every row sum starts at 0.0 and adds in ascending column order, compiled
 // without contraction, so the V-cycle equals the reference's bit for bit.
 #include <algorithm>
+#include <cmath>
 #include <functional>
 #include <memory>
-#include <thread>
-#include <cmath>
-#include <queue>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"

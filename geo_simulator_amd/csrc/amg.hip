@@ -383,8 +383,16 @@ DevCsr upload(gs_ctx *ctx, const HCsr &A) {
       ev[at] = A.val[p];
     }
   }
+  // rows longer than a slice is wide: when they are few and lead the sorted
+  // order, the slices that hold them go to one wavefront per row
+  if (!pos.empty()) {
+    int64_t nl = 0;
+    while (nl < n && len(pos[nl]) > 64) ++nl;
+    nl = std::min<int64_t>(n, (nl + 63) / 64 * 64);
+    if (nl > 0 && nl * 8 <= n) D.n_long = nl;
+  }
   int64_t longest = 0;
-  for (int64_t i = 0; i < n; ++i) longest = std::max(longest, len(i));
+  for (int64_t q = D.n_long; q < n; ++q) longest = std::max(longest, len(row_at(q)));
   D.ew = longest > 72 ? 8 : 4;
   D.ecb = to_device(ctx, cb);
   D.ecol = to_device(ctx, ec);
@@ -480,17 +488,66 @@ __global__ void k_amg_zero(int64_t n, double *__restrict__ x) {
   if (i < n) x[i] = 0.0;
 }
 
-// weighted_jacobi_step (:250-263): xo_i = xi_i + omega (r_i - (A xi)_i) dinv_i
-template <int W>
-__global__ void k_amg_jacobi(int64_t n, const Ell A, const double *__restrict__ dinv,
-                             const double *__restrict__ r, const double *__restrict__ xi,
-                             double *__restrict__ xo) {
-  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+// The V-cycle's row operations, each on (M x)_i of one operator M:
+//   JACOBI    weighted_jacobi_step (:250-263): xo_i = xi_i + omega (r_i - ax) dinv_i
+//   RESIDUAL  res_i = r_i - ax (x = xi)
+//   SPMV      y_i = ax (matvec, :230-247; restriction by P^T)
+//   PROLONG   x_i += (P e)_i, the correction formed first, then added (:168-175)
+enum AmgOp { OP_JACOBI, OP_RESIDUAL, OP_SPMV, OP_PROLONG };
+struct OpArgs {
+  const double *dinv, *r, *xi;  // xi: the vector M multiplies
+  double *xo;                   // the output (PROLONG: x, updated in place)
+};
+
+template <int OP>
+__device__ __forceinline__ void amg_epilogue(int64_t i, double ax, const OpArgs &a) {
+  if constexpr (OP == OP_JACOBI) {
+    const double t = AMG_OMEGA * (a.r[i] - ax);
+    a.xo[i] = a.xi[i] + t * a.dinv[i];
+  } else if constexpr (OP == OP_RESIDUAL) {
+    a.xo[i] = a.r[i] - ax;
+  } else if constexpr (OP == OP_SPMV) {
+    a.xo[i] = ax;
+  } else {
+    a.xo[i] = a.xo[i] + ax;
+  }
+}
+
+// positions [q0, n) of the slices, a lane per row
+template <int OP, int W>
+__global__ void k_amg_ell(int64_t q0, int64_t n, const Ell M, const OpArgs a) {
+  const int64_t q = q0 + (int64_t)blockIdx.x * BS + threadIdx.x;
   if (q >= n) return;
-  const int64_t i = ell_row(A, q);
-  const double ax = ell_sum<W>(A, q, xi);
-  const double t = AMG_OMEGA * (r[i] - ax);
-  xo[i] = xi[i] + t * dinv[i];
+  amg_epilogue<OP>(ell_row(M, q), ell_sum<W>(M, q, a.xi), a);
+}
+
+// The rows at positions [0, gridDim.x), longer than a slice is wide, a
+// wavefront each: the lanes form 64 products at a time side by side (the
+// row's CSR entries are contiguous) and lane 0 adds them in order, so the sum
+// is the same as a lane's, with one round trip per 64 entries instead of one
+// per W.
+template <int OP>
+__global__ __launch_bounds__(64) void k_amg_long(const int *__restrict__ pos,
+                                                 const int *__restrict__ rp,
+                                                 const int *__restrict__ col,
+                                                 const double *__restrict__ val,
+                                                 const OpArgs a) {
+  __shared__ double prod[64];
+  const int lane = threadIdx.x;
+  const int64_t i = pos[blockIdx.x];
+  const int b = rp[i], e = rp[i + 1];
+  double sum = 0.0;
+  for (int k0 = b; k0 < e; k0 += 64) {
+    const int k = k0 + lane;
+    prod[lane] = k < e ? val[k] * a.xi[col[k]] : 0.0;
+    __syncthreads();
+    if (lane == 0) {
+      const int m = min(64, e - k0);
+      for (int u = 0; u < m; ++u) sum = sum + prod[u];
+    }
+    __syncthreads();
+  }
+  if (lane == 0) amg_epilogue<OP>(i, sum, a);
 }
 
 // The first step from x = 0 with every operator entry finite: (A x)_i is
@@ -504,33 +561,20 @@ __global__ void k_amg_jacobi_zero(int64_t n, const double *__restrict__ dinv,
   xo[i] = 0.0 + t * dinv[i];
 }
 
-// res_i = r_i - (A x)_i
-template <int W>
-__global__ void k_amg_residual(int64_t n, const Ell A, const double *__restrict__ r,
-                               const double *__restrict__ x, double *__restrict__ res) {
-  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (q >= n) return;
-  const int64_t i = ell_row(A, q);
-  res[i] = r[i] - ell_sum<W>(A, q, x);
-}
-
-// y = M x (matvec, :230-247)
-template <int W>
-__global__ void k_amg_spmv(int64_t n, const Ell M, const double *__restrict__ x,
-                           double *__restrict__ y) {
-  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (q < n) y[ell_row(M, q)] = ell_sum<W>(M, q, x);
-}
-
-// x_i += (P e)_i  (the correction formed first, then added, :168-175)
-template <int W>
-__global__ void k_amg_prolong(int64_t n, const Ell P, const double *__restrict__ e,
-                              double *__restrict__ x) {
-  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
-  if (q >= n) return;
-  const int64_t i = ell_row(P, q);
-  const double c = ell_sum<W>(P, q, e);
-  x[i] = x[i] + c;
+// One row operation over every row of M.
+template <int OP>
+void amg_op(hipStream_t s, const DevCsr &M, const OpArgs &a) {
+  const int64_t n = M.rows;
+  if (M.n_long > 0)
+    hipLaunchKernelGGL(k_amg_long<OP>, (unsigned)M.n_long, 64, 0, s, M.erow.get(), M.rp.get(),
+                       M.col.get(), M.val.get(), a);
+  if (n > M.n_long) {
+    const unsigned g = nblk(n - M.n_long);
+    if (M.ew == 8)
+      hipLaunchKernelGGL((k_amg_ell<OP, 8>), g, BS, 0, s, M.n_long, n, ell(M), a);
+    else
+      hipLaunchKernelGGL((k_amg_ell<OP, 4>), g, BS, 0, s, M.n_long, n, ell(M), a);
+  }
 }
 
 // The last level's sweeps of its isolated rows: each row's ten updates at
@@ -663,32 +707,19 @@ static void vcycle(const gs_amg *M, size_t l, const double *r, double *x) {
       double *xo = (k & 1) ? x : t;
       if (k == 0 && from_zero && L.finite)
         hipLaunchKernelGGL(k_amg_jacobi_zero, g, BS, 0, s, n, dinv, r, xo);
-      else if (L.A.ew == 8)
-        hipLaunchKernelGGL(k_amg_jacobi<8>, g, BS, 0, s, n, ell(L.A), dinv, r, xi, xo);
       else
-        hipLaunchKernelGGL(k_amg_jacobi<4>, g, BS, 0, s, n, ell(L.A), dinv, r, xi, xo);
+        amg_op<OP_JACOBI>(s, L.A, OpArgs{dinv, r, xi, xo});
     }
   };
   static_assert(AMG_SMOOTH % 2 == 0, "the smoothing steps end in x");
   if (!L.finite) hipLaunchKernelGGL(k_amg_zero, g, BS, 0, s, n, x);
   smooth(true);
-  if (L.A.ew == 8)
-    hipLaunchKernelGGL(k_amg_residual<8>, g, BS, 0, s, n, ell(L.A), r, x, t);
-  else
-    hipLaunchKernelGGL(k_amg_residual<4>, g, BS, 0, s, n, ell(L.A), r, x, t);
+  amg_op<OP_RESIDUAL>(s, L.A, OpArgs{nullptr, r, x, t});
   const AmgLevel &C = M->lv[l + 1];
-  if (L.PT.ew == 8)
-    hipLaunchKernelGGL(k_amg_spmv<8>, nblk(L.coarse), BS, 0, s, L.coarse, ell(L.PT), t,
-                       C.r.get());
-  else
-    hipLaunchKernelGGL(k_amg_spmv<4>, nblk(L.coarse), BS, 0, s, L.coarse, ell(L.PT), t,
-                       C.r.get());
+  amg_op<OP_SPMV>(s, L.PT, OpArgs{nullptr, nullptr, t, C.r.get()});
   GS_LAUNCH_CHECK("k_amg_spmv");
   vcycle(M, l + 1, C.r.get(), C.x.get());
-  if (L.P.ew == 8)
-    hipLaunchKernelGGL(k_amg_prolong<8>, g, BS, 0, s, n, ell(L.P), C.x.get(), x);
-  else
-    hipLaunchKernelGGL(k_amg_prolong<4>, g, BS, 0, s, n, ell(L.P), C.x.get(), x);
+  amg_op<OP_PROLONG>(s, L.P, OpArgs{nullptr, nullptr, C.x.get(), x});
   smooth(false);
   GS_LAUNCH_CHECK("k_amg_jacobi");
 }

@@ -246,6 +246,7 @@ struct DevCsr {
   DArr<int> ecb, ecol, erow;
   DArr<double> eval;
   int ew = 4;  // entries per trip of the row sums (8 where rows are long)
+  int64_t n_long = 0;  // positions [0, n_long): rows summed a wavefront each
 };
 struct AmgLevel {
   DevCsr A, P, PT;      // P and PT empty where the level has none

@@ -613,6 +613,52 @@ __global__ __launch_bounds__(GS_NT) void k_amg_sweeps(
   }
 }
 
+// The same sweeps with the coupled rows staged in LDS (when they fit): their
+// operator rows in a local numbering that keeps the global order (so each row
+// still sums in ascending column), x, r and the inverse diagonal; every read
+// of a level step then stays on the CU.  Dynamic LDS, laid out as: x, r, dinv
+// (m doubles each), values (nnz doubles), row pointers (m + 1), columns (nnz),
+// level order (m) and level offsets (nlev + 1) as ints.
+__global__ __launch_bounds__(GS_NT) void k_amg_sweeps_lds(
+    int m, int nnz, int nlev, const int *__restrict__ grow, const int *__restrict__ lrp,
+    const int *__restrict__ lcol, const double *__restrict__ lval,
+    const int *__restrict__ lord, const int *__restrict__ lptr,
+    const double *__restrict__ dinv, const double *__restrict__ r, double *__restrict__ x) {
+  extern __shared__ double lds[];
+  double *sx = lds, *sr = sx + m, *sd = sr + m, *sv = sd + m;
+  int *srp = (int *)(sv + nnz);
+  int *sc = srp + m + 1, *so = sc + nnz, *sp = so + m;
+  for (int k = threadIdx.x; k < m; k += GS_NT) {
+    const int g = grow[k];
+    sx[k] = x[g];
+    sr[k] = r[g];
+    sd[k] = dinv[g];
+    so[k] = lord[k];
+  }
+  for (int k = threadIdx.x; k <= m; k += GS_NT) srp[k] = lrp[k];
+  for (int k = threadIdx.x; k < nnz; k += GS_NT) {
+    sc[k] = lcol[k];
+    sv[k] = lval[k];
+  }
+  for (int k = threadIdx.x; k <= nlev; k += GS_NT) sp[k] = lptr[k];
+  __syncthreads();
+  for (int s = 0; s < AMG_SWEEPS; ++s) {
+    for (int l = 0; l < nlev; ++l) {
+      for (int t = sp[l] + threadIdx.x; t < sp[l + 1]; t += GS_NT) {
+        const int i = so[t];
+        double ax = 0.0;
+        for (int p = srp[i]; p < srp[i + 1]; ++p) ax += sv[p] * sx[sc[p]];
+        const double d = (sr[i] - ax) * sd[i];
+        sx[i] = sx[i] + d;
+      }
+      __syncthreads();
+    }
+  }
+  for (int k = threadIdx.x; k < m; k += GS_NT) x[grow[k]] = sx[k];
+}
+
+constexpr size_t AMG_SWEEP_LDS = 64 << 10;  // bytes of the staged sweeps' LDS
+
 }  // namespace
 
 gs_amg *amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels, int64_t coarse_size,
@@ -663,6 +709,39 @@ gs_amg *amg_create(gs_ctx *ctx, const gs_csr *A, int64_t max_levels, int64_t coa
       L.nlev = (int)ptr.size() - 1;
       L.lptr = to_device(ctx, ptr);
       L.ord = to_device(ctx, ord);
+      // the coupled rows in a local numbering (ascending global row), for the
+      // LDS-staged sweeps when they fit
+      const int64_t m = (int64_t)ord.size();
+      std::vector<int> grow(ord.begin(), ord.end());
+      std::sort(grow.begin(), grow.end());
+      std::vector<int> loc((size_t)n, -1);
+      for (int64_t k = 0; k < m; ++k) loc[grow[k]] = (int)k;
+      std::vector<int> lrp((size_t)m + 1, 0), lcol, lord((size_t)m);
+      std::vector<double> lval;
+      for (int64_t k = 0; k < m; ++k) {
+        const int64_t i = grow[k];
+        for (int64_t p = cur.rp[i]; p < cur.rp[i + 1]; ++p) {
+          lcol.push_back(loc[cur.col[p]]);
+          lval.push_back(cur.val[p]);
+        }
+        lrp[k + 1] = (int)lcol.size();
+      }
+      for (int64_t t = 0; t < m; ++t) lord[t] = loc[ord[t]];
+      const int64_t nz = (int64_t)lcol.size();
+      const size_t bytes = (size_t)(3 * m + nz) * sizeof(double) +
+                           (size_t)((m + 1) + nz + m + (L.nlev + 1)) * sizeof(int);
+      // (a coupled row reads only coupled rows: a column j != i stored in row
+      // i makes j read by another row; checked here all the same)
+      const bool closed = std::all_of(lcol.begin(), lcol.end(), [](int c) { return c >= 0; });
+      if (m > 0 && closed && bytes <= AMG_SWEEP_LDS) {
+        L.sweep_lds = bytes;
+        L.lnnz = nz;
+        L.grow = to_device(ctx, grow);
+        L.lrp = to_device(ctx, lrp);
+        L.lcol = to_device(ctx, lcol);
+        L.lval = to_device(ctx, lval);
+        L.lord = to_device(ctx, lord);
+      }
     }
     if (!last) {
       const HCsr PT = transpose(P);
@@ -691,7 +770,11 @@ static void vcycle(const gs_amg *M, size_t l, const double *r, double *x) {
     if (L.n_iso > 0)
       hipLaunchKernelGGL(k_amg_sweeps_iso, nblk(L.n_iso), BS, 0, s, L.n_iso, L.iso.get(), rp,
                          col, val, dinv, r, x);
-    if (L.nlev > 0)
+    if (L.nlev > 0 && L.sweep_lds > 0)
+      hipLaunchKernelGGL(k_amg_sweeps_lds, 1, GS_NT, L.sweep_lds, s, (int)L.ord.n, (int)L.lnnz,
+                         L.nlev, L.grow.get(), L.lrp.get(), L.lcol.get(), L.lval.get(),
+                         L.lord.get(), L.lptr.get(), dinv, r, x);
+    else if (L.nlev > 0)
       hipLaunchKernelGGL(k_amg_sweeps, 1, GS_NT, 0, s, L.nlev, L.lptr.get(), L.ord.get(), rp,
                          col, val, dinv, r, x);
     GS_LAUNCH_CHECK("k_amg_sweeps");

@@ -260,6 +260,12 @@ struct AmgLevel {
   DArr<int> iso;
   int nlev = 0;
   DArr<int> lptr, ord;
+  // the coupled rows staged in LDS (sweep_lds bytes, 0: not staged): global
+  // row of each local row, local CSR and level order
+  size_t sweep_lds = 0;
+  int64_t lnnz = 0;
+  DArr<int> grow, lrp, lcol, lord;
+  DArr<double> lval;
 };
 }  // namespace gs
 

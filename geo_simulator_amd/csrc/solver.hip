@@ -72,6 +72,15 @@ struct CgState {
   long long iteration, max_iter;
   int stop, converged, pause, mixed;
   int brk_rel;   // break down only on p.Ap == 0 or a non-finite step
+  // flexible CG (mixed precision), outer steps decided on the device
+  // (k_fcg_decide, k_fcg_decide2): the outer residual norm after the last
+  // step, the residual the recursion predicted for it, the steps taken and
+  // the true-residual checks run, the next recursion's restart flag, a
+  // pending check, and how the solve ended (1 converged, 2 no descent left;
+  // stop is set with it).  Only those kernels set stop in a mixed solve.
+  double outer_rn, outer_pred;
+  long long outer_steps, outer_checks;
+  int outer_restart, outer_check, outer_end;
 };
 
 __device__ __forceinline__ bool active(const CgState *st, long long it) {
@@ -172,7 +181,8 @@ __global__ __launch_bounds__(BS) void k_sell_residual(
     int n, int ns, const int *__restrict__ cb, const IDX idx,
     const double *__restrict__ val, const double *__restrict__ x,
     const double *__restrict__ b, double *__restrict__ r,
-    double *__restrict__ part) {
+    double *__restrict__ part, const int *__restrict__ gate) {
+  if (gate && *gate == 0) return;  // a check the device did not ask for
   double acc[1] = {0.0};
   auto epi = [&](int row, double s) {
     const double ri = b[row] - s;
@@ -525,18 +535,22 @@ __global__ __launch_bounds__(BS) void k_fcg_start(int n, const double *__restric
 
 // Inner FP32 recursion (re)started from the FP64 residual: rs = r/||b||,
 // zs = D^-1 rs [and p = zs on a restart]; partials rs.zs
+// With dev (the device's outer-step decision), nothing once the solve has
+// stopped, and the restart flag from the state instead of RESTART.
 template <bool PREC, bool RESTART>
 __global__ __launch_bounds__(BS) void k_fcg_inner(
     int n, const double *__restrict__ r, double scale,
     const float *__restrict__ dinv, float *__restrict__ rs, float *__restrict__ zs,
-    float *__restrict__ p, double *__restrict__ part) {
+    float *__restrict__ p, double *__restrict__ part, const CgState *__restrict__ dev) {
+  if (dev && dev->stop) return;
+  const bool restart = dev ? dev->outer_restart != 0 : RESTART;
   double acc[1] = {0.0};
   for (int i = blockIdx.x * BS + threadIdx.x; i < n; i += gridDim.x * BS) {
     const float ri = (float)(r[i] * scale);
     rs[i] = ri;
     const float zi = PREC ? ri * dinv[i] : ri;
     zs[i] = zi;
-    if (RESTART) p[i] = zi;
+    if (restart) p[i] = zi;
     acc[0] += (double)ri * (double)zi;
   }
   put_partials<1>(acc, part);
@@ -546,7 +560,8 @@ __global__ __launch_bounds__(BS) void k_fcg_inner(
 // workgroup of FBS threads (the layout of k_cg2_step's partials).
 __global__ __launch_bounds__(FBS) void k_fcg_inner_scaled(
     int n, const double *__restrict__ r, double scale, const double *__restrict__ ssc,
-    float *__restrict__ rs, double *__restrict__ part) {
+    float *__restrict__ rs, double *__restrict__ part, const CgState *__restrict__ dev) {
+  if (dev && dev->stop) return;  // the device ended the solve
   double acc[1] = {0.0};
   for (int i = blockIdx.x * FBS + threadIdx.x; i < n; i += gridDim.x * FBS) {
     const float ri = (float)(ssc[i] * r[i] * scale);
@@ -564,10 +579,11 @@ __global__ __launch_bounds__(FBS) void k_fcg_inner_scaled(
 __global__ __launch_bounds__(FBS) void k_fin_inner(CgState *__restrict__ st,
                                                    const double *__restrict__ part,
                                                    int np, double rbase, double tol_ratio,
-                                                   int restart) {
+                                                   int restart, double outer_rn) {
   double tot[1];
   sum_partials<1>(part, np, tot);
   if (threadIdx.x != 0) return;
+  st->outer_rn = outer_rn;
   st->rz[st->iteration & 1] = tot[0];
   if (rbase < 0.0) {
     rbase = sqrt(tot[0]);
@@ -577,6 +593,73 @@ __global__ __launch_bounds__(FBS) void k_fin_inner(CgState *__restrict__ st,
   st->rbase = rbase;
   st->rnorm = rbase;
   st->pause = 0;
+}
+
+// k_fin_inner on the device's outer-step decision: the outer residual and
+// the restart flag come from the state (rbase = rn scale, or the scaled
+// recursion's own sqrt(r.r) with the pause threshold target / rn of it);
+// nothing once the solve has stopped.
+__global__ __launch_bounds__(FBS) void k_fin_inner_dev(CgState *__restrict__ st,
+                                                       const double *__restrict__ part, int np,
+                                                       int scaled, double scale,
+                                                       double target) {
+  if (st->stop) return;
+  double tot[1];
+  sum_partials<1>(part, np, tot);
+  if (threadIdx.x != 0) return;
+  const double rn = st->outer_rn;
+  st->rz[st->iteration & 1] = tot[0];
+  double rbase = rn * scale;
+  if (scaled) {
+    rbase = sqrt(tot[0]);
+    st->tol = rbase * (target / rn);
+  }
+  st->fresh_it = st->outer_restart ? st->iteration : -1;
+  st->rbase = rbase;
+  st->rnorm = rbase;
+  st->pause = 0;
+}
+
+// The outer step's decision on the device, first half (one thread), after
+// the step's scalars sc (gamma, alpha, p.Ap, taken, r.r): no descent left
+// ends the solve; else the new outer residual, the residual the recursion
+// predicted for it (its own reduction applied to the previous outer
+// residual), and whether to confirm convergence on b - A x.
+__global__ void k_fcg_decide(CgState *__restrict__ st, const double *__restrict__ sc,
+                             double target) {
+  if (sc[3] == 0.0) {
+    st->outer_end = 2;
+    st->stop = 1;
+    return;
+  }
+  st->outer_pred = st->rbase > 0.0 ? st->outer_rn * (st->rnorm / st->rbase) : 0.0;
+  const double rn = sqrt(sc[4]);
+  st->outer_rn = rn;
+  st->outer_steps += 1;
+  st->outer_check = rn < target ? 1 : 0;
+}
+
+// Second half, after the gated check (dots[0] = |b - A x|^2 when it ran):
+// converged ends the solve; else the recursion keeps its direction when the
+// step agreed with it (alpha near 1, the new residual near the predicted one,
+// a positive curvature) and the host saw no stall, and restarts otherwise.
+__global__ void k_fcg_decide2(CgState *__restrict__ st, const double *__restrict__ sc,
+                              const double *__restrict__ dots, double target, int stalled) {
+  if (st->stop) return;
+  double rn = st->outer_rn;
+  if (st->outer_check) {
+    st->outer_checks += 1;
+    rn = sqrt(dots[0]);
+    st->outer_rn = rn;
+    if (rn < target) {
+      st->outer_end = 1;
+      st->stop = 1;
+      return;
+    }
+  }
+  const bool agree = !stalled && fabs(sc[1] - 1.0) < 0.25 && rn < 2.0 * st->outer_pred &&
+                     !(st->pq == 0.0);
+  st->outer_restart = agree ? 0 : 1;
 }
 
 // d = ||b|| xs (the inner correction in FP64 and in units of b; S xs for
@@ -887,11 +970,14 @@ void launch_sell_spmv(hipStream_t s, const SellOp &o, const T *val, const T *x,
   }, sizeof(T));
 }
 
+// r = b - A x and partials r.r; with gate, only when *gate != 0 (a device
+// decision; the partials are then left as they were)
 void launch_residual(hipStream_t s, const SellOp &o, const double *val,
-                     const double *x, const double *b, double *r, double *part) {
+                     const double *x, const double *b, double *r, double *part,
+                     const int *gate = nullptr) {
   with_idx(*o.P, [&](auto idx) {
     hipLaunchKernelGGL((k_sell_residual<decltype(idx)>), o.grid, BS, 0, s, o.n(),
-                       o.ns(), o.P->cb.get(), idx, val, x, b, r, part);
+                       o.ns(), o.P->cb.get(), idx, val, x, b, r, part, gate);
   }, 8);
 }
 
@@ -1218,34 +1304,43 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
                      0.0, 1, itol, 0);
   hipLaunchKernelGGL((k_zero<float>), gv, BS, 0, s, n, xs);
   const unsigned gf = grid_fat(n);
-  auto start_inner = [&](bool restart, double rn) {
+  // (Re)start the recursion from the outer residual r.  Host path: restart
+  // and the outer residual rn given; device path (dev): both taken from the
+  // state, where k_fcg_decide2 left them, and nothing runs once the device
+  // has ended the solve.
+  auto start_inner = [&](bool restart, double rn, bool dev) {
+    const CgState *gate = dev ? iw.st : nullptr;
     if (scaled) {
       hipLaunchKernelGGL(k_fcg_inner_scaled, gf, FBS, 0, s, n, r, scale, ssc, iw.r,
-                         iw.part2);
+                         iw.part2, gate);
     } else if (dinv32) {
-      if (restart)
+      if (restart && !dev)
         hipLaunchKernelGGL((k_fcg_inner<true, true>), gv, BS, 0, s, n, r, scale,
-                           dinv32, iw.r, iw.z, iw.p, iw.part2);
+                           dinv32, iw.r, iw.z, iw.p, iw.part2, gate);
       else
         hipLaunchKernelGGL((k_fcg_inner<true, false>), gv, BS, 0, s, n, r, scale,
-                           dinv32, iw.r, iw.z, iw.p, iw.part2);
+                           dinv32, iw.r, iw.z, iw.p, iw.part2, gate);
     } else {
-      if (restart)
+      if (restart && !dev)
         hipLaunchKernelGGL((k_fcg_inner<false, true>), gv, BS, 0, s, n, r, scale,
-                           dinv32, iw.r, iw.z, iw.p, iw.part2);
+                           dinv32, iw.r, iw.z, iw.p, iw.part2, gate);
       else
         hipLaunchKernelGGL((k_fcg_inner<false, false>), gv, BS, 0, s, n, r, scale,
-                           dinv32, iw.r, iw.z, iw.p, iw.part2);
+                           dinv32, iw.r, iw.z, iw.p, iw.part2, gate);
     }
     // the scaled recursion measures its residual in its own norm ||S r||:
     // its base is its r.r and its target the same fraction of it as the
     // outer target is of ||r||
-    hipLaunchKernelGGL(k_fin_inner, 1, FBS, 0, s, iw.st, iw.part2,
-                       (int)(scaled ? gf : gv), scaled ? -1.0 : rn * scale, target / rn,
-                       restart ? 1 : 0);
+    if (dev)
+      hipLaunchKernelGGL(k_fin_inner_dev, 1, FBS, 0, s, iw.st, iw.part2,
+                         (int)(scaled ? gf : gv), scaled ? 1 : 0, scale, target);
+    else
+      hipLaunchKernelGGL(k_fin_inner, 1, FBS, 0, s, iw.st, iw.part2,
+                         (int)(scaled ? gf : gv), scaled ? -1.0 : rn * scale, target / rn,
+                         restart ? 1 : 0, rn);
     GS_LAUNCH_CHECK("k_fcg_inner");
   };
-  start_inner(true, bnorm);
+  start_inner(true, bnorm, false);
   double rn = bnorm;
   double *const sc = dots + 2;  // the outer step's device scalars
   bool first = true, checked = false;
@@ -1255,6 +1350,10 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
   // max(200, 2 x the longest cycle so far) passes has stalled on a stale
   // direction: it is cut short and the recursion restarts.
   long long longest = 0, cycle_start = 0;
+  // the cycle was restarted by the device's decision: its state is not read
+  // before its first batch (enqueued blind, Batcher's first size)
+  bool blind = false;
+  long long checks_seen = 0;
   while (res.outer < max_outer) {
     // FP32 passes until the recursion pauses, in pipelined batches (as in
     // pcg_core); the passes enqueued never go past the cycle's limits (the
@@ -1265,6 +1364,35 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
     long long best_at = hs.iteration;
     const long long it_begin = hs.iteration;
     long long next_it = hs.iteration;
+    auto enqueue_batch = [&](long long batch) {
+      const long long snap_after = batch > SNAP_TAIL ? batch - SNAP_TAIL : batch;
+      for (long long i = 0; i < batch; ++i) {
+        if (scaled)
+          enqueue_pass2<float, int>(s, o, valq, xs, iw.r, iw.p, iw.q, iw.st, next_it + i,
+                                    iw.part1, iw.part2);
+        else
+          enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
+                              next_it + i, iw.part1, iw.part2);
+        if (i + 1 == snap_after) snap_enqueue(ctx, iw.st);
+      }
+      GS_LAUNCH_CHECK("cg batch");
+      next_it += batch;
+      hs = snap_wait(ctx);
+    };
+    if (blind) {
+      blind = false;
+      long long first_batch = std::min(16ll, max_iter + 1 - next_it);
+      if (opt.max_inner_iterations > 0)
+        first_batch = std::min(first_batch, cycle_start + opt.max_inner_iterations - next_it);
+      if (first_batch > 0) {
+        enqueue_batch(first_batch);
+        // the cycle's starting residual (k_fin_inner_dev set rnorm = rbase),
+        // also the Batcher's first point
+        best = hs.rbase;
+        best_at = it_begin;
+        bt.next(hs.rbase, it_begin, 0.0, 1);
+      }
+    }
     while (!hs.pause && !hs.stop) {
       const long long window = std::max(200ll, 2 * longest);
       const bool fresh = next_it == hs.iteration;
@@ -1291,25 +1419,28 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
         next_it = std::max(next_it, hs.iteration);
         continue;
       }
-      const long long batch = std::min(need - ahead, limit - next_it);
-      const long long snap_after = batch > SNAP_TAIL ? batch - SNAP_TAIL : batch;
-      for (long long i = 0; i < batch; ++i) {
-        if (scaled)
-          enqueue_pass2<float, int>(s, o, valq, xs, iw.r, iw.p, iw.q, iw.st, next_it + i,
-                                    iw.part1, iw.part2);
-        else
-          enqueue_pass<float>(s, o, val32, xs, iw.r, iw.z, iw.p, iw.q, dinv32, iw.st,
-                              next_it + i, iw.part1, iw.part2);
-        if (i + 1 == snap_after) snap_enqueue(ctx, iw.st);
-      }
-      GS_LAUNCH_CHECK("cg batch");
-      next_it += batch;
-      hs = snap_wait(ctx);
+      enqueue_batch(std::min(need - ahead, limit - next_it));
     }
     if (next_it > hs.iteration) hs = read_state(ctx, iw.st);  // passes still in flight
     res.spmv += hs.iteration - it_begin;
+    res.spmv += hs.outer_checks - checks_seen;  // true-residual checks the device ran
+    checks_seen = hs.outer_checks;
     longest = std::max(longest, hs.iteration - cycle_start);
     res.iterations = hs.iteration;
+    if (hs.outer_end != 0) {
+      // the device ended the solve at its last outer step (taken on its
+      // decision): converged on the true residual, or no descent left
+      rn = hs.outer_rn;
+      if (hs.outer_end == 1) {
+        res.converged = true;
+        checked = true;
+      } else {
+        --res.outer;  // the step that found no descent was counted
+        checked = false;
+      }
+      break;
+    }
+    rn = hs.outer_rn;  // the outer residual this cycle started from
     // the residual the recursion predicts: its own reduction applied to the
     // outer residual it started from
     const double predicted = hs.rbase > 0.0 ? rn * (hs.rnorm / hs.rbase) : 0.0;
@@ -1328,6 +1459,30 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
     hipLaunchKernelGGL(k_fcg_update, gv, BS, 0, s, n, sc, p, ap, x, r, iw.part2);
     hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part2, (int)gv, sc + 4);
     GS_LAUNCH_CHECK("k_fcg_update");
+    // A step the recursion predicts will not reach the target, with passes
+    // and outer steps left, is decided on the device and the next cycle
+    // starts without a host round trip; the host learns the outcome from the
+    // next cycle's first snapshot.  A step that may converge or end the
+    // solve is decided here, on the scalars read back.
+    const bool dev = !(predicted < 4.0 * target) && hs.iteration < max_iter &&
+                     res.outer + 1 < max_outer;
+    if (dev) {
+      hipLaunchKernelGGL(k_fcg_decide, 1, 1, 0, s, iw.st, sc, target);
+      launch_residual(s, o, val, x, b, r, iw.part1, &iw.st->outer_check);
+      hipLaunchKernelGGL((k_fin_sum<1>), 1, FBS, 0, s, iw.part1, o.grid, dots);
+      hipLaunchKernelGGL(k_fcg_decide2, 1, 1, 0, s, iw.st, sc, dots, target,
+                         stalled ? 1 : 0);
+      GS_LAUNCH_CHECK("k_fcg_decide");
+      start_inner(false, 0.0, true);
+      first = false;
+      ++res.outer;
+      checked = false;
+      bt = Batcher{};
+      cycle_start = hs.iteration;
+      hs.pause = 0;  // as k_fin_inner_dev leaves it, unless the solve ended
+      blind = true;
+      continue;
+    }
     double hs4[4];  // alpha, p.Ap, step taken, r.r
     read_dots(ctx, sc + 1, hs4, 4);
     if (hs4[2] == 0.0) break;  // no descent left
@@ -1353,7 +1508,7 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
     if (hs.iteration >= max_iter) break;
     const bool agree = !stalled && std::fabs(alpha - 1.0) < 0.25 &&
                        rn < 2.0 * predicted && !(hs.pq == 0.0);
-    start_inner(!agree, rn);
+    start_inner(!agree, rn, false);
     bt = Batcher{};
     hs = read_state(ctx, iw.st);
     cycle_start = hs.iteration;

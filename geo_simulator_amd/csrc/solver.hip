@@ -1345,14 +1345,15 @@ PcgResult pcg_fcg(gs_ctx *ctx, gs_csr *A, const double *b, double *x,
   double *const sc = dots + 2;  // the outer step's device scalars
   bool first = true, checked = false;
   Batcher bt;
-  CgState hs = read_state(ctx, iw.st);
+  CgState hs{};  // pass 0, running: the state k_fin_inner leaves
   // A cycle whose recursion has not halved its best residual within
   // max(200, 2 x the longest cycle so far) passes has stalled on a stale
   // direction: it is cut short and the recursion restarts.
   long long longest = 0, cycle_start = 0;
-  // the cycle was restarted by the device's decision: its state is not read
-  // before its first batch (enqueued blind, Batcher's first size)
-  bool blind = false;
+  // the cycle was (re)started without the host reading its state (the
+  // first one, and those the device's outer-step decision restarted): its
+  // first batch is enqueued blind, at the Batcher's first size
+  bool blind = true;
   long long checks_seen = 0;
   while (res.outer < max_outer) {
     // FP32 passes until the recursion pauses, in pipelined batches (as in

@@ -18,17 +18,21 @@
 // built on host threads, each row by one thread.
 //
 // V-cycle (:132-185) on the device, one thread per row (operators and
-// prolongations sliced by 64 rows, a slice's lanes loading their rows'
-// entries side by side): weighted Jacobi (omega 0.67) twice, the residual,
-// restriction by P^T, the coarser cycle, prolongation by P, weighted Jacobi
-// twice.  The last level runs the
-// reference's ten in-place sweeps (Gauss-Seidel: rows in order, each reading
-// the entries already updated): rows that read no other row and that no
-// other row reads take their ten updates at once, one thread each; the
-// others run in one workgroup, grouped in dependency levels: row i comes
-// after every row j < i that it reads (its new value) and after every row
-// k < i that reads x_i (the old value), so rows of one level neither read
-// nor overwrite each other's entries and run at once.
+// prolongations sliced by 64 rows, rows sorted by length where that saves
+// padding, a slice's lanes loading their rows' entries side by side; the few
+// rows longer than 64 entries take a wavefront each, whose lanes form the
+// products side by side for one lane to add in order): weighted Jacobi
+// (omega 0.67) twice, the first step from x = 0 without the operator, the
+// residual, restriction by P^T, the coarser cycle, prolongation by P,
+// weighted Jacobi twice.  The last level runs the reference's ten in-place
+// sweeps (Gauss-Seidel: rows in order, each reading the entries already
+// updated): rows that read no other row and that no other row reads take
+// their ten updates at once, one thread each; the others run in one
+// workgroup, grouped in dependency levels: row i comes after every row
+// j < i that it reads (its new value) and after every row k < i that reads
+// x_i (the old value), so rows of one level neither read nor overwrite each
+// other's entries and run at once; their rows, vectors and schedule are
+// staged in LDS when they fit.
 // Every row sum starts at 0.0 and adds in ascending column order, compiled
 // without contraction, so the V-cycle equals the reference's bit for bit.
 #include <algorithm>

@@ -58,6 +58,11 @@ constexpr int GATHER_GROUP = 8;
 // workgroup.
 constexpr int ROW_LW = 68;
 static_assert(ROW_LW % SELL_CW == 0, "a pass covers whole SELL chunks");
+// The slices whose rows (with their lifts) need at most this many: the
+// midpoint rows of a Tet10 mesh, about three slices in four.  Half the LDS
+// per wavefront lets twice as many run per CU.
+constexpr int ROW_LW_SHORT = 32;
+static_assert(ROW_LW_SHORT % SELL_CW == 0, "a pass covers whole SELL chunks");
 // incidence rows whose loads a lane of the row gather issues together
 constexpr int ROW_UNROLL = 4;
 
@@ -548,7 +553,9 @@ __global__ void k_plan_fill(int64_t n, const int *__restrict__ rp,
 }
 
 // Numeric phase, thermal, in the order of the SELL view: a workgroup per
-// slice, a lane per SELL row (node r).  The element kernel left row a of
+// slice (slices[blockIdx], the plan's order: the short ones launched with
+// ROW_LW_SHORT accumulators, the rest with ROW_LW), a lane per SELL row
+// (node r).  The element kernel left row a of
 // every K_e (and f_e[a]) in the incidence slot of (e, a): step j of the
 // slice holds the j-th incidence (ascending element) of each of its 64
 // nodes side by side, so the lanes' loads of a step are contiguous.  The
@@ -567,7 +574,8 @@ __global__ void k_plan_fill(int64_t n, const int *__restrict__ rp,
 // workgroup leaves the largest |K_ii| in dmax[blockIdx.x].
 template <typename C, int LW>
 __global__ __launch_bounds__(64) void k_sys_rows_thermal(
-    int64_t n, const int *__restrict__ sperm, const int *__restrict__ cb,
+    const int *__restrict__ slices, int64_t n, const int *__restrict__ sperm,
+    const int *__restrict__ cb,
     const int *__restrict__ brp, const int *__restrict__ lptr,
     const int *__restrict__ diag, const int *__restrict__ ib, const int *__restrict__ icnt,
     const double *__restrict__ rows, const C *__restrict__ icode, int with_load,
@@ -576,7 +584,7 @@ __global__ __launch_bounds__(64) void k_sys_rows_thermal(
     unsigned long long *__restrict__ dmax) {
   __shared__ double acc[LW * 64];
   const int lane = threadIdx.x;
-  const int64_t sl = xcd_block();
+  const int64_t sl = slices[xcd_block()];
   const int64_t srow = sl * 64 + lane;
   int r = -1, nk = 0, nl = 0, lb = 0, ni = 0, kd = -1;
   if (srow < n) {
@@ -869,6 +877,24 @@ __global__ void k_inc_count(int64_t n, const int *__restrict__ diag_slot,
 }
 // Per slice: the incidence count of each SELL row (icnt) and the slice's
 // largest (nib[s], its steps of 64 incidence slots).
+// Accumulators slice sl's row gather needs: its longest row with its lift
+// entries, or its SELL chunks' entries.
+__global__ void k_slice_span(int64_t n, int64_t nslices, const int *__restrict__ sperm,
+                             const int *__restrict__ brp, const int *__restrict__ lptr,
+                             const int *__restrict__ cb, int *__restrict__ span) {
+  const int64_t sl = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (sl >= nslices) return;
+  int m = SELL_CW * (cb[sl + 1] - cb[sl]);
+  for (int l = 0; l < 64; ++l) {
+    const int64_t srow = sl * 64 + l;
+    if (srow < n) {
+      const int r = sperm[srow];
+      m = max(m, brp[r + 1] - brp[r] + lptr[r + 1] - lptr[r]);
+    }
+  }
+  span[sl] = m;
+}
+
 __global__ void k_slice_inc(int64_t n, int64_t nslices, const int *__restrict__ sperm,
                             const int *__restrict__ cnt, int *__restrict__ icnt,
                             int *__restrict__ nib) {
@@ -1381,6 +1407,21 @@ std::shared_ptr<SystemPlan> system_plan(gs_ctx *ctx, gs_mesh *m,
                          np.diag_slot.get(), np.contrib_ptr.get(), np.contrib.get(),
                          P->ib.get(), P->incpos.get());
       GS_LAUNCH_CHECK("k_inc_pos");
+      // the gather's slice order: short ones first (each part in slice order)
+      auto span = dalloc<int>(S.nslices);
+      hipLaunchKernelGGL(k_slice_span, nblk(S.nslices), BS, 0, s, n, S.nslices,
+                         S.sperm.get(), P->row_ptr.get(), P->lift_ptr.get(), S.cb.get(),
+                         span.get());
+      GS_LAUNCH_CHECK("k_slice_span");
+      const std::vector<int> hs = to_host(ctx, span.get(), S.nslices);
+      std::vector<int> order;
+      order.reserve((size_t)S.nslices);
+      for (int64_t sl = 0; sl < S.nslices; ++sl)
+        if (hs[sl] <= ROW_LW_SHORT) order.push_back((int)sl);
+      P->n_tshort = (int64_t)order.size();
+      for (int64_t sl = 0; sl < S.nslices; ++sl)
+        if (hs[sl] > ROW_LW_SHORT) order.push_back((int)sl);
+      P->tslices = to_device(ctx, order);
     }
     int maxlen = 0;
     if (n > 0) {
@@ -1449,12 +1490,19 @@ void system_gather(hipStream_t s, const gs_mesh *m, const SystemPlan &P, const d
   if (P.dpn == 1) {
     const SellPattern &S = *P.sell;
     auto go = [&](const auto *code) {
-      hipLaunchKernelGGL((k_sys_rows_thermal<std::remove_cv_t<std::remove_pointer_t<
-                              decltype(code)>>, ROW_LW>),
-                         (unsigned)S.nslices, 64, 0, s, P.n, S.sperm.get(), S.cb.get(),
-                         P.row_ptr.get(), P.lift_ptr.get(), P.diag.get(), P.ib.get(),
-                         P.icnt.get(),
-                         Ke, code, (int)with_load, sval, sval32, liftv, fload, dval, dmax);
+      using C = std::remove_cv_t<std::remove_pointer_t<decltype(code)>>;
+      const int64_t ns = P.n_tshort, nl = S.nslices - P.n_tshort;
+      if (ns > 0)
+        hipLaunchKernelGGL((k_sys_rows_thermal<C, ROW_LW_SHORT>), (unsigned)ns, 64, 0, s,
+                           P.tslices.get(), P.n, S.sperm.get(), S.cb.get(), P.row_ptr.get(),
+                           P.lift_ptr.get(), P.diag.get(), P.ib.get(), P.icnt.get(), Ke,
+                           code, (int)with_load, sval, sval32, liftv, fload, dval, dmax);
+      if (nl > 0)
+        hipLaunchKernelGGL((k_sys_rows_thermal<C, ROW_LW>), (unsigned)nl, 64, 0, s,
+                           P.tslices.get() + ns, P.n, S.sperm.get(), S.cb.get(),
+                           P.row_ptr.get(), P.lift_ptr.get(), P.diag.get(), P.ib.get(),
+                           P.icnt.get(), Ke, code, (int)with_load, sval, sval32, liftv,
+                           fload, dval, dmax + ns);
     };
     if (S.nslices > 0) {
       if (P.icode8) go((const uint8_t *)P.icode8.get());

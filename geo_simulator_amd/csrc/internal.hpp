@@ -173,6 +173,10 @@ struct SystemPlan {
   // 2-byte).
   DArr<int> incpos, ib, icnt;
   int64_t n_islots = 0;
+  // the row gather's slices, those whose rows and lifts fit the short pass
+  // (ROW_LW_SHORT accumulators) first: n_tshort of them
+  DArr<int> tslices;
+  int64_t n_tshort = 0;
   DArr<uint8_t> icode8;
   DArr<unsigned short> icode16;
   DArr<int> lift_slots;

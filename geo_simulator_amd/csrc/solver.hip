@@ -629,6 +629,7 @@ __global__ void k_fcg_decide(CgState *__restrict__ st, const double *__restrict_
                              double target) {
   if (sc[3] == 0.0) {
     st->outer_end = 2;
+    st->outer_check = 0;
     st->stop = 1;
     return;
   }

@@ -284,6 +284,35 @@ def test_mixed_inner_cap_and_outer_cap():
     assert not s2.converged and s2.outer_iterations == 1 and s2.relative_residual > 1e-14
 
 
+@pytest.mark.parametrize("case", ["thermal", "elastic"])
+@pytest.mark.parametrize("cap", [2, 4, 9])
+def test_mixed_many_device_decided_cycles(case, cap):
+    """Short recursion cycles (max_inner_iterations) make most outer steps
+    ones the recursion predicts will not converge, which the device decides
+    (restart flag, check, end) without a host read: the solve still
+    converges, reports the true residual of what it returns, counts at least
+    one SpMV per pass and per outer step, and matches a direct solve."""
+    if case == "thermal":
+        mesh, dof, _ = W.thermal_box(G, 6, 6, 4)
+        K, f = G.Assembler.assemble_thermal_system(mesh, dof, 3.0, 1e-6)
+    else:
+        mesh, dof, _ = W.elastic_box(G, 4, 4, 3)
+        mat = G.IsotropicElasticity(1e11, 0.25)
+        K, f = G.Assembler.assemble_elasticity_system(mesh, dof, mat, 3300.0, [0, 0, -9.81])
+    x, st = (G.ConjugateGradient().with_precision("mixed").with_tolerance(1e-11)
+             .with_abs_tolerance(0.0).with_max_inner_iterations(cap)
+             .with_max_outer_iterations(5000).with_max_iterations(100000).solve(K, f))
+    Ks = K.to_scipy()
+    fh, xh = f.cpu().numpy(), x.cpu().numpy()
+    true = np.linalg.norm(fh - Ks @ xh) / np.linalg.norm(fh)
+    assert st.converged and true < 1e-11 * (1 + 1e-3)
+    assert abs(true - st.relative_residual) <= 1e-3 * true + 1e-16
+    assert st.outer_iterations >= 2
+    assert st.spmv_count >= st.iterations + st.outer_iterations
+    xr = spla.spsolve(Ks.tocsc(), fh)
+    assert np.linalg.norm(xh - xr) / np.linalg.norm(xr) < 1e-8
+
+
 def block_random(nb, seed, wide=False, holes=0.3):
     """Square matrix of order 3 nb with 3x3-block structure, some entries of
     the stored blocks missing (holes), block rows of 1 to about 40 blocks."""

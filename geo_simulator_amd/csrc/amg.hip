@@ -334,6 +334,31 @@ void sweep_levels(const HCsr &A, std::vector<int> &iso, std::vector<int> &ptr,
     if (lv[i] >= 0) ord[(size_t)at[(size_t)lv[i]]++] = (int)i;
 }
 
+// The sliced copy of a level's rows: position q (row erow[q], or q) of slice
+// q / 64 takes its row's entries in order at (ecb[s] + k) * 64 + q % 64, and
+// column -1 with value 0 past its row's end up to the slice's width (the
+// positions past the last row included).
+__global__ void k_amg_fill_ell(int64_t n, int64_t npos, const int *__restrict__ rp,
+                               const int *__restrict__ col, const double *__restrict__ val,
+                               const int *__restrict__ ecb, const int *__restrict__ erow,
+                               int *__restrict__ ecol, double *__restrict__ eval) {
+  const int64_t q = (int64_t)blockIdx.x * BS + threadIdx.x;
+  if (q >= npos) return;
+  const int64_t sl = q >> 6, lane = q & 63;
+  int b = 0, len = 0;
+  if (q < n) {
+    const int64_t i = erow ? (int64_t)erow[q] : q;
+    b = rp[i];
+    len = rp[i + 1] - b;
+  }
+  const int c0 = ecb[sl], m = ecb[sl + 1] - c0;
+  for (int k = 0; k < m; ++k) {
+    const size_t at = (size_t)(c0 + k) * 64 + lane;
+    ecol[at] = k < len ? col[b + k] : -1;
+    eval[at] = k < len ? val[b + k] : 0.0;
+  }
+}
+
 DevCsr upload(gs_ctx *ctx, const HCsr &A) {
   if (A.nnz() >= INT32_MAX)
     throw Error(GS_ERR_TOO_LARGE, "AMG level with 2^31 or more stored entries");
@@ -377,16 +402,6 @@ DevCsr upload(gs_ctx *ctx, const HCsr &A) {
     cb[sl + 1] = cb[sl] + (int)m;
   }
   const size_t ne = (size_t)cb[ns] * 64;
-  std::vector<int> ec(ne, -1);
-  std::vector<double> ev(ne, 0.0);
-  for (int64_t q = 0; q < n; ++q) {
-    const int64_t i = row_at(q), sl = q / 64, lane = q % 64;
-    for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
-      const size_t at = (size_t)(cb[sl] + (p - A.rp[i])) * 64 + lane;
-      ec[at] = A.col[p];
-      ev[at] = A.val[p];
-    }
-  }
   // rows longer than a slice is wide: when they are few and lead the sorted
   // order, the slices that hold them go to one wavefront per row
   if (!pos.empty()) {
@@ -399,9 +414,16 @@ DevCsr upload(gs_ctx *ctx, const HCsr &A) {
   for (int64_t q = D.n_long; q < n; ++q) longest = std::max(longest, len(row_at(q)));
   D.ew = longest > 72 ? 8 : 4;
   D.ecb = to_device(ctx, cb);
-  D.ecol = to_device(ctx, ec);
-  D.eval = to_device(ctx, ev);
   if (!pos.empty()) D.erow = to_device(ctx, std::vector<int>(pos.begin(), pos.end()));
+  // the slices' entries, filled on the device from the uploaded rows
+  D.ecol = dalloc<int>((int64_t)ne);
+  D.eval = dalloc<double>((int64_t)ne);
+  if (ns > 0) {
+    hipLaunchKernelGGL(k_amg_fill_ell, nblk(ns * 64), BS, 0, ctx->stream, n, ns * 64,
+                       D.rp.get(), D.col.get(), D.val.get(), D.ecb.get(), D.erow.get(),
+                       D.ecol.get(), D.eval.get());
+    GS_LAUNCH_CHECK("k_amg_fill_ell");
+  }
   return D;
 }
 

@@ -10,9 +10,9 @@
 // dropped.  The coarsening is sequential by definition (each choice changes
 // the counts the next one reads), so the hierarchy is built on the host
 // from A in the caller's numbering, with loops that give the reference's
-// result at O(nnz log n) instead of its O(n^2) scans: buckets by count of
-// index min-heaps, stale entries skipped, stand in for the scan for the
-// largest count (smallest index first), and a sparse accumulator for the
+// result in about O(nnz) instead of its O(n^2) scans: a two-level bitmap of
+// point indices per count stands in for the scan for the largest count
+// (smallest index first), and a sparse accumulator for the
 // dense column loop of matrix_multiply (every product reaches its sum in the
 // same order).  Rows of the strength, prolongation and Galerkin matrices are
 // built on host threads, each row by one thread.
@@ -127,48 +127,78 @@ HCsr strength(const HCsr &A, double theta) {
   });
 }
 
+// A set of point indices as a two-level bitmap: insert and erase in O(1),
+// the smallest member by scanning one summary word per 4096 points.  The
+// bitmap is kept in pages of 4096 points, allocated when a member first
+// falls in them, so a set holds memory in proportion to its members.
+struct IndexSet {
+  std::vector<uint64_t> summary;                     // bit w: page s, word w nonzero
+  std::vector<std::unique_ptr<uint64_t[]>> page;     // 64 words each, or none
+  int64_t count = 0;
+  explicit IndexSet(int64_t n)
+      : summary((size_t)((n + 4095) / 4096), 0), page((size_t)((n + 4095) / 4096)) {}
+  void insert(int64_t i) {
+    const int64_t s = i >> 12, w = (i >> 6) & 63;
+    auto &pg = page[(size_t)s];
+    if (!pg) pg.reset(new uint64_t[64]());
+    pg[w] |= 1ull << (i & 63);
+    summary[(size_t)s] |= 1ull << w;
+    ++count;
+  }
+  void erase(int64_t i) {
+    const int64_t s = i >> 12, w = (i >> 6) & 63;
+    uint64_t *pg = page[(size_t)s].get();
+    pg[w] &= ~(1ull << (i & 63));
+    if (!pg[w]) summary[(size_t)s] &= ~(1ull << w);
+    --count;
+  }
+  int64_t smallest() const {
+    for (size_t s = 0; s < summary.size(); ++s)
+      if (summary[s]) {
+        const int w = __builtin_ctzll(summary[s]);
+        return (int64_t)s * 4096 + w * 64 + __builtin_ctzll(page[s][w]);
+      }
+    return -1;
+  }
+};
+
 // coarsen (:286-355) on the strength matrix S: 1 marks a coarse point.  The
-// undecided points with a positive count sit in buckets by count, each a
-// min-heap of point indices; an entry is stale once its point was decided or
-// its count grew (the point then also sits in the next bucket).  The top
-// bucket's smallest valid index is the point the reference's scan picks.
+// undecided points with a positive count sit in the set of their count; the
+// top set's smallest index is the point the reference's scan picks (the
+// largest count, the first index among equals).
 std::vector<unsigned char> coarsen(const HCsr &S, int64_t &num_coarse) {
   const int64_t n = S.rows;
   std::vector<int64_t> lam((size_t)n);
   std::vector<unsigned char> cf((size_t)n, 0), open((size_t)n, 1);
-  std::vector<std::vector<int>> bucket(1);
+  std::vector<std::unique_ptr<IndexSet>> bucket(1);
   int64_t top = 0;
-  auto push = [&](int64_t i) {
-    const int64_t l = lam[i];
+  auto set_of = [&](int64_t l) -> IndexSet & {
     if (l >= (int64_t)bucket.size()) bucket.resize((size_t)l + 1);
-    auto &h = bucket[(size_t)l];
-    h.push_back((int)i);
-    std::push_heap(h.begin(), h.end(), std::greater<int>());
-    top = std::max(top, l);
+    if (!bucket[(size_t)l]) bucket[(size_t)l].reset(new IndexSet(n));
+    return *bucket[(size_t)l];
   };
   for (int64_t i = 0; i < n; ++i) {
     int64_t c = 0;
     for (int64_t p = S.rp[i]; p < S.rp[i + 1]; ++p) c += S.col[p] != i;
     lam[i] = c;
-    if (c > 0) push(i);
+    if (c > 0) {
+      set_of(c).insert(i);
+      top = std::max(top, c);
+    }
   }
+  auto drop = [&](int64_t i) {  // i leaves the set of its count; an empty set is freed
+    auto &b = bucket[(size_t)lam[i]];
+    b->erase(i);
+    if (b->count == 0) b.reset();
+  };
+  auto decide = [&](int64_t i) {  // i leaves the undecided points
+    open[i] = 0;
+    if (lam[i] > 0) drop(i);
+  };
   int64_t nc = 0, left = n;
   while (left > 0) {
-    int64_t pick = -1;
-    while (top > 0 && pick < 0) {
-      auto &h = bucket[(size_t)top];
-      while (!h.empty()) {
-        std::pop_heap(h.begin(), h.end(), std::greater<int>());
-        const int i = h.back();
-        h.pop_back();
-        if (open[i] && lam[i] == top) {
-          pick = i;
-          break;
-        }
-      }
-      if (pick < 0) --top;
-    }
-    if (pick < 0) {  // no undecided point has a strong connection left
+    while (top > 0 && (!bucket[(size_t)top] || bucket[(size_t)top]->count == 0)) --top;
+    if (top == 0) {  // no undecided point has a strong connection left
       for (int64_t i = 0; i < n; ++i)
         if (open[i]) {
           cf[i] = 1;
@@ -177,20 +207,23 @@ std::vector<unsigned char> coarsen(const HCsr &S, int64_t &num_coarse) {
         }
       break;
     }
+    const int64_t pick = bucket[(size_t)top]->smallest();
     cf[pick] = 1;
-    open[pick] = 0;
+    decide(pick);
     ++nc;
     --left;
     for (int64_t p = S.rp[pick]; p < S.rp[pick + 1]; ++p) {
       const int64_t j = S.col[p];
       if (j == pick || !open[j]) continue;
-      open[j] = 0;  // fine
+      decide(j);  // fine
       --left;
       for (int64_t q = S.rp[j]; q < S.rp[j + 1]; ++q) {
         const int64_t k = S.col[q];
         if (k != j && open[k]) {
+          if (lam[k] > 0) drop(k);
           ++lam[k];
-          push(k);
+          set_of(lam[k]).insert(k);
+          top = std::max(top, lam[k]);
         }
       }
     }

@@ -289,30 +289,39 @@ HCsr transpose(const HCsr &A) {
 HCsr matmul(const HCsr &A, const HCsr &B) {
   const int64_t nb = std::max<int64_t>(1, B.cols);
   return build_rows(A.rows, B.cols, [&] {
-    // per thread: a dense accumulator over B's columns and the row's columns
+    // per thread: a dense accumulator over B's columns, the row that last
+    // touched each column, and the row's columns
     auto acc = std::make_shared<std::vector<double>>((size_t)nb);
-    auto seen = std::make_shared<std::vector<int64_t>>((size_t)nb, -1);
+    auto seen = std::make_shared<std::vector<int>>((size_t)nb, -1);
     auto cols = std::make_shared<std::vector<int>>();
     return [&, acc, seen, cols](int64_t i, std::vector<int> &col, std::vector<double> &val) {
-      cols->clear();
+      double *const ac = acc->data();
+      int *const sn = seen->data();
+      const int64_t *const brp = B.rp.data();
+      const int *const bcol = B.col.data();
+      const double *const bval = B.val.data();
+      const int row = (int)i;
+      std::vector<int> &cs = *cols;
+      cs.clear();
       for (int64_t p = A.rp[i]; p < A.rp[i + 1]; ++p) {
         const double a = A.val[p];
         const int64_t k = A.col[p];
-        for (int64_t q = B.rp[k]; q < B.rp[k + 1]; ++q) {
-          const int j = B.col[q];
-          if ((*seen)[j] != i) {
-            (*seen)[j] = i;
-            (*acc)[j] = 0.0;
-            cols->push_back(j);
+        const int64_t q1 = brp[k + 1];
+        for (int64_t q = brp[k]; q < q1; ++q) {
+          const int j = bcol[q];
+          if (sn[j] != row) {
+            sn[j] = row;
+            ac[j] = 0.0;
+            cs.push_back(j);
           }
-          (*acc)[j] += a * B.val[q];
+          ac[j] += a * bval[q];
         }
       }
-      std::sort(cols->begin(), cols->end());
-      for (int j : *cols)
-        if (std::fabs((*acc)[j]) > 1e-14) {
+      std::sort(cs.begin(), cs.end());
+      for (int j : cs)
+        if (std::fabs(ac[j]) > 1e-14) {
           col.push_back(j);
-          val.push_back((*acc)[j]);
+          val.push_back(ac[j]);
         }
     };
   });

@@ -87,13 +87,17 @@ def cpu_baseline(nx, ny, nz, threads):
     _, st = O.cg_solve(Kbc, fbc, max_iter=5000, tol=1e-8)
     t_cg = time.perf_counter() - t0
     per_iter = t_cg / max(1, st.iterations + 1)
+    # a parallel CG beside the reference's serial one (timing only)
+    _, sp = O.cg_solve_omp(Kbc, fbc, max_iter=5000, tol=1e-8, nthreads=threads)
+    t_cg_par = sp.solve_time
     t0 = time.perf_counter()
     _, sb = O.bicgstab_solve(Kbc, fbc, max_iter=5, tol=1e-8)
     bicg_per_iter = (time.perf_counter() - t0) / max(1, sb.iterations)
     t0 = time.perf_counter()
     _, sg = O.gmres_solve(Kbc, fbc, max_iter=10, restart=30, tol=1e-8)
     gm_per_step = (time.perf_counter() - t0) / max(1, sg.iterations)
-    return t_asm + t_cg, t_asm, t_cg, st, per_iter, bicg_per_iter, gm_per_step
+    return (t_asm + t_cg, t_asm, t_cg, st, per_iter, bicg_per_iter, gm_per_step,
+            (t_cg_par, sp.iterations))
 
 
 def make_cg(G, prec, tol, inner_tol=None, abs_tol=None, max_iter=200000):
@@ -600,7 +604,7 @@ def main():
     if not args.no_cpu_baseline:
         threads = min(os.cpu_count() or 1, 16)
         log(f"[bench] CPU baseline (oracle, {threads} threads for the element loop)")
-        total, t_asm, t_cg, cst, per_iter, bicg_per_iter, gm_per_step = cpu_baseline(
+        total, t_asm, t_cg, cst, per_iter, bicg_per_iter, gm_per_step, par = cpu_baseline(
             nx, ny, nz, threads)
         cpu = {"value": total, "unit": "s", "cores": threads, "kind": "port",
                "sample": (f"oracle C restatement on the same {nx}x{ny}x{nz} workload, "
@@ -611,6 +615,10 @@ def main():
                           f"{per_iter*1e3:.1f} ms/iter; GPU FP64: "
                           f"{results['fp64']['iterations']})"),
                "cg_iterations": cst.iterations, "assembly_s": t_asm, "cg_s": t_cg,
+               "cg_parallel": {"s": par[0], "iterations": par[1], "threads": threads,
+                               "note": "the same Jacobi-CG loop with OpenMP rows and "
+                                       "reductions (timing column; not the reference's "
+                                       "serial order)"},
                "bicgstab_s_per_pass": bicg_per_iter,
                "gmres30_s_per_step": gm_per_step,
                "host": host_cpu()}

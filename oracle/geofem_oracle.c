@@ -1142,6 +1142,91 @@ static double now_s(void) {
 /* src/linalg/iterative.rs:46-119 (+ :123-131 for the preconditioner choice),
  * statement by statement.  The iteration counter excludes the converging
  * pass; the recursive residual is never recomputed. */
+/* Timing column only, not part of the restatement: the loop of or_cg_solve
+ * with its SpMV rows and vector updates split over nthreads OpenMP threads
+ * and its dot products as OpenMP reductions (their summation order depends
+ * on the thread count, so iterates agree with or_cg_solve to rounding, not
+ * bit for bit).  It gives the CPU side a parallel CG beside the reference's
+ * serial one. */
+int or_cg_solve_omp(const or_csr *A, const double *b, int64_t max_iter, double tol,
+                    double abs_tol, int nthreads, double *x, or_stats *st) {
+  const int64_t n = A->rows;
+  const double t0 = now_s();
+  set_threads(nthreads);
+  double bb = 0.0;
+#pragma omp parallel for reduction(+ : bb) schedule(static)
+  for (int64_t i = 0; i < n; ++i) bb += b[i] * b[i];
+  const double b_norm = sqrt(bb);
+  memset(st, 0, sizeof(*st));
+  for (int64_t i = 0; i < n; ++i) x[i] = 0.0;
+  if (b_norm < 1e-25) {
+    st->converged = 1;
+    st->solve_time = now_s() - t0;
+    return OR_OK;
+  }
+  double *r = (double *)malloc((size_t)n * sizeof(double));
+  double *z = (double *)malloc((size_t)n * sizeof(double));
+  double *p = (double *)malloc((size_t)n * sizeof(double));
+  double *ap = (double *)malloc((size_t)n * sizeof(double));
+  double *dinv = (double *)malloc((size_t)n * sizeof(double));
+  if (!r || !z || !p || !ap || !dinv) {
+    free(r); free(z); free(p); free(ap); free(dinv);
+    return OR_ERR_OOM;
+  }
+  or_jacobi_diag_inv(A, dinv);
+  double rz = 0.0;
+#pragma omp parallel for reduction(+ : rz) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    r[i] = b[i];
+    z[i] = r[i] * dinv[i];
+    p[i] = z[i];
+    rz += r[i] * z[i];
+  }
+  int64_t it = 0;
+  int converged = 0;
+  double final_res = b_norm;
+  while (it < max_iter) {
+    double pap = 0.0;
+#pragma omp parallel for reduction(+ : pap) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      double s = 0.0;
+      for (int64_t k = A->indptr[i]; k < A->indptr[i + 1]; ++k) s += A->data[k] * p[A->indices[k]];
+      ap[i] = s;
+      pap += p[i] * s;
+    }
+    if (fabs(pap) < 1e-30) break;
+    const double alpha = rz / pap;
+    double rr = 0.0;
+#pragma omp parallel for reduction(+ : rr) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      x[i] += alpha * p[i];
+      r[i] -= alpha * ap[i];
+      rr += r[i] * r[i];
+    }
+    const double rn = sqrt(rr);
+    final_res = rn;
+    if (rn < tol * b_norm || rn < abs_tol) { converged = 1; break; }
+    double rz_new = 0.0;
+#pragma omp parallel for reduction(+ : rz_new) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+      z[i] = r[i] * dinv[i];
+      rz_new += r[i] * z[i];
+    }
+    const double beta = rz_new / rz;
+    rz = rz_new;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    ++it;
+  }
+  st->iterations = it;
+  st->residual_norm = final_res;
+  st->relative_residual = final_res / b_norm;
+  st->converged = converged;
+  st->solve_time = now_s() - t0;
+  free(r); free(z); free(p); free(ap); free(dinv);
+  return OR_OK;
+}
+
 int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
                 double abs_tol, int use_precond, double *x, or_stats *st) {
   const int64_t n = A->rows;

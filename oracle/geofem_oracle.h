@@ -152,6 +152,10 @@ typedef struct {
 
 void or_spmv(const or_csr *A, const double *x, double *y);
 void or_jacobi_diag_inv(const or_csr *A, double *dinv);
+/* Timing column only (not a restatement): or_cg_solve's loop on nthreads
+ * OpenMP threads, reductions in thread-dependent order. */
+int or_cg_solve_omp(const or_csr *A, const double *b, int64_t max_iter, double tol,
+                    double abs_tol, int nthreads, double *x, or_stats *st);
 int or_cg_solve(const or_csr *A, const double *b, int64_t max_iter, double tol,
                 double abs_tol, int use_precond, double *x, or_stats *st);
 int or_bicgstab_solve(const or_csr *A, const double *b, int64_t max_iter,

@@ -118,3 +118,19 @@ def test_elastic_patch_test_linear_field(oracle):
     Kbc, fbc = oracle.apply_dirichlet(K, np.zeros(3 * n), mask, vals)
     x, st = oracle.cg_solve(Kbc, fbc, max_iter=10000, tol=1e-14, abs_tol=0.0)
     assert np.max(np.abs(x.reshape(n, 3) - u_exact)) < 1e-12
+
+
+def test_parallel_cg_timing_column_matches_serial(oracle):
+    """The OpenMP CG (a timing column of the CPU baseline, not a restatement)
+    runs the serial loop's arithmetic up to the order of its reductions: the
+    same iteration count within one and the same solution to rounding."""
+    xyz, conn = oracle.generate_cube(10, 10, 6, 1000.0, 1000.0, 600.0)
+    mask, vals = oracle.thermal_dirichlet(xyz, 600.0)
+    K = oracle.assemble_thermal(xyz, conn, 3.0)
+    f = oracle.assemble_thermal_load(xyz, conn, 1e-6)
+    Kb, fb = oracle.apply_dirichlet(K, f, mask, vals)
+    x1, s1 = oracle.cg_solve(Kb, fb, max_iter=5000, tol=1e-10)
+    for threads in (1, 4):
+        x2, s2 = oracle.cg_solve_omp(Kb, fb, max_iter=5000, tol=1e-10, nthreads=threads)
+        assert s2.converged and abs(s2.iterations - s1.iterations) <= 1
+        assert np.linalg.norm(x2 - x1) <= 1e-10 * np.linalg.norm(x1)
